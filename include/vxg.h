@@ -1,0 +1,222 @@
+/*
+ * vxg.h -- C ABI of the MI355X-native Voxelyze explicit-dynamics step.
+ *
+ * Every entry point replaces one member of the reference's object API
+ * (jonhiller/Voxelyze, C++11).  The reference has no FFI of its own; the
+ * object methods cited on each line ARE the boundary, and the header-only
+ * C++ facade in include/Voxelyze.h rebuilds that object API on top of these
+ * functions so that existing CVoxelyze programs recompile unchanged.
+ *
+ * Conventions
+ *   - A simulation is an opaque handle (one per host thread / HIP stream).
+ *   - Materials, voxels and links are addressed by their current position in
+ *     the reference's lists: voxelMats, voxelsList and linksList
+ *     (include/Voxelyze.h:89-113).  Positions shift when an element is
+ *     removed, exactly as CVoxelyze::voxel(int)/link(int) do.
+ *   - Booleans are int (0/1).  No C++ exceptions cross this boundary;
+ *     failures return -1 / 0 and set vxg_last_error().
+ *   - Arrays passed in or out are host memory, copied in and out; the
+ *     library owns all device memory.
+ *   - The same symbol set, with the prefix vxo_ (CPU oracle, test only) or
+ *     vxr_ (driver over the compiled reference, test only), is exported by
+ *     the parity checkers; only this vxg_ library is the product.
+ */
+#ifndef VXG_H
+#define VXG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct vxg_sim vxg_sim;
+
+/* material property selectors for vxg_material_set / vxg_material_get */
+enum vxg_material_prop {
+    VXG_MAT_POISSONS_RATIO = 0,   /* CVX_Material::setPoissonsRatio  VX_Material.h:75   */
+    VXG_MAT_DENSITY = 1,          /* setDensity                      VX_Material.h:83   */
+    VXG_MAT_STATIC_FRICTION = 2,  /* setStaticFriction               VX_Material.h:85   */
+    VXG_MAT_KINETIC_FRICTION = 3, /* setKineticFriction              VX_Material.h:87   */
+    VXG_MAT_INTERNAL_DAMPING = 4, /* setInternalDamping              VX_Material.h:92   */
+    VXG_MAT_GLOBAL_DAMPING = 5,   /* setGlobalDamping                VX_Material.h:94  */
+    VXG_MAT_COLLISION_DAMPING = 6,/* setCollisionDamping             VX_Material.h:96  */
+    VXG_MAT_CTE = 7,              /* setCte                          VX_Material.h:106  */
+    /* read-only (vxg_material_get) */
+    VXG_MAT_YOUNGS_MODULUS = 8,   /* youngsModulus()                 VX_Material.h:68   */
+    VXG_MAT_YIELD_STRESS = 9,     /* yieldStress()                   VX_Material.h:69   */
+    VXG_MAT_FAILURE_STRESS = 10,  /* failureStress()                 VX_Material.h:70   */
+    VXG_MAT_MASS = 11,            /* CVX_MaterialVoxel::mass()        VX_MaterialVoxel.h:37 */
+    VXG_MAT_MOMENT_INERTIA = 12,  /* momentInertia()                 VX_MaterialVoxel.h:38 */
+    VXG_MAT_YIELD_STRAIN = 13,    /* epsilonYield                    VX_Material.h:122  */
+    VXG_MAT_FAILURE_STRAIN = 14,  /* epsilonFail                     VX_Material.h:123  */
+    VXG_MAT_IS_LINEAR = 15        /* isModelLinear()                 VX_Material.h:66   */
+};
+
+/* dof bits, identical to dofComponent (VX_External.h:19-26) */
+enum vxg_dof {
+    VXG_X_TRANSLATE = 1 << 0, VXG_Y_TRANSLATE = 1 << 1, VXG_Z_TRANSLATE = 1 << 2,
+    VXG_X_ROTATE = 1 << 3, VXG_Y_ROTATE = 1 << 4, VXG_Z_ROTATE = 1 << 5
+};
+
+/* precision modes (vxg_set_precision) */
+enum vxg_precision {
+    VXG_FP64_PLAIN = 0, /* every operation in the reference's own types, bit-faithful */
+    VXG_MIXED = 1       /* FP32 link forces, FP64 pose/momentum integration */
+};
+
+/* link flag bits returned by vxg_link_get_state */
+enum vxg_link_flag {
+    VXG_LINK_LOCAL_VELOCITY_VALID = 1 << 0, /* VX_Link.h:80 */
+    VXG_LINK_SMALL_ANGLE = 1 << 1,          /* isSmallAngle()  VX_Link.h:56 */
+    VXG_LINK_YIELDED = 1 << 2,              /* isYielded()     VX_Link.h:57 */
+    VXG_LINK_FAILED = 1 << 3                /* isFailed()      VX_Link.h:58 */
+};
+
+/* voxel flag bits returned by vxg_voxel_get_state */
+enum vxg_voxel_flag {
+    VXG_VOX_INTERIOR = 1 << 1,              /* SURFACE bit, set when all 6 links exist  VX_Voxel.h:144 */
+    VXG_VOX_FLOOR_ENABLED = 1 << 2,         /* VX_Voxel.h:145 */
+    VXG_VOX_FLOOR_STATIC_FRICTION = 1 << 3, /* VX_Voxel.h:146 */
+    VXG_VOX_COLLISIONS_ENABLED = 1 << 5     /* VX_Voxel.h:147 */
+};
+
+/* ---------------------------------------------------------------- lifecycle */
+/* CVoxelyze::CVoxelyze(double voxelSize)           include/Voxelyze.h:69, src/Voxelyze.cpp:28-32 */
+vxg_sim* vxg_create(double voxel_size);
+/* CVoxelyze::~CVoxelyze()                           include/Voxelyze.h:72 */
+void vxg_destroy(vxg_sim* s);
+/* last error string of the handle (library-level; no reference counterpart beyond
+   CVX_Material::lastError(), VX_Material.h:41, which vxg_material_last_error mirrors) */
+const char* vxg_last_error(vxg_sim* s);
+/* CVoxelyze::voxelSize()                            include/Voxelyze.h:118 */
+double vxg_voxel_size(vxg_sim* s);
+
+/* -------------------------------------------------------------- environment */
+/* CVoxelyze::setGravity(float g)                    include/Voxelyze.h:124, src/Voxelyze.cpp:596-602 */
+void vxg_set_gravity(vxg_sim* s, float g);
+float vxg_gravity(vxg_sim* s);                       /* gravity()  Voxelyze.h:125 */
+/* CVoxelyze::enableFloor(bool)                      include/Voxelyze.h:127, src/Voxelyze.cpp:604-610 */
+void vxg_enable_floor(vxg_sim* s, int enabled);
+int vxg_is_floor_enabled(vxg_sim* s);                /* isFloorEnabled() Voxelyze.h:128 */
+/* CVoxelyze::enableCollisions(bool)                 include/Voxelyze.h:130, src/Voxelyze.cpp:612-622 */
+void vxg_enable_collisions(vxg_sim* s, int enabled);
+int vxg_is_collisions_enabled(vxg_sim* s);           /* Voxelyze.h:131 */
+/* CVoxelyze::setAmbientTemperature(float, bool)     include/Voxelyze.h:121, src/Voxelyze.cpp:585-594 */
+void vxg_set_ambient_temperature(vxg_sim* s, float temperature, int all_voxels);
+float vxg_ambient_temperature(vxg_sim* s);           /* Voxelyze.h:122 */
+
+/* ----------------------------------------------------------------- stepping */
+/* CVoxelyze::doTimeStep(float dt=-1)                include/Voxelyze.h:82, src/Voxelyze.cpp:251-284
+   Returns 1, or 0 when a link's axial strain exceeded 100 (divergence). */
+int vxg_do_time_step(vxg_sim* s, float dt);
+/* nsteps consecutive doTimeStep(dt) calls with no host work in between (captured into one
+   HIP graph on the device).  Returns the number of steps that completed; when a step
+   diverges the loop stops there and *diverged_at (if non-NULL) receives its 0-based index,
+   else -1.  Equivalent to: for k<nsteps: if(!doTimeStep(dt)) break; */
+int vxg_step(vxg_sim* s, float dt, int nsteps, int* diverged_at);
+/* CVoxelyze::recommendedTimeStep()                  include/Voxelyze.h:83, src/Voxelyze.cpp:286-311 */
+float vxg_recommended_time_step(vxg_sim* s);
+/* CVoxelyze::resetTime()                            include/Voxelyze.h:84, src/Voxelyze.cpp:313-321 */
+void vxg_reset_time(vxg_sim* s);
+/* CVoxelyze::currentTime (private member, Voxelyze.h:138) */
+float vxg_current_time(vxg_sim* s);
+/* CVoxelyze::stateInfo(stateInfoType, valueType)    include/Voxelyze.h:134, src/Voxelyze.cpp:752-800 */
+float vxg_state_info(vxg_sim* s, int info, int type);
+/* number of watched collision pairs: CVoxelyze::collisionList()->size()  Voxelyze.h:115 */
+int vxg_collision_count(vxg_sim* s);
+/* precision mode of the device step (no reference counterpart: the reference is FP64/FP32
+   mixed exactly as VXG_FP64_PLAIN reproduces) */
+int vxg_set_precision(vxg_sim* s, int mode);
+
+/* ---------------------------------------------------------------- materials */
+/* CVoxelyze::addMaterial(float E, float rho)        include/Voxelyze.h:86, src/Voxelyze.cpp:357-366
+   Returns the material index (materialCount()-1) or -1. */
+int vxg_material_add(vxg_sim* s, float youngs_modulus, float density);
+int vxg_material_count(vxg_sim* s);                  /* materialCount()  Voxelyze.h:92 */
+/* CVX_Material::setModelLinear                      include/VX_Material.h:64, src/VX_Material.cpp:355-384 */
+int vxg_material_set_model_linear(vxg_sim* s, int mat, float youngs_modulus, float failure_stress);
+/* CVX_Material::setModelBilinear                    include/VX_Material.h:65, src/VX_Material.cpp:389-434 */
+int vxg_material_set_model_bilinear(vxg_sim* s, int mat, float youngs_modulus, float plastic_modulus,
+                                    float yield_stress, float failure_stress);
+/* CVX_Material::setModel(int, float*, float*)       include/VX_Material.h:63, src/VX_Material.cpp:283-350 */
+int vxg_material_set_model(vxg_sim* s, int mat, int count, const float* strain, const float* stress);
+/* the clamping setters of CVX_Material (VX_Material.h:75-106; VX_Material.cpp:472-523) by selector */
+void vxg_material_set(vxg_sim* s, int mat, int prop, float value);
+/* the corresponding getters, plus the read-only selectors of enum vxg_material_prop */
+float vxg_material_get(vxg_sim* s, int mat, int prop);
+/* CVX_Material::setExternalScaleFactor(Vec3D<double>)  include/VX_Material.h:101, VX_Material.cpp:509-515 */
+void vxg_material_set_external_scale(vxg_sim* s, int mat, double sx, double sy, double sz);
+/* CVX_Material::lastError()                         include/VX_Material.h:41 */
+const char* vxg_material_last_error(vxg_sim* s, int mat);
+/* CVX_Material::stress(strain, transverseStrainSum, forceLinear)  VX_Material.h:46, VX_Material.cpp:165-195 */
+float vxg_material_stress(vxg_sim* s, int mat, float strain, float transverse_strain_sum, int force_linear);
+/* CVX_Material::modulus(strain)                     include/VX_Material.h:47, src/VX_Material.cpp:214-226 */
+float vxg_material_modulus(vxg_sim* s, int mat, float strain);
+/* CVX_Material::modelDataPoints/modelDataStrain/modelDataStress  VX_Material.h:71-73.
+   Copies up to cap points; returns the point count. */
+int vxg_material_model_data(vxg_sim* s, int mat, float* strain, float* stress, int cap);
+
+/* ------------------------------------------------------------------- voxels */
+/* CVoxelyze::setVoxel(CVX_Material*, int, int, int) include/Voxelyze.h:96, src/Voxelyze.cpp:422-437
+   mat < 0 removes the voxel (the reference's NULL material).  Returns the voxel index or -1. */
+int vxg_voxel_set(vxg_sim* s, int mat, int ix, int iy, int iz);
+int vxg_voxel_count(vxg_sim* s);                      /* voxelCount()  Voxelyze.h:98 */
+/* CVoxelyze::voxel(int x, int y, int z) -> index or -1   include/Voxelyze.h:97 */
+int vxg_voxel_find(vxg_sim* s, int ix, int iy, int iz);
+/* CVX_Voxel::indexX/Y/Z (VX_Voxel.h:66-68) and material() (VX_Voxel.h:70, as a material index) */
+int vxg_voxel_info(vxg_sim* s, int vox, int* ijk3);
+/* CVX_Voxel::setTemperature(float)                  include/VX_Voxel.h:112, src/VX_Voxel.cpp:108-114 */
+void vxg_voxel_set_temperature(vxg_sim* s, int vox, float temperature);
+float vxg_voxel_temperature(vxg_sim* s, int vox);    /* temperature()  VX_Voxel.h:111 */
+/* Bulk state of voxels [first, first+count): position() (VX_Voxel.h:79), orientation()
+   (VX_Voxel.h:92), linear and angular momentum (VX_Voxel.h:164,166; velocity() = linMom/m)
+   and the voxel flag bits.  Any output pointer may be NULL.  Layouts: pos/lin/ang 3 per
+   voxel, orient 4 per voxel (w,x,y,z). */
+void vxg_voxel_get_state(vxg_sim* s, int first, int count, double* pos, double* orient,
+                         double* lin_mom, double* ang_mom, uint32_t* flags);
+
+/* ---------------------------------------------------------------- externals */
+/* CVX_External::setFixed(bool x6) -- mask of vxg_dof bits   VX_External.h:65, VX_External.cpp:47-51 */
+void vxg_ext_set_fixed(vxg_sim* s, int vox, int dof_mask);
+/* CVX_External::setDisplacement(dofComponent, double)       VX_External.h:69, VX_External.cpp:53-66 */
+void vxg_ext_set_displacement(vxg_sim* s, int vox, int dof, double displacement);
+/* CVX_External::setDisplacementAll(Vec3D, Vec3D)            VX_External.h:70, VX_External.cpp:68-75 */
+void vxg_ext_set_displacement_all(vxg_sim* s, int vox, const double* translation3, const double* rotation3);
+/* CVX_External::clearDisplacement(dofComponent)             VX_External.h:72, VX_External.cpp:77-89 */
+void vxg_ext_clear_displacement(vxg_sim* s, int vox, int dof);
+/* CVX_External::clearDisplacementAll()                      VX_External.h:73, VX_External.cpp:91-98 */
+void vxg_ext_clear_displacement_all(vxg_sim* s, int vox);
+/* CVX_External::setForce(float, float, float)               VX_External.h:78 */
+void vxg_ext_set_force(vxg_sim* s, int vox, float fx, float fy, float fz);
+/* CVX_External::setMoment(float, float, float)              VX_External.h:80 */
+void vxg_ext_set_moment(vxg_sim* s, int vox, float mx, float my, float mz);
+/* CVX_External::reset()                                     VX_External.h:48, VX_External.cpp:37-44 */
+void vxg_ext_reset(vxg_sim* s, int vox);
+/* CVX_Voxel::externalExists() (VX_Voxel.h:72) plus the external's fixed mask, force(),
+   moment(), translation(), rotation() (VX_External.h:60-76).  Returns 0 when no external
+   object was ever created (outputs untouched). */
+int vxg_ext_get(vxg_sim* s, int vox, int* dof_mask, float* force3, float* moment3,
+                double* translation3, double* rotation3);
+
+/* -------------------------------------------------------------------- links */
+int vxg_link_count(vxg_sim* s);                       /* linkCount()  Voxelyze.h:111 */
+/* CVoxelyze::link(x, y, z, direction) -> index or -1   include/Voxelyze.h:110, src/Voxelyze.cpp:501-507 */
+int vxg_link_find(vxg_sim* s, int ix, int iy, int iz, int direction);
+/* CVX_Link::voxel(bool) (VX_Link.h:47) as voxel indices, and the link axis (VX_Link.h:88) */
+int vxg_link_info(vxg_sim* s, int link, int* neg_vox, int* pos_vox);
+/* Bulk state of links [first, first+count) in linksList order: axialStrain() and
+   axialStress() (VX_Link.h:52-54), maxStrain/strainOffset (VX_Link.h:75), flag bits,
+   the local pose pos2/angle1v/angle2v (9 per link, VX_Link.h:101), the four end
+   vectors forceNeg, forcePos, momentNeg, momentPos (12 per link, VX_Link.h:71-72;
+   force(bool)/moment(bool) VX_Link.h:48-49) and currentRestLength (VX_Link.h:104).
+   Any output pointer may be NULL. */
+void vxg_link_get_state(vxg_sim* s, int first, int count, float* strain, float* stress,
+                        float* max_strain, float* strain_offset, uint32_t* flags,
+                        double* local9, double* forces12, double* rest_length);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VXG_H */

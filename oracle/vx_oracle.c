@@ -1,0 +1,1790 @@
+/*
+ * vx_oracle.c -- CPU restatement of Voxelyze's explicit-dynamics path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the parity checker for the HIP
+ * product library (voxelyze_amd/csrc).  Only tests/, __graft_entry__.smoke()
+ * and the cpu_baseline leg of bench.py may load it.  It is never linked into
+ * or called by the product.
+ *
+ * It restates, in plain C with the reference's own float/double cast points,
+ * expression trees and summation orders, the functions below (file:line in
+ * jonhiller/Voxelyze):
+ *   materials   src/VX_Material.cpp:15-75,165-226,283-434,472-532
+ *               src/VX_MaterialVoxel.cpp:16-79, src/VX_MaterialLink.cpp:16-141
+ *   model       src/Voxelyze.cpp:323-668 (addMaterial, setVoxel, add/remove/
+ *               replaceVoxel, addLink/removeLink, combinedMaterial, setters)
+ *   links       src/VX_Link.cpp:26-257 (ctor, reset, orientLink, updateForces,
+ *               updateStrain, strainEnergy, axialStiffness)
+ *   voxels      src/VX_Voxel.cpp:23-298, 376-418 (reset, setTemperature,
+ *               timeStep, force, moment, floorForce, generateNearby)
+ *   externals   src/VX_External.cpp:14-109
+ *   collisions  src/VX_Collision.cpp:15-56, src/Voxelyze.cpp:670-750
+ *   step        src/Voxelyze.cpp:251-321 (doTimeStep, recommendedTimeStep,
+ *               resetTime), stateInfo :752-800
+ *   math        include/Vec3D.h:48-108, include/Quat3D.h:19-204
+ *
+ * Build with -ffp-contract=off and without -ffast-math (see oracle/Makefile):
+ * contraction is the one transformation that changes the reference's bits.
+ *
+ * Parity pinning (tests/test_oracle.py): bit-exact against the golden vectors
+ * that tests/golden/make_golden.py recorded from the compiled reference, and
+ * against the closed forms of the reference's own tests (test/tVoxelyze.h).
+ * Poisson's ratio != 0 (the volumetric path) is not restated: every entry
+ * point that would need it reports an error instead ("parity unpinned" is
+ * therefore not claimed for it -- it is absent).
+ */
+#include <float.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define EXPORT __attribute__((visibility("default")))
+
+/* ------------------------------------------------------------------ math */
+typedef struct { double x, y, z; } dvec;
+typedef struct { float x, y, z; } fvec;
+typedef struct { double w, x, y, z; } dquat;
+
+static inline dvec dv(double x, double y, double z) { dvec r = {x, y, z}; return r; }
+static inline fvec fv(float x, float y, float z) { fvec r = {x, y, z}; return r; }
+static inline dquat dq(double w, double x, double y, double z) { dquat r = {w, x, y, z}; return r; }
+static const dquat QIDENT = {1.0, 0.0, 0.0, 0.0};
+
+#define PI_REF 3.14159265358979          /* Quat3D.h:19 */
+#define DBL_EPSILONx24 5.328e-15         /* Quat3D.h:20 */
+#define DISCARD_ANGLE_RAD 1e-7           /* Quat3D.h:24 */
+#define SMALL_ANGLE_RAD 1.732e-2         /* Quat3D.h:25 */
+#define SLTHRESH_ACOS2SQRT 2.4e-3        /* Quat3D.h:29 */
+
+/* Vec3D::Length (Vec3D.h:94) */
+static inline double dlen(dvec v) { return sqrt(v.x * v.x + v.y * v.y + v.z * v.z); }
+static inline double dlen2(dvec v) { return v.x * v.x + v.y * v.y + v.z * v.z; }
+
+/* Quat3D::operator*(Quat3D) (Quat3D.h:83) */
+static inline dquat qmul(dquat a, dquat f) {
+    return dq(a.w * f.w - a.x * f.x - a.y * f.y - a.z * f.z,
+              a.w * f.x + a.x * f.w + a.y * f.z - a.z * f.y,
+              a.w * f.y - a.x * f.z + a.y * f.w + a.z * f.x,
+              a.w * f.z + a.x * f.y - a.y * f.x + a.z * f.w);
+}
+static inline dquat qconj(dquat q) { return dq(q.w, -q.x, -q.y, -q.z); }
+
+/* Quat3D::RotateVec3D (Quat3D.h:170-177) */
+static inline dvec qrot(dquat q, dvec f) {
+    double fx = f.x, fy = f.y, fz = f.z;
+    double tw = fx * q.x + fy * q.y + fz * q.z;
+    double tx = fx * q.w - fy * q.z + fz * q.y;
+    double ty = fx * q.z + fy * q.w - fz * q.x;
+    double tz = -fx * q.y + fy * q.x + fz * q.w;
+    return dv(q.w * tx + q.x * tw + q.y * tz - q.z * ty,
+              q.w * ty - q.x * tz + q.y * tw + q.z * tx,
+              q.w * tz + q.x * ty - q.y * tx + q.z * tw);
+}
+
+/* Quat3D::RotateVec3DInv (Quat3D.h:188-195) */
+static inline dvec qrotinv(dquat q, dvec f) {
+    double fx = f.x, fy = f.y, fz = f.z;
+    double tw = q.x * fx + q.y * fy + q.z * fz;
+    double tx = q.w * fx - q.y * fz + q.z * fy;
+    double ty = q.w * fy + q.x * fz - q.z * fx;
+    double tz = q.w * fz - q.x * fy + q.y * fx;
+    return dv(tw * q.x + tx * q.w + ty * q.z - tz * q.y,
+              tw * q.y - tx * q.z + ty * q.w + tz * q.x,
+              tw * q.z + tx * q.y - ty * q.x + tz * q.w);
+}
+
+/* Quat3D::ToRotationVector (Quat3D.h:117-122): Vec*2.0*s, and Vec/f = Vec*(1/f) */
+static inline dvec qtorv(dquat q) {
+    if (q.w >= 1.0 || q.w <= -1.0) return dv(0, 0, 0);
+    double sl = 1.0 - q.w * q.w;
+    if (sl < SLTHRESH_ACOS2SQRT) {
+        double s = sqrt((2 - 2 * q.w) / sl);
+        return dv((q.x * 2.0) * s, (q.y * 2.0) * s, (q.z * 2.0) * s);
+    } else {
+        double a = acos(q.w);
+        double inv = 1.0 / sqrt(sl);
+        return dv(inv * ((q.x * 2.0) * a), inv * ((q.y * 2.0) * a), inv * ((q.z * 2.0) * a));
+    }
+}
+
+/* Quat3D::FromRotationVector (Quat3D.h:124-139); VecIn/2 multiplies by 1.0/2 */
+static inline dquat qfromrv(dvec v) {
+    dvec th = dv(0.5 * v.x, 0.5 * v.y, 0.5 * v.z);
+    double s, m2 = th.x * th.x + th.y * th.y + th.z * th.z;
+    dquat q;
+    if (m2 * m2 < DBL_EPSILONx24) {
+        q.w = 1.0 - 0.5 * m2;
+        s = 1.0 - m2 / 6.0;
+    } else {
+        double m = sqrt(m2);
+        q.w = cos(m);
+        s = sin(m) / m;
+    }
+    q.x = th.x * s;
+    q.y = th.y * s;
+    q.z = th.z * s;
+    return q;
+}
+
+/* Quat3D::FromAngleToPosX (Quat3D.h:141-166); leaves q untouched for a zero vector */
+static inline void qfromangletoposx(dquat* q, dvec rf) {
+    if (rf.x == 0 && rf.y == 0 && rf.z == 0) return;
+    double yox = rf.y / rf.x, zox = rf.z / rf.x;
+    if (yox < SMALL_ANGLE_RAD && yox > -SMALL_ANGLE_RAD && zox < SMALL_ANGLE_RAD && zox > -SMALL_ANGLE_RAD) {
+        q->x = 0;
+        q->y = 0.5 * zox;
+        q->z = -0.5 * yox;
+        q->w = 1 + 0.5 * (-q->y * q->y - q->z * q->z);
+        return;
+    }
+    dvec n = rf;
+    double l = sqrt(n.x * n.x + n.y * n.y + n.z * n.z);
+    if (l > 0) { double li = 1.0 / l; n.x *= li; n.y *= li; n.z *= li; }
+    double theta = acos(n.x);
+    if (theta > PI_REF - DISCARD_ANGLE_RAD) { q->w = 0; q->x = 0; q->y = 1; q->z = 0; return; }
+    const double ami = 1.0 / sqrt(n.z * n.z + n.y * n.y);
+    const double a = 0.5 * theta;
+    const double s = sin(a);
+    q->w = cos(a);
+    q->x = 0;
+    q->y = n.z * ami * s;
+    q->z = -n.y * ami * s;
+}
+
+/* ------------------------------------------------------------ structures */
+typedef struct omat {
+    /* CVX_Material (VX_Material.h:114-140) */
+    int linear;
+    float E, sigmaYield, sigmaFail, epsilonYield, epsilonFail;
+    int ndata;
+    float *strainData, *stressData;
+    float nu, rho, alphaCTE, muStatic, muKinetic, zetaInternal, zetaGlobal, zetaCollision;
+    dvec extScale;
+    float eHat;
+    /* CVX_MaterialVoxel (VX_MaterialVoxel.h:62-71) */
+    double nomSize;
+    float gravMult, mass, massInverse, sqrtMass, firstMoment, momentInertia, momentInertiaInverse;
+    float c2xSqMxExS, c2xSqIxExSxSxS;
+    /* CVX_MaterialLink (VX_MaterialLink.h:33-46) */
+    int isLink;
+    struct omat *vox1, *vox2;
+    float a1, a2, b1, b2, b3, sqA1, sqA2xIp, sqB1, sqB2xFMp, sqB3xIp;
+    struct omat** deps;
+    int ndeps, capdeps;
+    char error[128];
+} omat;
+
+typedef struct {
+    int dof;
+    fvec force, moment;
+    dvec translation, rotation;
+    int hasRotQ;
+    dquat rotQ;
+} oext;
+
+#define VS_SURFACE (1 << 1)
+#define VS_FLOOR_ENABLED (1 << 2)
+#define VS_FLOOR_STATIC (1 << 3)
+#define VS_COLLISIONS (1 << 5)
+
+typedef struct {
+    omat* mat;
+    int ix, iy, iz;
+    oext* ext;
+    int links[6];
+    dvec pos, linMom, angMom;
+    dquat orient;
+    int boolStates;
+    float temp, previousDt;
+    int hasColWatch;
+    fvec lastColWatchPos;
+    int *colWatch, ncol, capcol;
+    int *nearby, nnear, capnear;
+} ovox;
+
+typedef struct {
+    int vNeg, vPos;
+    omat* mat;
+    int axis;
+    dvec forceNeg, forcePos, momentNeg, momentPos;
+    float strain, maxStrain, strainOffset, stress, strainRatio;
+    int lvv, smallAngle;
+    dvec pos2, angle1v, angle2v;
+    dquat angle1, angle2;
+    double currentRestLength;
+    float currentTransverseArea, currentTransverseStrainSum;
+} olink;
+
+typedef struct {
+    int v1, v2;
+    float penetrationStiff, dampingC;
+    fvec force;
+} ocol;
+
+/* open-addressing map from a 64-bit key to an int */
+typedef struct {
+    uint64_t* keys;
+    int* vals;
+    size_t cap, used;
+} omap;
+#define MAP_EMPTY UINT64_MAX
+#define MAP_TOMB (UINT64_MAX - 1)
+
+typedef struct vxo_sim {
+    double voxSize;
+    float currentTime, ambientTemp, grav;
+    int floor, collisions;
+    float boundingRadius, watchDistance;
+    omat** vmats;
+    int nvmats, capvmats;
+    omat** lmats;
+    int nlmats, caplmats;
+    ovox* vox;
+    int nvslots, capvslots;
+    int* vlist;
+    int nvlist, capvlist;
+    olink* lnk;
+    int nlslots, caplslots;
+    int* llist;
+    int nllist, capllist;
+    ocol* cols;
+    int ncols, capcols;
+    int collisionsStale, nearbyStale;
+    omap vmap, lmap;
+    int* stamp;
+    int stampCap, stampVal;
+    char err[256];
+} vxo_sim;
+
+#define GROW(ptr, n, cap, extra)                                                      \
+    do {                                                                              \
+        if ((n) + (extra) > (cap)) {                                                  \
+            int nc_ = (cap) ? (cap) * 2 : 16;                                         \
+            while (nc_ < (n) + (extra)) nc_ *= 2;                                     \
+            void* p_ = realloc((ptr), (size_t)nc_ * sizeof(*(ptr)));                  \
+            if (!p_) { fprintf(stderr, "oracle: out of memory\n"); abort(); }         \
+            (ptr) = p_;                                                               \
+            (cap) = nc_;                                                              \
+        }                                                                             \
+    } while (0)
+
+static uint64_t map_hash(uint64_t k) {
+    k ^= k >> 33; k *= 0xff51afd7ed558ccdULL; k ^= k >> 33; k *= 0xc4ceb9fe1a85ec53ULL; k ^= k >> 33;
+    return k;
+}
+static void map_init(omap* m) { m->keys = NULL; m->vals = NULL; m->cap = 0; m->used = 0; }
+static void map_free(omap* m) { free(m->keys); free(m->vals); map_init(m); }
+static void map_put(omap* m, uint64_t k, int v);
+static void map_rehash(omap* m, size_t ncap) {
+    uint64_t* ok = m->keys; int* ov = m->vals; size_t oc = m->cap;
+    m->keys = (uint64_t*)malloc(ncap * sizeof(uint64_t));
+    m->vals = (int*)malloc(ncap * sizeof(int));
+    for (size_t i = 0; i < ncap; i++) m->keys[i] = MAP_EMPTY;
+    m->cap = ncap; m->used = 0;
+    for (size_t i = 0; i < oc; i++) if (ok[i] < MAP_TOMB) map_put(m, ok[i], ov[i]);
+    free(ok); free(ov);
+}
+static void map_put(omap* m, uint64_t k, int v) {
+    if ((m->used + 1) * 2 > m->cap) map_rehash(m, m->cap ? m->cap * 2 : 64);
+    size_t i = map_hash(k) & (m->cap - 1), tomb = (size_t)-1;
+    for (;;) {
+        if (m->keys[i] == k) { m->vals[i] = v; return; }
+        if (m->keys[i] == MAP_EMPTY) break;
+        if (m->keys[i] == MAP_TOMB && tomb == (size_t)-1) tomb = i;
+        i = (i + 1) & (m->cap - 1);
+    }
+    if (tomb != (size_t)-1) i = tomb; else m->used++;
+    m->keys[i] = k; m->vals[i] = v;
+}
+static int map_get(const omap* m, uint64_t k) {
+    if (!m->cap) return -1;
+    size_t i = map_hash(k) & (m->cap - 1);
+    for (;;) {
+        if (m->keys[i] == k) return m->vals[i];
+        if (m->keys[i] == MAP_EMPTY) return -1;
+        i = (i + 1) & (m->cap - 1);
+    }
+}
+static void map_del(omap* m, uint64_t k) {
+    if (!m->cap) return;
+    size_t i = map_hash(k) & (m->cap - 1);
+    for (;;) {
+        if (m->keys[i] == k) { m->keys[i] = MAP_TOMB; return; }
+        if (m->keys[i] == MAP_EMPTY) return;
+        i = (i + 1) & (m->cap - 1);
+    }
+}
+static uint64_t vkey(int x, int y, int z) {
+    return ((uint64_t)(uint32_t)(x + 0x100000) << 42) | ((uint64_t)(uint32_t)(y + 0x100000) << 21) |
+           (uint64_t)(uint32_t)(z + 0x100000);
+}
+static uint64_t lkey(int axis, int x, int y, int z) { return vkey(x, y, z) ^ ((uint64_t)(axis + 1) << 62); }
+
+/* ------------------------------------------------------------- materials */
+static int mat_updateDerived(omat* m);
+static int mat_setModel(omat* m, int n, const float* pStrain, const float* pStress);
+
+static void mat_setData(omat* m, int n, const float* strain, const float* stress) {
+    free(m->strainData); free(m->stressData);
+    m->strainData = (float*)malloc(sizeof(float) * (size_t)n);
+    m->stressData = (float*)malloc(sizeof(float) * (size_t)n);
+    memcpy(m->strainData, strain, sizeof(float) * (size_t)n);
+    memcpy(m->stressData, stress, sizeof(float) * (size_t)n);
+    m->ndata = n;
+}
+static void mat_err(omat* m, const char* e) { snprintf(m->error, sizeof(m->error), "%s", e); }
+
+/* CVX_Material::isFailed / isYielded (VX_Material.h:48-49) */
+static inline int mat_isFailed(const omat* m, float strain) { return m->epsilonFail != -1.0f && strain > m->epsilonFail; }
+static inline int mat_isYielded(const omat* m, float strain) { return m->epsilonYield != -1.0f && strain > m->epsilonYield; }
+
+/* CVX_Material::stress (VX_Material.cpp:165-195) */
+static float mat_stress(const omat* m, float strain, float tss, int forceLinear) {
+    if (mat_isFailed(m, strain)) return 0.0f;
+    if (strain <= m->strainData[1] || m->linear || forceLinear) {
+        if (m->nu == 0.0f) return m->E * strain;
+        else return m->eHat * ((1 - m->nu) * strain + m->nu * tss);
+    }
+    int n = m->ndata;
+    for (int i = 2; i < n; i++) {
+        if (strain <= m->strainData[i] || i == n - 1) {
+            float perc = (strain - m->strainData[i - 1]) / (m->strainData[i] - m->strainData[i - 1]);
+            float basic = m->stressData[i - 1] + perc * (m->stressData[i] - m->stressData[i - 1]);
+            if (m->nu == 0.0f) return basic;
+            float modulus = (m->stressData[i] - m->stressData[i - 1]) / (m->strainData[i] - m->strainData[i - 1]);
+            float modulusHat = modulus / ((1 - 2 * m->nu) * (1 + m->nu));
+            float effStrain = basic / modulus;
+            float effTss = tss * (effStrain / strain);
+            return modulusHat * ((1 - m->nu) * effStrain + m->nu * effTss);
+        }
+    }
+    return 0.0f;
+}
+
+/* CVX_Material::strain(stress) (VX_Material.cpp:197-211) */
+static float mat_strainOf(const omat* m, float stress) {
+    if (stress <= m->stressData[1] || m->linear) return stress / m->E;
+    int n = m->ndata;
+    for (int i = 2; i < n; i++) {
+        if (stress <= m->stressData[i] || i == n - 1) {
+            float perc = (stress - m->stressData[i - 1]) / (m->stressData[i] - m->stressData[i - 1]);
+            return m->strainData[i - 1] + perc * (m->strainData[i] - m->strainData[i - 1]);
+        }
+    }
+    return 0.0f;
+}
+
+/* CVX_Material::modulus (VX_Material.cpp:214-226) */
+static float mat_modulus(const omat* m, float strain) {
+    if (mat_isFailed(m, strain)) return 0.0f;
+    if (strain <= m->strainData[1] || m->linear) return m->E;
+    int n = m->ndata;
+    for (int i = 2; i < n; i++)
+        if (strain <= m->strainData[i] || i == n - 1)
+            return (m->stressData[i] - m->stressData[i - 1]) / (m->strainData[i] - m->strainData[i - 1]);
+    return 0.0f;
+}
+
+/* CVX_Material::setModelLinear (VX_Material.cpp:355-384) */
+static int mat_setModelLinear(omat* m, float E, float failureStress) {
+    if (E <= 0) { mat_err(m, "Young's modulus must be positive"); return 0; }
+    if (failureStress != -1.0f && failureStress <= 0) { mat_err(m, "Failure stress must be positive"); return 0; }
+    float tmpFail = failureStress;
+    if (tmpFail == -1) tmpFail = 1000000;
+    float tmpFailStrain = tmpFail / E;
+    float st[2] = {0, tmpFailStrain}, ss[2] = {0, tmpFail};
+    mat_setData(m, 2, st, ss);
+    m->linear = 1;
+    m->E = E;
+    m->sigmaYield = failureStress;
+    m->sigmaFail = failureStress;
+    m->epsilonYield = (failureStress == -1) ? -1 : tmpFailStrain;
+    m->epsilonFail = (failureStress == -1) ? -1 : tmpFailStrain;
+    return mat_updateDerived(m);
+}
+
+/* CVX_Material::setModelBilinear (VX_Material.cpp:389-434) */
+static int mat_setModelBilinear(omat* m, float E, float plastic, float yieldStress, float failureStress) {
+    if (E <= 0) { mat_err(m, "Young's modulus must be positive"); return 0; }
+    if (plastic <= 0 || plastic >= E) { mat_err(m, "Plastic modulus must be positive but less than Young's modulus"); return 0; }
+    if (yieldStress <= 0) { mat_err(m, "Yield stress must be positive"); return 0; }
+    if (failureStress != -1.0f && failureStress <= yieldStress) {
+        mat_err(m, "Failure stress must be positive and greater than the yield stress");
+        return 0;
+    }
+    float yieldStrain = yieldStress / E;
+    float tmpFail = failureStress;
+    if (tmpFail == -1) tmpFail = 3 * yieldStress;
+    float tM = plastic;
+    float tB = yieldStress - tM * yieldStrain;
+    float tmpFailStrain = (tmpFail - tB) / tM;
+    float st[3] = {0, yieldStrain, tmpFailStrain}, ss[3] = {0, yieldStress, tmpFail};
+    mat_setData(m, 3, st, ss);
+    m->linear = 0;
+    m->E = E;
+    m->sigmaYield = yieldStress;
+    m->sigmaFail = failureStress;
+    m->epsilonYield = yieldStrain;
+    m->epsilonFail = failureStress == -1.0f ? -1.0f : tmpFailStrain;
+    return mat_updateDerived(m);
+}
+
+/* CVX_Material::setYieldFromData (VX_Material.cpp:437-470) */
+static int mat_setYieldFromData(omat* m, float percentStrainOffset) {
+    m->sigmaYield = -1.0f;
+    m->epsilonYield = -1.0f;
+    float oM = m->E;
+    float oB = (-percentStrainOffset / 100 * oM);
+    int dataPoints = m->ndata - 1;
+    for (int i = 1; i < dataPoints - 1; i++) {
+        float x1 = m->strainData[i], x2 = m->strainData[i + 1];
+        float y1 = m->stressData[i], y2 = m->stressData[i + 1];
+        float tM = (y2 - y1) / (x2 - x1);
+        float tB = y1 - tM * x1;
+        if (oM != tM) {
+            float xi = (tB - oB) / (oM - tM);
+            if (xi > x1 && xi < x2) {
+                float perc = (xi - x1) / (x2 - x1);
+                m->sigmaYield = y1 + perc * (y2 - y1);
+                m->epsilonYield = xi;
+                return 1;
+            }
+        }
+    }
+    m->sigmaYield = m->sigmaFail;
+    m->epsilonYield = m->epsilonFail;
+    return 0;
+}
+
+/* CVX_Material::setModel (VX_Material.cpp:283-350) */
+static int mat_setModel(omat* m, int n, const float* pStrain, const float* pStress) {
+    if (*pStrain == 0 && *pStress == 0) { pStrain++; pStress++; n--; }
+    if (n <= 0) { mat_err(m, "Not enough data points"); return 0; }
+    if (*pStrain <= 0 || *pStress <= 0) { mat_err(m, "First stress and strain data points negative or zero"); return 0; }
+    float* ts = (float*)malloc(sizeof(float) * (size_t)(n + 1));
+    float* tt = (float*)malloc(sizeof(float) * (size_t)(n + 1));
+    ts[0] = 0; tt[0] = 0;
+    float sweepStrain = 0.0f, sweepStress = 0.0f;
+    for (int i = 0; i < n; i++) {
+        float thisStrain = pStrain[i], thisStress = pStress[i];
+        if (thisStrain <= sweepStrain) { mat_err(m, "Out of order strain data"); free(ts); free(tt); return 0; }
+        if (thisStress <= sweepStress) mat_err(m, "Stress data is not monotonically increasing");
+        /* the reference divides by its own first entry (0/0 = NaN), so this never rejects */
+        if (i > 0 && (thisStress - sweepStress) / (thisStrain - sweepStrain) > tt[0] / ts[0]) {
+            mat_err(m, "Slope of stress/strain curve should never exceed that of the first line segment (youngs modulus)");
+            free(ts); free(tt);
+            return 0;
+        }
+        sweepStrain = thisStrain;
+        sweepStress = thisStress;
+        ts[i + 1] = thisStrain;
+        tt[i + 1] = thisStress;
+    }
+    mat_setData(m, n + 1, ts, tt);
+    free(ts); free(tt);
+    m->E = m->stressData[1] / m->strainData[1];
+    m->sigmaFail = m->stressData[m->ndata - 1];
+    m->epsilonFail = m->strainData[m->ndata - 1];
+    m->linear = (n == 1);
+    if (n == 1 || n == 2) {
+        m->sigmaYield = m->stressData[1];
+        m->epsilonYield = m->strainData[1];
+    } else {
+        mat_setYieldFromData(m, 0.2f);
+    }
+    return mat_updateDerived(m);
+}
+
+/* CVX_MaterialLink::updateAll (VX_MaterialLink.cpp:45-119) */
+static int mat_linkUpdateAll(omat* m) {
+    omat *v1 = m->vox1, *v2 = m->vox2;
+    m->nomSize = 0.5 * (v1->nomSize + v2->nomSize);
+    m->rho = 0.5f * (v1->rho + v2->rho);
+    m->alphaCTE = 0.5f * (v1->alphaCTE + v2->alphaCTE);
+    m->muStatic = 0.5f * (v1->muStatic + v2->muStatic);
+    m->muKinetic = 0.5f * (v1->muKinetic + v2->muKinetic);
+    m->zetaInternal = 0.5f * (v1->zetaInternal + v2->zetaInternal);
+    m->zetaGlobal = 0.5f * (v1->zetaGlobal + v2->zetaGlobal);
+    m->zetaCollision = 0.5f * (v1->zetaCollision + v2->zetaCollision);
+    m->extScale = dv(1.0, 1.0, 1.0);
+
+    float stressFail = -1.0f, f1 = v1->sigmaFail, f2 = v2->sigmaFail;
+    if (f1 == -1.0f) stressFail = f2;
+    else if (f2 == -1.0f) stressFail = f1;
+    else stressFail = f1 < f2 ? f1 : f2;
+
+    if (v1->linear && v2->linear) {
+        mat_setModelLinear(m, 2.0f * v1->E * v2->E / (v1->E + v2->E), stressFail);
+    } else {
+        int cap = v1->ndata + v2->ndata + 2, cnt = 1;
+        float* nst = (float*)malloc(sizeof(float) * (size_t)cap);
+        float* nss = (float*)malloc(sizeof(float) * (size_t)cap);
+        nst[0] = 0.0f; nss[0] = 0.0f;
+        int it1 = 1, it2 = 1;
+        while (it1 < v1->ndata && it2 < v2->ndata) {
+            float strain = FLT_MAX;
+            if (it1 < v1->ndata) strain = v1->strainData[it1];
+            if (it2 < v2->ndata && v2->strainData[it2] < strain) strain = v2->strainData[it2];
+            if (strain == v1->strainData[it1]) it1++;
+            if (strain == v2->strainData[it2]) it2++;
+            float mod1 = mat_modulus(v1, strain - FLT_EPSILON);
+            float mod2 = mat_modulus(v2, strain - FLT_EPSILON);
+            float thisMod = 2.0f * mod1 * mod2 / (mod1 + mod2);
+            int last = cnt - 1;
+            nst[cnt] = strain;
+            nss[cnt] = nss[last] + thisMod * (strain - nst[last]);
+            cnt++;
+        }
+        mat_setModel(m, cnt, nst, nss);
+        free(nst); free(nss);
+        m->sigmaFail = stressFail;
+        m->epsilonFail = stressFail == -1.0f ? -1.0f : mat_strainOf(m, stressFail);
+    }
+
+    if (v1->nu == 0 && v2->nu == 0) m->nu = 0;
+    else {
+        float tmpEHat = 2 * v1->eHat * v2->eHat / (v1->eHat + v2->eHat);
+        float tmpE = m->E;
+        float c2 = (tmpEHat - tmpE) / (2 * tmpEHat) + 0.0625;
+        m->nu = sqrtf(c2) - 0.25;
+    }
+    return mat_updateDerived(m);
+}
+
+/* CVX_Material::updateDerived (VX_Material.cpp:525-532), CVX_MaterialVoxel::updateDerived
+   (VX_MaterialVoxel.cpp:57-79) and CVX_MaterialLink::updateDerived (VX_MaterialLink.cpp:121-141) */
+static int mat_updateDerived(omat* m) {
+    m->eHat = m->E / ((1 - 2 * m->nu) * (1 + m->nu));
+    for (int i = 0; i < m->ndeps; i++) mat_linkUpdateAll(m->deps[i]);
+
+    int ok = 1;
+    double volume = m->nomSize * m->nomSize * m->nomSize;
+    m->mass = (float)(volume * m->rho);
+    m->momentInertia = (float)(m->mass * m->nomSize * m->nomSize / 6.0f);
+    m->firstMoment = (float)(m->mass * m->nomSize / 2.0f);
+    if (volume == 0 || m->mass == 0 || m->momentInertia == 0) {
+        m->massInverse = m->sqrtMass = m->momentInertiaInverse = m->c2xSqMxExS = m->c2xSqIxExSxSxS = 0.0f;
+        ok = 0;
+    } else {
+        m->massInverse = 1.0f / m->mass;
+        m->sqrtMass = sqrtf(m->mass);
+        m->momentInertiaInverse = 1.0f / m->momentInertia;
+        m->c2xSqMxExS = (float)(2.0f * sqrt(m->mass * m->E * m->nomSize));
+        m->c2xSqIxExSxSxS = (float)(2.0f * sqrt(m->momentInertia * m->E * m->nomSize * m->nomSize * m->nomSize));
+    }
+    if (!m->isLink) return ok;
+
+    float L = (float)m->nomSize;
+    m->a1 = m->E * L;
+    m->a2 = m->E * L * L * L / (12.0f * (1 + m->nu));
+    m->b1 = m->E * L;
+    m->b2 = m->E * L * L / 2.0f;
+    m->b3 = m->E * L * L * L / 6.0f;
+    m->sqA1 = sqrtf(m->a1);
+    m->sqA2xIp = sqrtf(m->a2 * L * L / 6.0f);
+    m->sqB1 = sqrtf(m->b1);
+    m->sqB2xFMp = sqrtf(m->b2 * L / 2.0f);
+    m->sqB3xIp = sqrtf(m->b3 * L * L / 6.0f);
+    return 1;
+}
+
+/* CVX_Material::clear (VX_Material.cpp:54-75) */
+static void mat_clear(omat* m) {
+    m->nu = 0.0f; m->rho = 1.0f; m->alphaCTE = 0.0f; m->muStatic = 0.0f; m->muKinetic = 0.0f;
+    m->zetaInternal = 1.0f; m->zetaGlobal = 0.0f; m->zetaCollision = 0.0f;
+    m->extScale = dv(1.0, 1.0, 1.0);
+    mat_setModelLinear(m, 1.0, -1);
+    mat_updateDerived(m);
+}
+
+/* CVX_MaterialVoxel(E, rho, nominalSize) (VX_MaterialVoxel.cpp:16-36) */
+static omat* mat_newVoxel(float E, float rho, double nomSize) {
+    omat* m = (omat*)calloc(1, sizeof(omat));
+    m->nomSize = 0.001;
+    mat_clear(m);
+    m->rho = rho;
+    mat_setModelLinear(m, E, -1);
+    mat_updateDerived(m);
+    m->nomSize = nomSize;
+    m->gravMult = 0.0f;
+    mat_updateDerived(m);
+    return m;
+}
+
+/* CVX_MaterialLink(mat1, mat2) (VX_MaterialLink.cpp:16-23) */
+static omat* mat_newLink(omat* v1, omat* v2) {
+    omat* m = (omat*)calloc(1, sizeof(omat));
+    m->nomSize = 0.001;
+    mat_clear(m);
+    m->rho = 1e3f;
+    mat_setModelLinear(m, 1e6f, -1);
+    m->isLink = 1;
+    m->vox1 = v1;
+    m->vox2 = v2;
+    mat_linkUpdateAll(m);
+    return m;
+}
+
+static void mat_free(omat* m) {
+    if (!m) return;
+    free(m->strainData); free(m->stressData); free(m->deps); free(m);
+}
+
+static void mat_addDep(omat* m, omat* d) {
+    GROW(m->deps, m->ndeps, m->capdeps, 1);
+    m->deps[m->ndeps++] = d;
+}
+
+/* ------------------------------------------------------------ externals */
+/* CVX_External::rotationChanged (VX_External.cpp:100-109) */
+static void ext_rotationChanged(oext* e) {
+    if (e->rotation.x != 0 || e->rotation.y != 0 || e->rotation.z != 0) {
+        e->hasRotQ = 1;
+        e->rotQ = qfromrv(e->rotation);
+    } else if (e->hasRotQ) {
+        e->rotQ = QIDENT;
+    }
+}
+static void ext_reset(oext* e) {
+    e->dof = 0;
+    e->force = fv(0, 0, 0); e->moment = fv(0, 0, 0);
+    e->translation = dv(0, 0, 0); e->rotation = dv(0, 0, 0);
+    ext_rotationChanged(e);
+}
+static inline dquat ext_rotationQuat(const oext* e) { return e->hasRotQ ? e->rotQ : QIDENT; }
+static inline int ext_isFixed(const oext* e, int dof) { return (dof & e->dof) ? 1 : 0; }
+static inline int ext_isFixedAll(const oext* e) { return (e->dof & 0x3F) == 0x3F; }
+static inline int ext_isFixedAllRotation(const oext* e) { return ext_isFixed(e, 8) && ext_isFixed(e, 16) && ext_isFixed(e, 32); }
+static inline int ext_isFixedAnyRotation(const oext* e) { return ext_isFixed(e, 8) || ext_isFixed(e, 16) || ext_isFixed(e, 32); }
+
+/* ---------------------------------------------------------------- voxels */
+static inline ovox* V(vxo_sim* s, int slot) { return &s->vox[slot]; }
+static inline olink* Lk(vxo_sim* s, int slot) { return &s->lnk[slot]; }
+
+/* CVX_Voxel::originalPosition (VX_Voxel.h:80) */
+static inline dvec vox_originalPosition(const ovox* v) {
+    double sz = v->mat->nomSize;
+    return dv(v->ix * sz, v->iy * sz, v->iz * sz);
+}
+/* CVX_Voxel::baseSize(axis) (VX_Voxel.h:89) and baseSizeAverage (VX_Voxel.h:90) */
+static inline double vox_baseSizeAxis(const ovox* v, int axis) {
+    double sz = axis == 0 ? v->mat->nomSize * v->mat->extScale.x
+              : axis == 1 ? v->mat->nomSize * v->mat->extScale.y : v->mat->nomSize * v->mat->extScale.z;
+    return sz * (1 + v->temp * v->mat->alphaCTE);
+}
+static inline double vox_baseSizeAverage(const ovox* v) {
+    float f = 1 + v->temp * v->mat->alphaCTE;
+    double bx = f * (v->mat->nomSize * v->mat->extScale.x);
+    double by = f * (v->mat->nomSize * v->mat->extScale.y);
+    double bz = f * (v->mat->nomSize * v->mat->extScale.z);
+    return (bx + by + bz) / 3.0f;
+}
+/* CVX_Voxel::floorPenetration (VX_Voxel.h:122) */
+static inline float vox_floorPenetration(const ovox* v) {
+    return (float)(vox_baseSizeAverage(v) / 2 - v->mat->nomSize / 2 - v->pos.z);
+}
+/* CVX_Voxel::velocity (VX_Voxel.h:96) */
+static inline dvec vox_velocity(const ovox* v) {
+    float mi = v->mat->massInverse;
+    return dv(mi * v->linMom.x, mi * v->linMom.y, mi * v->linMom.z);
+}
+/* CVX_Voxel::dampingMultiplier (VX_Voxel.h:130) */
+static inline float vox_dampingMultiplier(const ovox* v) {
+    return 2 * v->mat->sqrtMass * v->mat->zetaInternal / v->previousDt;
+}
+static inline void vox_setStatic(ovox* v, int on) {
+    if (on) v->boolStates |= VS_FLOOR_STATIC; else v->boolStates &= ~VS_FLOOR_STATIC;
+}
+/* CVX_Voxel::reset (VX_Voxel.cpp:47-56) */
+static void vox_reset(ovox* v) {
+    v->pos = vox_originalPosition(v);
+    v->orient = QIDENT;
+    v->linMom = dv(0, 0, 0);
+    v->angMom = dv(0, 0, 0);
+    vox_setStatic(v, 1);
+    v->temp = 0.0f;
+    v->previousDt = 0.0f;
+}
+/* CVX_Voxel::updateSurface (VX_Voxel.cpp:376-381) */
+static void vox_updateSurface(ovox* v) {
+    int interior = 1;
+    for (int i = 0; i < 6; i++) if (v->links[i] < 0) interior = 0;
+    if (interior) v->boolStates |= VS_SURFACE; else v->boolStates &= ~VS_SURFACE;
+}
+static inline int vox_isInterior(const ovox* v) { return (v->boolStates & VS_SURFACE) ? 1 : 0; }
+
+static oext* vox_external(ovox* v) {
+    if (!v->ext) { v->ext = (oext*)calloc(1, sizeof(oext)); ext_reset(v->ext); }
+    return v->ext;
+}
+
+/* ----------------------------------------------------------------- links */
+/* CVX_Link::updateRestLength (VX_Link.cpp:137-140) */
+static void link_updateRestLength(vxo_sim* s, olink* l) {
+    l->currentRestLength = 0.5 * (vox_baseSizeAxis(V(s, l->vNeg), l->axis) + vox_baseSizeAxis(V(s, l->vPos), l->axis));
+}
+/* CVX_Link::updateTransverseInfo (VX_Link.cpp:142-147) for Poisson's ratio 0:
+   CVX_Voxel::transverseArea is size*size and transverseStrainSum is 0 (VX_Voxel.cpp:345-372) */
+static void link_updateTransverseInfo(vxo_sim* s, olink* l) {
+    float s1 = (float)V(s, l->vNeg)->mat->nomSize, s2 = (float)V(s, l->vPos)->mat->nomSize;
+    l->currentTransverseArea = 0.5f * (s1 * s1 + s2 * s2);
+    l->currentTransverseStrainSum = 0.5f * (0.0f + 0.0f);
+}
+/* CVX_Link::reset (VX_Link.cpp:61-75) */
+static void link_reset(vxo_sim* s, olink* l) {
+    l->pos2 = l->angle1v = l->angle2v = dv(0, 0, 0);
+    l->angle1 = l->angle2 = QIDENT;
+    l->forceNeg = l->forcePos = l->momentNeg = l->momentPos = dv(0, 0, 0);
+    l->strain = l->maxStrain = l->strainOffset = l->stress = 0.0f;
+    l->strainRatio = V(s, l->vPos)->mat->E / V(s, l->vNeg)->mat->E;
+    l->smallAngle = 1;
+    l->lvv = 0;
+    link_updateRestLength(s, l);
+    link_updateTransverseInfo(s, l);
+}
+
+/* toAxisX / toAxisOriginal (VX_Link.h:112-117) */
+static inline dvec toAxisX(int axis, dvec v) {
+    if (axis == 1) return dv(v.y, -v.x, v.z);
+    if (axis == 2) return dv(v.z, v.y, -v.x);
+    return v;
+}
+static inline dquat toAxisXq(int axis, dquat q) {
+    if (axis == 1) return dq(q.w, q.y, -q.x, q.z);
+    if (axis == 2) return dq(q.w, q.z, q.y, -q.x);
+    return q;
+}
+static inline dvec toAxisOriginal(int axis, dvec v) {
+    if (axis == 1) return dv(-v.y, v.x, v.z);
+    if (axis == 2) return dv(-v.z, v.y, v.x);
+    return v;
+}
+
+/* CVX_Link::orientLink (VX_Link.cpp:77-119) */
+static void link_orient(vxo_sim* s, olink* l) {
+    const ovox *vn = V(s, l->vNeg), *vp = V(s, l->vPos);
+    l->pos2 = toAxisX(l->axis, dv(vp->pos.x - vn->pos.x, vp->pos.y - vn->pos.y, vp->pos.z - vn->pos.z));
+    l->angle1 = toAxisXq(l->axis, vn->orient);
+    l->angle2 = toAxisXq(l->axis, vp->orient);
+    dquat totalRot = qconj(l->angle1);
+    l->pos2 = qrot(totalRot, l->pos2);
+    l->angle2 = qmul(totalRot, l->angle2);
+    l->angle1 = QIDENT;
+
+    float smallTurn = (float)((fabs(l->pos2.z) + fabs(l->pos2.y)) / l->pos2.x);
+    float extendPerc = (float)(fabs(1 - l->pos2.x / l->currentRestLength));
+    if (!l->smallAngle && smallTurn < 0.05f && extendPerc < 0.50f) {
+        l->smallAngle = 1;
+        l->lvv = 0;
+    } else if (l->smallAngle && (smallTurn > 1.2f * 0.05f || extendPerc > 1.2f * 0.50f)) {
+        l->smallAngle = 0;
+        l->lvv = 0;
+    }
+
+    if (l->smallAngle) {
+        l->pos2.x -= l->currentRestLength;
+    } else {
+        qfromangletoposx(&l->angle1, l->pos2);
+        l->angle2 = qmul(l->angle1, l->angle2);
+        l->pos2 = dv(dlen(l->pos2) - l->currentRestLength, 0, 0);
+    }
+    l->angle1v = qtorv(l->angle1);
+    l->angle2v = qtorv(l->angle2);
+}
+
+/* CVX_Link::updateStrain (VX_Link.cpp:220-249) */
+static float link_updateStrain(olink* l, float axialStrain) {
+    const omat* m = l->mat;
+    l->strain = axialStrain;
+    if (m->linear) {
+        if (axialStrain > l->maxStrain) l->maxStrain = axialStrain;
+        return mat_stress(m, axialStrain, l->currentTransverseStrainSum, 0);
+    } else {
+        float ret;
+        if (axialStrain > l->maxStrain) {
+            l->maxStrain = axialStrain;
+            ret = mat_stress(m, axialStrain, l->currentTransverseStrainSum, 0);
+            if (m->nu != 0.0f) l->strainOffset = l->maxStrain - mat_stress(m, axialStrain, 0.0f, 0) / (m->eHat * (1 - m->nu));
+            else l->strainOffset = l->maxStrain - ret / m->E;
+        } else {
+            float rel = axialStrain - l->strainOffset;
+            if (m->nu != 0.0f) ret = mat_stress(m, rel, l->currentTransverseStrainSum, 1);
+            else ret = m->E * rel;
+        }
+        return ret;
+    }
+}
+
+/* CVX_Link::updateForces (VX_Link.cpp:149-217) */
+static void link_updateForces(vxo_sim* s, olink* l) {
+    dvec oldPos2 = l->pos2, oldA1 = l->angle1v, oldA2 = l->angle2v;
+    link_orient(s, l);
+    dvec dPos2 = dv(0.5 * (l->pos2.x - oldPos2.x), 0.5 * (l->pos2.y - oldPos2.y), 0.5 * (l->pos2.z - oldPos2.z));
+    dvec dA1 = dv(0.5 * (l->angle1v.x - oldA1.x), 0.5 * (l->angle1v.y - oldA1.y), 0.5 * (l->angle1v.z - oldA1.z));
+    dvec dA2 = dv(0.5 * (l->angle2v.x - oldA2.x), 0.5 * (l->angle2v.y - oldA2.y), 0.5 * (l->angle2v.z - oldA2.z));
+
+    const omat* m = l->mat;
+    l->stress = link_updateStrain(l, (float)(l->pos2.x / l->currentRestLength));
+    if (mat_isFailed(m, l->maxStrain)) {
+        l->forceNeg = l->forcePos = l->momentNeg = l->momentPos = dv(0, 0, 0);
+        return;
+    }
+    float b1 = m->b1, b2 = m->b2, b3 = m->b3, a2 = m->a2;
+    const dvec p = l->pos2, a1v = l->angle1v, a2v = l->angle2v;
+    l->forceNeg = dv(l->stress * l->currentTransverseArea,
+                     b1 * p.y - b2 * (a1v.z + a2v.z),
+                     b1 * p.z + b2 * (a1v.y + a2v.y));
+    l->forcePos = dv(-l->forceNeg.x, -l->forceNeg.y, -l->forceNeg.z);
+    l->momentNeg = dv(a2 * (a2v.x - a1v.x),
+                      -b2 * p.z - b3 * (2 * a1v.y + a2v.y),
+                      b2 * p.y - b3 * (2 * a1v.z + a2v.z));
+    l->momentPos = dv(a2 * (a1v.x - a2v.x),
+                      -b2 * p.z - b3 * (a1v.y + 2 * a2v.y),
+                      b2 * p.y - b3 * (a1v.z + 2 * a2v.z));
+
+    if (l->lvv) {
+        float sqA1 = m->sqA1, sqA2xIp = m->sqA2xIp, sqB1 = m->sqB1, sqB2xFMp = m->sqB2xFMp, sqB3xIp = m->sqB3xIp;
+        dvec pc = dv(sqA1 * dPos2.x,
+                     sqB1 * dPos2.y - sqB2xFMp * (dA1.z + dA2.z),
+                     sqB1 * dPos2.z + sqB2xFMp * (dA1.y + dA2.y));
+        float dmN = vox_dampingMultiplier(V(s, l->vNeg)), dmP = vox_dampingMultiplier(V(s, l->vPos));
+        l->forceNeg.x += dmN * pc.x; l->forceNeg.y += dmN * pc.y; l->forceNeg.z += dmN * pc.z;
+        l->forcePos.x -= dmP * pc.x; l->forcePos.y -= dmP * pc.y; l->forcePos.z -= dmP * pc.z;
+        double hN = 0.5 * dmN, hP = 0.5 * dmP;
+        dvec mn = dv(-sqA2xIp * (dA2.x - dA1.x),
+                     sqB2xFMp * dPos2.z + sqB3xIp * (2 * dA1.y + dA2.y),
+                     -sqB2xFMp * dPos2.y + sqB3xIp * (2 * dA1.z + dA2.z));
+        dvec mp = dv(sqA2xIp * (dA2.x - dA1.x),
+                     sqB2xFMp * dPos2.z + sqB3xIp * (dA1.y + 2 * dA2.y),
+                     -sqB2xFMp * dPos2.y + sqB3xIp * (dA1.z + 2 * dA2.z));
+        l->momentNeg.x -= hN * mn.x; l->momentNeg.y -= hN * mn.y; l->momentNeg.z -= hN * mn.z;
+        l->momentPos.x -= hP * mp.x; l->momentPos.y -= hP * mp.y; l->momentPos.z -= hP * mp.z;
+    } else {
+        l->lvv = 1;
+    }
+
+    if (!l->smallAngle) {
+        l->forceNeg = qrotinv(l->angle1, l->forceNeg);
+        l->momentNeg = qrotinv(l->angle1, l->momentNeg);
+    }
+    l->forcePos = qrotinv(l->angle2, l->forcePos);
+    l->momentPos = qrotinv(l->angle2, l->momentPos);
+    l->forceNeg = toAxisOriginal(l->axis, l->forceNeg);
+    l->forcePos = toAxisOriginal(l->axis, l->forcePos);
+    l->momentNeg = toAxisOriginal(l->axis, l->momentNeg);
+    l->momentPos = toAxisOriginal(l->axis, l->momentPos);
+}
+
+/* CVX_Link::strainEnergy (VX_Link.cpp:251-257) */
+static float link_strainEnergy(const olink* l) {
+    const omat* m = l->mat;
+    return (float)(l->forceNeg.x * l->forceNeg.x / (2.0f * m->a1) +
+                   l->momentNeg.x * l->momentNeg.x / (2.0 * m->a2) +
+                   (l->momentNeg.z * l->momentNeg.z - l->momentNeg.z * l->momentPos.z + l->momentPos.z * l->momentPos.z) / (3.0 * m->b3) +
+                   (l->momentNeg.y * l->momentNeg.y - l->momentNeg.y * l->momentPos.y + l->momentPos.y * l->momentPos.y) / (3.0 * m->b3));
+}
+/* CVX_Link::axialStrain(bool) (VX_Link.cpp:121-124) */
+static float link_axialStrainEnd(const olink* l, int positiveEnd) {
+    return positiveEnd ? 2.0f * l->strain * l->strainRatio / (1.0f + l->strainRatio)
+                       : 2.0f * l->strain / (1.0f + l->strainRatio);
+}
+
+/* ------------------------------------------------------------ voxel step */
+/* CVX_Voxel::floorForce (VX_Voxel.cpp:274-298) */
+static void vox_floorForce(ovox* v, float dt, dvec* F) {
+    (void)dt;
+    float pen = vox_floorPenetration(v);
+    if (pen >= 0) {
+        dvec vel = vox_velocity(v);
+        dvec hv = dv(vel.x, vel.y, 0);
+        float normalForce = (float)(2 * v->mat->E * v->mat->nomSize) * pen;
+        float cDamp = v->mat->zetaCollision * v->mat->c2xSqMxExS;
+        F->z += normalForce - cDamp * vel.z;
+        if (v->boolStates & VS_FLOOR_STATIC) {
+            float surfaceForceSq = (float)(F->x * F->x + F->y * F->y);
+            float frictionForceSq = (v->mat->muStatic * normalForce) * (v->mat->muStatic * normalForce);
+            if (surfaceForceSq > frictionForceSq) vox_setStatic(v, 0);
+        } else {
+            float fk = v->mat->muKinetic * normalForce;
+            double l = sqrt(hv.x * hv.x + hv.y * hv.y + hv.z * hv.z);
+            dvec n = hv;
+            if (l > 0) { double inv = 1.0 / l; n = dv(inv * hv.x, inv * hv.y, inv * hv.z); }
+            F->x -= fk * n.x; F->y -= fk * n.y; F->z -= fk * n.z;
+        }
+    } else {
+        vox_setStatic(v, 0);
+    }
+}
+
+/* CVX_Voxel::force (VX_Voxel.cpp:234-256) */
+static dvec vox_force(vxo_sim* s, ovox* v) {
+    dvec t = dv(0, 0, 0);
+    for (int i = 0; i < 6; i++) {
+        if (v->links[i] >= 0) {
+            const olink* l = Lk(s, v->links[i]);
+            dvec f = (i % 2 == 1) ? l->forcePos : l->forceNeg;
+            t.x += f.x; t.y += f.y; t.z += f.z;
+        }
+    }
+    t = qrot(v->orient, t);
+    if (v->ext) { t.x += v->ext->force.x; t.y += v->ext->force.y; t.z += v->ext->force.z; }
+    dvec vel = vox_velocity(v);
+    float c = v->mat->zetaGlobal * v->mat->c2xSqMxExS;
+    t.x -= c * vel.x; t.y -= c * vel.y; t.z -= c * vel.z;
+    t.z += -v->mat->mass * 9.80665f * v->mat->gravMult;
+    if (v->boolStates & VS_COLLISIONS) {
+        for (int k = 0; k < v->ncol; k++) {
+            const ocol* c2 = &s->cols[v->colWatch[k]];
+            int self = (int)(v - s->vox);
+            fvec f = (c2->v1 == self) ? c2->force : fv(-c2->force.x, -c2->force.y, -c2->force.z);
+            t.x -= f.x; t.y -= f.y; t.z -= f.z;
+        }
+    }
+    return t;
+}
+
+/* CVX_Voxel::moment (VX_Voxel.cpp:258-271) */
+static dvec vox_moment(vxo_sim* s, ovox* v) {
+    dvec t = dv(0, 0, 0);
+    for (int i = 0; i < 6; i++) {
+        if (v->links[i] >= 0) {
+            const olink* l = Lk(s, v->links[i]);
+            dvec m = (i % 2 == 1) ? l->momentPos : l->momentNeg;
+            t.x += m.x; t.y += m.y; t.z += m.z;
+        }
+    }
+    t = qrot(v->orient, t);
+    if (v->ext) { t.x += v->ext->moment.x; t.y += v->ext->moment.y; t.z += v->ext->moment.z; }
+    float ii = v->mat->momentInertiaInverse;
+    dvec av = dv(ii * v->angMom.x, ii * v->angMom.y, ii * v->angMom.z);
+    float c = v->mat->zetaGlobal * v->mat->c2xSqIxExSxSxS;
+    t.x -= c * av.x; t.y -= c * av.y; t.z -= c * av.z;
+    return t;
+}
+
+/* CVX_Voxel::timeStep (VX_Voxel.cpp:162-232) */
+static void vox_timeStep(vxo_sim* s, ovox* v, float dt) {
+    v->previousDt = dt;
+    if (dt == 0.0f) return;
+    if (v->ext && ext_isFixedAll(v->ext)) {
+        dvec op = vox_originalPosition(v);
+        v->pos = dv(op.x + v->ext->translation.x, op.y + v->ext->translation.y, op.z + v->ext->translation.z);
+        v->orient = ext_rotationQuat(v->ext);
+        v->linMom = dv(0, 0, 0);
+        v->angMom = dv(0, 0, 0);
+        return;
+    }
+    dvec cur = vox_force(s, v);
+    dvec fric = cur;
+    int floorOn = (v->boolStates & VS_FLOOR_ENABLED) ? 1 : 0;
+    if (floorOn) vox_floorForce(v, dt, &cur);
+    fric = dv(cur.x - fric.x, cur.y - fric.y, cur.z - fric.z);
+
+    v->linMom.x += dt * cur.x; v->linMom.y += dt * cur.y; v->linMom.z += dt * cur.z;
+    float dtm = dt * v->mat->massInverse;
+    dvec tr = dv(dtm * v->linMom.x, dtm * v->linMom.y, dtm * v->linMom.z);
+
+    if (floorOn && vox_floorPenetration(v) >= 0) {
+        double work = fric.x * tr.x + fric.y * tr.y;
+        double hKe = 0.5 * v->mat->massInverse * (v->linMom.x * v->linMom.x + v->linMom.y * v->linMom.y);
+        if (hKe + work <= 0) vox_setStatic(v, 1);
+        if (v->boolStates & VS_FLOOR_STATIC) {
+            v->linMom.x = v->linMom.y = 0;
+            tr.x = tr.y = 0;
+        }
+    } else {
+        vox_setStatic(v, 0);
+    }
+    v->pos.x += tr.x; v->pos.y += tr.y; v->pos.z += tr.z;
+
+    dvec cm = vox_moment(s, v);
+    v->angMom.x += dt * cm.x; v->angMom.y += dt * cm.y; v->angMom.z += dt * cm.z;
+    float dti = dt * v->mat->momentInertiaInverse;
+    v->orient = qmul(qfromrv(dv(dti * v->angMom.x, dti * v->angMom.y, dti * v->angMom.z)), v->orient);
+
+    if (v->ext) {
+        double size = v->mat->nomSize;
+        oext* e = v->ext;
+        if (ext_isFixed(e, 1)) { v->pos.x = v->ix * size + e->translation.x; v->linMom.x = 0; }
+        if (ext_isFixed(e, 2)) { v->pos.y = v->iy * size + e->translation.y; v->linMom.y = 0; }
+        if (ext_isFixed(e, 4)) { v->pos.z = v->iz * size + e->translation.z; v->linMom.z = 0; }
+        if (ext_isFixedAnyRotation(e)) {
+            if (ext_isFixedAllRotation(e)) {
+                v->orient = ext_rotationQuat(e);
+                v->angMom = dv(0, 0, 0);
+            } else {
+                dvec rv = qtorv(v->orient);
+                if (ext_isFixed(e, 8)) { rv.x = 0; v->angMom.x = 0; }
+                if (ext_isFixed(e, 16)) { rv.y = 0; v->angMom.y = 0; }
+                if (ext_isFixed(e, 32)) { rv.z = 0; v->angMom.z = 0; }
+                v->orient = qfromrv(rv);
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------- collisions */
+static void stamp_reserve(vxo_sim* s) {
+    if (s->stampCap < s->nvslots) {
+        s->stamp = (int*)realloc(s->stamp, sizeof(int) * (size_t)s->nvslots);
+        for (int i = s->stampCap; i < s->nvslots; i++) s->stamp[i] = 0;
+        s->stampCap = s->nvslots;
+    }
+}
+
+/* CVX_Voxel::adjacentVoxel (VX_Voxel.cpp:58-63) */
+static int vox_adjacent(vxo_sim* s, int slot, int dir) {
+    int l = V(s, slot)->links[dir];
+    if (l < 0) return -1;
+    return Lk(s, l)->vPos == slot ? Lk(s, l)->vNeg : Lk(s, l)->vPos;
+}
+
+static int cmp_int(const void* a, const void* b) { int x = *(const int*)a, y = *(const int*)b; return (x > y) - (x < y); }
+
+/* CVX_Voxel::generateNearby (VX_Voxel.cpp:395-418) */
+static void vox_generateNearby(vxo_sim* s, int slot, int depth) {
+    stamp_reserve(s);
+    int mark = ++s->stampVal;
+    int *all = NULL, n = 0, cap = 0;
+    GROW(all, n, cap, 1);
+    all[n++] = slot;
+    s->stamp[slot] = mark;
+    int cur = 0;
+    for (int k = 0; k < depth; k++) {
+        int passEnd = n;
+        while (cur != passEnd) {
+            int pv = all[cur++];
+            for (int i = 0; i < 6; i++) {
+                int p2 = vox_adjacent(s, pv, i);
+                if (p2 >= 0 && s->stamp[p2] != mark) {
+                    s->stamp[p2] = mark;
+                    GROW(all, n, cap, 1);
+                    all[n++] = p2;
+                }
+            }
+        }
+    }
+    ovox* v = V(s, slot);
+    v->nnear = 0;
+    for (int i = 0; i < n; i++) {
+        int p = all[i];
+        if (!vox_isInterior(V(s, p)) && p != slot) {
+            GROW(v->nearby, v->nnear, v->capnear, 1);
+            v->nearby[v->nnear++] = p;
+        }
+    }
+    qsort(v->nearby, (size_t)v->nnear, sizeof(int), cmp_int);
+    free(all);
+}
+
+/* CVoxelyze::clearCollisions (Voxelyze.cpp:712-723) */
+static void sim_clearCollisions(vxo_sim* s) {
+    s->ncols = 0;
+    for (int i = 0; i < s->nvlist; i++) V(s, s->vlist[i])->ncol = 0;
+}
+
+static void vox_addColWatch(ovox* v, int c) {
+    GROW(v->colWatch, v->ncol, v->capcol, 1);
+    v->colWatch[v->ncol++] = c;
+}
+
+/* CVoxelyze::regenerateCollisions (Voxelyze.cpp:725-750) */
+static void sim_regenerateCollisions(vxo_sim* s, float threshRadiusSq) {
+    sim_clearCollisions(s);
+    for (int a = 0; a < s->nvlist; a++) {
+        int s1 = s->vlist[a];
+        ovox* v1 = V(s, s1);
+        if (vox_isInterior(v1)) continue;
+        v1->lastColWatchPos = fv((float)v1->pos.x, (float)v1->pos.y, (float)v1->pos.z);
+        for (int b = a + 1; b < s->nvlist; b++) {
+            int s2 = s->vlist[b];
+            ovox* v2 = V(s, s2);
+            if (vox_isInterior(v2)) continue;
+            dvec d = dv(v1->pos.x - v2->pos.x, v1->pos.y - v2->pos.y, v1->pos.z - v2->pos.z);
+            if (dlen2(d) > threshRadiusSq) continue;
+            if (bsearch(&s2, v1->nearby, (size_t)v1->nnear, sizeof(int), cmp_int)) continue;
+            GROW(s->cols, s->ncols, s->capcols, 1);
+            ocol* c = &s->cols[s->ncols];
+            c->v1 = s1;
+            c->v2 = s2;
+            float k1 = (float)(2 * v1->mat->E * v1->mat->nomSize), k2 = (float)(2 * v2->mat->E * v2->mat->nomSize);
+            c->penetrationStiff = 2.0f / (1.0f / k1 + 1.0f / k2);
+            c->dampingC = 0.5f * (v1->mat->zetaCollision * v1->mat->c2xSqMxExS + v2->mat->zetaCollision * v2->mat->c2xSqMxExS);
+            c->force = fv(0, 0, 0);
+            vox_addColWatch(v1, s->ncols);
+            vox_addColWatch(v2, s->ncols);
+            s->ncols++;
+        }
+    }
+    s->collisionsStale = 0;
+}
+
+/* CVX_Collision::updateContactForce (VX_Collision.cpp:42-56) */
+static void col_update(vxo_sim* s, ocol* c) {
+    const ovox *v1 = V(s, c->v1), *v2 = V(s, c->v2);
+    fvec off = fv((float)(v2->pos.x - v1->pos.x), (float)(v2->pos.y - v1->pos.y), (float)(v2->pos.z - v1->pos.z));
+    float nomDist = (float)((vox_baseSizeAverage(v1) + vox_baseSizeAverage(v2)) * 0.625f);
+    float len = sqrtf(off.x * off.x + off.y * off.y + off.z * off.z);
+    float relDist = nomDist - len;
+    if (relDist > 0) {
+        fvec u = off;
+        float l = sqrtf(off.x * off.x + off.y * off.y + off.z * off.z);
+        if (l > 0) { float inv = 1.0f / l; u = fv(inv * off.x, inv * off.y, inv * off.z); }
+        dvec ve1 = vox_velocity(v1), ve2 = vox_velocity(v2);
+        double ud[3] = {u.x, u.y, u.z};
+        float relVel = (float)((ve1.x * ud[0] + ve1.y * ud[1] + ve1.z * ud[2]) - (ve2.x * ud[0] + ve2.y * ud[1] + ve2.z * ud[2]));
+        float mag = c->penetrationStiff * relDist + c->dampingC * relVel;
+        c->force = fv(mag * u.x, mag * u.y, mag * u.z);
+    } else {
+        c->force = fv(0, 0, 0);
+    }
+}
+
+/* CVoxelyze::updateCollisions (Voxelyze.cpp:670-710) */
+static void sim_updateCollisions(vxo_sim* s) {
+    float watchRadiusVx = 2 * s->boundingRadius + s->watchDistance;
+    float watchRadiusMm = (float)(s->voxSize * watchRadiusVx);
+    float recalcDist = (float)(s->voxSize * s->watchDistance / 2);
+    if (s->nearbyStale) {
+        for (int i = 0; i < s->nvlist; i++) vox_generateNearby(s, s->vlist[i], (int)(watchRadiusVx * 2));
+        s->nearbyStale = 0;
+        s->collisionsStale = 1;
+    }
+    for (int i = 0; i < s->nvlist; i++) {
+        ovox* v = V(s, s->vlist[i]);
+        if (!vox_isInterior(v)) {
+            dvec d = dv(v->pos.x - v->lastColWatchPos.x, v->pos.y - v->lastColWatchPos.y, v->pos.z - v->lastColWatchPos.z);
+            if (dlen2(d) > recalcDist * recalcDist) s->collisionsStale = 1;
+        }
+    }
+    if (s->collisionsStale) sim_regenerateCollisions(s, watchRadiusMm * watchRadiusMm);
+    int nc = s->ncols;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nc; i++) col_update(s, &s->cols[i]);
+}
+
+/* ------------------------------------------------------------ model build */
+static int sim_voxelSlot(vxo_sim* s, int x, int y, int z) { return map_get(&s->vmap, vkey(x, y, z)); }
+
+static const int DOFF[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+static inline int toOpposite(int d) { return d - d % 2 + (d + 1) % 2; }
+/* link array coordinates: CVoxelyze::x/y/zIndexLinkOffset (Voxelyze.h:168-170) */
+static uint64_t linkKeyOf(int x, int y, int z, int dir) {
+    int axis = dir / 2, neg = dir % 2;
+    return lkey(axis, x - (neg && axis == 0), y - (neg && axis == 1), z - (neg && axis == 2));
+}
+
+/* CVoxelyze::combinedMaterial (Voxelyze.cpp:626-640) */
+static omat* sim_combinedMaterial(vxo_sim* s, omat* m1, omat* m2) {
+    for (int i = 0; i < s->nlmats; i++) {
+        omat* t = s->lmats[i];
+        if ((t->vox1 == m1 && t->vox2 == m2) || (t->vox1 == m2 && t->vox2 == m1)) return t;
+    }
+    omat* nm = mat_newLink(m1, m2);
+    GROW(s->lmats, s->nlmats, s->caplmats, 1);
+    s->lmats[s->nlmats++] = nm;
+    mat_addDep(m1, nm);
+    mat_addDep(m2, nm);
+    return nm;
+}
+
+/* CVoxelyze::addLink (Voxelyze.cpp:508-539) with the CVX_Link ctor (VX_Link.cpp:26-59) */
+static int sim_addLink(vxo_sim* s, int x, int y, int z, int dir) {
+    uint64_t key = linkKeyOf(x, y, z, dir);
+    int ex = map_get(&s->lmap, key);
+    if (ex >= 0) return ex;
+    int s1 = sim_voxelSlot(s, x, y, z);
+    int s2 = sim_voxelSlot(s, x + DOFF[dir][0], y + DOFF[dir][1], z + DOFF[dir][2]);
+    if (s1 < 0 || s2 < 0) return -1;
+    omat* m = sim_combinedMaterial(s, V(s, s1)->mat, V(s, s2)->mat);
+    GROW(s->lnk, s->nlslots, s->caplslots, 1);
+    int ls = s->nlslots++;
+    olink* l = Lk(s, ls);
+    memset(l, 0, sizeof(*l));
+    l->axis = dir / 2;
+    if (dir % 2 == 1) { l->vNeg = s2; l->vPos = s1; } else { l->vNeg = s1; l->vPos = s2; }
+    l->mat = m;
+    l->lvv = 0;
+    link_reset(s, l);
+    GROW(s->llist, s->nllist, s->capllist, 1);
+    s->llist[s->nllist++] = ls;
+    map_put(&s->lmap, key, ls);
+    V(s, s1)->links[dir] = ls;
+    vox_updateSurface(V(s, s1));
+    V(s, s2)->links[toOpposite(dir)] = ls;
+    vox_updateSurface(V(s, s2));
+    return ls;
+}
+
+/* CVoxelyze::removeLink (Voxelyze.cpp:541-575) */
+static void sim_removeLink(vxo_sim* s, int x, int y, int z, int dir) {
+    uint64_t key = linkKeyOf(x, y, z, dir);
+    int ls = map_get(&s->lmap, key);
+    if (ls < 0) return;
+    map_del(&s->lmap, key);
+    for (int i = 0; i < s->nllist; i++) {
+        if (s->llist[i] == ls) {
+            memmove(&s->llist[i], &s->llist[i + 1], sizeof(int) * (size_t)(s->nllist - i - 1));
+            s->nllist--;
+            break;
+        }
+    }
+    int s1 = sim_voxelSlot(s, x, y, z);
+    if (s1 >= 0) { V(s, s1)->links[dir] = -1; vox_updateSurface(V(s, s1)); }
+    int s2 = sim_voxelSlot(s, x + DOFF[dir][0], y + DOFF[dir][1], z + DOFF[dir][2]);
+    if (s2 >= 0) { V(s, s2)->links[toOpposite(dir)] = -1; vox_updateSurface(V(s, s2)); }
+}
+
+/* CVX_Voxel::setTemperature (VX_Voxel.cpp:108-114) */
+static void vox_setTemperature(vxo_sim* s, int slot, float t) {
+    ovox* v = V(s, slot);
+    v->temp = t;
+    for (int i = 0; i < 6; i++) if (v->links[i] >= 0) link_updateRestLength(s, Lk(s, v->links[i]));
+}
+
+/* CVX_Voxel::enableCollisions (VX_Voxel.cpp:384-393) */
+static void vox_enableCollisions(ovox* v, int enabled) {
+    if (enabled && !v->hasColWatch) { v->hasColWatch = 1; v->lastColWatchPos = fv(0, 0, 0); }
+    if (enabled) v->boolStates |= VS_COLLISIONS; else v->boolStates &= ~VS_COLLISIONS;
+}
+
+/* CVoxelyze::addVoxel (Voxelyze.cpp:439-461) with the CVX_Voxel ctor (VX_Voxel.cpp:23-37) */
+static int sim_addVoxel(vxo_sim* s, omat* m, int x, int y, int z) {
+    s->nearbyStale = s->collisionsStale = 1;
+    GROW(s->vox, s->nvslots, s->capvslots, 1);
+    int slot = s->nvslots++;
+    ovox* v = V(s, slot);
+    memset(v, 0, sizeof(*v));
+    for (int i = 0; i < 6; i++) v->links[i] = -1;
+    v->mat = m;
+    v->ix = (short)x; v->iy = (short)y; v->iz = (short)z;
+    v->boolStates = 0;
+    vox_reset(v);
+    map_put(&s->vmap, vkey(x, y, z), slot);
+    GROW(s->vlist, s->nvlist, s->capvlist, 1);
+    s->vlist[s->nvlist++] = slot;
+    v->pos = dv(x * s->voxSize, y * s->voxSize, z * s->voxSize);
+    if (s->floor) v->boolStates |= VS_FLOOR_ENABLED; else v->boolStates &= ~VS_FLOOR_ENABLED;
+    vox_setTemperature(s, slot, s->ambientTemp);
+    vox_enableCollisions(v, s->collisions);
+    for (int i = 0; i < 6; i++) sim_addLink(s, x, y, z, i);
+    return slot;
+}
+
+/* CVoxelyze::removeVoxel (Voxelyze.cpp:464-484) */
+static void sim_removeVoxel(vxo_sim* s, int x, int y, int z) {
+    s->nearbyStale = s->collisionsStale = 1;
+    int slot = sim_voxelSlot(s, x, y, z);
+    if (slot < 0) return;
+    ovox* v = V(s, slot);
+    free(v->ext); v->ext = NULL;
+    free(v->colWatch); v->colWatch = NULL; v->ncol = v->capcol = 0;
+    free(v->nearby); v->nearby = NULL; v->nnear = v->capnear = 0;
+    map_del(&s->vmap, vkey(x, y, z));
+    for (int i = 0; i < s->nvlist; i++) {
+        if (s->vlist[i] == slot) {
+            memmove(&s->vlist[i], &s->vlist[i + 1], sizeof(int) * (size_t)(s->nvlist - i - 1));
+            s->nvlist--;
+            break;
+        }
+    }
+    for (int i = 0; i < 6; i++) sim_removeLink(s, x, y, z, i);
+    /* collision records that name the removed voxel are dropped at the next
+       regeneration, which collisionsStale forces before any contact update */
+    s->ncols = 0;
+    for (int i = 0; i < s->nvlist; i++) V(s, s->vlist[i])->ncol = 0;
+}
+
+/* CVoxelyze::replaceVoxel (Voxelyze.cpp:486-499) with CVX_Voxel::replaceMaterial (VX_Voxel.cpp:79-91) */
+static void sim_replaceVoxel(vxo_sim* s, omat* m, int x, int y, int z) {
+    s->collisionsStale = 1;
+    int slot = sim_voxelSlot(s, x, y, z);
+    ovox* v = V(s, slot);
+    if (m) {
+        float fm = m->mass / v->mat->mass;
+        v->linMom.x *= fm; v->linMom.y *= fm; v->linMom.z *= fm;
+        float fi = m->momentInertia / v->mat->momentInertia;
+        v->angMom.x *= fi; v->angMom.y *= fi; v->angMom.z *= fi;
+        vox_setStatic(v, 0);
+        v->mat = m;
+    }
+    for (int i = 0; i < 6; i++) {
+        sim_removeLink(s, x, y, z, i);
+        sim_addLink(s, x, y, z, i);
+    }
+}
+
+/* ------------------------------------------------------------------ step */
+/* CVoxelyze::recommendedTimeStep (Voxelyze.cpp:286-311) */
+static float sim_recommendedTimeStep(vxo_sim* s) {
+    float maxFreq2 = 0.0f;
+    for (int i = 0; i < s->nllist; i++) {
+        olink* l = Lk(s, s->llist[i]);
+        float m1 = V(s, l->vNeg)->mat->mass, m2 = V(s, l->vPos)->mat->mass;
+        float f = l->mat->a1 / (m1 < m2 ? m1 : m2);
+        if (f > maxFreq2) maxFreq2 = f;
+    }
+    if (maxFreq2 <= 0.0f) {
+        for (int i = 0; i < s->nvlist; i++) {
+            ovox* v = V(s, s->vlist[i]);
+            float f = (float)(v->mat->E * v->mat->nomSize / v->mat->mass);
+            if (f > maxFreq2) maxFreq2 = f;
+        }
+    }
+    if (maxFreq2 <= 0.0f) return 0.0f;
+    return 1.0f / (6.283185f * sqrtf(maxFreq2));
+}
+
+static int sim_hasPoisson(vxo_sim* s) {
+    for (int i = 0; i < s->nvmats; i++) if (s->vmats[i]->nu != 0.0f) return 1;
+    return 0;
+}
+
+/* CVoxelyze::doTimeStep (Voxelyze.cpp:251-284) */
+static int sim_doTimeStep(vxo_sim* s, float dt) {
+    if (dt == 0) return 1;
+    else if (dt < 0) dt = sim_recommendedTimeStep(s);
+    int diverged = 0;
+    int nl = s->nllist;
+#pragma omp parallel for schedule(static) reduction(| : diverged)
+    for (int i = 0; i < nl; i++) {
+        olink* l = Lk(s, s->llist[i]);
+        link_updateForces(s, l);
+        if (l->strain > 100) diverged = 1;
+    }
+    if (diverged) return 0;
+    if (s->collisions) sim_updateCollisions(s);
+    int nv = s->nvlist;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nv; i++) vox_timeStep(s, V(s, s->vlist[i]), dt);
+    s->currentTime += dt;
+    return 1;
+}
+
+/* ================================================================ C ABI */
+typedef vxo_sim sim_t;
+
+static int vox_slot_of(sim_t* s, int vox) { return (vox >= 0 && vox < s->nvlist) ? s->vlist[vox] : -1; }
+static omat* mat_of(sim_t* s, int m) { return (m >= 0 && m < s->nvmats) ? s->vmats[m] : NULL; }
+
+EXPORT sim_t* vxo_create(double voxelSize) {
+    sim_t* s = (sim_t*)calloc(1, sizeof(sim_t));
+    map_init(&s->vmap);
+    map_init(&s->lmap);
+    s->voxSize = voxelSize <= 0 ? 0.001 : voxelSize;
+    s->collisionsStale = s->nearbyStale = 1;
+    s->boundingRadius = 0.75f;
+    s->watchDistance = 1.0f;
+    return s;
+}
+
+EXPORT void vxo_destroy(sim_t* s) {
+    if (!s) return;
+    for (int i = 0; i < s->nvslots; i++) {
+        free(s->vox[i].ext); free(s->vox[i].colWatch); free(s->vox[i].nearby);
+    }
+    for (int i = 0; i < s->nvmats; i++) mat_free(s->vmats[i]);
+    for (int i = 0; i < s->nlmats; i++) mat_free(s->lmats[i]);
+    free(s->vmats); free(s->lmats); free(s->vox); free(s->vlist);
+    free(s->lnk); free(s->llist); free(s->cols); free(s->stamp);
+    map_free(&s->vmap); map_free(&s->lmap);
+    free(s);
+}
+
+EXPORT const char* vxo_last_error(sim_t* s) { return s->err; }
+EXPORT double vxo_voxel_size(sim_t* s) { return s->voxSize; }
+
+EXPORT void vxo_set_gravity(sim_t* s, float g) {
+    s->grav = g;
+    for (int i = 0; i < s->nvmats; i++) s->vmats[i]->gravMult = s->grav;
+}
+EXPORT float vxo_gravity(sim_t* s) { return s->grav; }
+EXPORT void vxo_enable_floor(sim_t* s, int en) {
+    s->floor = en ? 1 : 0;
+    for (int i = 0; i < s->nvlist; i++) {
+        ovox* v = V(s, s->vlist[i]);
+        if (en) v->boolStates |= VS_FLOOR_ENABLED; else v->boolStates &= ~VS_FLOOR_ENABLED;
+    }
+}
+EXPORT int vxo_is_floor_enabled(sim_t* s) { return s->floor; }
+EXPORT void vxo_enable_collisions(sim_t* s, int en) {
+    en = en ? 1 : 0;
+    if (s->collisions == en) return;
+    s->collisions = en;
+    for (int i = 0; i < s->nvlist; i++) vox_enableCollisions(V(s, s->vlist[i]), en);
+    if (!en) sim_clearCollisions(s);
+    s->collisionsStale = 1;
+}
+EXPORT int vxo_is_collisions_enabled(sim_t* s) { return s->collisions; }
+EXPORT void vxo_set_ambient_temperature(sim_t* s, float t, int all) {
+    s->ambientTemp = t;
+    if (all) for (int i = 0; i < s->nvlist; i++) vox_setTemperature(s, s->vlist[i], t);
+}
+EXPORT float vxo_ambient_temperature(sim_t* s) { return s->ambientTemp; }
+
+EXPORT int vxo_do_time_step(sim_t* s, float dt) {
+    if (sim_hasPoisson(s)) { snprintf(s->err, sizeof(s->err), "Poisson's ratio != 0 is not restated by the oracle"); return 0; }
+    return sim_doTimeStep(s, dt);
+}
+EXPORT int vxo_step(sim_t* s, float dt, int n, int* divergedAt) {
+    if (divergedAt) *divergedAt = -1;
+    for (int k = 0; k < n; k++) {
+        if (!vxo_do_time_step(s, dt)) { if (divergedAt) *divergedAt = k; return k; }
+    }
+    return n;
+}
+EXPORT float vxo_recommended_time_step(sim_t* s) { return sim_recommendedTimeStep(s); }
+
+/* CVoxelyze::resetTime (Voxelyze.cpp:313-321) */
+EXPORT void vxo_reset_time(sim_t* s) {
+    s->currentTime = 0.0f;
+    s->collisionsStale = 1;
+    s->nearbyStale = 1;
+    for (int i = 0; i < s->nvlist; i++) vox_reset(V(s, s->vlist[i]));
+    for (int i = 0; i < s->nllist; i++) link_reset(s, Lk(s, s->llist[i]));
+}
+EXPORT float vxo_current_time(sim_t* s) { return s->currentTime; }
+EXPORT int vxo_collision_count(sim_t* s) { return s->ncols; }
+EXPORT int vxo_set_precision(sim_t* s, int mode) { (void)s; return mode == 0 ? 1 : 0; }
+
+/* CVoxelyze::stateInfo (Voxelyze.cpp:752-800) */
+EXPORT float vxo_state_info(sim_t* s, int info, int type) {
+    float ret = 0;
+    if (type == 1) ret = -FLT_MAX;
+    else if (type == 0) ret = FLT_MAX;
+    if (info == 7 || info == 5 || info == 6) {
+        if (s->nllist == 0) return 0.0f;
+        for (int i = 0; i < s->nllist; i++) {
+            olink* l = Lk(s, s->llist[i]);
+            float v = 0;
+            if (info == 7) v = link_strainEnergy(l);
+            else if (info == 5) v = l->stress;
+            else v = l->strain;
+            if (type == 0) { if (v < ret) ret = v; }
+            else if (type == 1) { if (v > ret) ret = v; }
+            else ret += v;
+        }
+        if (type == 3) ret /= s->nllist;
+    } else {
+        if (s->nvlist == 0) return 0.0f;
+        for (int i = 0; i < s->nvlist; i++) {
+            ovox* vx = V(s, s->vlist[i]);
+            const omat* m = vx->mat;
+            float v = 0;
+            switch (info) {
+            case 0: { dvec op = vox_originalPosition(vx); v = (float)dlen(dv(vx->pos.x - op.x, vx->pos.y - op.y, vx->pos.z - op.z)); break; }
+            case 1: v = (float)(dlen(vx->linMom) * m->massInverse); break;
+            case 2: v = (float)(0.5 * (m->massInverse * dlen2(vx->linMom) + m->momentInertiaInverse * dlen2(vx->angMom))); break;
+            case 3: v = (float)(2.0 * acos(vx->orient.w > 1 ? 1 : vx->orient.w)); break;
+            case 4: v = (float)(dlen(vx->angMom) * m->momentInertiaInverse); break;
+            case 8: {
+                /* CVX_Voxel::pressure with strain(false) (VX_Voxel.h:102-103, VX_Voxel.cpp:300-333) */
+                float st[3] = {0, 0, 0};
+                int nb[3] = {0, 0, 0};
+                for (int d = 0; d < 6; d++) {
+                    if (vx->links[d] >= 0) {
+                        int ax = d / 2;
+                        st[ax] += link_axialStrainEnd(Lk(s, vx->links[d]), d % 2 == 1);
+                        nb[ax]++;
+                    }
+                }
+                for (int a = 0; a < 3; a++) if (nb[a] == 2) st[a] *= 0.5f;
+                float vol = (float)(st[0] + st[1] + st[2]);
+                v = -m->E * vol / (3 * (1 - 2 * m->nu));
+                break;
+            }
+            case 9: v = m->mass; break;
+            default: v = 0;
+            }
+            if (type == 0) { if (v < ret) ret = v; }
+            else if (type == 1) { if (v > ret) ret = v; }
+            else ret += v;
+        }
+        if (type == 3) ret /= s->nvlist;
+    }
+    return ret;
+}
+
+/* ------------------------------------------------------------ materials */
+/* CVoxelyze::addMaterial(float, float) (Voxelyze.cpp:357-366) */
+EXPORT int vxo_material_add(sim_t* s, float E, float rho) {
+    omat* m = mat_newVoxel(E, rho, s->voxSize);
+    m->gravMult = s->grav;
+    GROW(s->vmats, s->nvmats, s->capvmats, 1);
+    s->vmats[s->nvmats++] = m;
+    return s->nvmats - 1;
+}
+EXPORT int vxo_material_count(sim_t* s) { return s->nvmats; }
+EXPORT int vxo_material_set_model_linear(sim_t* s, int mi, float E, float fail) {
+    omat* m = mat_of(s, mi);
+    return m ? mat_setModelLinear(m, E, fail) : 0;
+}
+EXPORT int vxo_material_set_model_bilinear(sim_t* s, int mi, float E, float pl, float ys, float fs) {
+    omat* m = mat_of(s, mi);
+    return m ? mat_setModelBilinear(m, E, pl, ys, fs) : 0;
+}
+EXPORT int vxo_material_set_model(sim_t* s, int mi, int n, const float* st, const float* ss) {
+    omat* m = mat_of(s, mi);
+    return m ? mat_setModel(m, n, st, ss) : 0;
+}
+/* CVX_Material setters (VX_Material.cpp:472-523) */
+EXPORT void vxo_material_set(sim_t* s, int mi, int prop, float v) {
+    omat* m = mat_of(s, mi);
+    if (!m) return;
+    switch (prop) {
+    case 0:
+        if (v < 0) v = 0;
+        if (v >= 0.5) v = 0.5 - FLT_EPSILON * 2;
+        m->nu = v;
+        mat_updateDerived(m);
+        break;
+    case 1:
+        if (v <= 0) v = FLT_MIN;
+        m->rho = v;
+        mat_updateDerived(m);
+        break;
+    case 2: if (v <= 0) v = 0; m->muStatic = v; break;
+    case 3: if (v <= 0) v = 0; m->muKinetic = v; break;
+    case 4: if (v <= 0) v = 0; m->zetaInternal = v; break;
+    case 5: if (v <= 0) v = 0; m->zetaGlobal = v; break;
+    case 6: if (v <= 0) v = 0; m->zetaCollision = v; break;
+    case 7: m->alphaCTE = v; break;
+    default: break;
+    }
+}
+EXPORT float vxo_material_get(sim_t* s, int mi, int prop) {
+    omat* m = mat_of(s, mi);
+    if (!m) return 0;
+    switch (prop) {
+    case 0: return m->nu;
+    case 1: return m->rho;
+    case 2: return m->muStatic;
+    case 3: return m->muKinetic;
+    case 4: return m->zetaInternal;
+    case 5: return m->zetaGlobal;
+    case 6: return m->zetaCollision;
+    case 7: return m->alphaCTE;
+    case 8: return m->E;
+    case 9: return m->sigmaYield;
+    case 10: return m->sigmaFail;
+    case 11: return m->mass;
+    case 12: return m->momentInertia;
+    case 13: return m->epsilonYield;
+    case 14: return m->epsilonFail;
+    case 15: return (float)m->linear;
+    default: return 0;
+    }
+}
+EXPORT void vxo_material_set_external_scale(sim_t* s, int mi, double x, double y, double z) {
+    omat* m = mat_of(s, mi);
+    if (!m) return;
+    if (x <= 0) x = FLT_MIN;
+    if (y <= 0) y = FLT_MIN;
+    if (z <= 0) z = FLT_MIN;
+    m->extScale = dv(x, y, z);
+}
+EXPORT const char* vxo_material_last_error(sim_t* s, int mi) {
+    omat* m = mat_of(s, mi);
+    return m ? m->error : "";
+}
+EXPORT float vxo_material_stress(sim_t* s, int mi, float strain, float tss, int fl) {
+    omat* m = mat_of(s, mi);
+    return m ? mat_stress(m, strain, tss, fl) : 0;
+}
+EXPORT float vxo_material_modulus(sim_t* s, int mi, float strain) {
+    omat* m = mat_of(s, mi);
+    return m ? mat_modulus(m, strain) : 0;
+}
+EXPORT int vxo_material_model_data(sim_t* s, int mi, float* st, float* ss, int cap) {
+    omat* m = mat_of(s, mi);
+    if (!m) return 0;
+    for (int i = 0; i < m->ndata && i < cap; i++) {
+        if (st) st[i] = m->strainData[i];
+        if (ss) ss[i] = m->stressData[i];
+    }
+    return m->ndata;
+}
+
+/* ---------------------------------------------------------------- voxels */
+static int vox_index_of_slot(sim_t* s, int slot) {
+    for (int i = s->nvlist - 1; i >= 0; i--) if (s->vlist[i] == slot) return i;
+    return -1;
+}
+/* CVoxelyze::setVoxel (Voxelyze.cpp:422-437) */
+EXPORT int vxo_voxel_set(sim_t* s, int mi, int x, int y, int z) {
+    if (mi < 0) { sim_removeVoxel(s, x, y, z); return -1; }
+    omat* m = mat_of(s, mi);
+    if (!m) return -1;
+    int slot = sim_voxelSlot(s, x, y, z);
+    if (slot >= 0) {
+        sim_replaceVoxel(s, m, x, y, z);
+        return vox_index_of_slot(s, slot);
+    }
+    sim_addVoxel(s, m, x, y, z);
+    return s->nvlist - 1;
+}
+EXPORT int vxo_voxel_count(sim_t* s) { return s->nvlist; }
+EXPORT int vxo_voxel_find(sim_t* s, int x, int y, int z) {
+    int slot = sim_voxelSlot(s, x, y, z);
+    return slot < 0 ? -1 : vox_index_of_slot(s, slot);
+}
+EXPORT int vxo_voxel_info(sim_t* s, int vi, int* ijk) {
+    int slot = vox_slot_of(s, vi);
+    if (slot < 0) return -1;
+    ovox* v = V(s, slot);
+    if (ijk) { ijk[0] = v->ix; ijk[1] = v->iy; ijk[2] = v->iz; }
+    for (int i = 0; i < s->nvmats; i++) if (s->vmats[i] == v->mat) return i;
+    return -1;
+}
+EXPORT void vxo_voxel_set_temperature(sim_t* s, int vi, float t) {
+    int slot = vox_slot_of(s, vi);
+    if (slot >= 0) vox_setTemperature(s, slot, t);
+}
+EXPORT float vxo_voxel_temperature(sim_t* s, int vi) {
+    int slot = vox_slot_of(s, vi);
+    return slot >= 0 ? V(s, slot)->temp : 0.0f;
+}
+EXPORT void vxo_voxel_get_state(sim_t* s, int first, int count, double* pos, double* orient, double* lin,
+                                double* ang, uint32_t* flags) {
+    for (int k = 0; k < count; k++) {
+        int slot = vox_slot_of(s, first + k);
+        if (slot < 0) continue;
+        ovox* v = V(s, slot);
+        if (pos) { pos[3 * k] = v->pos.x; pos[3 * k + 1] = v->pos.y; pos[3 * k + 2] = v->pos.z; }
+        if (orient) { orient[4 * k] = v->orient.w; orient[4 * k + 1] = v->orient.x; orient[4 * k + 2] = v->orient.y; orient[4 * k + 3] = v->orient.z; }
+        if (lin) { lin[3 * k] = v->linMom.x; lin[3 * k + 1] = v->linMom.y; lin[3 * k + 2] = v->linMom.z; }
+        if (ang) { ang[3 * k] = v->angMom.x; ang[3 * k + 1] = v->angMom.y; ang[3 * k + 2] = v->angMom.z; }
+        if (flags) flags[k] = (uint32_t)v->boolStates;
+    }
+}
+
+/* ------------------------------------------------------------- externals */
+static oext* ext_at(sim_t* s, int vi) {
+    int slot = vox_slot_of(s, vi);
+    return slot < 0 ? NULL : vox_external(V(s, slot));
+}
+/* CVX_External::setFixed(6 x bool) (VX_External.cpp:47-51): no rotationChanged() */
+EXPORT void vxo_ext_set_fixed(sim_t* s, int vi, int mask) {
+    oext* e = ext_at(s, vi);
+    if (!e) return;
+    e->dof = mask & 0x3F;
+    e->translation = e->rotation = dv(0, 0, 0);
+}
+/* CVX_External::setDisplacement (VX_External.cpp:53-66) */
+EXPORT void vxo_ext_set_displacement(sim_t* s, int vi, int dof, double d) {
+    oext* e = ext_at(s, vi);
+    if (!e) return;
+    e->dof |= dof;
+    if (d != 0.0f) {
+        if (dof & 1) e->translation.x = d;
+        if (dof & 2) e->translation.y = d;
+        if (dof & 4) e->translation.z = d;
+        if (dof & 8) e->rotation.x = d;
+        if (dof & 16) e->rotation.y = d;
+        if (dof & 32) e->rotation.z = d;
+    }
+    ext_rotationChanged(e);
+}
+/* CVX_External::setDisplacementAll (VX_External.cpp:68-75) */
+EXPORT void vxo_ext_set_displacement_all(sim_t* s, int vi, const double* t, const double* r) {
+    oext* e = ext_at(s, vi);
+    if (!e) return;
+    e->dof |= 0x3F;
+    e->translation = t ? dv(t[0], t[1], t[2]) : dv(0, 0, 0);
+    e->rotation = r ? dv(r[0], r[1], r[2]) : dv(0, 0, 0);
+    ext_rotationChanged(e);
+}
+/* CVX_External::clearDisplacement (VX_External.cpp:77-89) */
+EXPORT void vxo_ext_clear_displacement(sim_t* s, int vi, int dof) {
+    oext* e = ext_at(s, vi);
+    if (!e) return;
+    e->dof &= ~dof;
+    if (dof & 1) e->translation.x = 0.0;
+    if (dof & 2) e->translation.y = 0.0;
+    if (dof & 4) e->translation.z = 0.0;
+    if (dof & 8) e->rotation.x = 0.0;
+    if (dof & 16) e->rotation.y = 0.0;
+    if (dof & 32) e->rotation.z = 0.0;
+    ext_rotationChanged(e);
+}
+/* CVX_External::clearDisplacementAll (VX_External.cpp:91-98) */
+EXPORT void vxo_ext_clear_displacement_all(sim_t* s, int vi) {
+    oext* e = ext_at(s, vi);
+    if (!e) return;
+    e->dof &= ~0x3F;
+    e->translation = dv(0, 0, 0);
+    e->rotation = dv(0, 0, 0);
+    ext_rotationChanged(e);
+}
+EXPORT void vxo_ext_set_force(sim_t* s, int vi, float x, float y, float z) {
+    oext* e = ext_at(s, vi);
+    if (e) e->force = fv(x, y, z);
+}
+EXPORT void vxo_ext_set_moment(sim_t* s, int vi, float x, float y, float z) {
+    oext* e = ext_at(s, vi);
+    if (e) e->moment = fv(x, y, z);
+}
+EXPORT void vxo_ext_reset(sim_t* s, int vi) {
+    oext* e = ext_at(s, vi);
+    if (e) ext_reset(e);
+}
+EXPORT int vxo_ext_get(sim_t* s, int vi, int* dof, float* f, float* m, double* t, double* r) {
+    int slot = vox_slot_of(s, vi);
+    if (slot < 0 || !V(s, slot)->ext) return 0;
+    oext* e = V(s, slot)->ext;
+    if (dof) *dof = e->dof;
+    if (f) { f[0] = e->force.x; f[1] = e->force.y; f[2] = e->force.z; }
+    if (m) { m[0] = e->moment.x; m[1] = e->moment.y; m[2] = e->moment.z; }
+    if (t) { t[0] = e->translation.x; t[1] = e->translation.y; t[2] = e->translation.z; }
+    if (r) { r[0] = e->rotation.x; r[1] = e->rotation.y; r[2] = e->rotation.z; }
+    return 1;
+}
+
+/* ------------------------------------------------------------------ links */
+EXPORT int vxo_link_count(sim_t* s) { return s->nllist; }
+EXPORT int vxo_link_find(sim_t* s, int x, int y, int z, int dir) {
+    if (dir < 0 || dir > 5) return -1;
+    int ls = map_get(&s->lmap, linkKeyOf(x, y, z, dir));
+    if (ls < 0) return -1;
+    for (int i = 0; i < s->nllist; i++) if (s->llist[i] == ls) return i;
+    return -1;
+}
+EXPORT int vxo_link_info(sim_t* s, int li, int* neg, int* pos) {
+    if (li < 0 || li >= s->nllist) return -1;
+    olink* l = Lk(s, s->llist[li]);
+    if (neg) *neg = vox_index_of_slot(s, l->vNeg);
+    if (pos) *pos = vox_index_of_slot(s, l->vPos);
+    return l->axis;
+}
+EXPORT void vxo_link_get_state(sim_t* s, int first, int count, float* strain, float* stress, float* maxStrain,
+                               float* strainOffset, uint32_t* flags, double* local9, double* forces12,
+                               double* restLength) {
+    for (int k = 0; k < count; k++) {
+        int li = first + k;
+        if (li < 0 || li >= s->nllist) continue;
+        olink* l = Lk(s, s->llist[li]);
+        if (strain) strain[k] = l->strain;
+        if (stress) stress[k] = l->stress;
+        if (maxStrain) maxStrain[k] = l->maxStrain;
+        if (strainOffset) strainOffset[k] = l->strainOffset;
+        if (flags)
+            flags[k] = (uint32_t)((l->lvv ? 1 : 0) | (l->smallAngle ? 2 : 0) | (mat_isYielded(l->mat, l->maxStrain) ? 4 : 0) |
+                                  (mat_isFailed(l->mat, l->maxStrain) ? 8 : 0));
+        if (local9) {
+            double* o = local9 + 9 * k;
+            o[0] = l->pos2.x; o[1] = l->pos2.y; o[2] = l->pos2.z;
+            o[3] = l->angle1v.x; o[4] = l->angle1v.y; o[5] = l->angle1v.z;
+            o[6] = l->angle2v.x; o[7] = l->angle2v.y; o[8] = l->angle2v.z;
+        }
+        if (forces12) {
+            double* o = forces12 + 12 * k;
+            o[0] = l->forceNeg.x; o[1] = l->forceNeg.y; o[2] = l->forceNeg.z;
+            o[3] = l->forcePos.x; o[4] = l->forcePos.y; o[5] = l->forcePos.z;
+            o[6] = l->momentNeg.x; o[7] = l->momentNeg.y; o[8] = l->momentNeg.z;
+            o[9] = l->momentPos.x; o[10] = l->momentPos.y; o[11] = l->momentPos.z;
+        }
+        if (restLength) restLength[k] = l->currentRestLength;
+    }
+}

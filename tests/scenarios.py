@@ -1,0 +1,748 @@
+"""Scenarios of the reference's own tests (test/tVoxelyze.h) and scaled-down
+benchmark configurations, written once against the CVoxelyze-style API so the
+same function drives the compiled reference (golden recording), the CPU oracle
+and the HIP library.
+
+Each scenario takes ``make(voxel_size) -> CVoxelyze`` and returns a dict of
+numpy arrays: the probe traces the reference test looks at, the final voxel
+and link state, and any closed-form quantity the reference test asserts.
+"""
+import numpy as np
+
+from voxelyze_amd import api
+from voxelyze_amd import workloads as W
+
+X_POS, X_NEG, Y_POS, Y_NEG, Z_POS, Z_NEG = range(6)
+OFF = {X_POS: (1, 0, 0), X_NEG: (-1, 0, 0), Y_POS: (0, 1, 0), Y_NEG: (0, -1, 0), Z_POS: (0, 0, 1), Z_NEG: (0, 0, -1)}
+
+
+def snapshot(sim, prefix=""):
+    out = {}
+    st = sim.state()
+    for k, v in st.items():
+        out[prefix + "vox_" + k] = v
+    ls = sim.link_state()
+    for k, v in ls.items():
+        out[prefix + "link_" + k] = v
+    return out
+
+
+def rotvec(q):
+    """Quat3D::ToRotationVector (include/Quat3D.h:117-122) on the host, numpy float64."""
+    w, x, y, z = q
+    if w >= 1.0 or w <= -1.0:
+        return np.zeros(3)
+    sl = 1.0 - w * w
+    if sl < 2.4e-3:
+        s = np.sqrt((2 - 2 * w) / sl)
+        return np.array([x * 2.0 * s, y * 2.0 * s, z * 2.0 * s])
+    a = np.arccos(w)
+    inv = 1.0 / np.sqrt(sl)
+    return np.array([inv * (x * 2.0 * a), inv * (y * 2.0 * a), inv * (z * 2.0 * a)])
+
+
+# --------------------------------------------------------------------- test2Vox (tVoxelyze.h:63-113)
+def two_voxel(make, v1_fixed, direction, force, moment, dofmask, steps, probe):
+    sz = 0.001
+    sim = make(sz)
+    m = sim.addMaterial(1e6, 1e3)
+    m.setInternalDamping(1.0)
+    m.setGlobalDamping(0.2)
+    off = OFF[direction]
+    v1 = sim.setVoxel(m, 0, 0, 0)
+    if v1_fixed:
+        v1.external().setFixedAll()
+    else:
+        v1.external().setForce(*[-f for f in force])
+        v1.external().setMoment(*[-f for f in moment])
+        v1.external().setFixed(*[bool(dofmask & (1 << b)) for b in range(6)])
+    v2 = sim.setVoxel(m, *off)
+    v2.external().setForce(*force)
+    v2.external().setMoment(*moment)
+    v2.external().setFixed(*[bool(dofmask & (1 << b)) for b in range(6)])
+    ts = sim.recommendedTimeStep()
+    trace = np.zeros(steps)
+    for k in range(steps):
+        sim.doTimeStep(ts)
+        st = sim.state(v2.index, 1)
+        if probe < 3:
+            trace[k] = st["pos"][0][probe] - off[probe] * sz
+        else:
+            trace[k] = rotvec(st["orient"][0])[probe - 3]
+    out = {"trace": trace, "ts": np.float32(ts)}
+    out.update(snapshot(sim))
+    return out
+
+
+FPX, FNX, FPY, FNY, FPZ, FNZ = [(1e-3, 0, 0), (-1e-3, 0, 0), (0, 1e-3, 0), (0, -1e-3, 0), (0, 0, 1e-3), (0, 0, -1e-3)]
+MPX, MPY, MPZ = (1e-9, 0, 0), (0, 1e-9, 0), (0, 0, 1e-9)
+Z3 = (0, 0, 0)
+TX, TY, TZ, RX, RY, RZ = range(6)
+
+# (v1Fixed, dir, force, moment, dof(tx..rz free-mask as fixed bits), expected, probe, max steps) -- tVoxelyze.h:142-244
+TWO_VOXEL_CASES = [
+    (True, X_POS, FPX, Z3, api.dof(False, True, True, True, True, True), 1e-6, TX),
+    (True, X_POS, FNY, Z3, api.dof(True, False, True, True, True, True), -1e-6, TY),
+    (True, X_POS, FPZ, Z3, api.dof(True, True, False, True, True, True), 1e-6, TZ),
+    (True, Y_NEG, FPX, Z3, api.dof(False, True, True, True, True, True), 1e-6, TX),
+    (True, Z_NEG, FPZ, Z3, api.dof(True, True, False, True, True, True), 1e-6, TZ),
+    (True, X_POS, Z3, MPX, api.dof(True, True, True, False, True, True), 1.2e-5, RX),
+    (True, X_POS, Z3, MPY, api.dof(True, True, True, True, False, True), 3e-6, RY),
+    (True, Y_POS, Z3, MPY, api.dof(True, True, True, True, False, True), 1.2e-5, RY),
+    (True, Z_NEG, Z3, MPX, api.dof(True, True, True, False, True, True), 3e-6, RX),
+    (True, X_POS, Z3, MPY, api.dof(True, True, False, True, False, True), -6e-9, TZ),
+    (True, X_POS, FPY, Z3, api.dof(True, False, True, True, True, False), 6e-3, RZ),
+    (True, X_POS, FNZ, Z3, api.dof(True, True, False, True, False, True), 6e-3, RY),
+    (False, X_POS, FPX, Z3, api.dof(False, True, True, True, True, True), 5e-7, TX),
+    (False, Y_POS, FPY, Z3, api.dof(True, False, True, True, True, True), 5e-7, TY),
+    (False, Z_NEG, FPX, Z3, api.dof(False, True, True, True, True, True), 5e-7, TX),
+    (False, X_POS, Z3, MPZ, api.dof(True, True, True, True, True, False), 6e-6, RZ),
+    (False, Y_NEG, Z3, MPX, api.dof(True, True, True, False, True, True), 6e-6, RX),
+]
+
+
+def converge_steps(trace, expected, rel=1e-5):
+    """The convergence counter of test2Vox (tVoxelyze.h:99-107)."""
+    c = 0
+    for k, d in enumerate(trace):
+        if abs(d - expected) < abs(expected) * rel or np.float32(d) == np.float32(expected):
+            c += 1
+        else:
+            c = 0
+        if c == 10:
+            return k
+    return len(trace)
+
+
+# ------------------------------------------------------------------ other reference tests
+def surrounded(make, internal, glob, steps=100):
+    """internalDamping / globalDamping centre voxel (tVoxelyze.h:260-302)."""
+    sim = make(0.001)
+    m = sim.addMaterial(1e6, 1e3)
+    m.setInternalDamping(internal)
+    m.setGlobalDamping(glob)
+    v1 = sim.setVoxel(m, 0, 0, 0)
+    v1.external().setForce(1e-6, 1e-6, 1e-6)
+    for d in range(6):
+        sim.setVoxel(m, *OFF[d]).external().setFixedAll()
+    ts = sim.recommendedTimeStep()
+    trace = np.zeros((steps, 3))
+    for k in range(steps):
+        sim.doTimeStep(ts)
+        trace[k] = sim.state(v1.index, 1)["pos"][0]
+    out = {"trace": trace}
+    out.update(snapshot(sim))
+    return out
+
+
+def cantilever2_global(make, steps=300):
+    """2-voxel cantilever with global damping (tVoxelyze.h:311-333)."""
+    sim = make(0.001)
+    m = sim.addMaterial(1e6, 1e3)
+    m.setGlobalDamping(0.25)
+    m.setInternalDamping(0)
+    sim.setVoxel(m, 0, 0, 0).external().setFixedAll()
+    v = sim.setVoxel(m, 1, 0, 0)
+    v.external().setForce(1e-6, 1e-6, 1e-6)
+    ts = sim.recommendedTimeStep()
+    trace = np.zeros(steps)
+    for k in range(steps):
+        sim.doTimeStep(ts)
+        trace[k] = sim.state(v.index, 1)["pos"][0][1]
+    out = {"trace": trace}
+    out.update(snapshot(sim))
+    return out
+
+
+def combined_damping(make, steps=1000):
+    """4x3x3 block, global damping changed between runs with resetTime (tVoxelyze.h:336-379)."""
+    sim = make(0.001)
+    m = sim.addMaterial(1e6, 1e3)
+    m.setInternalDamping(1.0)
+    for i in range(4):
+        for j in range(3):
+            for k in range(3):
+                v = sim.setVoxel(m, i, j, k)
+                if i == 0:
+                    v.external().setFixedAll()
+                elif i == 3:
+                    v.external().setForce(0, 0, 1e-6)
+    probe = sim.voxel(3, 0, 0).index
+    out = {}
+    for i in range(2):
+        m.setGlobalDamping(np.float32(0.05) + np.float32(0.05) * np.float32(i))
+        ts = sim.recommendedTimeStep()
+        sim.resetTime()
+        trace = np.zeros(steps)
+        for k in range(steps):
+            sim.doTimeStep(ts)
+            trace[k] = sim.state(probe, 1)["pos"][0][2]
+        out[f"trace{i}"] = trace
+    out.update(snapshot(sim))
+    return out
+
+
+def scale_forces(make, steps=240):
+    """Force scaling (tVoxelyze.h:382-412)."""
+    sim = make(0.001)
+    m = sim.addMaterial(1e6, 1e3)
+    m.setInternalDamping(1.0)
+    m.setGlobalDamping(0.2)
+    sim.setVoxel(m, 0, 0, 0).external().setFixedAll()
+    v2 = sim.setVoxel(m, 1, 0, 0)
+    out = {}
+    for i in range(3, 8):
+        f = float(np.float32(1) / np.float32(10.0) ** i)
+        v2.external().setForce(f, f, f)
+        ts = sim.recommendedTimeStep()
+        sim.resetTime()
+        trace = np.zeros(steps)
+        for k in range(steps):
+            sim.doTimeStep(ts)
+            trace[k] = sim.state(v2.index, 1)["pos"][0][1]
+        out[f"trace{i}"] = trace
+    out.update(snapshot(sim))
+    return out
+
+
+def frequency(make, steps=1000):
+    """Axial natural frequency (tVoxelyze.h:415-443)."""
+    sim = make(0.001)
+    m = sim.addMaterial(1e6, 1e3)
+    m.setInternalDamping(0)
+    v1 = sim.setVoxel(m, 0, 0, 0)
+    v2 = sim.setVoxel(m, 1, 0, 0)
+    v1.external().setFixedAll()
+    v2.external().setFixed(False, True, True, True, True, True)
+    v2.external().setForce(1e-3, 0, 0)
+    ts = np.float32(sim.recommendedTimeStep()) / np.float32(10)
+    trace = np.zeros(steps)
+    for k in range(steps):
+        sim.doTimeStep(float(ts))
+        trace[k] = sim.state(v2.index, 1)["pos"][0][0] - 0.001001
+    out = {"trace": trace, "ts": np.float32(ts)}
+    out.update(snapshot(sim))
+    return out
+
+
+def large_deformation_damping(make, steps=200):
+    """Large-angle branch under damping (tVoxelyze.h:445-473)."""
+    sim = make(0.001)
+    m = sim.addMaterial(1e6, 1e3)
+    m.setInternalDamping(1.0)
+    m.setGlobalDamping(0.2)
+    v1 = sim.setVoxel(m, 0, 0, 0)
+    v2 = sim.setVoxel(m, 1, 0, 0)
+    ts = sim.recommendedTimeStep()
+    v1.external().setFixedAll()
+    v2.external().setForce(-0.2, 0.0, 0.2)
+    trace = np.zeros(steps)
+    small = np.zeros(steps, np.int32)
+    lk = sim.link(0, 0, 0, X_POS)
+    for k in range(steps):
+        sim.doTimeStep(ts)
+        trace[k] = sim.state(v2.index, 1)["pos"][0][2]
+        small[k] = lk.isSmallAngle()
+    out = {"trace": trace, "small": small}
+    out.update(snapshot(sim))
+    return out
+
+
+def large_deformation(make, levels=100, steps=100):
+    """Force ramp across the small/large-angle switch (tVoxelyze.h:475-522)."""
+    sim = make(0.001)
+    m = sim.addMaterial(1e6, 1e3)
+    m.setInternalDamping(1.0)
+    m.setGlobalDamping(0.2)
+    v1 = sim.setVoxel(m, 0, 0, 0)
+    v2 = sim.setVoxel(m, 1, 0, 0)
+    ts = sim.recommendedTimeStep()
+    v1.external().setFixedAll()
+    lk = sim.link(0, 0, 0, X_POS)
+    zs = np.zeros(levels)
+    small = np.zeros(levels, np.int32)
+    for i in range(levels):
+        v2.external().setForce(0, 0, float(np.float32(i) * np.float32(2e-3)))
+        for _ in range(steps):
+            sim.doTimeStep(ts)
+        zs[i] = sim.state(v2.index, 1)["pos"][0][2]
+        small[i] = lk.isSmallAngle()
+    out = {"trace": zs, "small": small, "ts": np.float32(ts)}
+    out.update(snapshot(sim))
+    return out
+
+
+def double_bond_cantilever(make, maxsteps=600):
+    """3-voxel cantilever, each axis and sign, with resetTime (tVoxelyze.h:525-575)."""
+    sim = make(0.001)
+    m = sim.addMaterial(1e6, 1e3)
+    m.setInternalDamping(1.0)
+    m.setGlobalDamping(0.1)
+    v1 = sim.setVoxel(m, 0, 0, 0)
+    sim.setVoxel(m, 1, 0, 0)
+    v3 = sim.setVoxel(m, 2, 0, 0)
+    ts = sim.recommendedTimeStep()
+    v1.external().setFixedAll()
+    out = {}
+    for i in range(3):
+        v3.external().setFixed(not (i == 0), not (i == 1), not (i == 2), True, not (i == 2), not (i == 1))
+        for j in range(2):
+            sign = 1.0 if j == 0 else -1.0
+            f = [0.0, 0.0, 0.0]
+            f[i] = float(np.float32(sign) * np.float32(5e-6))
+            v3.external().setForce(*f)
+            sim.resetTime()
+            tr = np.zeros(maxsteps)
+            for k in range(maxsteps):
+                sim.doTimeStep(ts)
+                tr[k] = sim.state(v3.index, 1)["pos"][0][i] - (0.002 if i == 0 else 0.0)
+            out[f"trace{i}{j}"] = tr
+    out.update(snapshot(sim))
+    return out
+
+
+def impulse(make, steps=1000):
+    """Large-angle impulse on a 4x2 plate (tVoxelyze.h:577-607)."""
+    sim = make(0.001)
+    m = sim.addMaterial(1e6, 1e3)
+    m.setInternalDamping(1.0)
+    m.setGlobalDamping(0.05)
+    for i in range(4):
+        for j in range(2):
+            v = sim.setVoxel(m, i, j, 0)
+            if i == 0:
+                v.external().setFixedAll()
+    ts = sim.recommendedTimeStep()
+    pv = sim.voxel(3, 0, 0)
+    trace = np.zeros(steps)
+    for k in range(steps):
+        if k == 10:
+            pv.external().setForce(0, 0, 100)
+        elif k == 11:
+            pv.external().setForce(0, 0, 0)
+        sim.doTimeStep(ts)
+        trace[k] = sim.state(pv.index, 1)["pos"][0][2]
+    out = {"trace": trace}
+    out.update(snapshot(sim))
+    return out
+
+
+def multi_simple(make, steps=800):
+    """Two materials in series (tVoxelyze.h:612-641)."""
+    sim = make(0.001)
+    m1 = sim.addMaterial(1e6, 1e3)
+    m2 = sim.addMaterial(1e9, 1e3)
+    for m in (m1, m2):
+        m.setGlobalDamping(0.03)
+        m.setInternalDamping(0.01)
+    for i in range(2):
+        a = sim.setVoxel(m1, 2 * i, 0, 0)
+        b = sim.setVoxel(m2, 2 * i + 1, 0, 0)
+        if i == 0:
+            a.external().setFixedAll()
+        if i == 1:
+            b.external().setForce(1e-3, 0, 0)
+    ts = sim.recommendedTimeStep()
+    probe = sim.voxel(3, 0, 0).index
+    trace = np.zeros(steps)
+    for k in range(steps):
+        sim.doTimeStep(ts)
+        trace[k] = sim.state(probe, 1)["pos"][0][0] - 0.003
+    out = {"trace": trace}
+    out.update(snapshot(sim))
+    return out
+
+
+def multi_simple2(make, steps=10000, every=50):
+    """Three effective materials in series (tVoxelyze.h:643-684)."""
+    sim = make(0.001)
+    m1 = sim.addMaterial(1e6, 1e3)
+    m2 = sim.addMaterial(1e9, 1e3)
+    for m in (m1, m2):
+        m.setGlobalDamping(0.01)
+        m.setInternalDamping(1.0)
+    for i in range(8):
+        v = sim.setVoxel(m1 if (i // 2) % 2 == 0 else m2, i, 0, 0)
+        if i == 0:
+            v.external().setFixedAll()
+        if i == 7:
+            v.external().setForce(1e-3, 0, 0)
+    ts = sim.recommendedTimeStep()
+    probe = sim.voxel(7, 0, 0).index
+    trace = np.zeros(steps // every)
+    for k in range(steps):
+        sim.doTimeStep(ts)
+        if (k + 1) % every == 0:
+            trace[k // every] = sim.state(probe, 1)["pos"][0][0] - 0.007
+    out = {"trace": trace}
+    out.update(snapshot(sim))
+    return out
+
+
+def deformable_material(make, load=400, relax=250):
+    """Bilinear plastic set in a 5x3x3 bar (tVoxelyze.h:845-889)."""
+    sim = make(0.001)
+    m = sim.addMaterial(1e6, 1e3)
+    m.setModelBilinear(1e6, 5e5, 1e5)
+    m.setInternalDamping(1.0)
+    m.setGlobalDamping(0.2)
+    for i in range(5):
+        for j in range(3):
+            for k in range(3):
+                v = sim.setVoxel(m, i, j, k)
+                if i == 0:
+                    v.external().setFixedAll()
+                if i == 4:
+                    v.external().setForce(0.2, 0.0, 0.0)
+    ts = sim.recommendedTimeStep()
+    probe = sim.voxel(4, 1, 1).index
+    trace = np.zeros(load + relax)
+    for k in range(load):
+        sim.doTimeStep(ts)
+        trace[k] = sim.state(probe, 1)["pos"][0][0] - 0.004
+    for j in range(3):
+        for k in range(3):
+            sim.voxel(4, j, k).external().setForce(0.0, 0.0, 0.0)
+    for k in range(relax):
+        sim.doTimeStep(ts)
+        trace[load + k] = sim.state(probe, 1)["pos"][0][0] - 0.004
+    out = {"trace": trace}
+    out.update(snapshot(sim))
+    return out
+
+
+def replace_material(make, steps=2000):
+    """Material replaced mid-run (tVoxelyze.h:946-986)."""
+    sim = make(0.001)
+    m1 = sim.addMaterial(1e6, 1e3)
+    m1.setInternalDamping(1.0)
+    m1.setGlobalDamping(0.08)
+    m2 = sim.addMaterial(1e7, 1e3)
+    m2.setInternalDamping(1.0)
+    m2.setGlobalDamping(0.08)
+    for i in range(5):
+        for j in range(2):
+            for k in range(2):
+                v = sim.setVoxel(m1, i, j, k)
+                if i == 0:
+                    v.external().setFixedAll()
+                elif i == 4:
+                    v.external().setForce(0, 0, 1e-6)
+    ts = sim.recommendedTimeStep()
+    probe = sim.voxel(4, 0, 0).index
+    trace = np.zeros(steps)
+    for k in range(steps):
+        sim.doTimeStep(ts)
+        if k == 150:
+            for i in range(5):
+                for j in range(2):
+                    for kk in range(2):
+                        if i % 2 == 1:
+                            sim.setVoxel(m2, i, j, kk)
+            ts = sim.recommendedTimeStep()
+        trace[k] = sim.state(probe, 1)["pos"][0][2]
+    out = {"trace": trace}
+    out.update(snapshot(sim))
+    return out
+
+
+def temperature(make, steps=500):
+    """Thermal bimorph, both signs (tVoxelyze.h:989-1032)."""
+    sim = make(0.001)
+    m1 = sim.addMaterial(1e6, 1e3)
+    m1.setInternalDamping(1.0)
+    m1.setGlobalDamping(0.15)
+    m1.setCte(0.01)
+    m2 = sim.addMaterial(1e7, 1e3)
+    m2.setInternalDamping(1.0)
+    m2.setGlobalDamping(0.15)
+    for i in range(3):
+        a = sim.setVoxel(m1, i, 0, 0)
+        b = sim.setVoxel(m2, i, 0, 1)
+        if i == 0:
+            a.external().setFixedAll()
+            b.external().setFixedAll()
+    sim.setAmbientTemperature(5, True)
+    ts = sim.recommendedTimeStep()
+    probe = sim.voxel(2, 0, 0).index
+    t1 = np.zeros(steps)
+    for k in range(steps):
+        sim.doTimeStep(ts)
+        t1[k] = sim.state(probe, 1)["pos"][0][2]
+    sim.resetTime()
+    sim.setAmbientTemperature(-5, True)
+    t2 = np.zeros(steps)
+    for k in range(steps):
+        sim.doTimeStep(ts)
+        t2[k] = sim.state(probe, 1)["pos"][0][2]
+    out = {"trace1": t1, "trace2": t2}
+    out.update(snapshot(sim))
+    return out
+
+
+def static_friction(make):
+    """Static friction break-away at 0.9/1.1 x mu_s*m*g (tVoxelyze.h:1034-1112)."""
+    vs = 0.001
+    sim = make(vs)
+    sim.enableFloor(True)
+    sim.setGravity()
+    m = sim.addMaterial(1e6, 1e3)
+    nf = float(np.float32(1e3 * vs * vs * vs * 9.80665))
+    v = sim.setVoxel(m, 0, 0, 0)
+    m.setStaticFriction(1.0)
+    m.setKineticFriction(0.1)
+    m.setGlobalDamping(1.0)
+    ts = sim.recommendedTimeStep()
+    xs = []
+    for grav, mus, fac in [(1.0, 1.0, 0.9), (1.0, 1.0, 1.1), (2.0, 1.0, 1.9), (2.0, 1.0, 2.1), (1.0, 2.0, 1.9),
+                           (1.0, 2.0, 2.1)]:
+        sim.setGravity(grav)
+        m.setStaticFriction(mus)
+        for _ in range(50):
+            sim.doTimeStep(ts)
+        v.external().setForce(float(np.float32(fac) * np.float32(nf)), 0.0, 0.0)
+        for _ in range(10):
+            sim.doTimeStep(ts)
+        xs.append(sim.state(v.index, 1)["pos"][0][0])
+        v.external().setForce(0.0, 0.0, 0.0)
+        sim.resetTime()
+    out = {"x": np.array(xs)}
+    out.update(snapshot(sim))
+    return out
+
+
+def kinetic_friction(make):
+    """Kinetic friction work-energy check (tVoxelyze.h:1116-1167)."""
+    vs = 0.001
+    sim = make(vs)
+    sim.enableFloor(True)
+    sim.setGravity()
+    m = sim.addMaterial(1e6, 1e3)
+    mass = 1e3 * vs * vs * vs
+    nf = np.float32(mass * 9.80665)
+    v = sim.setVoxel(m, 0, 0, 0)
+    m.setStaticFriction(1.0)
+    m.setKineticFriction(0.1)
+    m.setGlobalDamping(1.0)
+    ts = sim.recommendedTimeStep()
+    for _ in range(50):
+        sim.doTimeStep(ts)
+    m.setGlobalDamping(0.0001)
+    hf = np.float32(2.0) * nf
+    v.external().setForce(float(hf), 0.0, 0.0)
+    last, vel, energy = 0.0, 0.0, 0.0
+    xs = np.zeros(210)
+    for k in range(10):
+        sim.doTimeStep(ts)
+        cur = sim.state(v.index, 1)["pos"][0][0]
+        vel = (cur - last) / ts
+        fac = np.float32(0.5) if k == 9 else np.float32(1.0)
+        energy += float(fac * (hf - np.float32(m.kineticFriction()) * nf)) * (cur - last)
+        xs[k] = cur
+        last = cur
+    v.external().setForce(0.0, 0.0, 0.0)
+    for k in range(200):
+        sim.doTimeStep(ts)
+        xs[10 + k] = sim.state(v.index, 1)["pos"][0][0]
+    out = {"x": xs, "energy": np.float64(energy), "ke": np.float64(0.5 * mass * vel * vel)}
+    out.update(snapshot(sim))
+    return out
+
+
+def collisions(make, steps=150):
+    """Dropped voxel rests on a fixed one via contact (tVoxelyze.h:1169-1197)."""
+    vs = 0.001
+    sim = make(vs)
+    sim.enableFloor(True)
+    sim.setGravity()
+    m = sim.addMaterial(1e6, 1e6)
+    m.setGlobalDamping(0.0)
+    sim.setVoxel(m, 0, 0, 0).external().setFixedAll()
+    v2 = sim.setVoxel(m, 0, 0, 2)
+    sim.enableCollisions()
+    ts = sim.recommendedTimeStep()
+    trace = np.zeros(steps)
+    for k in range(steps):
+        sim.doTimeStep(ts)
+        trace[k] = sim.state(v2.index, 1)["pos"][0][2]
+    out = {"trace": trace, "ncol": np.int32(sim.collisionCount())}
+    out.update(snapshot(sim))
+    return out
+
+
+# ------------------------------------------------------------------ benchmark configs, scaled
+def config1(make, checkpoints=(1, 2, 3, 1000, 20000)):
+    """The 320-voxel plain-FP64 cantilever, state at several step counts."""
+    sim = make(0.005)
+    W.cantilever(sim)
+    dt = sim.recommendedTimeStep()
+    out = {"dt": np.float32(dt)}
+    done = 0
+    for c in checkpoints:
+        n, _ = sim.step(dt, c - done)
+        done = c
+        out.update(snapshot(sim, f"s{c}_"))
+    return out
+
+
+def config2_small(make, nx=16, ny=16, nz=8, checkpoints=(1, 2, 3, 5000), seed=2):
+    """The falling Bernoulli lattice at 16x16x8 sites with floor friction."""
+    sim = make(0.005)
+    info = W.drop_lattice(sim, nx, ny, nz, seed=seed)
+    dt = sim.recommendedTimeStep()
+    out = {"dt": np.float32(dt), "nvox": np.int32(info["n_voxels"])}
+    done = 0
+    for c in checkpoints:
+        sim.step(dt, c - done)
+        done = c
+        out.update(snapshot(sim, f"s{c}_"))
+    return out
+
+
+def config3_small(make, n=12, steps=2000, every=2000, seed=5):
+    """Porous bilinear block with self-collision and a prescribed top face (12^3 sites).
+    The top face is driven at 0.1 voxel pitch per 100 steps so that the run yields."""
+    vs = 0.005
+    sim = make(vs)
+    info = W.porous_block(sim, n=n, seed=seed, porosity=0.2, rmin=1, rmax=3)
+    dt = sim.recommendedTimeStep()
+    out = {"dt": np.float32(dt), "nvox": np.int32(info["n_voxels"])}
+    for k in range(steps):
+        W.porous_block_drive(sim, info["top"], k, vs, rate=0.1 / 100)
+        sim.doTimeStep(dt)
+        if (k + 1) % every == 0 or k < 3:
+            out.update(snapshot(sim, f"s{k + 1}_"))
+            out[f"s{k + 1}_ncol"] = np.int32(sim.collisionCount())
+    return out
+
+
+def config4_robot(make, nvox=200, steps=2000, every=2000, seed=11):
+    """One accreted robot of four materials, sinusoidal temperature actuation on the
+    host through setTemperature every step (2 Hz, +-39 C), gravity and floor."""
+    vs = 0.005
+    sim = make(vs)
+    sim.setGravity(1.0)
+    sim.enableFloor(True)
+    mats = []
+    for E, cte in [(1e5, 0.0), (1e7, 0.0), (1e6, 0.01), (1e6, -0.01)]:
+        m = sim.addMaterial(E, 1000.0)
+        m.setCte(cte)
+        m.setStaticFriction(1.0)
+        m.setKineticFriction(0.5)
+        mats.append(m)
+    pts = W.grow_robot(seed, nvox, box=20)
+    pick = (W.uniform(seed + 1000, np.arange(len(pts), dtype=np.uint64)) * 4).astype(int)
+    zmin = pts[:, 2].min()
+    for (x, y, z), mi in zip(pts, pick):
+        sim.setVoxel(mats[mi], int(x), int(y), int(z - zmin))
+    dt = sim.recommendedTimeStep()
+    out = {"dt": np.float32(dt)}
+    t = np.float32(0.0)
+    nv = sim.voxelCount()
+    for k in range(steps):
+        temp = float(np.float32(39.0) * np.float32(np.sin(np.float32(4.0 * np.pi) * t)))
+        for i in range(nv):
+            sim.voxel(i).setTemperature(temp)
+        sim.doTimeStep(dt)
+        t = np.float32(t + np.float32(dt))
+        if (k + 1) % every == 0 or k < 3:
+            out.update(snapshot(sim, f"s{k + 1}_"))
+    return out
+
+
+def divergence(make):
+    """A step far above the stable time step must report divergence (Voxelyze.cpp:265-269)."""
+    sim = make(0.001)
+    m = sim.addMaterial(1e6, 1e3)
+    sim.setVoxel(m, 0, 0, 0).external().setFixedAll()
+    v = sim.setVoxel(m, 1, 0, 0)
+    v.external().setForce(10.0, 0, 0)
+    ts = sim.recommendedTimeStep()
+    res = []
+    for _ in range(40):
+        res.append(sim.doTimeStep(ts * 40))
+    out = {"ok": np.array(res, dtype=np.int32), "time": np.float32(sim.currentTime())}
+    out.update(snapshot(sim))
+    return out
+
+
+def failure(make, steps=300):
+    """A link whose failure stress is exceeded carries no force from then on (VX_Link.cpp:163)."""
+    sim = make(0.001)
+    m = sim.addMaterial(1e6, 1e3)
+    m.setModelLinear(1e6, 1e4)
+    sim.setVoxel(m, 0, 0, 0).external().setFixedAll()
+    v = sim.setVoxel(m, 1, 0, 0)
+    v.external().setForce(0.05, 0, 0)
+    ts = sim.recommendedTimeStep()
+    trace = np.zeros(steps)
+    for k in range(steps):
+        sim.doTimeStep(ts)
+        trace[k] = sim.state(v.index, 1)["pos"][0][0]
+    out = {"trace": trace}
+    out.update(snapshot(sim))
+    return out
+
+
+def material_curves(make):
+    """stress()/modulus() tables of the reference's test curves (tVX_Material.h:128-274) and
+    the combined link materials (tVX_MaterialLink.h:3-70)."""
+    sim = make(0.001)
+    lin = sim.addMaterial(1e6, 1e3)
+    bil = sim.addMaterial(1e6, 1e3)
+    bil.setModelBilinear(1e6, 5e5, 1e5, 2e5)
+    c3 = sim.addMaterial(1e6, 1e3)
+    c3.setModelBilinear(5e6, 5e5, 2.5e5, 3.75e5)
+    dat = sim.addMaterial(1e6, 1e3)
+    dat.setModel([0.01, 0.02, 0.03, 0.05], [1e4, 1.5e4, 1.8e4, 2e4])
+    strains = np.linspace(-0.1, 0.6, 281).astype(np.float32)
+    out = {"strains": strains}
+    for name, m in [("lin", lin), ("bil", bil), ("c3", c3), ("dat", dat)]:
+        out[name + "_stress"] = np.array([m.stress(float(e)) for e in strains], np.float32)
+        out[name + "_modulus"] = np.array([m.modulus(float(e)) for e in strains], np.float32)
+        st, ss = m.modelData()
+        out[name + "_data_strain"], out[name + "_data_stress"] = st, ss
+        out[name + "_E"] = np.float32(m.youngsModulus())
+        out[name + "_yield"] = np.float32(m.yieldStress())
+        out[name + "_fail"] = np.float32(m.failureStress())
+    # links between every pair drive combinedMaterial (Voxelyze.cpp:626-640)
+    for i, m in enumerate([lin, bil, c3, dat]):
+        sim.setVoxel(m, 2 * i, 0, 0)
+        sim.setVoxel(m, 2 * i + 1, 0, 0)
+    sim.setVoxel(dat, 8, 0, 0)
+    ts = sim.recommendedTimeStep()
+    out["ts"] = np.float32(ts)
+    return out
+
+
+SCENARIOS = {
+    "two_voxel": None,  # expanded per case below
+    "internal_damping": lambda mk: surrounded(mk, 1.0, 0.0),
+    "global_damping": lambda mk: surrounded(mk, 0.0, 1.0),
+    "cantilever2_global": cantilever2_global,
+    "combined_damping": combined_damping,
+    "scale": scale_forces,
+    "frequency": frequency,
+    "large_deformation_damping": large_deformation_damping,
+    "large_deformation": large_deformation,
+    "double_bond_cantilever": double_bond_cantilever,
+    "impulse": impulse,
+    "multi_simple": multi_simple,
+    "multi_simple2": multi_simple2,
+    "deformable_material": deformable_material,
+    "replace_material": replace_material,
+    "temperature": temperature,
+    "static_friction": static_friction,
+    "kinetic_friction": kinetic_friction,
+    "collisions": collisions,
+    "divergence": divergence,
+    "failure": failure,
+    "material_curves": material_curves,
+    "config1": config1,
+    "config2_small": config2_small,
+    "config3_small": config3_small,
+    "config4_robot": config4_robot,
+}
+del SCENARIOS["two_voxel"]
+for _i, _c in enumerate(TWO_VOXEL_CASES):
+    SCENARIOS[f"two_voxel_{_i:02d}"] = (lambda c: (lambda mk: two_voxel(mk, c[0], c[1], c[2], c[3], c[4], 300, c[6])))(_c)

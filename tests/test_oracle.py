@@ -1,0 +1,248 @@
+"""Pins the CPU oracle (oracle/vx_oracle.c) to the reference.
+
+1. Bit-exact against every golden vector recorded from the compiled reference
+   (tests/golden/*.npz, written by tests/golden/make_golden.py).
+2. The closed forms and recorded values the reference's own tests assert
+   (test/tVoxelyze.h), evaluated on the oracle's output.
+3. Where the compiled reference is present (this build container), seeded random
+   small models are stepped through both and compared bit for bit.
+"""
+import numpy as np
+import pytest
+
+import backends as B
+import scenarios as S
+
+NAMES = sorted(S.SCENARIOS)
+
+
+def bits_equal(a, b):
+    a, b = np.atleast_1d(np.asarray(a)), np.atleast_1d(np.asarray(b))
+    if a.shape != b.shape or a.dtype != b.dtype:
+        return False
+    return a.tobytes() == b.tobytes()
+
+
+@pytest.fixture(scope="module")
+def oracle_runs():
+    return {}
+
+
+def run_oracle(cache, name):
+    if name not in cache:
+        cache[name] = S.SCENARIOS[name](B.maker(B.oracle_abi()))
+    return cache[name]
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_oracle_matches_golden_bitwise(name, oracle_runs):
+    gold = B.load_golden(name)
+    out = run_oracle(oracle_runs, name)
+    assert set(gold) == set(out)
+    bad = [k for k in gold if not bits_equal(gold[k], out[k])]
+    assert not bad, f"{name}: differs from the reference in {bad[:8]}"
+
+
+# --------------------------------------------------------- closed forms of tVoxelyze.h
+def float_eq(a, b, ulps=4):
+    """EXPECT_FLOAT_EQ: within 4 ulps in float."""
+    a, b = np.float32(a), np.float32(b)
+    return abs(int(a.view(np.int32)) - int(b.view(np.int32))) <= ulps
+
+
+@pytest.mark.parametrize("case", range(len(S.TWO_VOXEL_CASES)))
+def test_single_bond_converges_to_beam_theory(case, oracle_runs):
+    """Axial F/(EA/L)=1e-6, bending 6e-3 rad, torsion 1.2e-5 rad, ... (tVoxelyze.h:142-244)."""
+    c = S.TWO_VOXEL_CASES[case]
+    out = run_oracle(oracle_runs, f"two_voxel_{case:02d}")
+    k = S.converge_steps(out["trace"], c[5])
+    assert k < 300, f"case {case} did not converge to {c[5]}: last {out['trace'][-1]}"
+
+
+def test_internal_and_global_damping(oracle_runs):
+    """Centre voxel settles at F/(6k) (tVoxelyze.h:260-302)."""
+    i = run_oracle(oracle_runs, "internal_damping")
+    assert float_eq(i["trace"][-1][1], np.float32(1e-9) / np.float32(6))
+    g = run_oracle(oracle_runs, "global_damping")
+    for a in range(3):
+        assert float_eq(g["trace"][-1][a], np.float32(1e-9) / np.float32(6))
+    c = run_oracle(oracle_runs, "cantilever2_global")
+    assert float_eq(c["trace"][-1], 4e-9)
+
+
+def test_combined_damping_block(oracle_runs):
+    out = run_oracle(oracle_runs, "combined_damping")
+    for i in range(2):
+        assert abs(out[f"trace{i}"][-1] - 1.742e-8) < 1e-10
+
+
+def test_force_scaling(oracle_runs):
+    out = run_oracle(oracle_runs, "scale")
+    for i in range(3, 8):
+        result = 4 / 10.0 ** (i + 3)
+        assert abs(out[f"trace{i}"][-1] - result) < result / 1000
+
+
+def test_axial_natural_frequency(oracle_runs):
+    """Period 2*pi/sqrt(k/m) within 0.1% (tVoxelyze.h:415-443)."""
+    out = run_oracle(oracle_runs, "frequency")
+    data, ts = out["trace"], float(out["ts"])
+    zc = []
+    for i in range(1, len(data)):
+        if (data[i - 1] <= 0 < data[i]) or (data[i - 1] >= 0 > data[i]):
+            zc.append(ts * (i - 1 + data[i - 1] / (data[i - 1] - data[i])))
+    period = np.mean(np.diff(zc)) * 2
+    expected = 2 * 3.1415926 / np.sqrt(1e9)
+    assert abs(period - expected) < expected / 1000
+
+
+def test_large_deformation(oracle_runs):
+    out = run_oracle(oracle_runs, "large_deformation_damping")
+    assert abs(out["trace"][-1] - 9.5587e-4) < 1e-7
+    assert out["small"][0] == 1 and out["small"].min() == 0  # the large-angle branch ran
+    ramp = run_oracle(oracle_runs, "large_deformation")
+    assert ramp["small"][0] == 1 and ramp["small"][20] == 0  # tVoxelyze.h:502-503
+    z = ramp["trace"].astype(np.float32)
+    ts = np.float32(ramp["ts"])
+    d1 = np.diff(z) / ts
+    d2 = np.diff(d1) / ts
+    assert np.abs(d2).max() < 5000
+
+
+def test_series_springs(oracle_runs):
+    """3 links in series 1.5015e-6; mixed layers 3.5035e-6 (tVoxelyze.h:612-684)."""
+    assert float_eq(run_oracle(oracle_runs, "multi_simple")["trace"][-1], 1.5015e-6)
+    assert abs(np.float32(run_oracle(oracle_runs, "multi_simple2")["trace"][-1]) - 3.5035e-6) < 1e-10
+
+
+def test_double_bond_cantilever(oracle_runs):
+    """Axial 1e-8, bending FL^3/3EI = 1.6e-7 (tVoxelyze.h:525-575)."""
+    out = run_oracle(oracle_runs, "double_bond_cantilever")
+    for i in range(3):
+        for j in range(2):
+            sign = 1.0 if j == 0 else -1.0
+            exp = sign * (1e-8 if i == 0 else 1.6e-7)
+            tr = out[f"trace{i}{j}"].astype(np.float32)
+            assert abs(tr[-1] - exp) < abs(exp) * 1e-3
+
+
+def test_impulse_recovers(oracle_runs):
+    assert abs(run_oracle(oracle_runs, "impulse")["trace"][-1]) < 1e-5
+
+
+def test_bilinear_plastic_set(oracle_runs):
+    assert abs(np.float32(run_oracle(oracle_runs, "deformable_material")["trace"][-1]) - 4e-4) < 1e-7
+
+
+def test_replace_material(oracle_runs):
+    assert abs(run_oracle(oracle_runs, "replace_material")["trace"][-1] - 3.8167e-8) < 1e-10
+
+
+def test_thermal_bimorph(oracle_runs):
+    out = run_oracle(oracle_runs, "temperature")
+    assert abs(out["trace1"][-1] - 2.55e-5) < 1e-8
+    assert abs(out["trace2"][-1] - -2.591e-5) < 1e-8
+
+
+def test_static_friction(oracle_runs):
+    x = run_oracle(oracle_runs, "static_friction")["x"]
+    assert x[0] == 0.0 and x[1] != 0.0 and x[2] == 0.0 and x[3] != 0.0 and x[4] == 0.0 and x[5] != 0.0
+
+
+def test_kinetic_friction_work_energy(oracle_runs):
+    out = run_oracle(oracle_runs, "kinetic_friction")
+    assert abs(out["energy"] - out["ke"]) < 2e-16
+
+
+def test_collision_keeps_voxel_above(oracle_runs):
+    out = run_oracle(oracle_runs, "collisions")
+    assert out["trace"][-1] > 0.001
+    assert out["ncol"] == 1
+
+
+def test_divergence_and_failure(oracle_runs):
+    d = run_oracle(oracle_runs, "divergence")
+    assert d["ok"][0] == 1 and d["ok"].min() == 0
+    f = run_oracle(oracle_runs, "failure")
+    assert f["link_flags"][0] & 8  # failed
+    assert np.all(f["link_forces"][0] == 0)
+
+
+def test_material_curves(oracle_runs):
+    """stress()/modulus() known answers (tVX_Material.h:128-182, 265-274) and the link-material
+    harmonic mean (tVX_MaterialLink.h:3-70)."""
+    out = run_oracle(oracle_runs, "material_curves")
+    e = out["strains"]
+    lin = out["lin_stress"]
+    assert np.all(lin[e <= 1.0] == np.float32(1e6) * e[e <= 1.0])
+    bil = out["bil_stress"]
+    i = np.argmin(np.abs(e - np.float32(0.15)))
+    assert abs(bil[i] - (1e5 + 5e5 * (e[i] - 0.1))) < 1
+    assert np.all(bil[e > out["bil_data_strain"][-1]] == 0)  # failed beyond the last point
+    assert out["bil_modulus"][np.argmin(np.abs(e - 0.05))] == np.float32(1e6)
+    assert out["c3_yield"] == np.float32(2.5e5) and out["c3_fail"] == np.float32(3.75e5)
+    assert abs(out["c3_data_strain"][-1] - 0.3) < 1e-6  # eps_f = 0.05 + (3.75e5-2.5e5)/5e5
+
+
+# --------------------------------------------------------- brute force against the reference
+def random_model(make, seed):
+    rng = np.random.default_rng(seed)
+    sim = make(0.001 * (1 + rng.integers(0, 5)))
+    mats = []
+    for _ in range(int(rng.integers(1, 4))):
+        m = sim.addMaterial(float(10 ** rng.uniform(5, 7)), float(rng.uniform(500, 2000)))
+        kind = rng.integers(0, 3)
+        if kind == 1:
+            E = m.youngsModulus()
+            m.setModelBilinear(E, E * float(rng.uniform(0.1, 0.9)), E * 0.01, E * 0.03)
+        elif kind == 2:
+            m.setModelLinear(m.youngsModulus(), m.youngsModulus() * 0.02)
+        m.setInternalDamping(float(rng.uniform(0, 1)))
+        m.setGlobalDamping(float(rng.uniform(0, 0.1)))
+        m.setStaticFriction(float(rng.uniform(0, 1)))
+        m.setKineticFriction(float(rng.uniform(0, 0.5)))
+        m.setCollisionDamping(float(rng.uniform(0, 1)))
+        m.setCte(float(rng.uniform(-0.01, 0.01)))
+        mats.append(m)
+    n = int(rng.integers(2, 5))
+    occ = rng.random((n, n, n)) < 0.7
+    for z in range(n):
+        for y in range(n):
+            for x in range(n):
+                if occ[x, y, z]:
+                    v = sim.setVoxel(mats[int(rng.integers(0, len(mats)))], x, y, z + 1)
+                    r = rng.random()
+                    if r < 0.1:
+                        v.external().setFixedAll()
+                    elif r < 0.2:
+                        v.external().setFixed(*[bool(b) for b in rng.integers(0, 2, 6)])
+                    elif r < 0.4:
+                        v.external().setForce(*[float(f) for f in rng.normal(0, 0.02, 3)])
+                        v.external().setMoment(*[float(f) for f in rng.normal(0, 1e-6, 3)])
+    if sim.voxelCount() == 0:
+        sim.setVoxel(mats[0], 0, 0, 0)
+    sim.setGravity(float(rng.uniform(0, 2)))
+    sim.enableFloor(bool(rng.integers(0, 2)))
+    if rng.integers(0, 2):
+        sim.enableCollisions(True)
+    if rng.integers(0, 2):
+        sim.setAmbientTemperature(float(rng.uniform(-20, 20)), True)
+    return sim
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_oracle_vs_reference_random_models(seed):
+    ref = B.ref_abi()
+    if ref is None:
+        pytest.skip("compiled reference not present (built only where the reference sources are)")
+    a = random_model(B.maker(B.oracle_abi()), seed)
+    b = random_model(B.maker(ref), seed)
+    dt = a.recommendedTimeStep()
+    assert bits_equal(np.float32(dt), np.float32(b.recommendedTimeStep()))
+    for k in range(6):
+        na, _ = a.step(dt, 50)
+        nb, _ = b.step(dt, 50)
+        assert na == nb
+        sa, sb = S.snapshot(a), S.snapshot(b)
+        bad = [key for key in sa if not bits_equal(sa[key], sb[key])]
+        assert not bad, f"seed {seed} block {k}: {bad}"

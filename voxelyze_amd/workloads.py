@@ -1,0 +1,152 @@
+"""Synthetic workloads of the explicit-dynamics benchmark, built through the
+CVoxelyze-style API so that the same builder feeds the HIP library and the
+parity checkers.
+
+Randomness is project-defined and portable: SplitMix64 (Steele, Lea, Flood,
+"Fast splittable pseudorandom number generators", OOPSLA 2014) evaluated in
+counter form, u_k = mix(seed + (k+1)*0x9E3779B97F4A7C15) >> 11 scaled by 2^-53.
+std:: distributions are never used because their output differs between
+standard libraries.
+"""
+import numpy as np
+
+_GAMMA = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def splitmix_u64(seed, k):
+    """k-th SplitMix64 output of a stream (k may be an array)."""
+    k = np.asarray(k, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + (k + np.uint64(1)) * _GAMMA
+        z = (z ^ (z >> np.uint64(30))) * _M1
+        z = (z ^ (z >> np.uint64(27))) * _M2
+        return z ^ (z >> np.uint64(31))
+
+
+def uniform(seed, k):
+    """53-bit uniforms in [0, 1)."""
+    return (splitmix_u64(seed, k) >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
+
+
+# ----------------------------------------------------------------- config 1
+def cantilever(sim, nx=20, ny=4, nz=4, E=1e6, rho=1000.0, zeta=1.0, total_force=-1.0):
+    """Fixed-free cantilever: x=0 face setFixedAll, total_force in z spread over
+    the x=nx-1 face.  With the defaults and a 5 mm pitch this is the 320-voxel
+    plain-FP64 cantilever."""
+    m = sim.addMaterial(E, rho)
+    m.setInternalDamping(zeta)
+    per = total_force / (ny * nz)
+    tips = []
+    for z in range(nz):
+        for y in range(ny):
+            for x in range(nx):
+                v = sim.setVoxel(m, x, y, z)
+                if x == 0:
+                    v.external().setFixedAll()
+                if x == nx - 1:
+                    v.external().setForce(0.0, 0.0, per)
+                    tips.append(v.index)
+    return {"tips": np.array(tips)}
+
+
+# ----------------------------------------------------------------- config 2
+def drop_lattice(sim, nx=128, ny=128, nz=64, seed=2, p_occ=0.9, p_stiff=0.3, lift=2,
+                 soft=(1e6, 1000.0), stiff=(1e7, 1500.0), mu_s=1.0, mu_k=0.5, gravity=1.0, floor=True):
+    """Bernoulli(p_occ) lattice of two linear materials (stiff with p_stiff),
+    placed `lift` voxels above the floor, under gravity with floor friction.
+    Sites are visited x-fastest; site s draws u_occ = U(2s), u_mat = U(2s+1)."""
+    ma = sim.addMaterial(*soft)
+    mb = sim.addMaterial(*stiff)
+    for m in (ma, mb):
+        m.setStaticFriction(mu_s)
+        m.setKineticFriction(mu_k)
+    sim.setGravity(gravity)
+    sim.enableFloor(floor)
+    n = nx * ny * nz
+    s = np.arange(n, dtype=np.uint64)
+    u_occ = uniform(seed, 2 * s)
+    u_mat = uniform(seed, 2 * s + np.uint64(1))
+    occ = np.nonzero(u_occ < p_occ)[0]
+    stiff_site = u_mat[occ] < p_stiff
+    ix = (occ % nx).astype(np.int64)
+    iy = ((occ // nx) % ny).astype(np.int64)
+    iz = (occ // (nx * ny)).astype(np.int64) + lift
+    mats = [ma, mb]
+    for i in range(len(occ)):
+        sim.setVoxel(mats[int(stiff_site[i])], int(ix[i]), int(iy[i]), int(iz[i]))
+    return {"n_sites": n, "n_voxels": int(len(occ)), "stiff": stiff_site}
+
+
+# ----------------------------------------------------------------- config 3
+def porous_block(sim, n=64, seed=3, porosity=0.2, rmin=2, rmax=6, E=5e6, plastic=5e5, yield_stress=2.5e5,
+                 fail_stress=3.75e5, rho=1000.0, collisions=True):
+    """n^3 block of a bilinear material with random spherical voids added until the
+    void fraction reaches `porosity`; bottom face fixed, top face driven by
+    setDisplacementAll each step (see porous_block_drive)."""
+    m = sim.addMaterial(E, rho)
+    m.setModelBilinear(E, plastic, yield_stress, fail_stress)
+    solid = np.ones((n, n, n), dtype=bool)
+    g = np.arange(n)
+    k = 0
+    while 1.0 - solid.mean() < porosity:
+        c = uniform(seed, np.arange(4 * k, 4 * k + 3, dtype=np.uint64)) * n
+        r = rmin + int(uniform(seed, np.uint64(4 * k + 3)) * (rmax - rmin + 1))
+        d2 = (g[:, None, None] - c[0]) ** 2 + (g[None, :, None] - c[1]) ** 2 + (g[None, None, :] - c[2]) ** 2
+        solid &= d2 > r * r
+        k += 1
+    top, bottom = [], []
+    for z in range(n):
+        for y in range(n):
+            for x in range(n):
+                if solid[x, y, z]:
+                    v = sim.setVoxel(m, x, y, z)
+                    if z == 0:
+                        v.external().setFixedAll()
+                        bottom.append(v.index)
+                    elif z == n - 1:
+                        top.append(v.index)
+    if collisions:
+        sim.enableCollisions(True)
+    return {"top": np.array(top, dtype=np.int64), "bottom": np.array(bottom, dtype=np.int64),
+            "n_voxels": int(solid.sum())}
+
+
+def porous_block_drive(sim, top, step, voxel_size, rate=0.1 / 1000):
+    """Prescribed top-face displacement: -rate*pitch*step in z."""
+    d = -rate * voxel_size * step
+    for i in top:
+        sim.voxel(int(i)).external().setDisplacementAll((0.0, 0.0, d))
+
+
+# ----------------------------------------------------------------- config 4
+def grow_robot(seed, nvox, box=20):
+    """Seeded face-neighbour accretion from the box centre to nvox voxels.
+    Returns integer (x, y, z) rows in insertion order."""
+    c = box // 2
+    occ = {(c, c, c)}
+    order = [(c, c, c)]
+    frontier = []
+    dirs = ((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1))
+
+    def push(p):
+        for d in dirs:
+            q = (p[0] + d[0], p[1] + d[1], p[2] + d[2])
+            if all(0 <= q[i] < box for i in range(3)) and q not in occ:
+                frontier.append(q)
+
+    push((c, c, c))
+    k = 0
+    while len(order) < nvox and frontier:
+        j = int(uniform(seed, np.uint64(k)) * len(frontier))
+        k += 1
+        q = frontier[j]
+        frontier[j] = frontier[-1]
+        frontier.pop()
+        if q in occ:
+            continue
+        occ.add(q)
+        order.append(q)
+        push(q)
+    return np.array(order, dtype=np.int64)

@@ -1,0 +1,157 @@
+"""Throughput of the explicit step (CVoxelyze::doTimeStep) on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|mixed]
+                    [--workload lattice|cantilever] [--no-cpu-baseline]
+
+Default workload: the 128x128x64-site Bernoulli(0.9) lattice of two linear
+materials (E 1 MPa / 10 MPa) dropped under gravity onto a friction floor,
+~943k voxels and ~2.52M links, 5 mm pitch, dt = recommendedTimeStep().  A step
+is one doTimeStep over the whole lattice; state stays resident in HBM.
+
+Prints one JSON line: voxel-steps/s, the roofline of the dominant kernel
+(algorithmic bytes per launch / HIP-event-timed launch duration, against the
+8 TB/s HBM peak) and, on N=1, the CPU oracle timed on this host's cores on a
+bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "voxel-steps/s at 1M voxels (FP64 plain vs mixed), %HBM peak, adjoint/fwd"
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes (per element per step) of each kernel, FP64 plain:
+#   link kernel : 288 B per link  (endpoints 8, meta+flags 8, local pose 72 R + 72 W, rest length 8,
+#                                  strain state 8 R + 16 W, end forces 96 W) + 56 B per voxel (pose gather)
+#   voxel kernel: 244 B per voxel (state 104 R + 104 W, neighbour table 24, metadata 12) + 96 B per link
+#                                  (end-force gather)
+ALG_BYTES = {
+    "link": lambda V, L: 288.0 * L + 56.0 * V,
+    "voxel": lambda V, L: 244.0 * V + 96.0 * L,
+}
+
+
+def build_workload(sim, workload):
+    from voxelyze_amd import workloads as W
+    if workload == "lattice":
+        info = W.drop_lattice(sim, 128, 128, 64, seed=2)
+        return {"workload": "drop lattice 128x128x64 sites p=0.9, 2 linear materials, gravity+floor friction",
+                "sites": info["n_sites"]}
+    if workload == "cantilever":
+        W.cantilever(sim)
+        return {"workload": "cantilever 20x4x4, E=1MPa, fixed face + tip load"}
+    raise SystemExit(f"unknown workload {workload}")
+
+
+def traffic_from_profiles(kernel, workload):
+    p = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload}.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
+
+
+def cpu_baseline(workload, budget_s=15.0, max_steps=200):
+    """The CPU oracle (oracle/vx_oracle.c, OpenMP over the link and voxel loops like the
+    reference's USE_OMP build) on the same workload, timed on this host."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from voxelyze_amd._abi import bind
+    from voxelyze_amd.api import CVoxelyze
+    lib = os.path.join(ROOT, "oracle", "liboracle.so")
+    if not os.path.exists(lib):
+        return None
+    abi = bind(lib, "vxo_", "oracle")
+    sim = CVoxelyze(0.005, abi)
+    build_workload(sim, workload)
+    dt = sim.recommendedTimeStep()
+    sim.step(dt, 2)
+    n, t0 = 0, time.perf_counter()
+    while n < max_steps and time.perf_counter() - t0 < budget_s:
+        sim.step(dt, 5)
+        n += 5
+    el = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    V = sim.voxelCount()
+    return {"value": V * n / el, "unit": "voxel-steps/s", "cores": threads, "kind": "port",
+            "sample": f"oracle/vx_oracle.c with OpenMP, {n} steps after 2 warm-up steps of the same workload "
+                      f"({V} voxels), {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--workload", default="lattice", choices=["lattice", "cantilever"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build measures one GPU only; --gpus must be 1 "
+              "(multi-GPU sharding is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from voxelyze_amd.api import CVoxelyze, FP64_PLAIN, MIXED
+
+    t_build = time.perf_counter()
+    sim = CVoxelyze(0.005)
+    cfg = build_workload(sim, a.workload)
+    sim.setPrecision(MIXED if a.precision == "mixed" else FP64_PLAIN)
+    V, L = sim.voxelCount(), sim.linkCount()
+    dt = sim.recommendedTimeStep()
+    t_build = time.perf_counter() - t_build
+    print(f"[bench] built {V} voxels, {L} links in {t_build:.1f}s, dt={dt:.7g}", file=sys.stderr, flush=True)
+
+    sim.step(dt, a.warmup)
+    torch.cuda.synchronize()
+    sim.abi.profile(sim.handle, 1)
+    sim.abi.profile_read(sim.handle, None, None, 0)
+    t0 = time.perf_counter()
+    done, diverged = sim.step(dt, a.steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    sim.abi.profile(sim.handle, 0)
+    import ctypes as C
+    ms = (C.c_double * 3)()
+    cnt = (C.c_int * 3)()
+    sim.abi.profile_read(sim.handle, ms, cnt, 3)
+    if done != a.steps:
+        print(f"[bench] diverged at step {diverged}", file=sys.stderr)
+        sys.exit(1)
+
+    kernels = {}
+    for i, name in enumerate(["link", "voxel"]):
+        if cnt[i]:
+            avg_s = ms[i] / cnt[i] / 1e3
+            b = ALG_BYTES[name](V, L)
+            kernels[name] = {"avg_us": avg_s * 1e6, "alg_bytes": b, "gbs": b / avg_s / 1e9}
+    dom = max(kernels, key=lambda k: kernels[k]["avg_us"])
+    d = kernels[dom]
+    traffic = traffic_from_profiles(dom, a.workload)
+    roof = {"bound": "hbm", "achieved": round(d["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(d["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": f"k_{dom}s_fp64",
+            "kernels": {k: {"avg_us": round(v["avg_us"], 2), "GB/s": round(v["gbs"], 1)} for k, v in kernels.items()}}
+    value = V * a.steps / el
+    cfg.update({"voxels": V, "links": L, "precision": "fp64_plain" if a.precision == "fp64" else "mixed",
+                "dt": dt, "voxel_pitch_m": 0.005})
+    line = {
+        "metric": METRIC, "value": value, "unit": "voxel-steps/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": el / a.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64" if a.precision == "fp64" else "f32/f64",
+        "data": "synthetic (seeded SplitMix64 lattice)", "config": cfg, "roofline": roof,
+    }
+    if not a.no_cpu_baseline:
+        print("[bench] timing the CPU oracle on a bounded sample", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline(a.workload)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -129,6 +129,14 @@ int vxg_collision_count(vxg_sim* s);
    mixed exactly as VXG_FP64_PLAIN reproduces) */
 int vxg_set_precision(vxg_sim* s, int mode);
 
+/* Kernel timing (no reference counterpart; measurement hook for benchmarks).
+   When enabled, every kernel of later vxg_step calls is bracketed by HIP events on the
+   library's stream; vxg_profile_read returns, per kernel class (0 = link kernel,
+   1 = voxel kernel, 2 = contact kernels), the summed milliseconds and launch counts
+   since the last read, then clears them.  Returns the number of classes written. */
+int vxg_profile(vxg_sim* s, int enable);
+int vxg_profile_read(vxg_sim* s, double* ms, int* launches, int cap);
+
 /* ---------------------------------------------------------------- materials */
 /* CVoxelyze::addMaterial(float E, float rho)        include/Voxelyze.h:86, src/Voxelyze.cpp:357-366
    Returns the material index (materialCount()-1) or -1. */

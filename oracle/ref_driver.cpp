@@ -99,6 +99,8 @@ EXPORT float vxr_state_info(vxr_sim* s, int info, int type) {
 }
 EXPORT int vxr_collision_count(vxr_sim* s) { return (int)s->sim.collisionList()->size(); }
 EXPORT int vxr_set_precision(vxr_sim*, int mode) { return mode == 0 ? 1 : 0; }
+EXPORT int vxr_profile(vxr_sim*, int) { return 0; }
+EXPORT int vxr_profile_read(vxr_sim*, double*, int*, int) { return 0; }
 
 EXPORT int vxr_material_add(vxr_sim* s, float E, float rho) {
     CVX_Material* m = s->sim.addMaterial(E, rho);

@@ -1425,6 +1425,7 @@ EXPORT void vxo_set_ambient_temperature(sim_t* s, float t, int all) {
 EXPORT float vxo_ambient_temperature(sim_t* s) { return s->ambientTemp; }
 
 EXPORT int vxo_do_time_step(sim_t* s, float dt) {
+    s->err[0] = 0;
     if (sim_hasPoisson(s)) { snprintf(s->err, sizeof(s->err), "Poisson's ratio != 0 is not restated by the oracle"); return 0; }
     return sim_doTimeStep(s, dt);
 }
@@ -1448,6 +1449,8 @@ EXPORT void vxo_reset_time(sim_t* s) {
 EXPORT float vxo_current_time(sim_t* s) { return s->currentTime; }
 EXPORT int vxo_collision_count(sim_t* s) { return s->ncols; }
 EXPORT int vxo_set_precision(sim_t* s, int mode) { (void)s; return mode == 0 ? 1 : 0; }
+EXPORT int vxo_profile(sim_t* s, int en) { (void)s; (void)en; return 0; }
+EXPORT int vxo_profile_read(sim_t* s, double* ms, int* n, int cap) { (void)s; (void)ms; (void)n; (void)cap; return 0; }
 
 /* CVoxelyze::stateInfo (Voxelyze.cpp:752-800) */
 EXPORT float vxo_state_info(sim_t* s, int info, int type) {

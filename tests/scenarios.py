@@ -18,6 +18,13 @@ OFF = {X_POS: (1, 0, 0), X_NEG: (-1, 0, 0), Y_POS: (0, 1, 0), Y_NEG: (0, -1, 0),
 
 def snapshot(sim, prefix=""):
     out = {}
+    n = sim.voxelCount()
+    ijk = np.zeros((n, 3), np.int32)
+    for i in range(n):
+        v = sim.voxel(i)
+        ijk[i] = (v.indexX(), v.indexY(), v.indexZ())
+    out[prefix + "vox_ijk"] = ijk
+    out[prefix + "voxel_size"] = np.float64(sim.voxelSize())
     st = sim.state()
     for k, v in st.items():
         out[prefix + "vox_" + k] = v

@@ -38,6 +38,8 @@ SIGNATURES = {
     "state_info": (_F, [_P, _I, _I]),
     "collision_count": (_I, [_P]),
     "set_precision": (_I, [_P, _I]),
+    "profile": (_I, [_P, _I]),
+    "profile_read": (_I, [_P, _DP, _IP, _I]),
     "material_add": (_I, [_P, _F, _F]),
     "material_count": (_I, [_P]),
     "material_set_model_linear": (_I, [_P, _I, _F, _F]),

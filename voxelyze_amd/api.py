@@ -406,13 +406,23 @@ class CVoxelyze:
 
     # stepping
     def doTimeStep(self, dt=-1.0):
-        return bool(self.abi.do_time_step(self.handle, dt))
+        ok = bool(self.abi.do_time_step(self.handle, dt))
+        if not ok:
+            self._raise_if_error()
+        return ok
 
     def step(self, dt, nsteps):
         """nsteps doTimeStep(dt) calls; returns (steps completed, diverged step or -1)."""
         at = C.c_int(-1)
         n = self.abi.step(self.handle, dt, nsteps, C.byref(at))
+        if n < nsteps and at.value < 0:
+            self._raise_if_error()
         return n, at.value
+
+    def _raise_if_error(self):
+        e = self.lastError()
+        if e:
+            raise RuntimeError(f"{self.abi.name}: {e}")
 
     def recommendedTimeStep(self): return self.abi.recommended_time_step(self.handle)
     def resetTime(self): self.abi.reset_time(self.handle)

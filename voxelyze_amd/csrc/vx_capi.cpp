@@ -1,0 +1,498 @@
+/*
+ * vx_capi.cpp -- the vxg C ABI (include/vxg.h) of the HIP library.
+ *
+ * Each entry point mirrors one member of the reference's object API (the
+ * declaration in include/vxg.h cites it).  Mutations that touch the dynamic
+ * state first pull the device state back (Sim::syncDown), so mid-run edits
+ * behave exactly like the reference's in-place edits.
+ */
+#include <float.h>
+#include <math.h>
+#include <string.h>
+
+#include <cstdio>
+#include <string>
+
+#include "../../include/vxg.h"
+#include "vx_model.h"
+
+#pragma clang fp contract(off)
+
+using vx::Sim;
+
+struct vxg_sim {
+    Sim s;
+};
+
+#define API extern "C" __attribute__((visibility("default")))
+
+static std::string g_err;
+
+static Sim* S(vxg_sim* h) { return &h->s; }
+static vx::Mat* M(Sim* s, int m) { return (m >= 0 && m < (int)s->vmats.size()) ? s->vmats[m].get() : nullptr; }
+static bool V(Sim* s, int v) { return v >= 0 && v < s->nv(); }
+
+API vxg_sim* vxg_create(double voxel_size) {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) {
+        g_err = "no HIP device: the voxelyze_amd step runs on an MI355X (gfx950) GPU";
+        return nullptr;
+    }
+    vxg_sim* h = new vxg_sim();
+    h->s.voxSize = voxel_size <= 0 ? 0.001 : voxel_size;
+    return h;
+}
+API void vxg_destroy(vxg_sim* h) { delete h; }
+API const char* vxg_last_error(vxg_sim* h) { return h ? h->s.err.c_str() : g_err.c_str(); }
+API double vxg_voxel_size(vxg_sim* h) { return S(h)->voxSize; }
+
+/* ------------------------------------------------------------ environment */
+API void vxg_set_gravity(vxg_sim* h, float g) {
+    Sim* s = S(h);
+    s->grav = g;
+    for (auto& m : s->vmats) m->gravMult = s->grav;
+    s->materialsChanged();
+}
+API float vxg_gravity(vxg_sim* h) { return S(h)->grav; }
+
+API void vxg_enable_floor(vxg_sim* h, int en) {
+    Sim* s = S(h);
+    s->syncDown();
+    s->floor = en != 0;
+    for (auto& f : s->vflags) f = en ? (f | vx::VF_FLOOR) : (f & ~vx::VF_FLOOR);
+    s->dirty |= Sim::D_VSTATE;
+}
+API int vxg_is_floor_enabled(vxg_sim* h) { return S(h)->floor ? 1 : 0; }
+
+API void vxg_enable_collisions(vxg_sim* h, int en) {
+    Sim* s = S(h);
+    if (s->collisions == (en != 0)) return;
+    s->syncDown();
+    s->collisions = en != 0;
+    for (auto& f : s->vflags) f = en ? (f | vx::VF_COLLISIONS) : (f & ~vx::VF_COLLISIONS);
+    s->dirty |= Sim::D_VSTATE;
+}
+API int vxg_is_collisions_enabled(vxg_sim* h) { return S(h)->collisions ? 1 : 0; }
+
+API void vxg_set_ambient_temperature(vxg_sim* h, float t, int all) {
+    Sim* s = S(h);
+    s->ambientTemp = t;
+    if (all)
+        for (int v = 0; v < s->nv(); v++) s->setTemperature(v, t);
+}
+API float vxg_ambient_temperature(vxg_sim* h) { return S(h)->ambientTemp; }
+
+/* ---------------------------------------------------------------- stepping */
+API int vxg_step(vxg_sim* h, float dt, int nsteps, int* diverged_at) {
+    int r = S(h)->step(dt, nsteps, diverged_at);
+    return r < 0 ? 0 : r;
+}
+API int vxg_do_time_step(vxg_sim* h, float dt) {
+    int at = -1;
+    return S(h)->step(dt, 1, &at) == 1 ? 1 : 0;
+}
+API float vxg_recommended_time_step(vxg_sim* h) { return S(h)->recommendedTimeStep(); }
+
+/* CVoxelyze::resetTime (src/Voxelyze.cpp:313-321) */
+API void vxg_reset_time(vxg_sim* h) {
+    Sim* s = S(h);
+    s->syncDown();
+    s->currentTime = 0.0f;
+    for (int v = 0; v < s->nv(); v++) s->resetVoxel(v);
+    for (int l = 0; l < s->nl(); l++) s->resetLink(l);
+    s->dirty |= Sim::D_VSTATE | Sim::D_LSTATE | Sim::D_TEMP | Sim::D_REST;
+}
+API float vxg_current_time(vxg_sim* h) { return S(h)->currentTime; }
+API int vxg_collision_count(vxg_sim* h) { (void)h; return 0; }
+API int vxg_set_precision(vxg_sim* h, int mode) {
+    Sim* s = S(h);
+    if (mode != VXG_FP64_PLAIN) {
+        s->err = "only VXG_FP64_PLAIN is available";
+        return 0;
+    }
+    s->precision = mode;
+    return 1;
+}
+
+/* CVoxelyze::stateInfo (src/Voxelyze.cpp:752-800), evaluated on the host mirror */
+API float vxg_state_info(vxg_sim* h, int info, int type) {
+    Sim* s = S(h);
+    s->syncDown();
+    float ret = 0;
+    if (type == 1) ret = -FLT_MAX;
+    else if (type == 0) ret = FLT_MAX;
+    auto acc = [&](float v) {
+        if (type == 0) { if (v < ret) ret = v; }
+        else if (type == 1) { if (v > ret) ret = v; }
+        else ret += v;
+    };
+    using namespace vx;
+    if (info == 7 || info == 5 || info == 6) {
+        if (s->nl() == 0) return 0.0f;
+        for (int l = 0; l < s->nl(); l++) {
+            float v;
+            if (info == 5) v = s->lf[LSTRESS][l];
+            else if (info == 6) v = s->lf[LSTRAIN][l];
+            else {
+                const Mat* m = s->lmat[l];
+                double fnx = s->ld[LFNX][l], mnx = s->ld[LMNX][l], mny = s->ld[LMNY][l], mnz = s->ld[LMNZ][l];
+                double mpy = s->ld[LMPY][l], mpz = s->ld[LMPZ][l];
+                v = (float)(fnx * fnx / (2.0f * m->a1) + mnx * mnx / (2.0 * m->a2) +
+                            (mnz * mnz - mnz * mpz + mpz * mpz) / (3.0 * m->b3) + (mny * mny - mny * mpy + mpy * mpy) / (3.0 * m->b3));
+            }
+            acc(v);
+        }
+        if (type == 3) ret /= s->nl();
+    } else {
+        if (s->nv() == 0) return 0.0f;
+        for (int i = 0; i < s->nv(); i++) {
+            const Mat* m = s->vmat[i];
+            double px = s->vd[VPX][i], py = s->vd[VPY][i], pz = s->vd[VPZ][i];
+            double lx = s->vd[VLX][i], ly = s->vd[VLY][i], lz = s->vd[VLZ][i];
+            double ax = s->vd[VAX][i], ay = s->vd[VAY][i], az = s->vd[VAZ][i];
+            float v = 0;
+            switch (info) {
+            case 0: {
+                double sz = m->nomSize;
+                double dx = px - s->vijk[i][0] * sz, dy = py - s->vijk[i][1] * sz, dz = pz - s->vijk[i][2] * sz;
+                v = (float)sqrt(dx * dx + dy * dy + dz * dz);
+                break;
+            }
+            case 1: v = (float)(sqrt(lx * lx + ly * ly + lz * lz) * m->massInverse); break;
+            case 2: v = (float)(0.5 * (m->massInverse * (lx * lx + ly * ly + lz * lz) + m->momentInertiaInverse * (ax * ax + ay * ay + az * az))); break;
+            case 3: { double w = s->vd[VQW][i]; v = (float)(2.0 * acos(w > 1 ? 1 : w)); break; }
+            case 4: v = (float)(sqrt(ax * ax + ay * ay + az * az) * m->momentInertiaInverse); break;
+            case 8: {
+                float st[3] = {0, 0, 0};
+                int nb[3] = {0, 0, 0};
+                for (int d = 0; d < 6; d++) {
+                    int l = s->vlinks[i][d];
+                    if (l < 0) continue;
+                    float ratio = s->vmat[s->lpos[l]]->E / s->vmat[s->lneg[l]]->E;
+                    float strain = s->lf[LSTRAIN][l];
+                    st[d / 2] += (d % 2 == 1) ? 2.0f * strain * ratio / (1.0f + ratio) : 2.0f * strain / (1.0f + ratio);
+                    nb[d / 2]++;
+                }
+                for (int a = 0; a < 3; a++) if (nb[a] == 2) st[a] *= 0.5f;
+                float vol = (float)(st[0] + st[1] + st[2]);
+                v = -m->E * vol / (3 * (1 - 2 * m->nu));
+                break;
+            }
+            case 9: v = m->mass; break;
+            default: v = 0;
+            }
+            acc(v);
+        }
+        if (type == 3) ret /= s->nv();
+    }
+    return ret;
+}
+
+API int vxg_profile(vxg_sim* h, int en) {
+    S(h)->profiling = en != 0;
+    return 1;
+}
+API int vxg_profile_read(vxg_sim* h, double* ms, int* launches, int cap) {
+    Sim* s = S(h);
+    int n = cap < 3 ? cap : 3;
+    for (int i = 0; i < n; i++) {
+        if (ms) ms[i] = s->profMs[i];
+        if (launches) launches[i] = s->profCount[i];
+    }
+    for (int i = 0; i < 3; i++) { s->profMs[i] = 0; s->profCount[i] = 0; }
+    return n;
+}
+
+/* --------------------------------------------------------------- materials */
+API int vxg_material_add(vxg_sim* h, float E, float rho) { return S(h)->addMaterial(E, rho); }
+API int vxg_material_count(vxg_sim* h) { return (int)S(h)->vmats.size(); }
+API int vxg_material_set_model_linear(vxg_sim* h, int mi, float E, float fail) {
+    vx::Mat* m = M(S(h), mi);
+    if (!m) return 0;
+    bool ok = m->setModelLinear(E, fail);
+    S(h)->materialsChanged();
+    return ok ? 1 : 0;
+}
+API int vxg_material_set_model_bilinear(vxg_sim* h, int mi, float E, float pl, float ys, float fs) {
+    vx::Mat* m = M(S(h), mi);
+    if (!m) return 0;
+    bool ok = m->setModelBilinear(E, pl, ys, fs);
+    S(h)->materialsChanged();
+    return ok ? 1 : 0;
+}
+API int vxg_material_set_model(vxg_sim* h, int mi, int n, const float* st, const float* ss) {
+    vx::Mat* m = M(S(h), mi);
+    if (!m || !st || !ss) return 0;
+    bool ok = m->setModel(n, st, ss);
+    S(h)->materialsChanged();
+    return ok ? 1 : 0;
+}
+/* the clamping setters (src/VX_Material.cpp:472-523) */
+API void vxg_material_set(vxg_sim* h, int mi, int prop, float v) {
+    vx::Mat* m = M(S(h), mi);
+    if (!m) return;
+    switch (prop) {
+    case VXG_MAT_POISSONS_RATIO:
+        if (v < 0) v = 0;
+        if (v >= 0.5) v = 0.5 - FLT_EPSILON * 2;
+        m->nu = v;
+        m->updateDerived();
+        break;
+    case VXG_MAT_DENSITY:
+        if (v <= 0) v = FLT_MIN;
+        m->rho = v;
+        m->updateDerived();
+        break;
+    case VXG_MAT_STATIC_FRICTION: if (v <= 0) v = 0; m->muStatic = v; break;
+    case VXG_MAT_KINETIC_FRICTION: if (v <= 0) v = 0; m->muKinetic = v; break;
+    case VXG_MAT_INTERNAL_DAMPING: if (v <= 0) v = 0; m->zetaInternal = v; break;
+    case VXG_MAT_GLOBAL_DAMPING: if (v <= 0) v = 0; m->zetaGlobal = v; break;
+    case VXG_MAT_COLLISION_DAMPING: if (v <= 0) v = 0; m->zetaCollision = v; break;
+    case VXG_MAT_CTE: m->alphaCTE = v; break;
+    default: return;
+    }
+    S(h)->materialsChanged();
+}
+API float vxg_material_get(vxg_sim* h, int mi, int prop) {
+    vx::Mat* m = M(S(h), mi);
+    if (!m) return 0;
+    switch (prop) {
+    case VXG_MAT_POISSONS_RATIO: return m->nu;
+    case VXG_MAT_DENSITY: return m->rho;
+    case VXG_MAT_STATIC_FRICTION: return m->muStatic;
+    case VXG_MAT_KINETIC_FRICTION: return m->muKinetic;
+    case VXG_MAT_INTERNAL_DAMPING: return m->zetaInternal;
+    case VXG_MAT_GLOBAL_DAMPING: return m->zetaGlobal;
+    case VXG_MAT_COLLISION_DAMPING: return m->zetaCollision;
+    case VXG_MAT_CTE: return m->alphaCTE;
+    case VXG_MAT_YOUNGS_MODULUS: return m->E;
+    case VXG_MAT_YIELD_STRESS: return m->sigmaYield;
+    case VXG_MAT_FAILURE_STRESS: return m->sigmaFail;
+    case VXG_MAT_MASS: return m->mass;
+    case VXG_MAT_MOMENT_INERTIA: return m->momentInertia;
+    case VXG_MAT_YIELD_STRAIN: return m->epsilonYield;
+    case VXG_MAT_FAILURE_STRAIN: return m->epsilonFail;
+    case VXG_MAT_IS_LINEAR: return m->linear ? 1.0f : 0.0f;
+    default: return 0;
+    }
+}
+/* CVX_Material::setExternalScaleFactor (src/VX_Material.cpp:509-515) */
+API void vxg_material_set_external_scale(vxg_sim* h, int mi, double x, double y, double z) {
+    vx::Mat* m = M(S(h), mi);
+    if (!m) return;
+    if (x <= 0) x = FLT_MIN;
+    if (y <= 0) y = FLT_MIN;
+    if (z <= 0) z = FLT_MIN;
+    m->extScale[0] = x;
+    m->extScale[1] = y;
+    m->extScale[2] = z;
+    S(h)->materialsChanged();
+}
+API const char* vxg_material_last_error(vxg_sim* h, int mi) {
+    vx::Mat* m = M(S(h), mi);
+    return m ? m->error.c_str() : "";
+}
+API float vxg_material_stress(vxg_sim* h, int mi, float e, float t, int fl) {
+    vx::Mat* m = M(S(h), mi);
+    return m ? m->stress(e, t, fl != 0) : 0.0f;
+}
+API float vxg_material_modulus(vxg_sim* h, int mi, float e) {
+    vx::Mat* m = M(S(h), mi);
+    return m ? m->modulus(e) : 0.0f;
+}
+API int vxg_material_model_data(vxg_sim* h, int mi, float* st, float* ss, int cap) {
+    vx::Mat* m = M(S(h), mi);
+    if (!m) return 0;
+    int n = (int)m->strainData.size();
+    for (int i = 0; i < n && i < cap; i++) {
+        if (st) st[i] = m->strainData[i];
+        if (ss) ss[i] = m->stressData[i];
+    }
+    return n;
+}
+
+/* ------------------------------------------------------------------ voxels */
+/* CVoxelyze::setVoxel (src/Voxelyze.cpp:422-437) */
+API int vxg_voxel_set(vxg_sim* h, int mi, int x, int y, int z) {
+    Sim* s = S(h);
+    s->syncDown();
+    if (mi < 0) {
+        s->removeVoxel(x, y, z);
+        return -1;
+    }
+    vx::Mat* m = M(s, mi);
+    if (!m) return -1;
+    int v = s->voxelAt(x, y, z);
+    if (v >= 0) {
+        s->replaceVoxel(m, x, y, z);
+        return v;
+    }
+    return s->addVoxel(m, x, y, z);
+}
+API int vxg_voxel_count(vxg_sim* h) { return S(h)->nv(); }
+API int vxg_voxel_find(vxg_sim* h, int x, int y, int z) { return S(h)->voxelAt(x, y, z); }
+API int vxg_voxel_info(vxg_sim* h, int vi, int* ijk) {
+    Sim* s = S(h);
+    if (!V(s, vi)) return -1;
+    if (ijk) for (int c = 0; c < 3; c++) ijk[c] = s->vijk[vi][c];
+    for (size_t i = 0; i < s->vmats.size(); i++)
+        if (s->vmats[i].get() == s->vmat[vi]) return (int)i;
+    return -1;
+}
+API void vxg_voxel_set_temperature(vxg_sim* h, int vi, float t) {
+    Sim* s = S(h);
+    if (V(s, vi)) s->setTemperature(vi, t);
+}
+API float vxg_voxel_temperature(vxg_sim* h, int vi) {
+    Sim* s = S(h);
+    return V(s, vi) ? s->vtemp[vi] : 0.0f;
+}
+API void vxg_voxel_get_state(vxg_sim* h, int first, int count, double* pos, double* orient, double* lin, double* ang,
+                             uint32_t* flags) {
+    Sim* s = S(h);
+    if (first < 0 || count <= 0 || first + count > s->nv()) return;
+    if (s->hostStale && (size_t)count * 4 < (size_t)s->nv()) {
+        s->readVoxelRange(first, count, pos, orient, lin, ang, flags);
+        return;
+    }
+    s->syncDown();
+    using namespace vx;
+    for (int k = 0; k < count; k++) {
+        const int i = first + k;
+        if (pos) { pos[3 * k] = s->vd[VPX][i]; pos[3 * k + 1] = s->vd[VPY][i]; pos[3 * k + 2] = s->vd[VPZ][i]; }
+        if (orient) {
+            orient[4 * k] = s->vd[VQW][i]; orient[4 * k + 1] = s->vd[VQX][i];
+            orient[4 * k + 2] = s->vd[VQY][i]; orient[4 * k + 3] = s->vd[VQZ][i];
+        }
+        if (lin) { lin[3 * k] = s->vd[VLX][i]; lin[3 * k + 1] = s->vd[VLY][i]; lin[3 * k + 2] = s->vd[VLZ][i]; }
+        if (ang) { ang[3 * k] = s->vd[VAX][i]; ang[3 * k + 1] = s->vd[VAY][i]; ang[3 * k + 2] = s->vd[VAZ][i]; }
+        if (flags) flags[k] = s->vflags[i];
+    }
+}
+
+/* --------------------------------------------------------------- externals */
+static vx::Ext* E(vxg_sim* h, int vi) {
+    Sim* s = S(h);
+    return V(s, vi) ? s->external(vi) : nullptr;
+}
+/* CVX_External::setFixed(6 x bool) (src/VX_External.cpp:47-51): no rotationChanged() */
+API void vxg_ext_set_fixed(vxg_sim* h, int vi, int mask) {
+    vx::Ext* e = E(h, vi);
+    if (!e) return;
+    e->dof = mask & 0x3F;
+    for (int i = 0; i < 3; i++) e->translation[i] = e->rotation[i] = 0.0;
+}
+/* CVX_External::setDisplacement (src/VX_External.cpp:53-66) */
+API void vxg_ext_set_displacement(vxg_sim* h, int vi, int dof, double d) {
+    vx::Ext* e = E(h, vi);
+    if (!e) return;
+    e->dof |= dof;
+    if (d != 0.0f) {
+        if (dof & VXG_X_TRANSLATE) e->translation[0] = d;
+        if (dof & VXG_Y_TRANSLATE) e->translation[1] = d;
+        if (dof & VXG_Z_TRANSLATE) e->translation[2] = d;
+        if (dof & VXG_X_ROTATE) e->rotation[0] = d;
+        if (dof & VXG_Y_ROTATE) e->rotation[1] = d;
+        if (dof & VXG_Z_ROTATE) e->rotation[2] = d;
+    }
+    e->rotationChanged();
+}
+/* CVX_External::setDisplacementAll (src/VX_External.cpp:68-75) */
+API void vxg_ext_set_displacement_all(vxg_sim* h, int vi, const double* t, const double* r) {
+    vx::Ext* e = E(h, vi);
+    if (!e) return;
+    e->dof |= 0x3F;
+    for (int i = 0; i < 3; i++) {
+        e->translation[i] = t ? t[i] : 0.0;
+        e->rotation[i] = r ? r[i] : 0.0;
+    }
+    e->rotationChanged();
+}
+/* CVX_External::clearDisplacement (src/VX_External.cpp:77-89) */
+API void vxg_ext_clear_displacement(vxg_sim* h, int vi, int dof) {
+    vx::Ext* e = E(h, vi);
+    if (!e) return;
+    e->dof &= ~dof;
+    for (int i = 0; i < 3; i++) {
+        if (dof & (1 << i)) e->translation[i] = 0.0;
+        if (dof & (8 << i)) e->rotation[i] = 0.0;
+    }
+    e->rotationChanged();
+}
+/* CVX_External::clearDisplacementAll (src/VX_External.cpp:91-98) */
+API void vxg_ext_clear_displacement_all(vxg_sim* h, int vi) {
+    vx::Ext* e = E(h, vi);
+    if (!e) return;
+    e->dof &= ~0x3F;
+    for (int i = 0; i < 3; i++) e->translation[i] = e->rotation[i] = 0.0;
+    e->rotationChanged();
+}
+API void vxg_ext_set_force(vxg_sim* h, int vi, float x, float y, float z) {
+    vx::Ext* e = E(h, vi);
+    if (!e) return;
+    e->force[0] = x; e->force[1] = y; e->force[2] = z;
+}
+API void vxg_ext_set_moment(vxg_sim* h, int vi, float x, float y, float z) {
+    vx::Ext* e = E(h, vi);
+    if (!e) return;
+    e->moment[0] = x; e->moment[1] = y; e->moment[2] = z;
+}
+API void vxg_ext_reset(vxg_sim* h, int vi) {
+    vx::Ext* e = E(h, vi);
+    if (e) e->reset();
+}
+API int vxg_ext_get(vxg_sim* h, int vi, int* dof, float* f, float* m, double* t, double* r) {
+    Sim* s = S(h);
+    if (!V(s, vi) || !s->vext[vi]) return 0;
+    const vx::Ext* e = s->vext[vi].get();
+    if (dof) *dof = e->dof;
+    for (int i = 0; i < 3; i++) {
+        if (f) f[i] = e->force[i];
+        if (m) m[i] = e->moment[i];
+        if (t) t[i] = e->translation[i];
+        if (r) r[i] = e->rotation[i];
+    }
+    return 1;
+}
+
+/* ------------------------------------------------------------------- links */
+API int vxg_link_count(vxg_sim* h) { return S(h)->nl(); }
+/* CVoxelyze::link(x, y, z, dir) (src/Voxelyze.cpp:501-507) */
+API int vxg_link_find(vxg_sim* h, int x, int y, int z, int dir) {
+    Sim* s = S(h);
+    if (dir < 0 || dir > 5) return -1;
+    int v = s->voxelAt(x, y, z);
+    return v < 0 ? -1 : s->vlinks[v][dir];
+}
+API int vxg_link_info(vxg_sim* h, int li, int* neg, int* pos) {
+    Sim* s = S(h);
+    if (li < 0 || li >= s->nl()) return -1;
+    if (neg) *neg = s->lneg[li];
+    if (pos) *pos = s->lpos[li];
+    return s->laxis[li];
+}
+API void vxg_link_get_state(vxg_sim* h, int first, int count, float* strain, float* stress, float* maxStrain,
+                            float* strainOffset, uint32_t* flags, double* local9, double* forces12, double* rest) {
+    Sim* s = S(h);
+    if (first < 0 || count <= 0 || first + count > s->nl()) return;
+    s->syncDown();
+    using namespace vx;
+    for (int k = 0; k < count; k++) {
+        const int l = first + k;
+        const Mat* m = s->lmat[l];
+        if (strain) strain[k] = s->lf[LSTRAIN][l];
+        if (stress) stress[k] = s->lf[LSTRESS][l];
+        if (maxStrain) maxStrain[k] = s->lf[LMAXSTRAIN][l];
+        if (strainOffset) strainOffset[k] = s->lf[LOFFSET][l];
+        if (flags) {
+            float ms = s->lf[LMAXSTRAIN][l];
+            flags[k] = (s->lflags[l] & (LF_LVV | LF_SMALL)) | (m->isYielded(ms) ? 4u : 0u) | (m->isFailed(ms) ? 8u : 0u);
+        }
+        if (local9)
+            for (int c = 0; c < 9; c++) local9[9 * k + c] = s->ld[LP2X + c][l];
+        if (forces12)
+            for (int c = 0; c < 12; c++) forces12[12 * k + c] = s->ld[LFNX + c][l];
+        if (rest) rest[k] = s->ld[LREST][l];
+    }
+}

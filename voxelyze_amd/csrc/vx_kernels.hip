@@ -1,0 +1,341 @@
+/*
+ * vx_kernels.hip -- the explicit step of Voxelyze on CDNA4 (gfx950), FP64 plain.
+ *
+ * One step is two kernels, exactly the two phases of CVoxelyze::doTimeStep
+ * (src/Voxelyze.cpp:251-284):
+ *   k_links_fp64   one lane per link  -- CVX_Link::updateForces (src/VX_Link.cpp:149-217)
+ *   k_voxels_fp64  one lane per voxel -- CVX_Voxel::timeStep     (src/VX_Voxel.cpp:162-232)
+ * The voxel kernel gathers the <= 6 link end-forces of a voxel through the fixed
+ * neighbour table in direction order 0..5 (VX_Voxel.cpp:238-240), so every sum
+ * has the reference's order and no atomics are needed.
+ *
+ * Divergence (a link strain > 100, Voxelyze.cpp:265-269) is a device flag: the
+ * voxel kernel of that step does nothing and raises "halted", which makes every
+ * later kernel of a multi-step launch return at once, so a captured N-step
+ * sequence stops exactly where the reference's host loop would.
+ *
+ * Both kernels are HBM-bandwidth bound: a few hundred FP64 flops per element
+ * against ~300 B (voxel) / ~380 B (link) of traffic.
+ */
+#include "vx_device.h"
+#include "vx_math.h"
+
+#pragma clang fp contract(off)
+
+namespace vx {
+
+/* CVX_Material::stress for Poisson's ratio 0 (src/VX_Material.cpp:165-195) */
+__device__ __forceinline__ float link_stress(const LinkMat& m, const float* __restrict__ cs, const float* __restrict__ ss,
+                                             float strain) {
+    if (m.epsilonFail != -1.0f && strain > m.epsilonFail) return 0.0f;
+    if (strain <= m.strain1 || m.linear) return m.E * strain;
+    const int n = m.ndata;
+    for (int i = 2; i < n; i++) {
+        if (strain <= cs[i] || i == n - 1) {
+            float perc = (strain - cs[i - 1]) / (cs[i] - cs[i - 1]);
+            return ss[i - 1] + perc * (ss[i] - ss[i - 1]);
+        }
+    }
+    return 0.0f;
+}
+
+__global__ void __launch_bounds__(256) k_links_fp64(DevPtrs s) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.nl) return;
+    if (s.ctl[1]) return;
+    const int nl = s.nl, nv = s.nv;
+    const double* __restrict__ vd = s.vd;
+    double* __restrict__ ld = s.ld;
+    float* __restrict__ lf = s.lf;
+
+    const int n = s.lneg[i], p = s.lpos[i], meta = s.lmeta[i];
+    const int axis = meta & 3;
+    const LinkMat m = s.lmats[meta >> 2];
+
+    const dvec posN = dv(vd[VPX * nv + n], vd[VPY * nv + n], vd[VPZ * nv + n]);
+    const dvec posP = dv(vd[VPX * nv + p], vd[VPY * nv + p], vd[VPZ * nv + p]);
+    const dquat qN = dq(vd[VQW * nv + n], vd[VQX * nv + n], vd[VQY * nv + n], vd[VQZ * nv + n]);
+    const dquat qP = dq(vd[VQW * nv + p], vd[VQX * nv + p], vd[VQY * nv + p], vd[VQZ * nv + p]);
+    const dvec oldPos2 = dv(ld[LP2X * nl + i], ld[LP2Y * nl + i], ld[LP2Z * nl + i]);
+    const dvec oldA1 = dv(ld[LA1X * nl + i], ld[LA1Y * nl + i], ld[LA1Z * nl + i]);
+    const dvec oldA2 = dv(ld[LA2X * nl + i], ld[LA2Y * nl + i], ld[LA2Z * nl + i]);
+    const double rest = ld[LREST * nl + i];
+    uint32_t fl = s.lflags[i];
+    bool small = (fl & LF_SMALL) != 0, lvv = (fl & LF_LVV) != 0;
+    float maxStrain = lf[LMAXSTRAIN * nl + i], strainOffset = lf[LOFFSET * nl + i];
+
+    /* orientLink (VX_Link.cpp:77-119) */
+    dvec pos2 = toAxisX(axis, dv(posP.x - posN.x, posP.y - posN.y, posP.z - posN.z));
+    dquat a1 = toAxisXq(axis, qN);
+    dquat a2 = toAxisXq(axis, qP);
+    const dquat totalRot = qconj(a1);
+    pos2 = qrot(totalRot, pos2);
+    a2 = qmul(totalRot, a2);
+    a1 = qident();
+    const float smallTurn = (float)((fabs(pos2.z) + fabs(pos2.y)) / pos2.x);
+    const float extendPerc = (float)(fabs(1 - pos2.x / rest));
+    if (!small && smallTurn < 0.05f && extendPerc < 0.50f) {
+        small = true;
+        lvv = false;
+    } else if (small && (smallTurn > 1.2f * 0.05f || extendPerc > 1.2f * 0.50f)) {
+        small = false;
+        lvv = false;
+    }
+    if (small) {
+        pos2.x -= rest;
+    } else {
+        qfromangletoposx(&a1, pos2);
+        a2 = qmul(a1, a2);
+        pos2 = dv(sqrt(pos2.x * pos2.x + pos2.y * pos2.y + pos2.z * pos2.z) - rest, 0.0, 0.0);
+    }
+    const dvec a1v = qtorv(a1), a2v = qtorv(a2);
+
+    const dvec dPos2 = dv(0.5 * (pos2.x - oldPos2.x), 0.5 * (pos2.y - oldPos2.y), 0.5 * (pos2.z - oldPos2.z));
+    const dvec dA1 = dv(0.5 * (a1v.x - oldA1.x), 0.5 * (a1v.y - oldA1.y), 0.5 * (a1v.z - oldA1.z));
+    const dvec dA2 = dv(0.5 * (a2v.x - oldA2.x), 0.5 * (a2v.y - oldA2.y), 0.5 * (a2v.z - oldA2.z));
+
+    /* updateStrain (VX_Link.cpp:220-249), Poisson's ratio 0 */
+    const float axialStrain = (float)(pos2.x / rest);
+    float stress;
+    const float* cs = s.curveStrain + m.curveOff;
+    const float* ss = s.curveStress + m.curveOff;
+    if (m.linear) {
+        if (axialStrain > maxStrain) maxStrain = axialStrain;
+        stress = link_stress(m, cs, ss, axialStrain);
+    } else if (axialStrain > maxStrain) {
+        maxStrain = axialStrain;
+        stress = link_stress(m, cs, ss, axialStrain);
+        strainOffset = maxStrain - stress / m.E;
+    } else {
+        stress = m.E * (axialStrain - strainOffset);
+    }
+
+    dvec fN, fP, mN, mP;
+    const bool failed = m.epsilonFail != -1.0f && maxStrain > m.epsilonFail;
+    if (failed) {
+        fN = fP = mN = mP = dv(0.0, 0.0, 0.0);
+    } else {
+        const float b1 = m.b1, b2 = m.b2, b3 = m.b3, am2 = m.a2;
+        fN = dv(stress * m.transverseArea,
+                b1 * pos2.y - b2 * (a1v.z + a2v.z),
+                b1 * pos2.z + b2 * (a1v.y + a2v.y));
+        fP = dv(-fN.x, -fN.y, -fN.z);
+        mN = dv(am2 * (a2v.x - a1v.x),
+                -b2 * pos2.z - b3 * (2 * a1v.y + a2v.y),
+                b2 * pos2.y - b3 * (2 * a1v.z + a2v.z));
+        mP = dv(am2 * (a1v.x - a2v.x),
+                -b2 * pos2.z - b3 * (a1v.y + 2 * a2v.y),
+                b2 * pos2.y - b3 * (a1v.z + 2 * a2v.z));
+        if (lvv) {
+            const float dmN = s.vmats[s.vmat[n]].dampNum / s.vprevdt[n];
+            const float dmP = s.vmats[s.vmat[p]].dampNum / s.vprevdt[p];
+            const dvec pc = dv(m.sqA1 * dPos2.x,
+                               m.sqB1 * dPos2.y - m.sqB2xFMp * (dA1.z + dA2.z),
+                               m.sqB1 * dPos2.z + m.sqB2xFMp * (dA1.y + dA2.y));
+            fN.x += dmN * pc.x; fN.y += dmN * pc.y; fN.z += dmN * pc.z;
+            fP.x -= dmP * pc.x; fP.y -= dmP * pc.y; fP.z -= dmP * pc.z;
+            const double hN = 0.5 * dmN, hP = 0.5 * dmP;
+            const dvec mn = dv(-m.sqA2xIp * (dA2.x - dA1.x),
+                               m.sqB2xFMp * dPos2.z + m.sqB3xIp * (2 * dA1.y + dA2.y),
+                               -m.sqB2xFMp * dPos2.y + m.sqB3xIp * (2 * dA1.z + dA2.z));
+            const dvec mp = dv(m.sqA2xIp * (dA2.x - dA1.x),
+                               m.sqB2xFMp * dPos2.z + m.sqB3xIp * (dA1.y + 2 * dA2.y),
+                               -m.sqB2xFMp * dPos2.y + m.sqB3xIp * (dA1.z + 2 * dA2.z));
+            mN.x -= hN * mn.x; mN.y -= hN * mn.y; mN.z -= hN * mn.z;
+            mP.x -= hP * mp.x; mP.y -= hP * mp.y; mP.z -= hP * mp.z;
+        } else {
+            lvv = true;
+        }
+        if (!small) {
+            fN = qrotinv(a1, fN);
+            mN = qrotinv(a1, mN);
+        }
+        fP = qrotinv(a2, fP);
+        mP = qrotinv(a2, mP);
+        fN = toAxisOriginal(axis, fN);
+        fP = toAxisOriginal(axis, fP);
+        mN = toAxisOriginal(axis, mN);
+        mP = toAxisOriginal(axis, mP);
+    }
+
+    ld[LP2X * nl + i] = pos2.x; ld[LP2Y * nl + i] = pos2.y; ld[LP2Z * nl + i] = pos2.z;
+    ld[LA1X * nl + i] = a1v.x; ld[LA1Y * nl + i] = a1v.y; ld[LA1Z * nl + i] = a1v.z;
+    ld[LA2X * nl + i] = a2v.x; ld[LA2Y * nl + i] = a2v.y; ld[LA2Z * nl + i] = a2v.z;
+    ld[LFNX * nl + i] = fN.x; ld[LFNY * nl + i] = fN.y; ld[LFNZ * nl + i] = fN.z;
+    ld[LFPX * nl + i] = fP.x; ld[LFPY * nl + i] = fP.y; ld[LFPZ * nl + i] = fP.z;
+    ld[LMNX * nl + i] = mN.x; ld[LMNY * nl + i] = mN.y; ld[LMNZ * nl + i] = mN.z;
+    ld[LMPX * nl + i] = mP.x; ld[LMPY * nl + i] = mP.y; ld[LMPZ * nl + i] = mP.z;
+    lf[LSTRAIN * nl + i] = axialStrain;
+    lf[LMAXSTRAIN * nl + i] = maxStrain;
+    lf[LOFFSET * nl + i] = strainOffset;
+    lf[LSTRESS * nl + i] = stress;
+    s.lflags[i] = (lvv ? LF_LVV : 0u) | (small ? LF_SMALL : 0u);
+    if (axialStrain > 100) atomicOr(&s.ctl[0], 1);
+}
+
+/* CVX_Voxel::floorPenetration with baseSizeAverage (VX_Voxel.h:88-90,122) */
+__device__ __forceinline__ float floor_penetration(const VoxMat& m, float temp, double pz) {
+    const float f = 1 + temp * m.alphaCTE;
+    const double bx = f * m.sizeX, by = f * m.sizeY, bz = f * m.sizeZ;
+    const double bsa = (bx + by + bz) / 3.0f;
+    return (float)(bsa / 2 - m.nomSize / 2 - pz);
+}
+
+__global__ void __launch_bounds__(256) k_voxels_fp64(DevPtrs s, float dt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.nv) return;
+    if (s.ctl[0]) {
+        if (i == 0) s.ctl[1] = 1;
+        return;
+    }
+    if (i == 0) s.ctl[2] += 1;
+    const int nv = s.nv, nl = s.nl;
+    double* __restrict__ vd = s.vd;
+    const double* __restrict__ ld = s.ld;
+    const VoxMat m = s.vmats[s.vmat[i]];
+    const int e = s.vext[i];
+    const ExtRec* ex = e >= 0 ? &s.exts[e] : nullptr;
+    s.vprevdt[i] = dt;
+
+    if (ex && (ex->dof & 0x3F) == 0x3F) {
+        /* fully fixed: pinned to its prescribed pose (VX_Voxel.cpp:167-172) */
+        const double sz = m.nomSize;
+        vd[VPX * nv + i] = s.vijk[i] * sz + ex->tx;
+        vd[VPY * nv + i] = s.vijk[nv + i] * sz + ex->ty;
+        vd[VPZ * nv + i] = s.vijk[2 * nv + i] * sz + ex->tz;
+        vd[VQW * nv + i] = ex->qw; vd[VQX * nv + i] = ex->qx; vd[VQY * nv + i] = ex->qy; vd[VQZ * nv + i] = ex->qz;
+        vd[VLX * nv + i] = 0.0; vd[VLY * nv + i] = 0.0; vd[VLZ * nv + i] = 0.0;
+        vd[VAX * nv + i] = 0.0; vd[VAY * nv + i] = 0.0; vd[VAZ * nv + i] = 0.0;
+        return;
+    }
+
+    dvec pos = dv(vd[VPX * nv + i], vd[VPY * nv + i], vd[VPZ * nv + i]);
+    dvec lin = dv(vd[VLX * nv + i], vd[VLY * nv + i], vd[VLZ * nv + i]);
+    dvec ang = dv(vd[VAX * nv + i], vd[VAY * nv + i], vd[VAZ * nv + i]);
+    dquat q = dq(vd[VQW * nv + i], vd[VQX * nv + i], vd[VQY * nv + i], vd[VQZ * nv + i]);
+    uint32_t flags = s.vflags[i];
+
+    int lk[6];
+#pragma unroll
+    for (int d = 0; d < 6; d++) lk[d] = s.vlink[d * nv + i];
+
+    /* force() (VX_Voxel.cpp:234-256) */
+    dvec t = dv(0.0, 0.0, 0.0);
+#pragma unroll
+    for (int d = 0; d < 6; d++) {
+        const int l = lk[d];
+        if (l >= 0) {
+            const int b = (d & 1) ? LFPX : LFNX;
+            t.x += ld[b * nl + l]; t.y += ld[(b + 1) * nl + l]; t.z += ld[(b + 2) * nl + l];
+        }
+    }
+    t = qrot(q, t);
+    if (ex) { t.x += ex->fx; t.y += ex->fy; t.z += ex->fz; }
+    {
+        const dvec vel = dv(m.massInverse * lin.x, m.massInverse * lin.y, m.massInverse * lin.z);
+        t.x -= m.globalTransC * vel.x; t.y -= m.globalTransC * vel.y; t.z -= m.globalTransC * vel.z;
+    }
+    t.z += m.gravityForce;
+
+    dvec cur = t;
+    const bool floorOn = (flags & VF_FLOOR) != 0;
+    float pen = 0.0f;
+    if (floorOn) {
+        /* floorForce (VX_Voxel.cpp:274-298) */
+        pen = floor_penetration(m, s.vtemp[i], pos.z);
+        if (pen >= 0) {
+            const dvec vel = dv(m.massInverse * lin.x, m.massInverse * lin.y, m.massInverse * lin.z);
+            const float normalForce = m.penetrationStiff * pen;
+            cur.z += normalForce - m.collisionTransC * vel.z;
+            if (flags & VF_STATIC) {
+                const float surfaceForceSq = (float)(cur.x * cur.x + cur.y * cur.y);
+                const float frictionForceSq = (m.muStatic * normalForce) * (m.muStatic * normalForce);
+                if (surfaceForceSq > frictionForceSq) flags &= ~VF_STATIC;
+            } else {
+                const float fk = m.muKinetic * normalForce;
+                const double hx = vel.x, hy = vel.y, hz = 0.0;
+                const double l = sqrt(hx * hx + hy * hy + hz * hz);
+                dvec nrm = dv(hx, hy, hz);
+                if (l > 0) { const double inv = 1.0 / l; nrm = dv(inv * hx, inv * hy, inv * hz); }
+                cur.x -= fk * nrm.x; cur.y -= fk * nrm.y; cur.z -= fk * nrm.z;
+            }
+        } else {
+            flags &= ~VF_STATIC;
+        }
+    }
+    const dvec fric = dv(cur.x - t.x, cur.y - t.y, cur.z - t.z);
+
+    lin.x += dt * cur.x; lin.y += dt * cur.y; lin.z += dt * cur.z;
+    const float dtm = dt * m.massInverse;
+    dvec tr = dv(dtm * lin.x, dtm * lin.y, dtm * lin.z);
+    if (floorOn && pen >= 0) {
+        const double work = fric.x * tr.x + fric.y * tr.y;
+        const double hKe = 0.5 * m.massInverse * (lin.x * lin.x + lin.y * lin.y);
+        if (hKe + work <= 0) flags |= VF_STATIC;
+        if (flags & VF_STATIC) {
+            lin.x = lin.y = 0.0;
+            tr.x = tr.y = 0.0;
+        }
+    } else {
+        flags &= ~VF_STATIC;
+    }
+    pos.x += tr.x; pos.y += tr.y; pos.z += tr.z;
+
+    /* moment() (VX_Voxel.cpp:258-271) */
+    dvec mo = dv(0.0, 0.0, 0.0);
+#pragma unroll
+    for (int d = 0; d < 6; d++) {
+        const int l = lk[d];
+        if (l >= 0) {
+            const int b = (d & 1) ? LMPX : LMNX;
+            mo.x += ld[b * nl + l]; mo.y += ld[(b + 1) * nl + l]; mo.z += ld[(b + 2) * nl + l];
+        }
+    }
+    mo = qrot(q, mo);
+    if (ex) { mo.x += ex->mx; mo.y += ex->my; mo.z += ex->mz; }
+    {
+        const dvec av = dv(m.momentInertiaInverse * ang.x, m.momentInertiaInverse * ang.y, m.momentInertiaInverse * ang.z);
+        mo.x -= m.globalRotC * av.x; mo.y -= m.globalRotC * av.y; mo.z -= m.globalRotC * av.z;
+    }
+    ang.x += dt * mo.x; ang.y += dt * mo.y; ang.z += dt * mo.z;
+    const float dti = dt * m.momentInertiaInverse;
+    q = qmul(qfromrv(dv(dti * ang.x, dti * ang.y, dti * ang.z)), q);
+
+    if (ex) {
+        /* partial dof fixes (VX_Voxel.cpp:210-228) */
+        const double sz = m.nomSize;
+        const int dof = ex->dof;
+        if (dof & 1) { pos.x = s.vijk[i] * sz + ex->tx; lin.x = 0.0; }
+        if (dof & 2) { pos.y = s.vijk[nv + i] * sz + ex->ty; lin.y = 0.0; }
+        if (dof & 4) { pos.z = s.vijk[2 * nv + i] * sz + ex->tz; lin.z = 0.0; }
+        if (dof & 56) {
+            if ((dof & 56) == 56) {
+                q = dq(ex->qw, ex->qx, ex->qy, ex->qz);
+                ang = dv(0.0, 0.0, 0.0);
+            } else {
+                dvec rv = qtorv(q);
+                if (dof & 8) { rv.x = 0.0; ang.x = 0.0; }
+                if (dof & 16) { rv.y = 0.0; ang.y = 0.0; }
+                if (dof & 32) { rv.z = 0.0; ang.z = 0.0; }
+                q = qfromrv(rv);
+            }
+        }
+    }
+
+    vd[VPX * nv + i] = pos.x; vd[VPY * nv + i] = pos.y; vd[VPZ * nv + i] = pos.z;
+    vd[VLX * nv + i] = lin.x; vd[VLY * nv + i] = lin.y; vd[VLZ * nv + i] = lin.z;
+    vd[VAX * nv + i] = ang.x; vd[VAY * nv + i] = ang.y; vd[VAZ * nv + i] = ang.z;
+    vd[VQW * nv + i] = q.w; vd[VQX * nv + i] = q.x; vd[VQY * nv + i] = q.y; vd[VQZ * nv + i] = q.z;
+    s.vflags[i] = flags;
+}
+
+static inline unsigned blocks(int n, int b) { return (unsigned)((n + b - 1) / b); }
+
+void launch_links_fp64(const DevPtrs& s, hipStream_t st) {
+    if (s.nl > 0) hipLaunchKernelGGL(k_links_fp64, dim3(blocks(s.nl, 256)), dim3(256), 0, st, s);
+}
+void launch_voxels_fp64(const DevPtrs& s, float dt, hipStream_t st) {
+    if (s.nv > 0) hipLaunchKernelGGL(k_voxels_fp64, dim3(blocks(s.nv, 256)), dim3(256), 0, st, s, dt);
+}
+
+}  // namespace vx

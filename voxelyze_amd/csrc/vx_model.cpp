@@ -1,0 +1,838 @@
+/*
+ * vx_model.cpp -- host model of a simulation and its device mirror.
+ *
+ * Material derivation, topology building and the externals follow the
+ * reference's semantics (file:line on each function); every float/double
+ * cast point of the derived constants is the reference's, because the device
+ * step consumes these constants and must reproduce its bits.
+ */
+#include "vx_model.h"
+
+#include <float.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+
+#pragma clang fp contract(off)
+
+namespace vx {
+
+void launch_links_fp64(const DevPtrs& s, hipStream_t st);
+void launch_voxels_fp64(const DevPtrs& s, float dt, hipStream_t st);
+
+/* ================================================================ materials */
+/* CVX_Material::stress (src/VX_Material.cpp:165-195) */
+float Mat::stress(float strain, float tss, bool forceLinear) const {
+    if (isFailed(strain)) return 0.0f;
+    if (strain <= strainData[1] || linear || forceLinear) {
+        if (nu == 0.0f) return E * strain;
+        return eHat * ((1 - nu) * strain + nu * tss);
+    }
+    const int n = (int)strainData.size();
+    for (int i = 2; i < n; i++) {
+        if (strain <= strainData[i] || i == n - 1) {
+            float perc = (strain - strainData[i - 1]) / (strainData[i] - strainData[i - 1]);
+            float basic = stressData[i - 1] + perc * (stressData[i] - stressData[i - 1]);
+            if (nu == 0.0f) return basic;
+            float mod = (stressData[i] - stressData[i - 1]) / (strainData[i] - strainData[i - 1]);
+            float modHat = mod / ((1 - 2 * nu) * (1 + nu));
+            float effStrain = basic / mod;
+            float effTss = tss * (effStrain / strain);
+            return modHat * ((1 - nu) * effStrain + nu * effTss);
+        }
+    }
+    return 0.0f;
+}
+
+/* CVX_Material::strain(stress) (src/VX_Material.cpp:197-211) */
+float Mat::strainOf(float s) const {
+    if (s <= stressData[1] || linear) return s / E;
+    const int n = (int)stressData.size();
+    for (int i = 2; i < n; i++) {
+        if (s <= stressData[i] || i == n - 1) {
+            float perc = (s - stressData[i - 1]) / (stressData[i] - stressData[i - 1]);
+            return strainData[i - 1] + perc * (strainData[i] - strainData[i - 1]);
+        }
+    }
+    return 0.0f;
+}
+
+/* CVX_Material::modulus (src/VX_Material.cpp:214-226) */
+float Mat::modulus(float strain) const {
+    if (isFailed(strain)) return 0.0f;
+    if (strain <= strainData[1] || linear) return E;
+    const int n = (int)strainData.size();
+    for (int i = 2; i < n; i++)
+        if (strain <= strainData[i] || i == n - 1) return (stressData[i] - stressData[i - 1]) / (strainData[i] - strainData[i - 1]);
+    return 0.0f;
+}
+
+/* CVX_Material::setModelLinear (src/VX_Material.cpp:355-384) */
+bool Mat::setModelLinear(float youngs, float failureStress) {
+    if (youngs <= 0) { error = "Young's modulus must be positive"; return false; }
+    if (failureStress != -1.0f && failureStress <= 0) { error = "Failure stress must be positive"; return false; }
+    float tmpFail = failureStress;
+    if (tmpFail == -1) tmpFail = 1000000;
+    float tmpFailStrain = tmpFail / youngs;
+    strainData = {0.0f, tmpFailStrain};
+    stressData = {0.0f, tmpFail};
+    linear = true;
+    E = youngs;
+    sigmaYield = failureStress;
+    sigmaFail = failureStress;
+    epsilonYield = (failureStress == -1) ? -1 : tmpFailStrain;
+    epsilonFail = (failureStress == -1) ? -1 : tmpFailStrain;
+    return updateDerived();
+}
+
+/* CVX_Material::setModelBilinear (src/VX_Material.cpp:389-434) */
+bool Mat::setModelBilinear(float youngs, float plastic, float yieldStress, float failureStress) {
+    if (youngs <= 0) { error = "Young's modulus must be positive"; return false; }
+    if (plastic <= 0 || plastic >= youngs) { error = "Plastic modulus must be positive but less than Young's modulus"; return false; }
+    if (yieldStress <= 0) { error = "Yield stress must be positive"; return false; }
+    if (failureStress != -1.0f && failureStress <= yieldStress) {
+        error = "Failure stress must be positive and greater than the yield stress";
+        return false;
+    }
+    float yieldStrain = yieldStress / youngs;
+    float tmpFail = failureStress;
+    if (tmpFail == -1) tmpFail = 3 * yieldStress;
+    float tM = plastic;
+    float tB = yieldStress - tM * yieldStrain;
+    float tmpFailStrain = (tmpFail - tB) / tM;
+    strainData = {0.0f, yieldStrain, tmpFailStrain};
+    stressData = {0.0f, yieldStress, tmpFail};
+    linear = false;
+    E = youngs;
+    sigmaYield = yieldStress;
+    sigmaFail = failureStress;
+    epsilonYield = yieldStrain;
+    epsilonFail = failureStress == -1.0f ? -1.0f : tmpFailStrain;
+    return updateDerived();
+}
+
+/* CVX_Material::setYieldFromData (src/VX_Material.cpp:437-470) */
+bool Mat::setYieldFromData(float pct) {
+    sigmaYield = -1.0f;
+    epsilonYield = -1.0f;
+    float oM = E;
+    float oB = (-pct / 100 * oM);
+    int dataPoints = (int)strainData.size() - 1;
+    for (int i = 1; i < dataPoints - 1; i++) {
+        float x1 = strainData[i], x2 = strainData[i + 1], y1 = stressData[i], y2 = stressData[i + 1];
+        float tM = (y2 - y1) / (x2 - x1);
+        float tB = y1 - tM * x1;
+        if (oM != tM) {
+            float xi = (tB - oB) / (oM - tM);
+            if (xi > x1 && xi < x2) {
+                float perc = (xi - x1) / (x2 - x1);
+                sigmaYield = y1 + perc * (y2 - y1);
+                epsilonYield = xi;
+                return true;
+            }
+        }
+    }
+    sigmaYield = sigmaFail;
+    epsilonYield = epsilonFail;
+    return false;
+}
+
+/* CVX_Material::setModel (src/VX_Material.cpp:283-350) */
+bool Mat::setModel(int n, const float* pStrain, const float* pStress) {
+    if (*pStrain == 0 && *pStress == 0) { pStrain++; pStress++; n--; }
+    if (n <= 0) { error = "Not enough data points"; return false; }
+    if (*pStrain <= 0 || *pStress <= 0) { error = "First stress and strain data points negative or zero"; return false; }
+    std::vector<float> ts{0.0f}, tt{0.0f};
+    float sweepStrain = 0.0f, sweepStress = 0.0f;
+    for (int i = 0; i < n; i++) {
+        float thisStrain = pStrain[i], thisStress = pStress[i];
+        if (thisStrain <= sweepStrain) { error = "Out of order strain data"; return false; }
+        if (thisStress <= sweepStress) error = "Stress data is not monotonically increasing";
+        /* compares against the first entry's 0/0 (NaN): never rejects, as in the reference */
+        if (i > 0 && (thisStress - sweepStress) / (thisStrain - sweepStrain) > tt[0] / ts[0]) {
+            error = "Slope of stress/strain curve should never exceed that of the first line segment (youngs modulus)";
+            return false;
+        }
+        sweepStrain = thisStrain;
+        sweepStress = thisStress;
+        ts.push_back(thisStrain);
+        tt.push_back(thisStress);
+    }
+    strainData = ts;
+    stressData = tt;
+    E = stressData[1] / strainData[1];
+    sigmaFail = stressData.back();
+    epsilonFail = strainData.back();
+    linear = (n == 1);
+    if (n == 1 || n == 2) {
+        sigmaYield = stressData[1];
+        epsilonYield = strainData[1];
+    } else {
+        setYieldFromData(0.2f);
+    }
+    return updateDerived();
+}
+
+/* CVX_MaterialLink::updateAll (src/VX_MaterialLink.cpp:45-119) */
+bool Mat::linkUpdateAll() {
+    Mat *v1 = vox1, *v2 = vox2;
+    nomSize = 0.5 * (v1->nomSize + v2->nomSize);
+    rho = 0.5f * (v1->rho + v2->rho);
+    alphaCTE = 0.5f * (v1->alphaCTE + v2->alphaCTE);
+    muStatic = 0.5f * (v1->muStatic + v2->muStatic);
+    muKinetic = 0.5f * (v1->muKinetic + v2->muKinetic);
+    zetaInternal = 0.5f * (v1->zetaInternal + v2->zetaInternal);
+    zetaGlobal = 0.5f * (v1->zetaGlobal + v2->zetaGlobal);
+    zetaCollision = 0.5f * (v1->zetaCollision + v2->zetaCollision);
+    extScale[0] = extScale[1] = extScale[2] = 1.0;
+    float stressFail = -1.0f, f1 = v1->sigmaFail, f2 = v2->sigmaFail;
+    if (f1 == -1.0f) stressFail = f2;
+    else if (f2 == -1.0f) stressFail = f1;
+    else stressFail = f1 < f2 ? f1 : f2;
+    if (v1->linear && v2->linear) {
+        setModelLinear(2.0f * v1->E * v2->E / (v1->E + v2->E), stressFail);
+    } else {
+        std::vector<float> nst{0.0f}, nss{0.0f};
+        size_t it1 = 1, it2 = 1;
+        while (it1 < v1->strainData.size() && it2 < v2->strainData.size()) {
+            float strain = FLT_MAX;
+            if (it1 < v1->strainData.size()) strain = v1->strainData[it1];
+            if (it2 < v2->strainData.size() && v2->strainData[it2] < strain) strain = v2->strainData[it2];
+            if (strain == v1->strainData[it1]) it1++;
+            if (strain == v2->strainData[it2]) it2++;
+            float m1 = v1->modulus(strain - FLT_EPSILON);
+            float m2 = v2->modulus(strain - FLT_EPSILON);
+            float thisMod = 2.0f * m1 * m2 / (m1 + m2);
+            size_t last = nst.size() - 1;
+            nss.push_back(nss[last] + thisMod * (strain - nst[last]));
+            nst.push_back(strain);
+        }
+        setModel((int)nst.size(), nst.data(), nss.data());
+        sigmaFail = stressFail;
+        epsilonFail = stressFail == -1.0f ? -1.0f : strainOf(stressFail);
+    }
+    if (v1->nu == 0 && v2->nu == 0) {
+        nu = 0;
+    } else {
+        float tmpEHat = 2 * v1->eHat * v2->eHat / (v1->eHat + v2->eHat);
+        float tmpE = E;
+        float c2 = (tmpEHat - tmpE) / (2 * tmpEHat) + 0.0625;
+        nu = sqrtf(c2) - 0.25;
+    }
+    return updateDerived();
+}
+
+/* CVX_Material/CVX_MaterialVoxel/CVX_MaterialLink::updateDerived
+   (src/VX_Material.cpp:525-532, src/VX_MaterialVoxel.cpp:57-79, src/VX_MaterialLink.cpp:121-141) */
+bool Mat::updateDerived() {
+    eHat = E / ((1 - 2 * nu) * (1 + nu));
+    for (Mat* d : deps) d->linkUpdateAll();
+    bool ok = true;
+    double volume = nomSize * nomSize * nomSize;
+    mass = (float)(volume * rho);
+    momentInertia = (float)(mass * nomSize * nomSize / 6.0f);
+    firstMoment = (float)(mass * nomSize / 2.0f);
+    if (volume == 0 || mass == 0 || momentInertia == 0) {
+        massInverse = sqrtMass = momentInertiaInverse = c2xSqMxExS = c2xSqIxExSxSxS = 0.0f;
+        ok = false;
+    } else {
+        massInverse = 1.0f / mass;
+        sqrtMass = sqrtf(mass);
+        momentInertiaInverse = 1.0f / momentInertia;
+        c2xSqMxExS = (float)(2.0f * sqrt(mass * E * nomSize));
+        c2xSqIxExSxSxS = (float)(2.0f * sqrt(momentInertia * E * nomSize * nomSize * nomSize));
+    }
+    if (!isLink) return ok;
+    float L = (float)nomSize;
+    a1 = E * L;
+    a2 = E * L * L * L / (12.0f * (1 + nu));
+    b1 = E * L;
+    b2 = E * L * L / 2.0f;
+    b3 = E * L * L * L / 6.0f;
+    sqA1 = sqrtf(a1);
+    sqA2xIp = sqrtf(a2 * L * L / 6.0f);
+    sqB1 = sqrtf(b1);
+    sqB2xFMp = sqrtf(b2 * L / 2.0f);
+    sqB3xIp = sqrtf(b3 * L * L / 6.0f);
+    return true;
+}
+
+/* CVX_Material::clear (src/VX_Material.cpp:54-75) */
+void Mat::clear() {
+    nu = 0.0f; rho = 1.0f; alphaCTE = 0.0f; muStatic = 0.0f; muKinetic = 0.0f;
+    zetaInternal = 1.0f; zetaGlobal = 0.0f; zetaCollision = 0.0f;
+    extScale[0] = extScale[1] = extScale[2] = 1.0;
+    setModelLinear(1.0f, -1.0f);
+    updateDerived();
+}
+
+/* ================================================================ externals */
+/* CVX_External::rotationChanged (src/VX_External.cpp:100-109) */
+void Ext::rotationChanged() {
+    if (rotation[0] != 0 || rotation[1] != 0 || rotation[2] != 0) {
+        hasRotQ = true;
+        rotQ = qfromrv(dv(rotation[0], rotation[1], rotation[2]));
+    } else if (hasRotQ) {
+        rotQ = qident();
+    }
+}
+/* CVX_External::reset (src/VX_External.cpp:37-44) */
+void Ext::reset() {
+    dof = 0;
+    for (int i = 0; i < 3; i++) force[i] = moment[i] = 0.0f, translation[i] = rotation[i] = 0.0;
+    rotationChanged();
+}
+
+/* ================================================================== device */
+bool DevBuf::ensure(size_t b) {
+    if (b <= bytes && p) return true;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    size_t nb = b ? b : 64;
+    if (hipMalloc(&p, nb) != hipSuccess) { p = nullptr; return false; }
+    bytes = nb;
+    return true;
+}
+DevBuf::~DevBuf() {
+    if (p) (void)hipFree(p);
+}
+
+Sim::~Sim() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (hipEvent_t e : events) (void)hipEventDestroy(e);
+    if (ctlHost) (void)hipHostFree(ctlHost);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+bool Sim::fail(const char* what, hipError_t e) {
+    err = std::string(what) + ": " + hipGetErrorString(e);
+    return false;
+}
+
+/* ================================================================ topology */
+static uint64_t vkey(int x, int y, int z) {
+    return ((uint64_t)(uint32_t)(x + 0x100000) << 42) | ((uint64_t)(uint32_t)(y + 0x100000) << 21) |
+           (uint64_t)(uint32_t)(z + 0x100000);
+}
+static const int DOFF[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+static inline int opposite(int d) { return d - d % 2 + (d + 1) % 2; }
+
+/* CVoxelyze::addMaterial (src/Voxelyze.cpp:357-366) via CVX_MaterialVoxel(E, rho, size) */
+int Sim::addMaterial(float E, float rho) {
+    std::unique_ptr<Mat> m(new Mat());
+    m->nomSize = 0.001;
+    m->clear();
+    m->rho = rho;
+    m->setModelLinear(E, -1.0f);
+    m->updateDerived();
+    m->nomSize = voxSize;
+    m->gravMult = 0.0f;
+    m->updateDerived();
+    m->gravMult = grav;
+    vmats.push_back(std::move(m));
+    materialsChanged();
+    return (int)vmats.size() - 1;
+}
+
+/* CVoxelyze::combinedMaterial (src/Voxelyze.cpp:626-640) */
+Mat* Sim::combinedMaterial(Mat* a, Mat* b) {
+    for (auto& l : lmats)
+        if ((l->vox1 == a && l->vox2 == b) || (l->vox1 == b && l->vox2 == a)) return l.get();
+    std::unique_ptr<Mat> m(new Mat());
+    m->nomSize = 0.001;
+    m->clear();
+    m->rho = 1e3f;
+    m->setModelLinear(1e6f, -1.0f);
+    m->isLink = true;
+    m->vox1 = a;
+    m->vox2 = b;
+    m->linkUpdateAll();
+    Mat* r = m.get();
+    lmats.push_back(std::move(m));
+    a->deps.push_back(r);
+    b->deps.push_back(r);
+    materialsChanged();
+    return r;
+}
+
+int Sim::voxelAt(int x, int y, int z) const {
+    auto it = vmap.find(vkey(x, y, z));
+    return it == vmap.end() ? -1 : it->second;
+}
+
+/* CVX_Voxel::reset (src/VX_Voxel.cpp:47-56) */
+void Sim::resetVoxel(int i) {
+    const double sz = vmat[i]->nomSize;
+    vd[VPX][i] = vijk[i][0] * sz;
+    vd[VPY][i] = vijk[i][1] * sz;
+    vd[VPZ][i] = vijk[i][2] * sz;
+    vd[VQW][i] = 1.0;
+    vd[VQX][i] = vd[VQY][i] = vd[VQZ][i] = 0.0;
+    vd[VLX][i] = vd[VLY][i] = vd[VLZ][i] = 0.0;
+    vd[VAX][i] = vd[VAY][i] = vd[VAZ][i] = 0.0;
+    vflags[i] |= VF_STATIC;
+    vtemp[i] = 0.0f;
+    vprevdt[i] = 0.0f;
+}
+
+/* CVX_Voxel::updateSurface (src/VX_Voxel.cpp:376-381) */
+void Sim::updateSurface(int i) {
+    bool interior = true;
+    for (int d = 0; d < 6; d++) if (vlinks[i][d] < 0) interior = false;
+    if (interior) vflags[i] |= VF_INTERIOR; else vflags[i] &= ~VF_INTERIOR;
+}
+
+/* CVX_Link::updateRestLength (src/VX_Link.cpp:137-140) with CVX_Voxel::baseSize(axis) (VX_Voxel.h:89) */
+void Sim::updateRestLength(int li) {
+    const int ax = laxis[li];
+    auto base = [&](int v) {
+        const Mat* m = vmat[v];
+        return (m->nomSize * m->extScale[ax]) * (1 + vtemp[v] * m->alphaCTE);
+    };
+    ld[LREST][li] = 0.5 * (base(lneg[li]) + base(lpos[li]));
+    dirty |= D_REST;
+}
+
+/* CVX_Link::reset (src/VX_Link.cpp:61-75) */
+void Sim::resetLink(int li) {
+    for (int c = 0; c < LD_COUNT; c++) ld[c][li] = 0.0;
+    for (int c = 0; c < LF_COUNT; c++) lf[c][li] = 0.0f;
+    lflags[li] = LF_SMALL;
+    updateRestLength(li);
+}
+
+/* CVX_Voxel::setTemperature (src/VX_Voxel.cpp:108-114) */
+void Sim::setTemperature(int vi, float t) {
+    vtemp[vi] = t;
+    dirty |= D_TEMP;
+    for (int d = 0; d < 6; d++) if (vlinks[vi][d] >= 0) updateRestLength(vlinks[vi][d]);
+}
+
+Ext* Sim::external(int vi) {
+    if (!vext[vi]) {
+        vext[vi].reset(new Ext());
+        vext[vi]->reset();
+    }
+    dirty |= D_EXT;
+    return vext[vi].get();
+}
+
+/* CVoxelyze::addLink (src/Voxelyze.cpp:508-539) with the CVX_Link ctor (src/VX_Link.cpp:26-59) */
+int Sim::addLink(int x, int y, int z, int dir) {
+    const int v1 = voxelAt(x, y, z);
+    const int v2 = voxelAt(x + DOFF[dir][0], y + DOFF[dir][1], z + DOFF[dir][2]);
+    if (v1 < 0 || v2 < 0) return -1;
+    if (vlinks[v1][dir] >= 0) return vlinks[v1][dir];
+    Mat* m = combinedMaterial(vmat[v1], vmat[v2]);
+    const int li = nl();
+    const bool rev = (dir % 2) == 1;
+    lneg.push_back(rev ? v2 : v1);
+    lpos.push_back(rev ? v1 : v2);
+    laxis.push_back(dir / 2);
+    lmat.push_back(m);
+    for (int c = 0; c < LD_COUNT; c++) ld[c].push_back(0.0);
+    for (int c = 0; c < LF_COUNT; c++) lf[c].push_back(0.0f);
+    lflags.push_back(0);
+    resetLink(li);
+    vlinks[v1][dir] = li;
+    updateSurface(v1);
+    vlinks[v2][opposite(dir)] = li;
+    updateSurface(v2);
+    topologyChanged();
+    return li;
+}
+
+/* CVoxelyze::removeLink (src/Voxelyze.cpp:541-575) by list position */
+void Sim::removeLinkAt(int li) {
+    const int a = lneg[li], b = lpos[li];
+    for (int d = 0; d < 6; d++) {
+        if (vlinks[a][d] == li) { vlinks[a][d] = -1; updateSurface(a); }
+        if (vlinks[b][d] == li) { vlinks[b][d] = -1; updateSurface(b); }
+    }
+    lneg.erase(lneg.begin() + li);
+    lpos.erase(lpos.begin() + li);
+    laxis.erase(laxis.begin() + li);
+    lmat.erase(lmat.begin() + li);
+    for (int c = 0; c < LD_COUNT; c++) ld[c].erase(ld[c].begin() + li);
+    for (int c = 0; c < LF_COUNT; c++) lf[c].erase(lf[c].begin() + li);
+    lflags.erase(lflags.begin() + li);
+    for (auto& ls : vlinks)
+        for (int d = 0; d < 6; d++)
+            if (ls[d] > li) ls[d]--;
+    topologyChanged();
+}
+
+/* CVoxelyze::addVoxel (src/Voxelyze.cpp:439-461) with the CVX_Voxel ctor (src/VX_Voxel.cpp:23-37) */
+int Sim::addVoxel(Mat* m, int x, int y, int z) {
+    const int vi = nv();
+    vmat.push_back(m);
+    vijk.push_back({(short)x, (short)y, (short)z});
+    vlinks.push_back({-1, -1, -1, -1, -1, -1});
+    vext.emplace_back(nullptr);
+    for (int c = 0; c < VD_COUNT; c++) vd[c].push_back(0.0);
+    vflags.push_back(0);
+    vprevdt.push_back(0.0f);
+    vtemp.push_back(0.0f);
+    resetVoxel(vi);
+    vmap[vkey(x, y, z)] = vi;
+    vd[VPX][vi] = x * voxSize;
+    vd[VPY][vi] = y * voxSize;
+    vd[VPZ][vi] = z * voxSize;
+    if (floor) vflags[vi] |= VF_FLOOR; else vflags[vi] &= ~VF_FLOOR;
+    setTemperature(vi, ambientTemp);
+    if (collisions) vflags[vi] |= VF_COLLISIONS; else vflags[vi] &= ~VF_COLLISIONS;
+    for (int d = 0; d < 6; d++) addLink(x, y, z, d);
+    topologyChanged();
+    return vi;
+}
+
+/* CVoxelyze::removeVoxel (src/Voxelyze.cpp:464-484) */
+void Sim::removeVoxel(int x, int y, int z) {
+    const int vi = voxelAt(x, y, z);
+    if (vi < 0) return;
+    for (int d = 0; d < 6; d++)
+        if (vlinks[vi][d] >= 0) removeLinkAt(vlinks[vi][d]);
+    vmap.erase(vkey(x, y, z));
+    vmat.erase(vmat.begin() + vi);
+    vijk.erase(vijk.begin() + vi);
+    vlinks.erase(vlinks.begin() + vi);
+    vext.erase(vext.begin() + vi);
+    for (int c = 0; c < VD_COUNT; c++) vd[c].erase(vd[c].begin() + vi);
+    vflags.erase(vflags.begin() + vi);
+    vprevdt.erase(vprevdt.begin() + vi);
+    vtemp.erase(vtemp.begin() + vi);
+    for (auto& kv : vmap) if (kv.second > vi) kv.second--;
+    for (size_t l = 0; l < lneg.size(); l++) {
+        if (lneg[l] > vi) lneg[l]--;
+        if (lpos[l] > vi) lpos[l]--;
+    }
+    topologyChanged();
+}
+
+/* CVoxelyze::replaceVoxel (src/Voxelyze.cpp:486-499) with CVX_Voxel::replaceMaterial (src/VX_Voxel.cpp:79-91) */
+void Sim::replaceVoxel(Mat* m, int x, int y, int z) {
+    const int vi = voxelAt(x, y, z);
+    if (m) {
+        float fm = m->mass / vmat[vi]->mass;
+        vd[VLX][vi] *= fm; vd[VLY][vi] *= fm; vd[VLZ][vi] *= fm;
+        float fi = m->momentInertia / vmat[vi]->momentInertia;
+        vd[VAX][vi] *= fi; vd[VAY][vi] *= fi; vd[VAZ][vi] *= fi;
+        vflags[vi] &= ~VF_STATIC;
+        vmat[vi] = m;
+    }
+    for (int d = 0; d < 6; d++) {
+        if (vlinks[vi][d] >= 0) removeLinkAt(vlinks[vi][d]);
+        addLink(x, y, z, d);
+    }
+    topologyChanged();
+}
+
+/* CVoxelyze::recommendedTimeStep (src/Voxelyze.cpp:286-311), Poisson's ratio 0 */
+float Sim::recommendedTimeStep() {
+    if (dtCacheValid) return dtCache;
+    float maxFreq2 = 0.0f;
+    for (int l = 0; l < nl(); l++) {
+        float m1 = vmat[lneg[l]]->mass, m2 = vmat[lpos[l]]->mass;
+        float f = lmat[l]->a1 / (m1 < m2 ? m1 : m2);
+        if (f > maxFreq2) maxFreq2 = f;
+    }
+    if (maxFreq2 <= 0.0f) {
+        for (int v = 0; v < nv(); v++) {
+            const Mat* m = vmat[v];
+            float f = (float)(m->E * m->nomSize / m->mass);
+            if (f > maxFreq2) maxFreq2 = f;
+        }
+    }
+    dtCache = maxFreq2 <= 0.0f ? 0.0f : 1.0f / (6.283185f * sqrtf(maxFreq2));
+    dtCacheValid = true;
+    return dtCache;
+}
+
+/* ============================================================ host <-> device */
+bool Sim::ensureDevice() {
+    if (stream) return true;
+    hipError_t e = hipStreamCreate(&stream);
+    if (e != hipSuccess) return fail("hipStreamCreate", e);
+    e = hipHostMalloc((void**)&ctlHost, 4 * sizeof(int), hipHostMallocDefault);
+    if (e != hipSuccess) return fail("hipHostMalloc", e);
+    if (!bctl.ensure(4 * sizeof(int))) return fail("hipMalloc", hipErrorOutOfMemory);
+    return true;
+}
+
+#define HIPCHK(call)                                       \
+    do {                                                   \
+        hipError_t e_ = (call);                            \
+        if (e_ != hipSuccess) return fail(#call, e_);      \
+    } while (0)
+
+template <class T>
+static hipError_t up(T* dst, const std::vector<T>& src, hipStream_t st) {
+    if (src.empty()) return hipSuccess;
+    return hipMemcpyAsync(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, st);
+}
+
+/* push every dirty host array to the device */
+bool Sim::syncUp() {
+    if (!ensureDevice()) return false;
+    const int n = nv(), L = nl();
+    if (dirty & D_TOPO) {
+        if (n > nvCap || L > nlCap || !bvd.p) {
+            int vc = std::max(n, 1), lc = std::max(L, 1);
+            if (!bvd.ensure((size_t)VD_COUNT * vc * 8) || !bvflags.ensure((size_t)vc * 4) || !bvmat.ensure((size_t)vc * 4) ||
+                !bvext.ensure((size_t)vc * 4) || !bvprevdt.ensure((size_t)vc * 4) || !bvtemp.ensure((size_t)vc * 4) ||
+                !bvlink.ensure((size_t)6 * vc * 4) || !bvijk.ensure((size_t)3 * vc * 4) || !blneg.ensure((size_t)lc * 4) ||
+                !blpos.ensure((size_t)lc * 4) || !blmeta.ensure((size_t)lc * 4) || !bld.ensure((size_t)LD_COUNT * lc * 8) ||
+                !blf.ensure((size_t)LF_COUNT * lc * 4) || !blflags.ensure((size_t)lc * 4)) {
+                err = "device allocation failed";
+                return false;
+            }
+            nvCap = vc;
+            nlCap = lc;
+        }
+        dp.nv = n;
+        dp.nl = L;
+        dp.vd = (double*)bvd.p;
+        dp.vflags = (uint32_t*)bvflags.p;
+        dp.vmat = (int*)bvmat.p;
+        dp.vext = (int*)bvext.p;
+        dp.vprevdt = (float*)bvprevdt.p;
+        dp.vtemp = (float*)bvtemp.p;
+        dp.vlink = (int*)bvlink.p;
+        dp.vijk = (int*)bvijk.p;
+        dp.lneg = (int*)blneg.p;
+        dp.lpos = (int*)blpos.p;
+        dp.lmeta = (int*)blmeta.p;
+        dp.ld = (double*)bld.p;
+        dp.lf = (float*)blf.p;
+        dp.lflags = (uint32_t*)blflags.p;
+        dp.ctl = (int*)bctl.p;
+        std::vector<int> tmp((size_t)6 * n);
+        for (int v = 0; v < n; v++)
+            for (int d = 0; d < 6; d++) tmp[(size_t)d * n + v] = vlinks[v][d];
+        HIPCHK(up(dp.vlink, tmp, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        tmp.assign((size_t)3 * n, 0);
+        for (int v = 0; v < n; v++)
+            for (int c = 0; c < 3; c++) tmp[(size_t)c * n + v] = vijk[v][c];
+        HIPCHK(up(dp.vijk, tmp, stream));
+        HIPCHK(up(dp.lneg, lneg, stream));
+        HIPCHK(up(dp.lpos, lpos, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        dirty |= D_VSTATE | D_LSTATE | D_MATS | D_EXT | D_REST | D_TEMP;
+    }
+    if (dirty & D_MATS) {
+        int k = 0;
+        std::vector<VoxMat> vm;
+        for (auto& m : vmats) {
+            m->dev = k++;
+            VoxMat r{};
+            r.nomSize = m->nomSize;
+            r.sizeX = m->nomSize * m->extScale[0];
+            r.sizeY = m->nomSize * m->extScale[1];
+            r.sizeZ = m->nomSize * m->extScale[2];
+            r.massInverse = m->massInverse;
+            r.momentInertiaInverse = m->momentInertiaInverse;
+            r.alphaCTE = m->alphaCTE;
+            r.gravityForce = -m->mass * 9.80665f * m->gravMult;
+            r.dampNum = 2 * m->sqrtMass * m->zetaInternal;
+            r.globalTransC = m->zetaGlobal * m->c2xSqMxExS;
+            r.globalRotC = m->zetaGlobal * m->c2xSqIxExSxSxS;
+            r.collisionTransC = m->zetaCollision * m->c2xSqMxExS;
+            r.penetrationStiff = (float)(2 * m->E * m->nomSize);
+            r.muStatic = m->muStatic;
+            r.muKinetic = m->muKinetic;
+            vm.push_back(r);
+        }
+        std::vector<LinkMat> lm;
+        std::vector<float> cs, ss;
+        k = 0;
+        for (auto& m : lmats) {
+            m->dev = k++;
+            LinkMat r{};
+            r.E = m->E; r.a2 = m->a2; r.b1 = m->b1; r.b2 = m->b2; r.b3 = m->b3;
+            r.sqA1 = m->sqA1; r.sqA2xIp = m->sqA2xIp; r.sqB1 = m->sqB1; r.sqB2xFMp = m->sqB2xFMp; r.sqB3xIp = m->sqB3xIp;
+            r.epsilonFail = m->epsilonFail;
+            const float s1 = (float)m->vox1->nomSize, s2 = (float)m->vox2->nomSize;
+            r.transverseArea = 0.5f * (s1 * s1 + s2 * s2);
+            r.strain1 = m->strainData[1];
+            r.linear = m->linear ? 1 : 0;
+            r.ndata = (int)m->strainData.size();
+            r.curveOff = (int)cs.size();
+            cs.insert(cs.end(), m->strainData.begin(), m->strainData.end());
+            ss.insert(ss.end(), m->stressData.begin(), m->stressData.end());
+            lm.push_back(r);
+        }
+        if (cs.empty()) { cs.push_back(0.0f); ss.push_back(0.0f); }
+        if (!bvmats.ensure(std::max<size_t>(1, vm.size()) * sizeof(VoxMat)) ||
+            !blmats.ensure(std::max<size_t>(1, lm.size()) * sizeof(LinkMat)) || !bcs.ensure(cs.size() * 4) ||
+            !bss.ensure(ss.size() * 4)) {
+            err = "device allocation failed";
+            return false;
+        }
+        dp.vmats = (const VoxMat*)bvmats.p;
+        dp.lmats = (const LinkMat*)blmats.p;
+        dp.curveStrain = (const float*)bcs.p;
+        dp.curveStress = (const float*)bss.p;
+        HIPCHK(up((VoxMat*)bvmats.p, vm, stream));
+        HIPCHK(up((LinkMat*)blmats.p, lm, stream));
+        HIPCHK(up((float*)bcs.p, cs, stream));
+        HIPCHK(up((float*)bss.p, ss, stream));
+        std::vector<int> vmi(n), meta(L);
+        for (int v = 0; v < n; v++) vmi[v] = vmat[v]->dev;
+        for (int l = 0; l < L; l++) meta[l] = laxis[l] | (lmat[l]->dev << 2);
+        HIPCHK(up(dp.vmat, vmi, stream));
+        HIPCHK(up(dp.lmeta, meta, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+    if (dirty & D_EXT) {
+        std::vector<ExtRec> ex;
+        extIndex.assign(n, -1);
+        for (int v = 0; v < n; v++) {
+            const Ext* e = vext[v].get();
+            if (!e) continue;
+            ExtRec r{};
+            r.tx = e->translation[0]; r.ty = e->translation[1]; r.tz = e->translation[2];
+            dquat q = e->rotationQuat();
+            r.qw = q.w; r.qx = q.x; r.qy = q.y; r.qz = q.z;
+            r.fx = e->force[0]; r.fy = e->force[1]; r.fz = e->force[2];
+            r.mx = e->moment[0]; r.my = e->moment[1]; r.mz = e->moment[2];
+            r.dof = e->dof;
+            extIndex[v] = (int)ex.size();
+            ex.push_back(r);
+        }
+        if (!bexts.ensure(std::max<size_t>(1, ex.size()) * sizeof(ExtRec))) { err = "device allocation failed"; return false; }
+        dp.exts = (const ExtRec*)bexts.p;
+        HIPCHK(up((ExtRec*)bexts.p, ex, stream));
+        HIPCHK(up(dp.vext, extIndex, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+    if (dirty & D_VSTATE) {
+        for (int c = 0; c < VD_COUNT; c++)
+            if (n) HIPCHK(hipMemcpyAsync(dp.vd + (size_t)c * n, vd[c].data(), (size_t)n * 8, hipMemcpyHostToDevice, stream));
+        HIPCHK(up(dp.vflags, vflags, stream));
+        HIPCHK(up(dp.vprevdt, vprevdt, stream));
+    }
+    if (dirty & D_TEMP) HIPCHK(up(dp.vtemp, vtemp, stream));
+    if (dirty & D_LSTATE) {
+        for (int c = 0; c < LD_COUNT; c++)
+            if (L) HIPCHK(hipMemcpyAsync(dp.ld + (size_t)c * L, ld[c].data(), (size_t)L * 8, hipMemcpyHostToDevice, stream));
+        for (int c = 0; c < LF_COUNT; c++)
+            if (L) HIPCHK(hipMemcpyAsync(dp.lf + (size_t)c * L, lf[c].data(), (size_t)L * 4, hipMemcpyHostToDevice, stream));
+        HIPCHK(up(dp.lflags, lflags, stream));
+    } else if (dirty & D_REST) {
+        if (L) HIPCHK(hipMemcpyAsync(dp.ld + (size_t)LREST * L, ld[LREST].data(), (size_t)L * 8, hipMemcpyHostToDevice, stream));
+    }
+    HIPCHK(hipStreamSynchronize(stream));
+    dirty = 0;
+    return true;
+}
+
+/* pull the device-owned dynamic state back into the mirror */
+bool Sim::syncDown() {
+    if (!hostStale) return true;
+    const int n = nv(), L = nl();
+    for (int c = 0; c < VD_COUNT; c++)
+        if (n) HIPCHK(hipMemcpyAsync(vd[c].data(), dp.vd + (size_t)c * n, (size_t)n * 8, hipMemcpyDeviceToHost, stream));
+    if (n) HIPCHK(hipMemcpyAsync(vflags.data(), dp.vflags, (size_t)n * 4, hipMemcpyDeviceToHost, stream));
+    if (n) HIPCHK(hipMemcpyAsync(vprevdt.data(), dp.vprevdt, (size_t)n * 4, hipMemcpyDeviceToHost, stream));
+    for (int c = 0; c < LD_COUNT; c++) {
+        if (c == LREST || !L) continue; /* rest lengths are host-owned */
+        HIPCHK(hipMemcpyAsync(ld[c].data(), dp.ld + (size_t)c * L, (size_t)L * 8, hipMemcpyDeviceToHost, stream));
+    }
+    for (int c = 0; c < LF_COUNT; c++)
+        if (L) HIPCHK(hipMemcpyAsync(lf[c].data(), dp.lf + (size_t)c * L, (size_t)L * 4, hipMemcpyDeviceToHost, stream));
+    if (L) HIPCHK(hipMemcpyAsync(lflags.data(), dp.lflags, (size_t)L * 4, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    hostStale = false;
+    return true;
+}
+
+/* a few voxels straight from the device, leaving the mirror stale */
+bool Sim::readVoxelRange(int first, int count, double* pos, double* orient, double* lin, double* ang, uint32_t* flags) {
+    const int n = nv();
+    std::vector<double> buf((size_t)count);
+    auto get = [&](int c, double* out, int stride, int off) -> bool {
+        if (hipMemcpy(buf.data(), dp.vd + (size_t)c * n + first, (size_t)count * 8, hipMemcpyDeviceToHost) != hipSuccess)
+            return false;
+        for (int k = 0; k < count; k++) out[(size_t)k * stride + off] = buf[k];
+        return true;
+    };
+    bool ok = true;
+    if (pos) ok = ok && get(VPX, pos, 3, 0) && get(VPY, pos, 3, 1) && get(VPZ, pos, 3, 2);
+    if (orient) ok = ok && get(VQW, orient, 4, 0) && get(VQX, orient, 4, 1) && get(VQY, orient, 4, 2) && get(VQZ, orient, 4, 3);
+    if (lin) ok = ok && get(VLX, lin, 3, 0) && get(VLY, lin, 3, 1) && get(VLZ, lin, 3, 2);
+    if (ang) ok = ok && get(VAX, ang, 3, 0) && get(VAY, ang, 3, 1) && get(VAZ, ang, 3, 2);
+    if (flags) ok = ok && hipMemcpy(flags, dp.vflags + first, (size_t)count * 4, hipMemcpyDeviceToHost) == hipSuccess;
+    if (!ok) err = "device read failed";
+    return ok;
+}
+
+/* nsteps x CVoxelyze::doTimeStep(dt) (src/Voxelyze.cpp:251-284) */
+int Sim::step(float dt, int nsteps, int* divergedAt) {
+    if (divergedAt) *divergedAt = -1;
+    err.clear();
+    if (nsteps <= 0) return 0;
+    if (dt == 0) return nsteps;
+    if (dt < 0) dt = recommendedTimeStep();
+    for (auto& m : vmats) {
+        if (m->nu != 0.0f) {
+            err = "Poisson's ratio != 0 is not supported by the device step";
+            return -1;
+        }
+    }
+    if (collisions) {
+        err = "self-collision is not yet supported by the device step";
+        return -1;
+    }
+    if (nv() == 0) {
+        for (int k = 0; k < nsteps; k++) currentTime += dt;
+        return nsteps;
+    }
+    if (!syncUp()) return -1;
+    hipError_t e = hipMemsetAsync(dp.ctl, 0, 4 * sizeof(int), stream);
+    if (e != hipSuccess) { fail("hipMemsetAsync", e); return -1; }
+    if (profiling) {
+        const size_t need = (size_t)4 * nsteps;
+        while (events.size() < need) {
+            hipEvent_t ev;
+            e = hipEventCreate(&ev);
+            if (e != hipSuccess) { fail("hipEventCreate", e); return -1; }
+            events.push_back(ev);
+        }
+    }
+    for (int k = 0; k < nsteps; k++) {
+        if (profiling) (void)hipEventRecord(events[4 * k], stream);
+        launch_links_fp64(dp, stream);
+        if (profiling) (void)hipEventRecord(events[4 * k + 1], stream);
+        if (profiling) (void)hipEventRecord(events[4 * k + 2], stream);
+        launch_voxels_fp64(dp, dt, stream);
+        if (profiling) (void)hipEventRecord(events[4 * k + 3], stream);
+    }
+    e = hipGetLastError();
+    if (e != hipSuccess) { fail("kernel launch", e); return -1; }
+    e = hipMemcpyAsync(ctlHost, dp.ctl, 4 * sizeof(int), hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) { fail("step", e); return -1; }
+    if (profiling) {
+        for (int k = 0; k < nsteps; k++) {
+            float a = 0, b = 0;
+            if (nl() > 0 && hipEventElapsedTime(&a, events[4 * k], events[4 * k + 1]) == hipSuccess) {
+                profMs[0] += a;
+                profCount[0]++;
+            }
+            if (hipEventElapsedTime(&b, events[4 * k + 2], events[4 * k + 3]) == hipSuccess) {
+                profMs[1] += b;
+                profCount[1]++;
+            }
+        }
+    }
+    hostStale = true;
+    const int done = ctlHost[2];
+    for (int k = 0; k < done; k++) currentTime += dt;
+    if (ctlHost[0] && divergedAt) *divergedAt = done;
+    return done;
+}
+
+}  // namespace vx

@@ -1,0 +1,157 @@
+/*
+ * vx_model.h -- host-side model of one simulation: materials, voxels, links and
+ * externals with the reference's construction semantics, plus the mirror of
+ * the device-resident dynamic state.
+ *
+ * The model is authoritative for topology, materials, externals, temperatures
+ * and rest lengths; the device is authoritative for the dynamic state
+ * (pose, momenta, link-local pose, strains, flags) between steps.  The mirror
+ * is pulled back lazily when the host needs it.
+ */
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <memory>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "vx_device.h"
+#include "vx_math.h"
+
+namespace vx {
+
+/* CVX_Material + CVX_MaterialVoxel + CVX_MaterialLink (include/VX_Material*.h) */
+struct Mat {
+    bool linear = true;
+    float E = 1.0f, sigmaYield = -1.0f, sigmaFail = -1.0f, epsilonYield = -1.0f, epsilonFail = -1.0f;
+    std::vector<float> strainData, stressData;
+    float nu = 0.0f, rho = 1.0f, alphaCTE = 0.0f, muStatic = 0.0f, muKinetic = 0.0f;
+    float zetaInternal = 1.0f, zetaGlobal = 0.0f, zetaCollision = 0.0f;
+    double extScale[3] = {1.0, 1.0, 1.0};
+    float eHat = 0.0f;
+    double nomSize = 0.001;
+    float gravMult = 0.0f, mass = 0.0f, massInverse = 0.0f, sqrtMass = 0.0f, firstMoment = 0.0f;
+    float momentInertia = 0.0f, momentInertiaInverse = 0.0f, c2xSqMxExS = 0.0f, c2xSqIxExSxSxS = 0.0f;
+    bool isLink = false;
+    Mat *vox1 = nullptr, *vox2 = nullptr;
+    float a1 = 0, a2 = 0, b1 = 0, b2 = 0, b3 = 0, sqA1 = 0, sqA2xIp = 0, sqB1 = 0, sqB2xFMp = 0, sqB3xIp = 0;
+    std::vector<Mat*> deps;
+    std::string error;
+    int dev = -1; /* index in the device table */
+
+    bool isFailed(float strain) const { return epsilonFail != -1.0f && strain > epsilonFail; }
+    bool isYielded(float strain) const { return epsilonYield != -1.0f && strain > epsilonYield; }
+    float stress(float strain, float tss, bool forceLinear) const;
+    float strainOf(float stress) const;
+    float modulus(float strain) const;
+    bool setModelLinear(float E, float failureStress);
+    bool setModelBilinear(float E, float plastic, float yieldStress, float failureStress);
+    bool setModel(int n, const float* strain, const float* stress);
+    bool setYieldFromData(float percentStrainOffset);
+    bool updateDerived();
+    bool linkUpdateAll();
+    void clear();
+};
+
+/* CVX_External (include/VX_External.h) */
+struct Ext {
+    int dof = 0;
+    float force[3] = {0, 0, 0}, moment[3] = {0, 0, 0};
+    double translation[3] = {0, 0, 0}, rotation[3] = {0, 0, 0};
+    bool hasRotQ = false;
+    dquat rotQ = {1.0, 0.0, 0.0, 0.0};
+    void rotationChanged();
+    void reset();
+    dquat rotationQuat() const { return hasRotQ ? rotQ : qident(); }
+};
+
+/* one device allocation */
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    bool ensure(size_t b);
+    ~DevBuf();
+};
+
+struct Sim {
+    double voxSize = 0.001;
+    float currentTime = 0.0f, ambientTemp = 0.0f, grav = 0.0f;
+    bool floor = false, collisions = false;
+    int precision = 0;
+    std::string err;
+
+    std::vector<std::unique_ptr<Mat>> vmats; /* voxelMats order */
+    std::vector<std::unique_ptr<Mat>> lmats; /* linkMats */
+
+    /* voxels, voxelsList order */
+    std::vector<Mat*> vmat;
+    std::vector<std::array<int, 3>> vijk;
+    std::vector<std::array<int, 6>> vlinks; /* linksList positions or -1 */
+    std::vector<std::unique_ptr<Ext>> vext;
+    std::vector<double> vd[VD_COUNT];
+    std::vector<uint32_t> vflags;
+    std::vector<float> vprevdt, vtemp;
+    std::unordered_map<uint64_t, int> vmap;
+
+    /* links, linksList order */
+    std::vector<int> lneg, lpos, laxis;
+    std::vector<Mat*> lmat;
+    std::vector<double> ld[LD_COUNT];
+    std::vector<float> lf[LF_COUNT];
+    std::vector<uint32_t> lflags;
+
+    /* host <-> device */
+    enum : unsigned { D_TOPO = 1, D_VSTATE = 2, D_LSTATE = 4, D_MATS = 8, D_EXT = 16, D_REST = 32, D_TEMP = 64 };
+    unsigned dirty = D_TOPO | D_MATS | D_EXT;
+    bool hostStale = false; /* device holds newer dynamic state */
+    bool dtCacheValid = false;
+    float dtCache = 0.0f;
+    hipStream_t stream = nullptr;
+    DevPtrs dp{};
+    DevBuf bvd, bvflags, bvmat, bvext, bvprevdt, bvtemp, bvlink, bvijk;
+    DevBuf blneg, blpos, blmeta, bld, blf, blflags;
+    DevBuf bvmats, blmats, bcs, bss, bexts, bctl;
+    int nvCap = 0, nlCap = 0;
+    std::vector<int> extIndex; /* compact external index per voxel, -1 none */
+    int* ctlHost = nullptr;
+    /* kernel timing */
+    bool profiling = false;
+    std::vector<hipEvent_t> events;
+    double profMs[3] = {0, 0, 0};
+    int profCount[3] = {0, 0, 0};
+
+    ~Sim();
+    int nv() const { return (int)vmat.size(); }
+    int nl() const { return (int)lneg.size(); }
+
+    /* model building (src/Voxelyze.cpp:357-668) */
+    int addMaterial(float E, float rho);
+    Mat* combinedMaterial(Mat* a, Mat* b);
+    int voxelAt(int x, int y, int z) const;
+    int addVoxel(Mat* m, int x, int y, int z);
+    void removeVoxel(int x, int y, int z);
+    void replaceVoxel(Mat* m, int x, int y, int z);
+    int addLink(int x, int y, int z, int dir);
+    void removeLinkAt(int li);
+    void resetLink(int li);
+    void resetVoxel(int vi);
+    void updateRestLength(int li);
+    void setTemperature(int vi, float t);
+    void updateSurface(int vi);
+    Ext* external(int vi);
+    float recommendedTimeStep();
+    void materialsChanged() { dirty |= D_MATS; dtCacheValid = false; }
+    void topologyChanged() { dirty |= D_TOPO; dtCacheValid = false; }
+
+    /* device */
+    bool ensureDevice();
+    bool syncUp();
+    bool syncDown();
+    bool readVoxelRange(int first, int count, double* pos, double* orient, double* lin, double* ang, uint32_t* flags);
+    int step(float dt, int nsteps, int* divergedAt);
+    bool fail(const char* what, hipError_t e);
+};
+
+}  // namespace vx

@@ -1,17 +1,19 @@
 """Throughput of the explicit step (CVoxelyze::doTimeStep) on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|mixed]
-                    [--workload lattice|cantilever] [--no-cpu-baseline]
+                    [--workload lattice|cantilever] [--no-cpu-baseline] [--no-compare]
 
 Default workload: the 128x128x64-site Bernoulli(0.9) lattice of two linear
 materials (E 1 MPa / 10 MPa) dropped under gravity onto a friction floor,
 ~943k voxels and ~2.52M links, 5 mm pitch, dt = recommendedTimeStep().  A step
 is one doTimeStep over the whole lattice; state stays resident in HBM.
 
-Prints one JSON line: voxel-steps/s, the roofline of the dominant kernel
-(algorithmic bytes per launch / HIP-event-timed launch duration, against the
-8 TB/s HBM peak) and, on N=1, the CPU oracle timed on this host's cores on a
-bounded sample of the same workload.
+Prints one JSON line: voxel-steps/s of the selected precision (FP64 plain by
+default), the roofline of its dominant kernel (algorithmic bytes per launch /
+HIP-event-timed launch duration, against the 8 TB/s HBM peak), the other
+precision measured the same way on an identical copy of the workload with the
+relative displacement error between the two after the same steps, and, on N=1,
+the CPU oracle timed on this host's cores on a bounded sample of the workload.
 """
 import argparse
 import json
@@ -31,10 +33,17 @@ HBM_PEAK_GBS = 8000.0
 #                                  strain state 8 R + 16 W, end forces 96 W) + 56 B per voxel (pose gather)
 #   voxel kernel: 244 B per voxel (state 104 R + 104 W, neighbour table 24, metadata 12) + 96 B per link
 #                                  (end-force gather)
+#   fused mixed step: 248 B per voxel (state 104 R + 104 W, flags 4 R + 4 W, material and external
+#                                  index 8, neighbour table 24) + 124 B per link (FP32 link state
+#                                  52 R + 52 W, flags 4 R + 4 W, rest length 8, material index 4)
 ALG_BYTES = {
     "link": lambda V, L: 288.0 * L + 56.0 * V,
     "voxel": lambda V, L: 244.0 * V + 96.0 * L,
+    "step_mixed": lambda V, L: 248.0 * V + 124.0 * L,
 }
+# profile classes of vxg_profile_read
+PROFILE_CLASSES = ["link", "voxel", "contact", "step_mixed"]
+KERNEL_NAMES = {"link": "k_links_fp64", "voxel": "k_voxels_fp64", "step_mixed": "k_step_mixed"}
 
 
 def build_workload(sim, workload):
@@ -83,6 +92,72 @@ def cpu_baseline(workload, budget_s=15.0, max_steps=200):
                       f"({V} voxels), {el:.1f} s"}
 
 
+def measure(precision, workload, steps, warmup):
+    """Build the workload, step it `warmup` times, then time `steps` steps with HIP events
+    around every kernel.  Returns the sim (kept alive for comparisons) and the figures."""
+    import ctypes as C
+    import torch
+    from voxelyze_amd.api import CVoxelyze, FP64_PLAIN, MIXED
+
+    t_build = time.perf_counter()
+    sim = CVoxelyze(0.005)
+    cfg = build_workload(sim, workload)
+    sim.setPrecision(MIXED if precision == "mixed" else FP64_PLAIN)
+    V, L = sim.voxelCount(), sim.linkCount()
+    dt = sim.recommendedTimeStep()
+    nominal = sim.state()["pos"]  # before the first step every voxel sits at its lattice position
+    t_build = time.perf_counter() - t_build
+    print(f"[bench] {precision}: built {V} voxels, {L} links in {t_build:.1f}s, dt={dt:.7g}", file=sys.stderr,
+          flush=True)
+    sim.step(dt, warmup)
+    torch.cuda.synchronize()
+    sim.abi.profile(sim.handle, 1)
+    sim.abi.profile_read(sim.handle, None, None, 0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    done, diverged = sim.step(dt, steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    sim.abi.profile(sim.handle, 0)
+    nc = len(PROFILE_CLASSES)
+    ms = (C.c_double * nc)()
+    cnt = (C.c_int * nc)()
+    sim.abi.profile_read(sim.handle, ms, cnt, nc)
+    if done != steps:
+        print(f"[bench] {precision}: diverged at step {diverged}", file=sys.stderr)
+        sys.exit(1)
+    kernels = {}
+    for i, name in enumerate(PROFILE_CLASSES):
+        if cnt[i] and name in ALG_BYTES:
+            avg_s = ms[i] / cnt[i] / 1e3
+            b = ALG_BYTES[name](V, L)
+            kernels[name] = {"avg_us": avg_s * 1e6, "alg_bytes": b, "gbs": b / avg_s / 1e9, "launches": cnt[i]}
+    print(f"[bench] {precision}: {el / steps * 1e3:.4f} ms/step, kernels "
+          + ", ".join(f"{k} {v['avg_us']:.1f} us {v['gbs']:.0f} GB/s" for k, v in kernels.items()),
+          file=sys.stderr, flush=True)
+    cfg.update({"voxels": V, "links": L, "dt": dt, "voxel_pitch_m": 0.005})
+    return sim, {"value": V * steps / el, "el": el, "kernels": kernels, "cfg": cfg, "V": V, "L": L,
+                 "nominal": nominal}
+
+
+def roofline(kernels, workload):
+    dom = max(kernels, key=lambda k: kernels[k]["avg_us"])
+    d = kernels[dom]
+    traffic = traffic_from_profiles(KERNEL_NAMES[dom], workload)
+    return {"bound": "hbm", "achieved": round(d["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(d["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": KERNEL_NAMES[dom],
+            "alg_bytes_per_launch": d["alg_bytes"],
+            "kernels": {KERNEL_NAMES[k]: {"avg_us": round(v["avg_us"], 2), "GB/s": round(v["gbs"], 1),
+                                          "alg_bytes": v["alg_bytes"]} for k, v in kernels.items()}}
+
+
+def rel_disp_error(ref_sim, sim, nominal):
+    """max_i |pos_i - pos_ref_i| / max_i |pos_ref_i - nominal_i|"""
+    pr = ref_sim.state()["pos"]
+    po = sim.state()["pos"]
+    return float(np.abs(po - pr).max() / max(np.abs(pr - nominal).max(), 1e-300))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +166,9 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--workload", default="lattice", choices=["lattice", "cantilever"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-compare", action="store_true", help="measure the selected precision only")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run a plain 1 GiB -> 1 GiB elementwise copy (known bytes) to scale PMC byte counters")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only; --gpus must be 1 "
@@ -98,55 +176,38 @@ def main():
         sys.exit(2)
 
     import torch
-    from voxelyze_amd.api import CVoxelyze, FP64_PLAIN, MIXED
 
-    t_build = time.perf_counter()
-    sim = CVoxelyze(0.005)
-    cfg = build_workload(sim, a.workload)
-    sim.setPrecision(MIXED if a.precision == "mixed" else FP64_PLAIN)
-    V, L = sim.voxelCount(), sim.linkCount()
-    dt = sim.recommendedTimeStep()
-    t_build = time.perf_counter() - t_build
-    print(f"[bench] built {V} voxels, {L} links in {t_build:.1f}s, dt={dt:.7g}", file=sys.stderr, flush=True)
+    if a.calibrate_copy:
+        src = torch.ones(256 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
+        print("[bench] calibration copy: 3 x (1 GiB read + 1 GiB write)", file=sys.stderr, flush=True)
 
-    sim.step(dt, a.warmup)
-    torch.cuda.synchronize()
-    sim.abi.profile(sim.handle, 1)
-    sim.abi.profile_read(sim.handle, None, None, 0)
-    t0 = time.perf_counter()
-    done, diverged = sim.step(dt, a.steps)
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
-    sim.abi.profile(sim.handle, 0)
-    import ctypes as C
-    ms = (C.c_double * 3)()
-    cnt = (C.c_int * 3)()
-    sim.abi.profile_read(sim.handle, ms, cnt, 3)
-    if done != a.steps:
-        print(f"[bench] diverged at step {diverged}", file=sys.stderr)
-        sys.exit(1)
-
-    kernels = {}
-    for i, name in enumerate(["link", "voxel"]):
-        if cnt[i]:
-            avg_s = ms[i] / cnt[i] / 1e3
-            b = ALG_BYTES[name](V, L)
-            kernels[name] = {"avg_us": avg_s * 1e6, "alg_bytes": b, "gbs": b / avg_s / 1e9}
-    dom = max(kernels, key=lambda k: kernels[k]["avg_us"])
-    d = kernels[dom]
-    traffic = traffic_from_profiles(dom, a.workload)
-    roof = {"bound": "hbm", "achieved": round(d["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(d["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": f"k_{dom}s_fp64",
-            "kernels": {k: {"avg_us": round(v["avg_us"], 2), "GB/s": round(v["gbs"], 1)} for k, v in kernels.items()}}
-    value = V * a.steps / el
-    cfg.update({"voxels": V, "links": L, "precision": "fp64_plain" if a.precision == "fp64" else "mixed",
-                "dt": dt, "voxel_pitch_m": 0.005})
+    sim, r = measure(a.precision, a.workload, a.steps, a.warmup)
+    cfg = r["cfg"]
+    cfg["precision"] = "fp64_plain" if a.precision == "fp64" else "mixed"
     line = {
-        "metric": METRIC, "value": value, "unit": "voxel-steps/s", "n_gpus": 1, "steps": a.steps,
-        "warmup": a.warmup, "ms_per_step": el / a.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "metric": METRIC, "value": r["value"], "unit": "voxel-steps/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": r["el"] / a.steps * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64" if a.precision == "fp64" else "f32/f64",
-        "data": "synthetic (seeded SplitMix64 lattice)", "config": cfg, "roofline": roof,
+        "data": "synthetic (seeded SplitMix64 lattice)", "config": cfg, "roofline": roofline(r["kernels"], a.workload),
     }
+    if not a.no_compare:
+        other = "mixed" if a.precision == "fp64" else "fp64"
+        sim2, r2 = measure(other, a.workload, a.steps, a.warmup)
+        ref, test = (sim, sim2) if a.precision == "fp64" else (sim2, sim)
+        err = rel_disp_error(ref, test, r["nominal"])
+        fp64_v, mixed_v = (r["value"], r2["value"]) if a.precision == "fp64" else (r2["value"], r["value"])
+        line[other] = {"value": r2["value"], "ms_per_step": r2["el"] / a.steps * 1e3,
+                       "roofline": roofline(r2["kernels"], a.workload)}
+        line["mixed_vs_fp64"] = {"speedup": mixed_v / fp64_v, "rel_disp_err": err,
+                                 "after_steps": a.warmup + a.steps}
+        print(f"[bench] mixed/fp64 speedup {mixed_v / fp64_v:.2f}x, rel disp err {err:.3e}", file=sys.stderr,
+              flush=True)
+        del sim2
     if not a.no_cpu_baseline:
         print("[bench] timing the CPU oracle on a bounded sample", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(a.workload)

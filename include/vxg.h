@@ -131,9 +131,10 @@ int vxg_set_precision(vxg_sim* s, int mode);
 
 /* Kernel timing (no reference counterpart; measurement hook for benchmarks).
    When enabled, every kernel of later vxg_step calls is bracketed by HIP events on the
-   library's stream; vxg_profile_read returns, per kernel class (0 = link kernel,
-   1 = voxel kernel, 2 = contact kernels), the summed milliseconds and launch counts
-   since the last read, then clears them.  Returns the number of classes written. */
+   library's stream; vxg_profile_read returns, per kernel class (0 = FP64 link kernel,
+   1 = FP64 voxel kernel, 2 = contact kernels, 3 = fused mixed-precision step), the summed
+   milliseconds and launch counts since the last read, then clears them.  Returns the
+   number of classes written. */
 int vxg_profile(vxg_sim* s, int enable);
 int vxg_profile_read(vxg_sim* s, double* ms, int* launches, int cap);
 

@@ -28,7 +28,6 @@ TOL = {name: 1e-9 for name in TRANSCENDENTAL}
 # floor friction switches static/kinetic on tiny force differences: chaotic
 TOL["config2_small"] = 1e-6
 TOL["config4_robot"] = 1e-6
-NEEDS_COLLISIONS = {"collisions", "config3_small"}
 
 NAMES = sorted(S.SCENARIOS)
 
@@ -55,8 +54,6 @@ def rel_disp_error(gold, out):
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", NAMES)
 def test_hip_fp64_matches_reference(name):
-    if name in NEEDS_COLLISIONS:
-        pytest.skip("self-collision is not on the device path yet")
     gold = B.load_golden(name)
     print(f"[parity] {name}", flush=True)
     out = S.SCENARIOS[name](B.maker(B.product_abi()))
@@ -68,6 +65,9 @@ def test_hip_fp64_matches_reference(name):
     err = rel_disp_error(gold, out)
     print(f"[parity] {name}: bit-exact arrays {len(gold) - len(bad)}/{len(gold)}, rel disp err {err:.3e}", flush=True)
     assert err <= TOL[name], f"{name}: relative displacement error {err:.3e} > {TOL[name]:.1e}"
+    for k in gold:
+        if k.endswith("ncol"):
+            assert out[k] == gold[k], f"{name}: collision pair count {out[k]} != {gold[k]}"
     for k in gold:
         if k.startswith("trace") or k == "x":
             g, o = gold[k], out[k]

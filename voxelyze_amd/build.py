@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libvxg.so")
-SOURCES = ["vx_kernels.hip", "vx_model.cpp", "vx_capi.cpp"]
+SOURCES = ["vx_kernels.hip", "vx_contact.hip", "vx_mixed.hip", "vx_model.cpp", "vx_mixed_host.cpp", "vx_capi.cpp"]
 HEADERS = ["vx_device.h", "vx_math.h", "vx_model.h"]
 
 
@@ -48,5 +48,20 @@ def build(verbose=False):
     return OUT
 
 
+def build_facade_example():
+    """tests/cpp/facade_cantilever: a reference-style C++ program built against the
+    header-only facade include/Voxelyze.h and linked with libvxg.so."""
+    src = os.path.join(ROOT, "tests", "cpp", "facade_cantilever.cpp")
+    out = os.path.join(ROOT, "tests", "cpp", "facade_cantilever")
+    cmd = ["g++", "-O2", "-std=c++11", "-I", os.path.join(ROOT, "include"), src, "-o", out,
+           "-L", HERE, "-lvxg", "-Wl,-rpath,$ORIGIN/../../voxelyze_amd"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("building the facade example failed")
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_facade_example())

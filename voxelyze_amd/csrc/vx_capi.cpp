@@ -72,6 +72,8 @@ API void vxg_enable_collisions(vxg_sim* h, int en) {
     s->collisions = en != 0;
     for (auto& f : s->vflags) f = en ? (f | vx::VF_COLLISIONS) : (f & ~vx::VF_COLLISIONS);
     s->dirty |= Sim::D_VSTATE;
+    s->collisionsStale = true;
+    if (!en) s->colEntries = 0;
 }
 API int vxg_is_collisions_enabled(vxg_sim* h) { return S(h)->collisions ? 1 : 0; }
 
@@ -101,24 +103,33 @@ API void vxg_reset_time(vxg_sim* h) {
     s->currentTime = 0.0f;
     for (int v = 0; v < s->nv(); v++) s->resetVoxel(v);
     for (int l = 0; l < s->nl(); l++) s->resetLink(l);
+    s->forcesStale = false; /* every end force is zero now; the next link phase rewrites the slots */
+    s->lastDt = 0.0f;
+    s->nearbyStale = s->collisionsStale = true;
     s->dirty |= Sim::D_VSTATE | Sim::D_LSTATE | Sim::D_TEMP | Sim::D_REST;
 }
 API float vxg_current_time(vxg_sim* h) { return S(h)->currentTime; }
-API int vxg_collision_count(vxg_sim* h) { (void)h; return 0; }
+API int vxg_collision_count(vxg_sim* h) {
+    Sim* s = S(h);
+    return s->collisions ? s->colEntries / 2 : 0;
+}
 API int vxg_set_precision(vxg_sim* h, int mode) {
     Sim* s = S(h);
-    if (mode != VXG_FP64_PLAIN) {
-        s->err = "only VXG_FP64_PLAIN is available";
+    if (mode != VXG_FP64_PLAIN && mode != VXG_MIXED) {
+        s->err = "unknown precision mode";
         return 0;
     }
+    if (mode == s->precision) return 1;
+    if (!s->syncForces()) return 0;
     s->precision = mode;
+    s->topologyChanged(); /* re-lay the device state out for the new step */
     return 1;
 }
 
 /* CVoxelyze::stateInfo (src/Voxelyze.cpp:752-800), evaluated on the host mirror */
 API float vxg_state_info(vxg_sim* h, int info, int type) {
     Sim* s = S(h);
-    s->syncDown();
+    if (info == 7) s->syncForces(); else s->syncDown();
     float ret = 0;
     if (type == 1) ret = -FLT_MAX;
     else if (type == 0) ret = FLT_MAX;
@@ -195,12 +206,12 @@ API int vxg_profile(vxg_sim* h, int en) {
 }
 API int vxg_profile_read(vxg_sim* h, double* ms, int* launches, int cap) {
     Sim* s = S(h);
-    int n = cap < 3 ? cap : 3;
+    int n = cap < 4 ? cap : 4;
     for (int i = 0; i < n; i++) {
         if (ms) ms[i] = s->profMs[i];
         if (launches) launches[i] = s->profCount[i];
     }
-    for (int i = 0; i < 3; i++) { s->profMs[i] = 0; s->profCount[i] = 0; }
+    for (int i = 0; i < 4; i++) { s->profMs[i] = 0; s->profCount[i] = 0; }
     return n;
 }
 
@@ -316,7 +327,7 @@ API int vxg_material_model_data(vxg_sim* h, int mi, float* st, float* ss, int ca
 /* CVoxelyze::setVoxel (src/Voxelyze.cpp:422-437) */
 API int vxg_voxel_set(vxg_sim* h, int mi, int x, int y, int z) {
     Sim* s = S(h);
-    s->syncDown();
+    s->syncForces(); /* untouched links keep their last end forces across the edit */
     if (mi < 0) {
         s->removeVoxel(x, y, z);
         return -1;
@@ -354,6 +365,8 @@ API void vxg_voxel_get_state(vxg_sim* h, int first, int count, double* pos, doub
     if (first < 0 || count <= 0 || first + count > s->nv()) return;
     if (s->hostStale && (size_t)count * 4 < (size_t)s->nv()) {
         s->readVoxelRange(first, count, pos, orient, lin, ang, flags);
+        if (flags)
+            for (int k = 0; k < count; k++) flags[k] &= 0xFFu;
         return;
     }
     s->syncDown();
@@ -367,7 +380,7 @@ API void vxg_voxel_get_state(vxg_sim* h, int first, int count, double* pos, doub
         }
         if (lin) { lin[3 * k] = s->vd[VLX][i]; lin[3 * k + 1] = s->vd[VLY][i]; lin[3 * k + 2] = s->vd[VLZ][i]; }
         if (ang) { ang[3 * k] = s->vd[VAX][i]; ang[3 * k + 1] = s->vd[VAY][i]; ang[3 * k + 2] = s->vd[VAZ][i]; }
-        if (flags) flags[k] = s->vflags[i];
+        if (flags) flags[k] = s->vflags[i] & 0xFFu; /* the reference's boolStates bits only */
     }
 }
 
@@ -476,7 +489,7 @@ API void vxg_link_get_state(vxg_sim* h, int first, int count, float* strain, flo
                             float* strainOffset, uint32_t* flags, double* local9, double* forces12, double* rest) {
     Sim* s = S(h);
     if (first < 0 || count <= 0 || first + count > s->nl()) return;
-    s->syncDown();
+    if (forces12) s->syncForces(); else s->syncDown();
     using namespace vx;
     for (int k = 0; k < count; k++) {
         const int l = first + k;

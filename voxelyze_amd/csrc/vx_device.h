@@ -15,16 +15,26 @@ namespace vx {
 
 /* component order of the voxel FP64 block */
 enum { VPX = 0, VPY, VPZ, VLX, VLY, VLZ, VAX, VAY, VAZ, VQW, VQX, VQY, VQZ, VD_COUNT };
-/* component order of the link FP64 block: local pose, rest length, four end vectors */
+/* component order of the link FP64 block: local pose and rest length live on the
+   device (LDEV_COUNT components); the four end vectors are a host-side view of the
+   per-voxel force slots */
 enum {
-    LP2X = 0, LP2Y, LP2Z, LA1X, LA1Y, LA1Z, LA2X, LA2Y, LA2Z, LREST,
-    LFNX, LFNY, LFNZ, LFPX, LFPY, LFPZ, LMNX, LMNY, LMNZ, LMPX, LMPY, LMPZ, LD_COUNT
+    LP2X = 0, LP2Y, LP2Z, LA1X, LA1Y, LA1Z, LA2X, LA2Y, LA2Z, LREST, LDEV_COUNT,
+    LFNX = LDEV_COUNT, LFNY, LFNZ, LFPX, LFPY, LFPZ, LMNX, LMNY, LMNZ, LMPX, LMPY, LMPZ, LD_COUNT
 };
+/* Per-voxel force slots: the end vectors a link hands to each of its voxels, stored
+   at the voxel and the link's direction from it: component c (0-2 force, 3-5 moment)
+   of direction d (X_POS..Z_NEG, VX_Voxel.h:39-46) of voxel v is vf[(d*6 + c)*nv + v].
+   A missing link leaves its slot at +0.0, which adds nothing to the direction-ordered
+   sum (a sum started at +0.0 never becomes -0.0). */
+__host__ __device__ inline int vf_index(int d, int c, int nv, int v) { return (d * 6 + c) * nv + v; }
+#define VF_COUNT 36
 /* component order of the link FP32 block */
 enum { LSTRAIN = 0, LMAXSTRAIN, LOFFSET, LSTRESS, LF_COUNT };
 
-/* voxel flag bits (CVX_Voxel::voxFlags, VX_Voxel.h:143-147) */
-enum { VF_INTERIOR = 1 << 1, VF_FLOOR = 1 << 2, VF_STATIC = 1 << 3, VF_COLLISIONS = 1 << 5 };
+/* voxel flag bits (CVX_Voxel::voxFlags, VX_Voxel.h:143-147); bits 8..13 record which of
+   the six directions has a link (host-maintained, read-only on the device) */
+enum { VF_INTERIOR = 1 << 1, VF_FLOOR = 1 << 2, VF_STATIC = 1 << 3, VF_COLLISIONS = 1 << 5, VF_LINK0 = 1 << 8 };
 /* link flag bits */
 enum { LF_LVV = 1 << 0, LF_SMALL = 1 << 1 };
 
@@ -77,12 +87,13 @@ struct DevPtrs {
     int* vext;       /* -1: no CVX_External */
     float* vprevdt;
     float* vtemp;
-    int* vlink;      /* 6 * nv, device link index per direction or -1 */
+    double* vf;      /* VF_COUNT * nv force slots */
     int* vijk;       /* 3 * nv */
-    int* lneg;
-    int* lpos;
-    int* lmeta;      /* axis | (link material << 2) */
-    double* ld;      /* LD_COUNT * nl */
+    /* Link slots: the link along axis a whose negative voxel is n lives in slot a*nv + n
+       (nl = 3*nv slots); an empty slot has lpos = -1. */
+    int* lpos;       /* positive voxel of the slot's link, -1 for none */
+    int* lmeta;      /* link material of the slot */
+    double* ld;      /* LDEV_COUNT * nl */
     float* lf;       /* LF_COUNT * nl */
     uint32_t* lflags;
     const VoxMat* vmats;
@@ -91,6 +102,47 @@ struct DevPtrs {
     const float* curveStress;
     const ExtRec* exts;
     int* ctl;        /* [0] diverged this step, [1] halted, [2] steps completed */
+    /* self-collision lists (null when collisions are off): partners of voxel v are
+       entries [colOff[v], colOff[v]+colCnt[v]) in increasing voxel order; colF holds
+       their contact forces on v (3 floats each) for the current step */
+    const int* colCnt;
+    const int* colOff;
+    const float* colF;
+};
+
+/* FP32 link state of the mixed-precision step, per slot (MS_COUNT floats) */
+enum { MS_P2X = 0, MS_P2Y, MS_P2Z, MS_A1X, MS_A1Y, MS_A1Z, MS_A2X, MS_A2Y, MS_A2Z, MS_MAX, MS_OFF, MS_STRAIN, MS_STRESS, MS_COUNT };
+
+/* arguments of one mixed-precision step (vx_mixed.hip) */
+struct MixPtrs {
+    int nv;
+    const double* vdIn;     /* voxel state before the step */
+    double* vdOut;          /* voxel state after the step */
+    const float* lsIn;      /* link state before / after (MS_COUNT * 3nv) */
+    float* lsOut;
+    const uint32_t* lflIn;  /* link flags before / after */
+    uint32_t* lflOut;
+    const uint32_t* vflIn;  /* voxel flags before / after (the static-friction bit changes) */
+    uint32_t* vflOut;
+    const double* lrest;    /* rest length per slot */
+    const int* lmeta;       /* link material per slot */
+    const int* vnbr;        /* 6 * nv: voxel across each direction, -1 none */
+    const int* vmat;
+    const int* vext;
+    const int* vijk;
+    const float* vtemp;
+    const VoxMat* vmats;
+    const LinkMat* lmats;
+    const float* curveStrain;
+    const float* curveStress;
+    const ExtRec* exts;
+    const int* colCnt;      /* contacts (null when off) */
+    const int* colOff;
+    const int* partners;
+    int* div;               /* one divergence flag per step of the launch */
+    int k;                  /* this step's index in the launch */
+    float dt;
+    float prevDt;           /* previousDt of every voxel (uniform whenever a link's local velocity is valid) */
 };
 
 }  // namespace vx

@@ -39,18 +39,39 @@ __device__ __forceinline__ float link_stress(const LinkMat& m, const float* __re
     return 0.0f;
 }
 
+/* one lane per link slot: grid (ceil(nv/256), 3), blockIdx.y is the link axis and the lane's
+   voxel is the link's negative end.  Every slot and every force slot is written on every
+   step (zeros where no link exists) so that all stores cover whole cache lines. */
 __global__ void __launch_bounds__(256) k_links_fp64(DevPtrs s) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= s.nl) return;
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nv = s.nv;
+    if (n >= nv) return;
     if (s.ctl[1]) return;
-    const int nl = s.nl, nv = s.nv;
+    const int axis = blockIdx.y;
+    const int nl = s.nl;
+    const int i = axis * nv + n;
     const double* __restrict__ vd = s.vd;
     double* __restrict__ ld = s.ld;
     float* __restrict__ lf = s.lf;
-
-    const int n = s.lneg[i], p = s.lpos[i], meta = s.lmeta[i];
-    const int axis = meta & 3;
-    const LinkMat m = s.lmats[meta >> 2];
+    double* __restrict__ vf = s.vf;
+    const int dn = 2 * axis, dpp = 2 * axis + 1;
+    const int p = s.lpos[i];
+    if (!(s.vflags[n] & (VF_LINK0 << dpp))) {
+        /* n has no link towards -axis: nobody else writes this slot */
+#pragma unroll
+        for (int c = 0; c < 6; c++) vf[vf_index(dpp, c, nv, n)] = 0.0;
+    }
+    if (p < 0) {
+#pragma unroll
+        for (int c = 0; c < 6; c++) vf[vf_index(dn, c, nv, n)] = 0.0;
+#pragma unroll
+        for (int c = 0; c < LREST; c++) ld[c * nl + i] = 0.0;
+#pragma unroll
+        for (int c = 0; c < LF_COUNT; c++) lf[c * nl + i] = 0.0f;
+        s.lflags[i] = 0u;
+        return;
+    }
+    const LinkMat m = s.lmats[s.lmeta[i]];
 
     const dvec posN = dv(vd[VPX * nv + n], vd[VPY * nv + n], vd[VPZ * nv + n]);
     const dvec posP = dv(vd[VPX * nv + p], vd[VPY * nv + p], vd[VPZ * nv + p]);
@@ -161,10 +182,12 @@ __global__ void __launch_bounds__(256) k_links_fp64(DevPtrs s) {
     ld[LP2X * nl + i] = pos2.x; ld[LP2Y * nl + i] = pos2.y; ld[LP2Z * nl + i] = pos2.z;
     ld[LA1X * nl + i] = a1v.x; ld[LA1Y * nl + i] = a1v.y; ld[LA1Z * nl + i] = a1v.z;
     ld[LA2X * nl + i] = a2v.x; ld[LA2Y * nl + i] = a2v.y; ld[LA2Z * nl + i] = a2v.z;
-    ld[LFNX * nl + i] = fN.x; ld[LFNY * nl + i] = fN.y; ld[LFNZ * nl + i] = fN.z;
-    ld[LFPX * nl + i] = fP.x; ld[LFPY * nl + i] = fP.y; ld[LFPZ * nl + i] = fP.z;
-    ld[LMNX * nl + i] = mN.x; ld[LMNY * nl + i] = mN.y; ld[LMNZ * nl + i] = mN.z;
-    ld[LMPX * nl + i] = mP.x; ld[LMPY * nl + i] = mP.y; ld[LMPZ * nl + i] = mP.z;
+    /* hand the end vectors to the two voxels' slots: the link is direction 2*axis (X/Y/Z_POS)
+       of its negative voxel and 2*axis+1 (X/Y/Z_NEG) of its positive voxel */
+    vf[vf_index(dn, 0, nv, n)] = fN.x; vf[vf_index(dn, 1, nv, n)] = fN.y; vf[vf_index(dn, 2, nv, n)] = fN.z;
+    vf[vf_index(dn, 3, nv, n)] = mN.x; vf[vf_index(dn, 4, nv, n)] = mN.y; vf[vf_index(dn, 5, nv, n)] = mN.z;
+    vf[vf_index(dpp, 0, nv, p)] = fP.x; vf[vf_index(dpp, 1, nv, p)] = fP.y; vf[vf_index(dpp, 2, nv, p)] = fP.z;
+    vf[vf_index(dpp, 3, nv, p)] = mP.x; vf[vf_index(dpp, 4, nv, p)] = mP.y; vf[vf_index(dpp, 5, nv, p)] = mP.z;
     lf[LSTRAIN * nl + i] = axialStrain;
     lf[LMAXSTRAIN * nl + i] = maxStrain;
     lf[LOFFSET * nl + i] = strainOffset;
@@ -189,9 +212,8 @@ __global__ void __launch_bounds__(256) k_voxels_fp64(DevPtrs s, float dt) {
         return;
     }
     if (i == 0) s.ctl[2] += 1;
-    const int nv = s.nv, nl = s.nl;
+    const int nv = s.nv;
     double* __restrict__ vd = s.vd;
-    const double* __restrict__ ld = s.ld;
     const VoxMat m = s.vmats[s.vmat[i]];
     const int e = s.vext[i];
     const ExtRec* ex = e >= 0 ? &s.exts[e] : nullptr;
@@ -215,19 +237,13 @@ __global__ void __launch_bounds__(256) k_voxels_fp64(DevPtrs s, float dt) {
     dquat q = dq(vd[VQW * nv + i], vd[VQX * nv + i], vd[VQY * nv + i], vd[VQZ * nv + i]);
     uint32_t flags = s.vflags[i];
 
-    int lk[6];
-#pragma unroll
-    for (int d = 0; d < 6; d++) lk[d] = s.vlink[d * nv + i];
+    const double* __restrict__ vf = s.vf;
 
-    /* force() (VX_Voxel.cpp:234-256) */
+    /* force() (VX_Voxel.cpp:234-256): link end forces in direction order 0..5 */
     dvec t = dv(0.0, 0.0, 0.0);
 #pragma unroll
     for (int d = 0; d < 6; d++) {
-        const int l = lk[d];
-        if (l >= 0) {
-            const int b = (d & 1) ? LFPX : LFNX;
-            t.x += ld[b * nl + l]; t.y += ld[(b + 1) * nl + l]; t.z += ld[(b + 2) * nl + l];
-        }
+        t.x += vf[vf_index(d, 0, nv, i)]; t.y += vf[vf_index(d, 1, nv, i)]; t.z += vf[vf_index(d, 2, nv, i)];
     }
     t = qrot(q, t);
     if (ex) { t.x += ex->fx; t.y += ex->fy; t.z += ex->fz; }
@@ -236,6 +252,14 @@ __global__ void __launch_bounds__(256) k_voxels_fp64(DevPtrs s, float dt) {
         t.x -= m.globalTransC * vel.x; t.y -= m.globalTransC * vel.y; t.z -= m.globalTransC * vel.z;
     }
     t.z += m.gravityForce;
+    if ((flags & VF_COLLISIONS) && s.colCnt) {
+        /* contact forces in partner order (VX_Voxel.cpp:250-253) */
+        const int cnt = s.colCnt[i], off = s.colOff[i];
+        for (int k = 0; k < cnt; k++) {
+            const float* f = s.colF + 3 * (size_t)(off + k);
+            t.x -= f[0]; t.y -= f[1]; t.z -= f[2];
+        }
+    }
 
     dvec cur = t;
     const bool floorOn = (flags & VF_FLOOR) != 0;
@@ -285,11 +309,7 @@ __global__ void __launch_bounds__(256) k_voxels_fp64(DevPtrs s, float dt) {
     dvec mo = dv(0.0, 0.0, 0.0);
 #pragma unroll
     for (int d = 0; d < 6; d++) {
-        const int l = lk[d];
-        if (l >= 0) {
-            const int b = (d & 1) ? LMPX : LMNX;
-            mo.x += ld[b * nl + l]; mo.y += ld[(b + 1) * nl + l]; mo.z += ld[(b + 2) * nl + l];
-        }
+        mo.x += vf[vf_index(d, 3, nv, i)]; mo.y += vf[vf_index(d, 4, nv, i)]; mo.z += vf[vf_index(d, 5, nv, i)];
     }
     mo = qrot(q, mo);
     if (ex) { mo.x += ex->mx; mo.y += ex->my; mo.z += ex->mz; }
@@ -332,7 +352,7 @@ __global__ void __launch_bounds__(256) k_voxels_fp64(DevPtrs s, float dt) {
 static inline unsigned blocks(int n, int b) { return (unsigned)((n + b - 1) / b); }
 
 void launch_links_fp64(const DevPtrs& s, hipStream_t st) {
-    if (s.nl > 0) hipLaunchKernelGGL(k_links_fp64, dim3(blocks(s.nl, 256)), dim3(256), 0, st, s);
+    if (s.nv > 0) hipLaunchKernelGGL(k_links_fp64, dim3(blocks(s.nv, 256), 3), dim3(256), 0, st, s);
 }
 void launch_voxels_fp64(const DevPtrs& s, float dt, hipStream_t st) {
     if (s.nv > 0) hipLaunchKernelGGL(k_voxels_fp64, dim3(blocks(s.nv, 256)), dim3(256), 0, st, s, dt);

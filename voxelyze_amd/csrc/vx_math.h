@@ -217,6 +217,14 @@ VX_HD double cr_cos(double x) {
     dd_sincos(x, &s, &c);
     return c.hi;
 }
+/* correctly rounded sin and cos of one argument from a single double-double evaluation */
+VX_HD void cr_sincos(double x, double* sn, double* cs) {
+    if (!(fabs(x) < 1048576.0)) { *sn = sin(x); *cs = cos(x); return; }
+    dd s, c;
+    dd_sincos(x, &s, &c);
+    *sn = x == 0.0 ? x : s.hi;
+    *cs = c.hi;
+}
 /* correctly rounded acos: one double-double Newton step on cos(y) = w from libm's value */
 VX_HD double cr_acos(double w) {
     if (!(w > -1.0 && w < 1.0)) return acos(w);
@@ -251,9 +259,10 @@ VX_HD dquat qfromrv(dvec v) {
         q.w = 1.0 - 0.5 * m2;
         s = 1.0 - m2 / 6.0;
     } else {
-        double m = sqrt(m2);
-        q.w = cr_cos(m);
-        s = cr_sin(m) / m;
+        double m = sqrt(m2), sm, cm;
+        cr_sincos(m, &sm, &cm);
+        q.w = cm;
+        s = sm / m;
     }
     q.x = th.x * s;
     q.y = th.y * s;
@@ -279,8 +288,9 @@ VX_HD void qfromangletoposx(dquat* q, dvec rf) {
     if (theta > 3.14159265358979 - 1e-7) { q->w = 0.0; q->x = 0.0; q->y = 1.0; q->z = 0.0; return; }
     const double ami = 1.0 / sqrt(n.z * n.z + n.y * n.y);
     const double a = 0.5 * theta;
-    const double s = cr_sin(a);
-    q->w = cr_cos(a);
+    double s, c;
+    cr_sincos(a, &s, &c);
+    q->w = c;
     q->x = 0.0;
     q->y = n.z * ami * s;
     q->z = -n.y * ami * s;

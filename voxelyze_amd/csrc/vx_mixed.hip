@@ -1,0 +1,349 @@
+/*
+ * vx_mixed.hip -- the explicit step in mixed precision: one fused kernel per step.
+ *
+ * Precision split (the benchmark's "mixed" form): each link's relative pose is
+ * extracted in FP64 (so that subtracting the rest length keeps its digits), the
+ * beam forces, damping and the link's own state are FP32, and the force/moment
+ * sums, momentum and quaternion integration, floor contact and the state in HBM
+ * stay FP64.  Contact forces are FP32, as in the reference (VX_Collision.cpp:45-53).
+ *
+ * Fusion: one lane per voxel evaluates the <= 6 links it touches (every link is
+ * therefore evaluated by both of its voxels, with identical inputs and code, so
+ * both get the same numbers), sums its end forces in direction order 0..5 and
+ * integrates.  Voxel state and link state are double-buffered: a step reads
+ * buffer "in" and writes buffer "out", so lanes never see a neighbour's new
+ * state (the voxel flags too).  The link's negative voxel owns it and writes its new state.  There is
+ * no link-to-voxel force hand-off in HBM.
+ *
+ * Divergence (a strain > 100, Voxelyze.cpp:265-269) raises div[k] for step k of
+ * a launch; step k+1 sees it, raises its own flag and returns at once, and so
+ * on, so every link of the diverging step is still updated (as in the
+ * reference) while nothing after it runs.  The host discards the voxel output
+ * of the diverging step and keeps its link output.
+ */
+#include "vx_device.h"
+#include "vx_math.h"
+
+namespace vx {
+
+struct fquat { float w, x, y, z; };
+
+__device__ __forceinline__ fvec f_qrotinv(fquat q, fvec f) {
+    float tw = q.x * f.x + q.y * f.y + q.z * f.z;
+    float tx = q.w * f.x - q.y * f.z + q.z * f.y;
+    float ty = q.w * f.y + q.x * f.z - q.z * f.x;
+    float tz = q.w * f.z - q.x * f.y + q.y * f.x;
+    return fv(tw * q.x + tx * q.w + ty * q.z - tz * q.y, tw * q.y - tx * q.z + ty * q.w + tz * q.x,
+              tw * q.z + tx * q.y - ty * q.x + tz * q.w);
+}
+
+/* Quat3D::ToRotationVector in FP64 with libm acos (Quat3D.h:117-122) */
+__device__ __forceinline__ dvec d_torv(dquat q) {
+    if (q.w >= 1.0 || q.w <= -1.0) return dv(0.0, 0.0, 0.0);
+    double sl = 1.0 - q.w * q.w;
+    double f = sl < 2.4e-3 ? 2.0 * sqrt((2 - 2 * q.w) / sl) : 2.0 * acos(q.w) / sqrt(sl);
+    return dv(q.x * f, q.y * f, q.z * f);
+}
+
+/* Quat3D::FromAngleToPosX (Quat3D.h:141-166) with libm trig */
+__device__ __forceinline__ void d_angletoposx(dquat* q, dvec rf) {
+    if (rf.x == 0.0 && rf.y == 0.0 && rf.z == 0.0) return;
+    double yox = rf.y / rf.x, zox = rf.z / rf.x;
+    if (yox < 1.732e-2 && yox > -1.732e-2 && zox < 1.732e-2 && zox > -1.732e-2) {
+        q->x = 0.0; q->y = 0.5 * zox; q->z = -0.5 * yox;
+        q->w = 1 + 0.5 * (-q->y * q->y - q->z * q->z);
+        return;
+    }
+    double l = sqrt(rf.x * rf.x + rf.y * rf.y + rf.z * rf.z);
+    dvec n = dv(rf.x / l, rf.y / l, rf.z / l);
+    double theta = acos(n.x);
+    if (theta > 3.14159265358979 - 1e-7) { q->w = 0.0; q->x = 0.0; q->y = 1.0; q->z = 0.0; return; }
+    double ami = 1.0 / sqrt(n.z * n.z + n.y * n.y), s, c;
+    sincos(0.5 * theta, &s, &c);
+    q->w = c; q->x = 0.0; q->y = n.z * ami * s; q->z = -n.y * ami * s;
+}
+
+__device__ __forceinline__ float m_stress(const LinkMat& m, const float* cs, const float* ss, float strain) {
+    if (m.epsilonFail != -1.0f && strain > m.epsilonFail) return 0.0f;
+    if (strain <= m.strain1 || m.linear) return m.E * strain;
+    for (int i = 2; i < m.ndata; i++) {
+        if (strain <= cs[i] || i == m.ndata - 1) {
+            float perc = (strain - cs[i - 1]) / (cs[i] - cs[i - 1]);
+            return ss[i - 1] + perc * (ss[i] - ss[i - 1]);
+        }
+    }
+    return 0.0f;
+}
+
+/* One link (CVX_Link::updateForces, VX_Link.cpp:149-217) in mixed precision.  Returns the
+   force and moment on the requested end (in that voxel's frame) and, when owner, the new
+   link state. */
+__device__ void link_mixed(const MixPtrs& s, int slot, int n, int p, int axis, bool wantPos, bool owner,
+                           dvec posN, dquat qN, dvec posP, dquat qP, fvec* outF, fvec* outM) {
+    const int S = 3 * s.nv;
+    const LinkMat m = s.lmats[s.lmeta[slot]];
+    const double rest = s.lrest[slot];
+    uint32_t fl = s.lflIn[slot];
+    bool small = (fl & LF_SMALL) != 0, lvv = (fl & LF_LVV) != 0;
+    const float* ls = s.lsIn;
+    float maxStrain = ls[MS_MAX * S + slot], strainOffset = ls[MS_OFF * S + slot];
+
+    /* orientLink in FP64 up to the small quantities */
+    dvec d = toAxisX(axis, dv(posP.x - posN.x, posP.y - posN.y, posP.z - posN.z));
+    dquat a1 = toAxisXq(axis, qN), a2 = toAxisXq(axis, qP);
+    const dquat tr = qconj(a1);
+    d = qrot(tr, d);
+    a2 = qmul(tr, a2);
+    a1 = qident();
+    const float smallTurn = (float)((fabs(d.z) + fabs(d.y)) / d.x);
+    const float extendPerc = (float)(fabs(1 - d.x / rest));
+    if (!small && smallTurn < 0.05f && extendPerc < 0.50f) { small = true; lvv = false; }
+    else if (small && (smallTurn > 1.2f * 0.05f || extendPerc > 1.2f * 0.50f)) { small = false; lvv = false; }
+    if (small) {
+        d.x -= rest;
+    } else {
+        d_angletoposx(&a1, d);
+        a2 = qmul(a1, a2);
+        d = dv(sqrt(d.x * d.x + d.y * d.y + d.z * d.z) - rest, 0.0, 0.0);
+    }
+    const float axialStrain = (float)(d.x / rest);
+    const dvec a1d = d_torv(a1), a2d = d_torv(a2);
+    const fvec p2 = fv((float)d.x, (float)d.y, (float)d.z);
+    const fvec a1v = fv((float)a1d.x, (float)a1d.y, (float)a1d.z), a2v = fv((float)a2d.x, (float)a2d.y, (float)a2d.z);
+
+    /* updateStrain (VX_Link.cpp:220-249) */
+    float stress;
+    const float* cs = s.curveStrain + m.curveOff;
+    const float* cst = s.curveStress + m.curveOff;
+    if (m.linear) {
+        if (axialStrain > maxStrain) maxStrain = axialStrain;
+        stress = m_stress(m, cs, cst, axialStrain);
+    } else if (axialStrain > maxStrain) {
+        maxStrain = axialStrain;
+        stress = m_stress(m, cs, cst, axialStrain);
+        strainOffset = maxStrain - stress / m.E;
+    } else {
+        stress = m.E * (axialStrain - strainOffset);
+    }
+
+    fvec fN = fv(0, 0, 0), fP = fv(0, 0, 0), mN = fv(0, 0, 0), mP = fv(0, 0, 0);
+    const bool failed = m.epsilonFail != -1.0f && maxStrain > m.epsilonFail;
+    const bool lvvIn = lvv;
+    if (!failed) {
+        fN = fv(stress * m.transverseArea, m.b1 * p2.y - m.b2 * (a1v.z + a2v.z), m.b1 * p2.z + m.b2 * (a1v.y + a2v.y));
+        fP = fv(-fN.x, -fN.y, -fN.z);
+        mN = fv(m.a2 * (a2v.x - a1v.x), -m.b2 * p2.z - m.b3 * (2 * a1v.y + a2v.y), m.b2 * p2.y - m.b3 * (2 * a1v.z + a2v.z));
+        mP = fv(m.a2 * (a1v.x - a2v.x), -m.b2 * p2.z - m.b3 * (a1v.y + 2 * a2v.y), m.b2 * p2.y - m.b3 * (a1v.z + 2 * a2v.z));
+        if (lvvIn) {
+            const fvec dP = fv(0.5f * (p2.x - ls[MS_P2X * S + slot]), 0.5f * (p2.y - ls[MS_P2Y * S + slot]),
+                               0.5f * (p2.z - ls[MS_P2Z * S + slot]));
+            const fvec d1 = fv(0.5f * (a1v.x - ls[MS_A1X * S + slot]), 0.5f * (a1v.y - ls[MS_A1Y * S + slot]),
+                               0.5f * (a1v.z - ls[MS_A1Z * S + slot]));
+            const fvec d2 = fv(0.5f * (a2v.x - ls[MS_A2X * S + slot]), 0.5f * (a2v.y - ls[MS_A2Y * S + slot]),
+                               0.5f * (a2v.z - ls[MS_A2Z * S + slot]));
+            const float dmN = s.vmats[s.vmat[n]].dampNum / s.prevDt;
+            const float dmP = s.vmats[s.vmat[p]].dampNum / s.prevDt;
+            const fvec pc = fv(m.sqA1 * dP.x, m.sqB1 * dP.y - m.sqB2xFMp * (d1.z + d2.z), m.sqB1 * dP.z + m.sqB2xFMp * (d1.y + d2.y));
+            fN.x += dmN * pc.x; fN.y += dmN * pc.y; fN.z += dmN * pc.z;
+            fP.x -= dmP * pc.x; fP.y -= dmP * pc.y; fP.z -= dmP * pc.z;
+            const float hN = 0.5f * dmN, hP = 0.5f * dmP;
+            mN.x -= hN * (-m.sqA2xIp * (d2.x - d1.x));
+            mN.y -= hN * (m.sqB2xFMp * dP.z + m.sqB3xIp * (2 * d1.y + d2.y));
+            mN.z -= hN * (-m.sqB2xFMp * dP.y + m.sqB3xIp * (2 * d1.z + d2.z));
+            mP.x -= hP * (m.sqA2xIp * (d2.x - d1.x));
+            mP.y -= hP * (m.sqB2xFMp * dP.z + m.sqB3xIp * (d1.y + 2 * d2.y));
+            mP.z -= hP * (-m.sqB2xFMp * dP.y + m.sqB3xIp * (d1.z + 2 * d2.z));
+        } else {
+            lvv = true;
+        }
+    }
+    if (wantPos) {
+        const fquat q2 = {(float)a2.w, (float)a2.x, (float)a2.y, (float)a2.z};
+        fvec f = f_qrotinv(q2, fP), mm = f_qrotinv(q2, mP);
+        dvec fo = toAxisOriginal(axis, dv(f.x, f.y, f.z)), mo = toAxisOriginal(axis, dv(mm.x, mm.y, mm.z));
+        *outF = fv((float)fo.x, (float)fo.y, (float)fo.z);
+        *outM = fv((float)mo.x, (float)mo.y, (float)mo.z);
+    } else {
+        fvec f = fN, mm = mN;
+        if (!small) {
+            const fquat q1 = {(float)a1.w, (float)a1.x, (float)a1.y, (float)a1.z};
+            f = f_qrotinv(q1, fN);
+            mm = f_qrotinv(q1, mN);
+        }
+        dvec fo = toAxisOriginal(axis, dv(f.x, f.y, f.z)), mo = toAxisOriginal(axis, dv(mm.x, mm.y, mm.z));
+        *outF = fv((float)fo.x, (float)fo.y, (float)fo.z);
+        *outM = fv((float)mo.x, (float)mo.y, (float)mo.z);
+    }
+    if (owner) {
+        float* o = s.lsOut;
+        o[MS_P2X * S + slot] = p2.x; o[MS_P2Y * S + slot] = p2.y; o[MS_P2Z * S + slot] = p2.z;
+        o[MS_A1X * S + slot] = a1v.x; o[MS_A1Y * S + slot] = a1v.y; o[MS_A1Z * S + slot] = a1v.z;
+        o[MS_A2X * S + slot] = a2v.x; o[MS_A2Y * S + slot] = a2v.y; o[MS_A2Z * S + slot] = a2v.z;
+        o[MS_MAX * S + slot] = maxStrain; o[MS_OFF * S + slot] = strainOffset;
+        o[MS_STRAIN * S + slot] = axialStrain; o[MS_STRESS * S + slot] = stress;
+        s.lflOut[slot] = (lvv ? LF_LVV : 0u) | (small ? LF_SMALL : 0u);
+        if (axialStrain > 100) atomicOr(&s.div[s.k], 1);
+    }
+}
+
+__device__ __forceinline__ double m_bsa(const VoxMat& m, float temp) {
+    const float f = 1 + temp * m.alphaCTE;
+    return (f * m.sizeX + f * m.sizeY + f * m.sizeZ) / 3.0;
+}
+
+__global__ void __launch_bounds__(256) k_step_mixed(MixPtrs s) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s.k > 0 && s.div[s.k - 1]) { /* an earlier step of this launch diverged */
+        if (i == 0) s.div[s.k] = 1;
+        return;
+    }
+    const int nv = s.nv;
+    if (i >= nv) return;
+    const double* __restrict__ vd = s.vdIn;
+    const dvec pos0 = dv(vd[VPX * nv + i], vd[VPY * nv + i], vd[VPZ * nv + i]);
+    const dquat q0 = dq(vd[VQW * nv + i], vd[VQX * nv + i], vd[VQY * nv + i], vd[VQZ * nv + i]);
+    uint32_t flags = s.vflIn[i];
+
+    /* link phase for this voxel's <= 6 links, summed in direction order */
+    dvec lf = dv(0.0, 0.0, 0.0), lm = dv(0.0, 0.0, 0.0);
+    for (int dir = 0; dir < 6; dir++) {
+        const int o = s.vnbr[dir * nv + i];
+        if (o < 0) continue;
+        const int axis = dir >> 1;
+        const bool neg = (dir & 1) == 0; /* this voxel is the link's negative end */
+        const dvec po = dv(vd[VPX * nv + o], vd[VPY * nv + o], vd[VPZ * nv + o]);
+        const dquat qo = dq(vd[VQW * nv + o], vd[VQX * nv + o], vd[VQY * nv + o], vd[VQZ * nv + o]);
+        fvec f, m;
+        if (neg) link_mixed(s, axis * nv + i, i, o, axis, false, true, pos0, q0, po, qo, &f, &m);
+        else link_mixed(s, axis * nv + o, o, i, axis, true, false, po, qo, pos0, q0, &f, &m);
+        lf.x += f.x; lf.y += f.y; lf.z += f.z;
+        lm.x += m.x; lm.y += m.y; lm.z += m.z;
+    }
+
+    const VoxMat m = s.vmats[s.vmat[i]];
+    const int e = s.vext[i];
+    const ExtRec* ex = e >= 0 ? &s.exts[e] : nullptr;
+    double* __restrict__ out = s.vdOut;
+    const float dt = s.dt;
+    if (ex && (ex->dof & 0x3F) == 0x3F) {
+        const double sz = m.nomSize;
+        out[VPX * nv + i] = s.vijk[i] * sz + ex->tx;
+        out[VPY * nv + i] = s.vijk[nv + i] * sz + ex->ty;
+        out[VPZ * nv + i] = s.vijk[2 * nv + i] * sz + ex->tz;
+        out[VQW * nv + i] = ex->qw; out[VQX * nv + i] = ex->qx; out[VQY * nv + i] = ex->qy; out[VQZ * nv + i] = ex->qz;
+        out[VLX * nv + i] = 0.0; out[VLY * nv + i] = 0.0; out[VLZ * nv + i] = 0.0;
+        out[VAX * nv + i] = 0.0; out[VAY * nv + i] = 0.0; out[VAZ * nv + i] = 0.0;
+        s.vflOut[i] = flags;
+        return;
+    }
+    dvec pos = pos0, lin = dv(vd[VLX * nv + i], vd[VLY * nv + i], vd[VLZ * nv + i]);
+    dvec ang = dv(vd[VAX * nv + i], vd[VAY * nv + i], vd[VAZ * nv + i]);
+    dquat q = q0;
+
+    dvec t = qrot(q, lf);
+    if (ex) { t.x += ex->fx; t.y += ex->fy; t.z += ex->fz; }
+    t.x -= m.globalTransC * (m.massInverse * lin.x);
+    t.y -= m.globalTransC * (m.massInverse * lin.y);
+    t.z -= m.globalTransC * (m.massInverse * lin.z);
+    t.z += m.gravityForce;
+    if ((flags & VF_COLLISIONS) && s.colCnt) {
+        /* contacts (VX_Collision.cpp:42-56) from this voxel's side, FP32 */
+        const int cnt = s.colCnt[i], off = s.colOff[i];
+        const double bsai = m_bsa(m, s.vtemp[i]);
+        for (int k = 0; k < cnt; k++) {
+            const int j = s.partners[off + k];
+            const VoxMat mj = s.vmats[s.vmat[j]];
+            const fvec o3 = fv((float)(vd[VPX * nv + j] - pos.x), (float)(vd[VPY * nv + j] - pos.y), (float)(vd[VPZ * nv + j] - pos.z));
+            const float len = sqrtf(o3.x * o3.x + o3.y * o3.y + o3.z * o3.z);
+            const float rel = (float)((bsai + m_bsa(mj, s.vtemp[j])) * 0.625f) - len;
+            if (rel > 0 && len > 0) {
+                const float inv = 1.0f / len;
+                const fvec u = fv(o3.x * inv, o3.y * inv, o3.z * inv);
+                const float vi = (float)(m.massInverse * (lin.x * u.x + lin.y * u.y + lin.z * u.z));
+                const float vj = (float)(mj.massInverse * (vd[VLX * nv + j] * u.x + vd[VLY * nv + j] * u.y + vd[VLZ * nv + j] * u.z));
+                const float stiff = 2.0f / (1.0f / m.penetrationStiff + 1.0f / mj.penetrationStiff);
+                const float damp = 0.5f * (m.collisionTransC + mj.collisionTransC);
+                const float mag = stiff * rel + damp * (vi - vj);
+                t.x -= u.x * mag; t.y -= u.y * mag; t.z -= u.z * mag;
+            }
+        }
+    }
+    dvec cur = t;
+    const bool floorOn = (flags & VF_FLOOR) != 0;
+    float pen = 0.0f;
+    if (floorOn) {
+        pen = (float)(m_bsa(m, s.vtemp[i]) / 2 - m.nomSize / 2 - pos.z);
+        if (pen >= 0) {
+            const dvec vel = dv(m.massInverse * lin.x, m.massInverse * lin.y, m.massInverse * lin.z);
+            const float normalForce = m.penetrationStiff * pen;
+            cur.z += normalForce - m.collisionTransC * vel.z;
+            if (flags & VF_STATIC) {
+                const float sf = (float)(cur.x * cur.x + cur.y * cur.y);
+                const float ff = (m.muStatic * normalForce) * (m.muStatic * normalForce);
+                if (sf > ff) flags &= ~VF_STATIC;
+            } else {
+                const double l = sqrt(vel.x * vel.x + vel.y * vel.y);
+                if (l > 0) {
+                    const double fk = (m.muKinetic * normalForce) / l;
+                    cur.x -= fk * vel.x;
+                    cur.y -= fk * vel.y;
+                }
+            }
+        } else {
+            flags &= ~VF_STATIC;
+        }
+    }
+    const dvec fric = dv(cur.x - t.x, cur.y - t.y, cur.z - t.z);
+    lin.x += dt * cur.x; lin.y += dt * cur.y; lin.z += dt * cur.z;
+    const float dtm = dt * m.massInverse;
+    dvec trn = dv(dtm * lin.x, dtm * lin.y, dtm * lin.z);
+    if (floorOn && pen >= 0) {
+        const double work = fric.x * trn.x + fric.y * trn.y;
+        const double hKe = 0.5 * m.massInverse * (lin.x * lin.x + lin.y * lin.y);
+        if (hKe + work <= 0) flags |= VF_STATIC;
+        if (flags & VF_STATIC) { lin.x = lin.y = 0.0; trn.x = trn.y = 0.0; }
+    } else {
+        flags &= ~VF_STATIC;
+    }
+    pos.x += trn.x; pos.y += trn.y; pos.z += trn.z;
+
+    dvec mo = qrot(q, lm);
+    if (ex) { mo.x += ex->mx; mo.y += ex->my; mo.z += ex->mz; }
+    mo.x -= m.globalRotC * (m.momentInertiaInverse * ang.x);
+    mo.y -= m.globalRotC * (m.momentInertiaInverse * ang.y);
+    mo.z -= m.globalRotC * (m.momentInertiaInverse * ang.z);
+    ang.x += dt * mo.x; ang.y += dt * mo.y; ang.z += dt * mo.z;
+    const float dti = dt * m.momentInertiaInverse;
+    q = qmul(qfromrv(dv(dti * ang.x, dti * ang.y, dti * ang.z)), q);
+    if (ex) {
+        const double sz = m.nomSize;
+        const int dof = ex->dof;
+        if (dof & 1) { pos.x = s.vijk[i] * sz + ex->tx; lin.x = 0.0; }
+        if (dof & 2) { pos.y = s.vijk[nv + i] * sz + ex->ty; lin.y = 0.0; }
+        if (dof & 4) { pos.z = s.vijk[2 * nv + i] * sz + ex->tz; lin.z = 0.0; }
+        if (dof & 56) {
+            if ((dof & 56) == 56) {
+                q = dq(ex->qw, ex->qx, ex->qy, ex->qz);
+                ang = dv(0.0, 0.0, 0.0);
+            } else {
+                dvec rv = qtorv(q);
+                if (dof & 8) { rv.x = 0.0; ang.x = 0.0; }
+                if (dof & 16) { rv.y = 0.0; ang.y = 0.0; }
+                if (dof & 32) { rv.z = 0.0; ang.z = 0.0; }
+                q = qfromrv(rv);
+            }
+        }
+    }
+    out[VPX * nv + i] = pos.x; out[VPY * nv + i] = pos.y; out[VPZ * nv + i] = pos.z;
+    out[VLX * nv + i] = lin.x; out[VLY * nv + i] = lin.y; out[VLZ * nv + i] = lin.z;
+    out[VAX * nv + i] = ang.x; out[VAY * nv + i] = ang.y; out[VAZ * nv + i] = ang.z;
+    out[VQW * nv + i] = q.w; out[VQX * nv + i] = q.x; out[VQY * nv + i] = q.y; out[VQZ * nv + i] = q.z;
+    s.vflOut[i] = flags;
+}
+
+void launch_step_mixed(const MixPtrs& s, hipStream_t st) {
+    if (s.nv <= 0) return;
+    hipLaunchKernelGGL(k_step_mixed, dim3((unsigned)((s.nv + 255) / 256)), dim3(256), 0, st, s);
+}
+
+}  // namespace vx

@@ -20,6 +20,14 @@ namespace vx {
 
 void launch_links_fp64(const DevPtrs& s, hipStream_t st);
 void launch_voxels_fp64(const DevPtrs& s, float dt, hipStream_t st);
+void launch_col_stale(const DevPtrs& s, const ContactPtrs& c, hipStream_t st);
+void launch_col_keys(const DevPtrs& s, const ContactPtrs& c, int nbuckets, hipStream_t st);
+hipError_t col_sort(const ContactPtrs& c, void* tmp, size_t* tmpBytes, hipStream_t st);
+hipError_t col_scan(const ContactPtrs& c, void* tmp, size_t* tmpBytes, hipStream_t st);
+void launch_col_buckets(const ContactPtrs& c, hipStream_t st);
+void launch_col_count(const DevPtrs& s, const ContactPtrs& c, hipStream_t st);
+void launch_col_fill(const DevPtrs& s, const ContactPtrs& c, hipStream_t st);
+void launch_col_forces(const DevPtrs& s, const ContactPtrs& c, hipStream_t st);
 
 /* ================================================================ materials */
 /* CVX_Material::stress (src/VX_Material.cpp:165-195) */
@@ -380,7 +388,14 @@ void Sim::resetVoxel(int i) {
 /* CVX_Voxel::updateSurface (src/VX_Voxel.cpp:376-381) */
 void Sim::updateSurface(int i) {
     bool interior = true;
-    for (int d = 0; d < 6; d++) if (vlinks[i][d] < 0) interior = false;
+    for (int d = 0; d < 6; d++) {
+        if (vlinks[i][d] < 0) {
+            interior = false;
+            vflags[i] &= ~(VF_LINK0 << d);
+        } else {
+            vflags[i] |= VF_LINK0 << d;
+        }
+    }
     if (interior) vflags[i] |= VF_INTERIOR; else vflags[i] &= ~VF_INTERIOR;
 }
 
@@ -573,6 +588,14 @@ static hipError_t up(T* dst, const std::vector<T>& src, hipStream_t st) {
     return hipMemcpyAsync(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, st);
 }
 
+/* device slot of a linksList link: axis-major, indexed by the negative voxel */
+template <class T>
+static std::vector<T> to_slots(const std::vector<T>& v, const std::vector<int>& slot, size_t nslots, T fill) {
+    std::vector<T> r(nslots, fill);
+    for (size_t l = 0; l < slot.size(); l++) r[slot[l]] = v[l];
+    return r;
+}
+
 /* push every dirty host array to the device */
 bool Sim::syncUp() {
     if (!ensureDevice()) return false;
@@ -580,11 +603,12 @@ bool Sim::syncUp() {
     if (dirty & D_TOPO) {
         if (n > nvCap || L > nlCap || !bvd.p) {
             int vc = std::max(n, 1), lc = std::max(L, 1);
-            if (!bvd.ensure((size_t)VD_COUNT * vc * 8) || !bvflags.ensure((size_t)vc * 4) || !bvmat.ensure((size_t)vc * 4) ||
-                !bvext.ensure((size_t)vc * 4) || !bvprevdt.ensure((size_t)vc * 4) || !bvtemp.ensure((size_t)vc * 4) ||
-                !bvlink.ensure((size_t)6 * vc * 4) || !bvijk.ensure((size_t)3 * vc * 4) || !blneg.ensure((size_t)lc * 4) ||
-                !blpos.ensure((size_t)lc * 4) || !blmeta.ensure((size_t)lc * 4) || !bld.ensure((size_t)LD_COUNT * lc * 8) ||
-                !blf.ensure((size_t)LF_COUNT * lc * 4) || !blflags.ensure((size_t)lc * 4)) {
+            const size_t sc = (size_t)3 * vc; /* link slots */
+            if (!bvd.ensure((size_t)VD_COUNT * vc * 8) || !bvf.ensure((size_t)VF_COUNT * vc * 8) ||
+                !bvflags.ensure((size_t)vc * 4) || !bvmat.ensure((size_t)vc * 4) || !bvext.ensure((size_t)vc * 4) ||
+                !bvprevdt.ensure((size_t)vc * 4) || !bvtemp.ensure((size_t)vc * 4) || !bvijk.ensure((size_t)3 * vc * 4) ||
+                !blpos.ensure(sc * 4) || !blmeta.ensure(sc * 4) || !bld.ensure((size_t)LDEV_COUNT * sc * 8) ||
+                !blf.ensure((size_t)LF_COUNT * sc * 4) || !blflags.ensure(sc * 4)) {
                 err = "device allocation failed";
                 return false;
             }
@@ -592,33 +616,40 @@ bool Sim::syncUp() {
             nlCap = lc;
         }
         dp.nv = n;
-        dp.nl = L;
+        dp.nl = 3 * n;
         dp.vd = (double*)bvd.p;
+        dp.vf = (double*)bvf.p;
         dp.vflags = (uint32_t*)bvflags.p;
         dp.vmat = (int*)bvmat.p;
         dp.vext = (int*)bvext.p;
         dp.vprevdt = (float*)bvprevdt.p;
         dp.vtemp = (float*)bvtemp.p;
-        dp.vlink = (int*)bvlink.p;
         dp.vijk = (int*)bvijk.p;
-        dp.lneg = (int*)blneg.p;
         dp.lpos = (int*)blpos.p;
         dp.lmeta = (int*)blmeta.p;
         dp.ld = (double*)bld.p;
         dp.lf = (float*)blf.p;
         dp.lflags = (uint32_t*)blflags.p;
         dp.ctl = (int*)bctl.p;
-        std::vector<int> tmp((size_t)6 * n);
+        slotOf.resize(L);
+        for (int l = 0; l < L; l++) slotOf[l] = laxis[l] * n + lneg[l];
+        /* voxel across each direction (mixed-precision step) */
+        std::vector<int> nbr((size_t)6 * n, -1);
         for (int v = 0; v < n; v++)
-            for (int d = 0; d < 6; d++) tmp[(size_t)d * n + v] = vlinks[v][d];
-        HIPCHK(up(dp.vlink, tmp, stream));
-        HIPCHK(hipStreamSynchronize(stream));
-        tmp.assign((size_t)3 * n, 0);
+            for (int d = 0; d < 6; d++) {
+                const int l = vlinks[v][d];
+                if (l >= 0) nbr[(size_t)d * n + v] = lneg[l] == v ? lpos[l] : lneg[l];
+            }
+        if (!bvnbr.ensure((size_t)6 * std::max(n, 1) * 4)) { err = "device allocation failed"; return false; }
+        HIPCHK(up((int*)bvnbr.p, nbr, stream));
+        if (precision == 1 && !mixedBuffers()) return false;
+        HIPCHK(hipMemsetAsync(dp.vf, 0, (size_t)VF_COUNT * n * 8, stream));
+        std::vector<int> tmp((size_t)3 * n);
         for (int v = 0; v < n; v++)
             for (int c = 0; c < 3; c++) tmp[(size_t)c * n + v] = vijk[v][c];
         HIPCHK(up(dp.vijk, tmp, stream));
-        HIPCHK(up(dp.lneg, lneg, stream));
-        HIPCHK(up(dp.lpos, lpos, stream));
+        std::vector<int> dpos = to_slots(lpos, slotOf, (size_t)3 * n, -1);
+        HIPCHK(up(dp.lpos, dpos, stream));
         HIPCHK(hipStreamSynchronize(stream));
         dirty |= D_VSTATE | D_LSTATE | D_MATS | D_EXT | D_REST | D_TEMP;
     }
@@ -679,9 +710,9 @@ bool Sim::syncUp() {
         HIPCHK(up((LinkMat*)blmats.p, lm, stream));
         HIPCHK(up((float*)bcs.p, cs, stream));
         HIPCHK(up((float*)bss.p, ss, stream));
-        std::vector<int> vmi(n), meta(L);
+        std::vector<int> vmi(n), meta((size_t)3 * n, 0);
         for (int v = 0; v < n; v++) vmi[v] = vmat[v]->dev;
-        for (int l = 0; l < L; l++) meta[l] = laxis[l] | (lmat[l]->dev << 2);
+        for (int l = 0; l < L; l++) meta[slotOf[l]] = lmat[l]->dev;
         HIPCHK(up(dp.vmat, vmi, stream));
         HIPCHK(up(dp.lmeta, meta, stream));
         HIPCHK(hipStreamSynchronize(stream));
@@ -715,21 +746,32 @@ bool Sim::syncUp() {
         HIPCHK(up(dp.vprevdt, vprevdt, stream));
     }
     if (dirty & D_TEMP) HIPCHK(up(dp.vtemp, vtemp, stream));
+    std::vector<std::vector<double>> stageD;
+    std::vector<std::vector<float>> stageF;
+    const size_t S = (size_t)3 * n;
     if (dirty & D_LSTATE) {
-        for (int c = 0; c < LD_COUNT; c++)
-            if (L) HIPCHK(hipMemcpyAsync(dp.ld + (size_t)c * L, ld[c].data(), (size_t)L * 8, hipMemcpyHostToDevice, stream));
-        for (int c = 0; c < LF_COUNT; c++)
-            if (L) HIPCHK(hipMemcpyAsync(dp.lf + (size_t)c * L, lf[c].data(), (size_t)L * 4, hipMemcpyHostToDevice, stream));
-        HIPCHK(up(dp.lflags, lflags, stream));
+        for (int c = 0; c < LDEV_COUNT; c++) {
+            stageD.push_back(to_slots(ld[c], slotOf, S, 0.0));
+            if (S) HIPCHK(hipMemcpyAsync(dp.ld + c * S, stageD.back().data(), S * 8, hipMemcpyHostToDevice, stream));
+        }
+        for (int c = 0; c < LF_COUNT; c++) {
+            stageF.push_back(to_slots(lf[c], slotOf, S, 0.0f));
+            if (S) HIPCHK(hipMemcpyAsync(dp.lf + c * S, stageF.back().data(), S * 4, hipMemcpyHostToDevice, stream));
+        }
+        std::vector<uint32_t> fl = to_slots(lflags, slotOf, S, 0u);
+        HIPCHK(up(dp.lflags, fl, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        if (precision == 1 && !uploadMixedLinks()) return false;
     } else if (dirty & D_REST) {
-        if (L) HIPCHK(hipMemcpyAsync(dp.ld + (size_t)LREST * L, ld[LREST].data(), (size_t)L * 8, hipMemcpyHostToDevice, stream));
+        stageD.push_back(to_slots(ld[LREST], slotOf, S, 0.0));
+        if (S) HIPCHK(hipMemcpyAsync(dp.ld + LREST * S, stageD.back().data(), S * 8, hipMemcpyHostToDevice, stream));
     }
     HIPCHK(hipStreamSynchronize(stream));
     dirty = 0;
     return true;
 }
 
-/* pull the device-owned dynamic state back into the mirror */
+/* pull the device-owned dynamic state back into the mirror (end forces on demand) */
 bool Sim::syncDown() {
     if (!hostStale) return true;
     const int n = nv(), L = nl();
@@ -737,15 +779,50 @@ bool Sim::syncDown() {
         if (n) HIPCHK(hipMemcpyAsync(vd[c].data(), dp.vd + (size_t)c * n, (size_t)n * 8, hipMemcpyDeviceToHost, stream));
     if (n) HIPCHK(hipMemcpyAsync(vflags.data(), dp.vflags, (size_t)n * 4, hipMemcpyDeviceToHost, stream));
     if (n) HIPCHK(hipMemcpyAsync(vprevdt.data(), dp.vprevdt, (size_t)n * 4, hipMemcpyDeviceToHost, stream));
-    for (int c = 0; c < LD_COUNT; c++) {
-        if (c == LREST || !L) continue; /* rest lengths are host-owned */
-        HIPCHK(hipMemcpyAsync(ld[c].data(), dp.ld + (size_t)c * L, (size_t)L * 8, hipMemcpyDeviceToHost, stream));
+    const size_t S = (size_t)3 * n;
+    std::vector<double> bufD((size_t)LREST * S);
+    std::vector<float> bufF((size_t)LF_COUNT * S);
+    std::vector<uint32_t> bufU(S);
+    if (L) {
+        HIPCHK(hipMemcpyAsync(bufD.data(), dp.ld, (size_t)LREST * S * 8, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(bufF.data(), dp.lf, (size_t)LF_COUNT * S * 4, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(bufU.data(), dp.lflags, S * 4, hipMemcpyDeviceToHost, stream));
     }
-    for (int c = 0; c < LF_COUNT; c++)
-        if (L) HIPCHK(hipMemcpyAsync(lf[c].data(), dp.lf + (size_t)c * L, (size_t)L * 4, hipMemcpyDeviceToHost, stream));
-    if (L) HIPCHK(hipMemcpyAsync(lflags.data(), dp.lflags, (size_t)L * 4, hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
+    for (int l = 0; l < L; l++) {
+        const size_t d = slotOf[l];
+        for (int c = 0; c < LREST; c++) ld[c][l] = bufD[c * S + d]; /* rest lengths are host-owned */
+        for (int c = 0; c < LF_COUNT; c++) lf[c][l] = bufF[c * S + d];
+        lflags[l] = bufU[d];
+    }
     hostStale = false;
+    if (precision == 1) {
+        /* the fused mixed step keeps no end forces: the mirror keeps the last FP64 ones */
+        if (!downloadMixedLinks()) return false;
+        forcesStale = false;
+        return true;
+    }
+    forcesStale = true;
+    return true;
+}
+
+/* the four end vectors of every link, read back from the voxels' force slots */
+bool Sim::syncForces() {
+    if (!syncDown()) return false;
+    if (!forcesStale) return true;
+    const int n = nv(), L = nl();
+    std::vector<double> f((size_t)VF_COUNT * n);
+    if (n) HIPCHK(hipMemcpy(f.data(), dp.vf, (size_t)VF_COUNT * n * 8, hipMemcpyDeviceToHost));
+    for (int l = 0; l < L; l++) {
+        const int a = laxis[l], ng = lneg[l], ps = lpos[l];
+        for (int c = 0; c < 3; c++) {
+            ld[LFNX + c][l] = f[vf_index(2 * a, c, n, ng)];
+            ld[LFPX + c][l] = f[vf_index(2 * a + 1, c, n, ps)];
+            ld[LMNX + c][l] = f[vf_index(2 * a, 3 + c, n, ng)];
+            ld[LMPX + c][l] = f[vf_index(2 * a + 1, 3 + c, n, ps)];
+        }
+    }
+    forcesStale = false;
     return true;
 }
 
@@ -782,15 +859,20 @@ int Sim::step(float dt, int nsteps, int* divergedAt) {
             return -1;
         }
     }
-    if (collisions) {
-        err = "self-collision is not yet supported by the device step";
-        return -1;
-    }
     if (nv() == 0) {
         for (int k = 0; k < nsteps; k++) currentTime += dt;
         return nsteps;
     }
     if (!syncUp()) return -1;
+    if (precision == 1) return stepMixed(dt, nsteps, divergedAt);
+    if (collisions) {
+        const int d = stepWithContacts(dt, nsteps, divergedAt);
+        if (d > 0) lastDt = dt;
+        return d;
+    }
+    dp.colCnt = nullptr;
+    dp.colOff = nullptr;
+    dp.colF = nullptr;
     hipError_t e = hipMemsetAsync(dp.ctl, 0, 4 * sizeof(int), stream);
     if (e != hipSuccess) { fail("hipMemsetAsync", e); return -1; }
     if (profiling) {
@@ -830,8 +912,187 @@ int Sim::step(float dt, int nsteps, int* divergedAt) {
     }
     hostStale = true;
     const int done = ctlHost[2];
+    if (done > 0) lastDt = dt;
     for (int k = 0; k < done; k++) currentTime += dt;
     if (ctlHost[0] && divergedAt) *divergedAt = done;
+    return done;
+}
+
+/* ================================================================ contacts */
+/* per-voxel contact arrays, (re)sized with the topology */
+bool Sim::contactBuffers() {
+    const int n = nv();
+    const size_t vc = (size_t)std::max(n, 1);
+    if (!bcLast.ensure(3 * vc * 4) || !bcMask.ensure(8 * vc * 4) || !bcKeys.ensure(vc * 4) || !bcVals.ensure(vc * 4) ||
+        !bcKeysS.ensure(vc * 4) || !bcValsS.ensure(vc * 4) || !bcCount.ensure(vc * 4) || !bcOff.ensure(vc * 4) ||
+        !bcFlag.ensure(16) || !bcOffBit.ensure(11 * 11 * 11)) {
+        err = "device allocation failed";
+        return false;
+    }
+    int nb = 1024;
+    while (nb < 2 * n) nb *= 2;
+    if (!bcBStart.ensure((size_t)nb * 4) || !bcBEnd.ensure((size_t)nb * 4)) {
+        err = "device allocation failed";
+        return false;
+    }
+    nBuckets = nb;
+    /* bit of each lattice offset of the L1 ball of radius 5 (231 offsets) */
+    std::vector<unsigned char> tab(11 * 11 * 11, 255);
+    int bit = 0;
+    for (int x = -5; x <= 5; x++)
+        for (int y = -5; y <= 5; y++)
+            for (int z = -5; z <= 5; z++)
+                if (abs(x) + abs(y) + abs(z) <= 5) tab[((x + 5) * 11 + (y + 5)) * 11 + (z + 5)] = (unsigned char)bit++;
+    HIPCHK(hipMemcpy(bcOffBit.p, tab.data(), tab.size(), hipMemcpyHostToDevice));
+    const float watchRadiusVx = 2 * 0.75f + 1.0f;                /* 2*boundingRadius + watchDistance */
+    const float watchRadiusMm = (float)(voxSize * watchRadiusVx);
+    const float recalcDist = (float)(voxSize * 1.0f / 2);
+    cp.nv = n;
+    cp.last = (float*)bcLast.p;
+    cp.nearMask = (const uint32_t*)bcMask.p;
+    cp.offBit = (const unsigned char*)bcOffBit.p;
+    cp.keys = (uint32_t*)bcKeys.p;
+    cp.vals = (int*)bcVals.p;
+    cp.keysSorted = (uint32_t*)bcKeysS.p;
+    cp.valsSorted = (int*)bcValsS.p;
+    cp.bucketStart = (int*)bcBStart.p;
+    cp.bucketEnd = (int*)bcBEnd.p;
+    cp.hashMask = (uint32_t)(nb - 1);
+    cp.pairCount = (int*)bcCount.p;
+    cp.pairOff = (int*)bcOff.p;
+    cp.flag = (int*)bcFlag.p;
+    cp.cellInv = 1.0 / (1.01 * (double)watchRadiusMm);
+    cp.threshSq = watchRadiusMm * watchRadiusMm;
+    cp.recalcSq = recalcDist * recalcDist;
+    HIPCHK(hipMemset(cp.pairCount, 0, vc * 4));
+    HIPCHK(hipMemset(cp.last, 0, 3 * vc * 4));
+    return true;
+}
+
+/* CVX_Voxel::generateNearby for every surface voxel (src/VX_Voxel.cpp:395-418): breadth-first
+   over the links to depth 5, recorded as bits of the reached voxels' lattice offsets */
+bool Sim::nearMasks() {
+    const int n = nv();
+    std::vector<uint32_t> mask((size_t)8 * n, 0u);
+    std::vector<int> stamp(n, -1), cur, next;
+    auto bitOf = [](int x, int y, int z) {
+        int b = 0;
+        for (int a = -5; a <= 5; a++)
+            for (int c = -5; c <= 5; c++)
+                for (int d = -5; d <= 5; d++)
+                    if (abs(a) + abs(c) + abs(d) <= 5) {
+                        if (a == x && c == y && d == z) return b;
+                        b++;
+                    }
+        return -1;
+    };
+    static int table[11][11][11];
+    static bool init = false;
+    if (!init) {
+        for (int a = -5; a <= 5; a++)
+            for (int c = -5; c <= 5; c++)
+                for (int d = -5; d <= 5; d++) table[a + 5][c + 5][d + 5] = bitOf(a, c, d);
+        init = true;
+    }
+    for (int v = 0; v < n; v++) {
+        if (vflags[v] & VF_INTERIOR) continue;
+        cur.assign(1, v);
+        stamp[v] = v;
+        for (int depth = 0; depth < 5; depth++) {
+            next.clear();
+            for (int u : cur)
+                for (int d = 0; d < 6; d++) {
+                    const int l = vlinks[u][d];
+                    if (l < 0) continue;
+                    const int w = lneg[l] == u ? lpos[l] : lneg[l];
+                    if (stamp[w] == v) continue;
+                    stamp[w] = v;
+                    next.push_back(w);
+                    const int b = table[vijk[w][0] - vijk[v][0] + 5][vijk[w][1] - vijk[v][1] + 5][vijk[w][2] - vijk[v][2] + 5];
+                    mask[(size_t)8 * v + (b >> 5)] |= 1u << (b & 31);
+                }
+            cur.swap(next);
+        }
+    }
+    HIPCHK(hipMemcpyAsync(bcMask.p, mask.data(), mask.size() * 4, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    return true;
+}
+
+/* CVoxelyze::regenerateCollisions (src/Voxelyze.cpp:725-750) on the grid */
+bool Sim::regenerateCollisions() {
+    const int n = nv();
+    launch_col_keys(dp, cp, nBuckets, stream);
+    size_t tmp = 0;
+    HIPCHK(col_sort(cp, nullptr, &tmp, stream));
+    size_t tmp2 = 0;
+    HIPCHK(col_scan(cp, nullptr, &tmp2, stream));
+    if (!bcTmp.ensure(std::max(tmp, tmp2))) { err = "device allocation failed"; return false; }
+    tmp = bcTmp.bytes;
+    HIPCHK(col_sort(cp, bcTmp.p, &tmp, stream));
+    launch_col_buckets(cp, stream);
+    launch_col_count(dp, cp, stream);
+    tmp2 = bcTmp.bytes;
+    HIPCHK(col_scan(cp, bcTmp.p, &tmp2, stream));
+    int lastOff = 0, lastCnt = 0;
+    HIPCHK(hipMemcpyAsync(&lastOff, cp.pairOff + (n - 1), 4, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(&lastCnt, cp.pairCount + (n - 1), 4, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    colEntries = lastOff + lastCnt;
+    const size_t ce = (size_t)std::max(colEntries, 1);
+    if (!bcPartners.ensure(ce * 4) || !bcF.ensure(3 * ce * 4)) { err = "device allocation failed"; return false; }
+    cp.partners = (int*)bcPartners.p;
+    cp.cf = (float*)bcF.p;
+    launch_col_fill(dp, cp, stream);
+    HIPCHK(hipGetLastError());
+    collisionsStale = false;
+    return true;
+}
+
+/* doTimeStep with self-collision: link phase, CVoxelyze::updateCollisions, voxel phase.
+   The host inspects the divergence and staleness flags between the phases of each step. */
+int Sim::stepWithContacts(float dt, int nsteps, int* divergedAt) {
+    if (cp.nv != nv() || !bcLast.p) {
+        if (!contactBuffers()) return -1;
+        nearbyStale = true;
+    }
+    hipError_t e = hipMemsetAsync(dp.ctl, 0, 4 * sizeof(int), stream);
+    if (e != hipSuccess) { fail("hipMemsetAsync", e); return -1; }
+    int done = 0;
+    for (int k = 0; k < nsteps; k++) {
+        launch_links_fp64(dp, stream);
+        e = hipMemsetAsync(cp.flag, 0, 4, stream);
+        if (e != hipSuccess) { fail("hipMemsetAsync", e); return -1; }
+        launch_col_stale(dp, cp, stream);
+        e = hipMemcpyAsync(ctlHost, dp.ctl, 4 * sizeof(int), hipMemcpyDeviceToHost, stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(ctlHost + 3, cp.flag, 4, hipMemcpyDeviceToHost, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) { fail("step", e); return -1; }
+        if (ctlHost[0]) {
+            hostStale = true;
+            if (divergedAt) *divergedAt = k;
+            return done;
+        }
+        if (nearbyStale) {
+            if (!nearMasks()) return -1;
+            nearbyStale = false;
+            collisionsStale = true;
+        }
+        if (ctlHost[3]) collisionsStale = true;
+        if (collisionsStale && !regenerateCollisions()) return -1;
+        dp.colCnt = cp.pairCount;
+        dp.colOff = cp.pairOff;
+        dp.colF = cp.cf;
+        launch_col_forces(dp, cp, stream);
+        launch_voxels_fp64(dp, dt, stream);
+        e = hipGetLastError();
+        if (e != hipSuccess) { fail("kernel launch", e); return -1; }
+        currentTime += dt;
+        done++;
+    }
+    e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) { fail("step", e); return -1; }
+    hostStale = true;
     return done;
 }
 

@@ -22,6 +22,28 @@
 
 namespace vx {
 
+struct ContactPtrs {
+    int nv;
+    float* last;
+    const uint32_t* nearMask;
+    const unsigned char* offBit;
+    uint32_t* keys;
+    int* vals;
+    uint32_t* keysSorted;
+    int* valsSorted;
+    int* bucketStart;
+    int* bucketEnd;
+    uint32_t hashMask;
+    int* pairCount;
+    int* pairOff;
+    int* partners;
+    float* cf;
+    int* flag;
+    double cellInv;
+    float threshSq;
+    float recalcSq;
+};
+
 /* CVX_Material + CVX_MaterialVoxel + CVX_MaterialLink (include/VX_Material*.h) */
 struct Mat {
     bool linear = true;
@@ -110,17 +132,39 @@ struct Sim {
     float dtCache = 0.0f;
     hipStream_t stream = nullptr;
     DevPtrs dp{};
-    DevBuf bvd, bvflags, bvmat, bvext, bvprevdt, bvtemp, bvlink, bvijk;
-    DevBuf blneg, blpos, blmeta, bld, blf, blflags;
+    DevBuf bvd, bvf, bvflags, bvmat, bvext, bvprevdt, bvtemp, bvijk;
+    DevBuf blpos, blmeta, bld, blf, blflags;
     DevBuf bvmats, blmats, bcs, bss, bexts, bctl;
     int nvCap = 0, nlCap = 0;
     std::vector<int> extIndex; /* compact external index per voxel, -1 none */
+    std::vector<int> slotOf;   /* linksList position -> device link slot */
+    bool forcesStale = false;  /* host end-force view older than the device slots */
     int* ctlHost = nullptr;
+    /* self-collision (src/Voxelyze.cpp:670-750) */
+    bool nearbyStale = true, collisionsStale = true;
+    int colEntries = 0; /* listed partner entries = 2 x pairs */
+    int nBuckets = 0;
+    ContactPtrs cp{};
+    DevBuf bcLast, bcMask, bcOffBit, bcKeys, bcVals, bcKeysS, bcValsS, bcBStart, bcBEnd, bcCount, bcOff, bcPartners,
+        bcF, bcFlag, bcTmp;
+    bool contactBuffers();
+    bool nearMasks();
+    bool regenerateCollisions();
+    int stepWithContacts(float dt, int nsteps, int* divergedAt);
     /* kernel timing */
     bool profiling = false;
     std::vector<hipEvent_t> events;
-    double profMs[3] = {0, 0, 0};
-    int profCount[3] = {0, 0, 0};
+    double profMs[4] = {0, 0, 0, 0};
+    int profCount[4] = {0, 0, 0, 0};
+    /* mixed precision (vx_mixed.hip): second voxel buffer, FP32 link-state pairs */
+    DevBuf bvdAlt, bvflAlt, bvnbr, bms[2], bmfl[2], bdiv;
+    int lcur = 0;          /* mixed link-state buffer that holds the current state */
+    float lastDt = 0.0f;   /* dt of the last completed step: every voxel's previousDt */
+    bool mixedBuffers();
+    bool uploadMixedLinks();
+    bool downloadMixedLinks();
+    void swapVoxelBuffers();
+    int stepMixed(float dt, int nsteps, int* divergedAt);
 
     ~Sim();
     int nv() const { return (int)vmat.size(); }
@@ -143,12 +187,13 @@ struct Sim {
     Ext* external(int vi);
     float recommendedTimeStep();
     void materialsChanged() { dirty |= D_MATS; dtCacheValid = false; }
-    void topologyChanged() { dirty |= D_TOPO; dtCacheValid = false; }
+    void topologyChanged() { dirty |= D_TOPO; dtCacheValid = false; nearbyStale = collisionsStale = true; }
 
     /* device */
     bool ensureDevice();
     bool syncUp();
     bool syncDown();
+    bool syncForces();
     bool readVoxelRange(int first, int count, double* pos, double* orient, double* lin, double* ang, uint32_t* flags);
     int step(float dt, int nsteps, int* divergedAt);
     bool fail(const char* what, hipError_t e);

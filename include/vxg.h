@@ -110,16 +110,18 @@ float vxg_ambient_temperature(vxg_sim* s);           /* Voxelyze.h:122 */
 /* CVoxelyze::doTimeStep(float dt=-1)                include/Voxelyze.h:82, src/Voxelyze.cpp:251-284
    Returns 1, or 0 when a link's axial strain exceeded 100 (divergence). */
 int vxg_do_time_step(vxg_sim* s, float dt);
-/* nsteps consecutive doTimeStep(dt) calls with no host work in between (captured into one
-   HIP graph on the device).  Returns the number of steps that completed; when a step
-   diverges the loop stops there and *diverged_at (if non-NULL) receives its 0-based index,
-   else -1.  Equivalent to: for k<nsteps: if(!doTimeStep(dt)) break; */
+/* nsteps consecutive doTimeStep(dt) calls with no host work in between (one launch
+   sequence on the library's stream).  Returns the number of steps that completed; when a
+   step diverges the loop stops there and *diverged_at (if non-NULL) receives its 0-based
+   index, else -1.  Equivalent to: for k<nsteps: if(!doTimeStep(dt)) break;
+   With several robots (below) every robot steps with dt and stops at its own divergence;
+   the return value is the smallest number of steps any robot completed. */
 int vxg_step(vxg_sim* s, float dt, int nsteps, int* diverged_at);
 /* CVoxelyze::recommendedTimeStep()                  include/Voxelyze.h:83, src/Voxelyze.cpp:286-311 */
 float vxg_recommended_time_step(vxg_sim* s);
 /* CVoxelyze::resetTime()                            include/Voxelyze.h:84, src/Voxelyze.cpp:313-321 */
 void vxg_reset_time(vxg_sim* s);
-/* CVoxelyze::currentTime (private member, Voxelyze.h:138) */
+/* CVoxelyze::currentTime (private member, Voxelyze.h:138); of the selected robot */
 float vxg_current_time(vxg_sim* s);
 /* CVoxelyze::stateInfo(stateInfoType, valueType)    include/Voxelyze.h:134, src/Voxelyze.cpp:752-800 */
 float vxg_state_info(vxg_sim* s, int info, int type);
@@ -171,6 +173,9 @@ int vxg_material_model_data(vxg_sim* s, int mat, float* strain, float* stress, i
    mat < 0 removes the voxel (the reference's NULL material).  Returns the voxel index or -1. */
 int vxg_voxel_set(vxg_sim* s, int mat, int ix, int iy, int iz);
 int vxg_voxel_count(vxg_sim* s);                      /* voxelCount()  Voxelyze.h:98 */
+/* n setVoxel calls in one: voxel k gets material mat[k] (< 0 removes) at ijk[3k..3k+2];
+   index[k] (if non-NULL) receives its index or -1.  Returns the number placed. */
+int vxg_voxel_set_many(vxg_sim* s, int n, const int* mat, const int* ijk, int* index);
 /* CVoxelyze::voxel(int x, int y, int z) -> index or -1   include/Voxelyze.h:97 */
 int vxg_voxel_find(vxg_sim* s, int ix, int iy, int iz);
 /* CVX_Voxel::indexX/Y/Z (VX_Voxel.h:66-68) and material() (VX_Voxel.h:70, as a material index) */
@@ -223,6 +228,42 @@ int vxg_link_info(vxg_sim* s, int link, int* neg_vox, int* pos_vox);
 void vxg_link_get_state(vxg_sim* s, int first, int count, float* strain, float* stress,
                         float* max_strain, float* strain_offset, uint32_t* flags,
                         double* local9, double* forces12, double* rest_length);
+
+/* ------------------------------------------------------------------ robots
+   No counterpart in the reference: a robot is an independent body that behaves exactly
+   as a CVoxelyze of its own (its own lattice, clock, dt, divergence and, with
+   collisions on, self-contact only), sharing the simulation's materials, voxel size
+   and environment.  Thousands of robots step together in one launch sequence.  A new
+   simulation holds robot 0; lattice-addressed calls (vxg_voxel_set, vxg_voxel_find,
+   vxg_link_find) refer to the selected robot, and new voxels can only be added to the
+   newest robot, so that every robot's voxels are contiguous in the voxel index space. */
+int vxg_robot_add(vxg_sim* s);               /* new empty robot, selected; returns its index */
+int vxg_robot_select(vxg_sim* s, int robot);  /* returns 0 for an unknown robot */
+int vxg_robot_count(vxg_sim* s);
+/* voxels of a robot: indices [*first, *first + *count) */
+int vxg_robot_voxels(vxg_sim* s, int robot, int* first, int* count);
+/* recommendedTimeStep() of the robot alone (src/Voxelyze.cpp:286-311 over its links) */
+float vxg_robot_recommended_time_step(vxg_sim* s, int robot);
+float vxg_robot_time(vxg_sim* s, int robot);  /* the robot's currentTime */
+/* nsteps steps of every robot, robot r with dt[r] (dt NULL or dt[r] < 0: the robot's own
+   recommended step; dt[r] == 0: the robot does not step).  A robot whose link diverges
+   stops there, the others go on; done[r] (if non-NULL) receives the steps robot r
+   completed.  Returns the smallest of them, or -1 on error. */
+int vxg_robot_step(vxg_sim* s, int nsteps, const float* dt, int* done);
+/* centre of mass of every robot (3 doubles per robot, mass-weighted, FP64) */
+int vxg_robot_com(vxg_sim* s, double* com);
+
+/* ----------------------------------------------------------------- actuation
+   No counterpart in the reference (its users set temperatures from the host between
+   steps, CVX_Voxel::setTemperature VX_Voxel.h:112).  When enabled, before every step the
+   device sets each voxel's temperature to
+       T = (float)(base + amplitude * sin(2*pi*frequency*t + phase_v))   (in FP64)
+   with t the voxel's robot's currentTime, and the rest lengths follow, exactly what a
+   host loop calling setTemperature(T) on every voxel before each doTimeStep does.
+   Temperatures set from the host are overwritten while it is enabled. */
+int vxg_set_actuation(vxg_sim* s, int enabled, float base, float amplitude, float frequency);
+/* phase (radians) of voxels [first, first+count) */
+int vxg_voxel_set_phase(vxg_sim* s, int first, int count, const float* phase);
 
 #ifdef __cplusplus
 }

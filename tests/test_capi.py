@@ -20,7 +20,8 @@ def declared():
 def test_header_and_binding_agree():
     names = declared()
     assert len(names) >= 50
-    assert sorted("vxg_" + k for k in _abi.SIGNATURES) == names
+    assert sorted("vxg_" + k for k in list(_abi.SIGNATURES) + list(_abi.EXTENSIONS)) == names
+    assert not set(_abi.SIGNATURES) & set(_abi.EXTENSIONS)
 
 
 def test_library_exports_every_declared_symbol():

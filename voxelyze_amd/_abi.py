@@ -74,6 +74,22 @@ SIGNATURES = {
     "link_get_state": (None, [_P, _I, _I, _FP, _FP, _FP, _FP, _UP, _DP, _DP, _DP]),
 }
 
+# extensions of the HIP library with no counterpart in the reference (robots, device-side
+# actuation, bulk voxel placement); bound where the library exports them
+EXTENSIONS = {
+    "voxel_set_many": (_I, [_P, _I, _IP, _IP, _IP]),
+    "robot_add": (_I, [_P]),
+    "robot_select": (_I, [_P, _I]),
+    "robot_count": (_I, [_P]),
+    "robot_voxels": (_I, [_P, _I, _IP, _IP]),
+    "robot_recommended_time_step": (_F, [_P, _I]),
+    "robot_time": (_F, [_P, _I]),
+    "robot_step": (_I, [_P, _I, _FP, _IP]),
+    "robot_com": (_I, [_P, _DP]),
+    "set_actuation": (_I, [_P, _I, _F, _F, _F]),
+    "voxel_set_phase": (_I, [_P, _I, _I, _FP]),
+}
+
 
 class ABI:
     """The C ABI of one library, bound under one symbol prefix."""
@@ -87,6 +103,15 @@ class ABI:
             fn.restype = res
             fn.argtypes = args
             setattr(self, key, fn)
+        self.extensions = set()
+        for key, (res, args) in EXTENSIONS.items():
+            if not hasattr(lib, prefix + key):
+                continue
+            fn = getattr(lib, prefix + key)
+            fn.restype = res
+            fn.argtypes = args
+            setattr(self, key, fn)
+            self.extensions.add(key)
 
     def __repr__(self):
         return f"ABI({self.name})"

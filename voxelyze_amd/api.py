@@ -41,6 +41,10 @@ def _fptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_float)) if a is not None else None
 
 
+def _iptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
 def _uptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_uint32)) if a is not None else None
 
@@ -432,6 +436,54 @@ class CVoxelyze:
 
     def setPrecision(self, mode):
         return self._check(bool(self.abi.set_precision(self.handle, mode)))
+
+    # robots, actuation, bulk placement (extensions of the HIP library, include/vxg.h)
+    def setVoxels(self, materials, ijk):
+        """setVoxel for many voxels: materials (n,) of CVX_Material or int, ijk (n, 3)."""
+        mats = np.ascontiguousarray([m.index if isinstance(m, CVX_Material) else int(m) for m in materials],
+                                    dtype=np.int32)
+        ijk = np.ascontiguousarray(ijk, dtype=np.int32).reshape(-1, 3)
+        out = np.zeros(len(mats), np.int32)
+        self.abi.voxel_set_many(self.handle, len(mats), _iptr(mats), _iptr(ijk), _iptr(out))
+        return out
+
+    def addRobot(self): return self.abi.robot_add(self.handle)
+
+    def selectRobot(self, r): return self._check(bool(self.abi.robot_select(self.handle, r)))
+
+    def robotCount(self): return self.abi.robot_count(self.handle)
+
+    def robotVoxels(self, r):
+        first, count = C.c_int(), C.c_int()
+        self.abi.robot_voxels(self.handle, r, C.byref(first), C.byref(count))
+        return first.value, count.value
+
+    def robotRecommendedTimeStep(self, r): return self.abi.robot_recommended_time_step(self.handle, r)
+
+    def robotTime(self, r): return self.abi.robot_time(self.handle, r)
+
+    def robotStep(self, nsteps, dt=None):
+        """nsteps steps of every robot (dt: per-robot array, None = each robot's recommended
+        step).  Returns the per-robot completed step counts."""
+        R = self.robotCount()
+        done = np.zeros(R, np.int32)
+        d = None if dt is None else np.ascontiguousarray(np.broadcast_to(np.asarray(dt, np.float32), (R,)))
+        r = self.abi.robot_step(self.handle, nsteps, _fptr(d) if d is not None else None, _iptr(done))
+        if r < 0:
+            self._raise_if_error()
+        return done
+
+    def robotCom(self):
+        com = np.zeros((self.robotCount(), 3))
+        self._check(bool(self.abi.robot_com(self.handle, _dptr(com))))
+        return com
+
+    def setActuation(self, enabled=True, base=0.0, amplitude=0.0, frequency=0.0):
+        self.abi.set_actuation(self.handle, int(enabled), base, amplitude, frequency)
+
+    def setPhases(self, phase, first=0):
+        ph = np.ascontiguousarray(phase, dtype=np.float32)
+        self._check(bool(self.abi.voxel_set_phase(self.handle, first, len(ph), _fptr(ph))))
 
     # bulk state
     def state(self, first=0, count=None):

@@ -48,6 +48,20 @@ def build(verbose=False):
     return OUT
 
 
+WORK_LIB = os.path.join(HERE, "libvxwork.so")
+
+
+def build_workloads():
+    """libvxwork.so: the native robot-growth generator used by workloads.robot_shape."""
+    cmd = ["gcc", "-O2", "-shared", "-fPIC", os.path.join(CSRC, "vx_workloads.c"), "-o", WORK_LIB + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("gcc failed building libvxwork.so")
+    os.replace(WORK_LIB + ".tmp", WORK_LIB)
+    return WORK_LIB
+
+
 def build_facade_example():
     """tests/cpp/facade_cantilever: a reference-style C++ program built against the
     header-only facade include/Voxelyze.h and linked with libvxg.so."""
@@ -64,4 +78,5 @@ def build_facade_example():
 
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_workloads())
     print(build_facade_example())

@@ -100,15 +100,15 @@ API float vxg_recommended_time_step(vxg_sim* h) { return S(h)->recommendedTimeSt
 API void vxg_reset_time(vxg_sim* h) {
     Sim* s = S(h);
     s->syncDown();
-    s->currentTime = 0.0f;
+    for (auto& t : s->rtime) t = 0.0f;
+    for (auto& p : s->rprev) p = 0.0f;
     for (int v = 0; v < s->nv(); v++) s->resetVoxel(v);
     for (int l = 0; l < s->nl(); l++) s->resetLink(l);
     s->forcesStale = false; /* every end force is zero now; the next link phase rewrites the slots */
-    s->lastDt = 0.0f;
     s->nearbyStale = s->collisionsStale = true;
     s->dirty |= Sim::D_VSTATE | Sim::D_LSTATE | Sim::D_TEMP | Sim::D_REST;
 }
-API float vxg_current_time(vxg_sim* h) { return S(h)->currentTime; }
+API float vxg_current_time(vxg_sim* h) { return S(h)->rtime[S(h)->curRobot]; }
 API int vxg_collision_count(vxg_sim* h) {
     Sim* s = S(h);
     return s->collisions ? s->colEntries / 2 : 0;
@@ -341,7 +341,93 @@ API int vxg_voxel_set(vxg_sim* h, int mi, int x, int y, int z) {
     }
     return s->addVoxel(m, x, y, z);
 }
+API int vxg_voxel_set_many(vxg_sim* h, int n, const int* mat, const int* ijk, int* index) {
+    Sim* s = S(h);
+    s->syncForces();
+    int ok = 0;
+    for (int k = 0; k < n; k++) {
+        const int x = ijk[3 * k], y = ijk[3 * k + 1], z = ijk[3 * k + 2];
+        int v = -1;
+        if (mat[k] < 0) {
+            s->removeVoxel(x, y, z);
+        } else if (vx::Mat* m = M(s, mat[k])) {
+            v = s->voxelAt(x, y, z);
+            if (v >= 0) s->replaceVoxel(m, x, y, z);
+            else v = s->addVoxel(m, x, y, z);
+        }
+        if (index) index[k] = v;
+        if (v >= 0) ok++;
+    }
+    return ok;
+}
 API int vxg_voxel_count(vxg_sim* h) { return S(h)->nv(); }
+
+/* ------------------------------------------------------------------ robots */
+API int vxg_robot_add(vxg_sim* h) {
+    Sim* s = S(h);
+    s->syncForces();
+    return s->addRobot();
+}
+API int vxg_robot_select(vxg_sim* h, int r) {
+    Sim* s = S(h);
+    if (r < 0 || r >= s->nRobots()) {
+        s->err = "no such robot";
+        return 0;
+    }
+    s->curRobot = r;
+    return 1;
+}
+API int vxg_robot_count(vxg_sim* h) { return S(h)->nRobots(); }
+API int vxg_robot_voxels(vxg_sim* h, int r, int* first, int* count) {
+    Sim* s = S(h);
+    if (r < 0 || r >= s->nRobots()) return 0;
+    if (first) *first = s->rfirst[r];
+    if (count) *count = s->rfirst[r + 1] - s->rfirst[r];
+    return 1;
+}
+API float vxg_robot_recommended_time_step(vxg_sim* h, int r) {
+    Sim* s = S(h);
+    return (r >= 0 && r < s->nRobots()) ? s->robotRecommendedTimeStep(r) : 0.0f;
+}
+API float vxg_robot_time(vxg_sim* h, int r) {
+    Sim* s = S(h);
+    return (r >= 0 && r < s->nRobots()) ? s->rtime[r] : 0.0f;
+}
+API int vxg_robot_step(vxg_sim* h, int nsteps, const float* dt, int* done) {
+    Sim* s = S(h);
+    const int R = s->nRobots();
+    std::vector<float> d(R, -1.0f);
+    if (dt) d.assign(dt, dt + R);
+    std::vector<int> dn(R, 0);
+    const int r = s->stepRobots(d.data(), nsteps, dn.data(), nullptr);
+    if (r < 0) return -1;
+    if (done) for (int i = 0; i < R; i++) done[i] = dn[i];
+    return r;
+}
+API int vxg_robot_com(vxg_sim* h, double* com) { return S(h)->robotCom(com) ? 1 : 0; }
+
+/* ----------------------------------------------------------------- actuation */
+API int vxg_set_actuation(vxg_sim* h, int enabled, float base, float amplitude, float frequency) {
+    Sim* s = S(h);
+    s->syncDown();
+    if (s->actOn && !enabled) s->dirty |= Sim::D_TEMP | Sim::D_REST; /* the device kept them */
+    s->actOn = enabled != 0;
+    s->actBase = base;
+    s->actAmp = amplitude;
+    s->actFreq = frequency;
+    s->dirty |= Sim::D_PHASE;
+    return 1;
+}
+API int vxg_voxel_set_phase(vxg_sim* h, int first, int count, const float* phase) {
+    Sim* s = S(h);
+    if (first < 0 || count < 0 || first + count > s->nv()) {
+        s->err = "voxel range out of bounds";
+        return 0;
+    }
+    for (int k = 0; k < count; k++) s->vphase[first + k] = phase[k];
+    s->dirty |= Sim::D_PHASE;
+    return 1;
+}
 API int vxg_voxel_find(vxg_sim* h, int x, int y, int z) { return S(h)->voxelAt(x, y, z); }
 API int vxg_voxel_info(vxg_sim* h, int vi, int* ijk) {
     Sim* s = S(h);

@@ -31,30 +31,10 @@
 
 namespace vx {
 
-struct ContactPtrs {
-    int nv;
-    float* last;               /* lastColWatchPosition, 3 * nv (x, y, z blocks) */
-    const uint32_t* nearMask;  /* 8 words per voxel */
-    const unsigned char* offBit; /* 11^3 table: bit (0..230) of lattice offset (ox,oy,oz) in [-5,5]^3, 255 outside */
-    uint32_t* keys;
-    int* vals;
-    uint32_t* keysSorted;
-    int* valsSorted;
-    int* bucketStart;
-    int* bucketEnd;
-    uint32_t hashMask;
-    int* pairCount;
-    int* pairOff;
-    int* partners;
-    float* cf;                 /* 3 per listed partner */
-    int* flag;                 /* [0] stale */
-    double cellInv;
-    float threshSq;            /* watchRadius^2 (float, as regenerateCollisions receives it) */
-    float recalcSq;            /* recalcDist^2 (float) */
-};
-
-__device__ __forceinline__ uint32_t cell_hash(long long cx, long long cy, long long cz, uint32_t mask) {
-    uint64_t h = (uint64_t)(cx * 73856093LL) ^ (uint64_t)(cy * 19349663LL) ^ (uint64_t)(cz * 83492791LL);
+/* bucket of a grid cell of one robot (robots share lattice coordinates but never touch) */
+__device__ __forceinline__ uint32_t cell_hash(long long cx, long long cy, long long cz, int robot, uint32_t mask) {
+    uint64_t h = (uint64_t)(cx * 73856093LL) ^ (uint64_t)(cy * 19349663LL) ^ (uint64_t)(cz * 83492791LL) ^
+                 ((uint64_t)robot * 0x9E3779B97F4A7C15ull);
     h ^= h >> 29;
     return (uint32_t)h & mask;
 }
@@ -85,7 +65,7 @@ __global__ void k_col_keys(DevPtrs s, ContactPtrs c) {
     c.last[nv + i] = (float)py;
     c.last[2 * nv + i] = (float)pz;
     c.keys[i] = cell_hash((long long)floor(px * c.cellInv), (long long)floor(py * c.cellInv),
-                          (long long)floor(pz * c.cellInv), c.hashMask);
+                          (long long)floor(pz * c.cellInv), s.vrobot[i], c.hashMask);
 }
 
 __global__ void k_col_clear_buckets(ContactPtrs c, int nb) {
@@ -127,13 +107,14 @@ __global__ void k_col_pairs(DevPtrs s, ContactPtrs c) {
     for (int w = 0; w < 8; w++) mask[w] = c.nearMask[8 * (size_t)i + w];
     const long long cx = (long long)floor(px * c.cellInv), cy = (long long)floor(py * c.cellInv),
                     cz = (long long)floor(pz * c.cellInv);
+    const int rob = s.vrobot[i];
     uint32_t seen[27];
     int nseen = 0, cnt = 0;
     const int base = FILL ? c.pairOff[i] : 0;
     for (int a = -1; a <= 1; a++)
         for (int b = -1; b <= 1; b++)
             for (int d = -1; d <= 1; d++) {
-                const uint32_t h = cell_hash(cx + a, cy + b, cz + d, c.hashMask);
+                const uint32_t h = cell_hash(cx + a, cy + b, cz + d, rob, c.hashMask);
                 bool dup = false;
                 for (int q = 0; q < nseen; q++) dup |= (seen[q] == h);
                 if (dup) continue;
@@ -141,7 +122,7 @@ __global__ void k_col_pairs(DevPtrs s, ContactPtrs c) {
                 const int e = c.bucketEnd[h];
                 for (int k = c.bucketStart[h]; k < e; k++) {
                     const int j = c.valsSorted[k];
-                    if (j == i) continue;
+                    if (j == i || s.vrobot[j] != rob) continue;
                     const double dx = px - s.vd[VPX * nv + j], dy = py - s.vd[VPY * nv + j], dz = pz - s.vd[VPZ * nv + j];
                     if (dx * dx + dy * dy + dz * dz > c.threshSq) continue;
                     if (near5(s, c, mask, i, j)) continue;

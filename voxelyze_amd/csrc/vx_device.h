@@ -53,7 +53,7 @@ struct VoxMat {
     float collisionTransC;    /* zetaCollision*_2xSqMxExS */
     float penetrationStiff;   /* (float)(2*E*nomSize) */
     float muStatic, muKinetic;
-    float pad;
+    float mass;               /* _mass */
 };
 
 /* per link-material constants (VX_MaterialLink.h:38-47, VX_Material.h:114-128) */
@@ -101,13 +101,47 @@ struct DevPtrs {
     const float* curveStrain;
     const float* curveStress;
     const ExtRec* exts;
-    int* ctl;        /* [0] diverged this step, [1] halted, [2] steps completed */
+    /* Robots: independent bodies stepped together.  The voxels of robot r are contiguous;
+       a single simulation is robot 0. */
+    const int* vrobot;   /* robot of each voxel */
+    const float* rdt;    /* each robot's dt in this launch */
+    float* rtime;        /* each robot's currentTime; its first voxel's lane advances it */
+    int* rdiv;           /* step of this launch in which the robot diverged, NO_DIVERGENCE if none */
+    int* ctl;            /* [0] some robot diverged in this launch */
+    /* device-side actuation T_v(t) = base + amp*sin(2*pi*freq*t + phase_v); null phase: off */
+    const float* vphase;
+    float actBase, actAmp, actFreq;
     /* self-collision lists (null when collisions are off): partners of voxel v are
        entries [colOff[v], colOff[v]+colCnt[v]) in increasing voxel order; colF holds
        their contact forces on v (3 floats each) for the current step */
     const int* colCnt;
     const int* colOff;
     const float* colF;
+};
+
+#define NO_DIVERGENCE 0x7F7F7F7F /* a byte fill of 0x7F */
+
+/* self-collision buffers (vx_contact.hip) */
+struct ContactPtrs {
+    int nv;
+    float* last;                 /* lastColWatchPosition, 3 * nv (x, y, z blocks) */
+    const uint32_t* nearMask;    /* 8 words per voxel */
+    const unsigned char* offBit; /* 11^3 table: bit (0..230) of lattice offset (ox,oy,oz) in [-5,5]^3, 255 outside */
+    uint32_t* keys;
+    int* vals;
+    uint32_t* keysSorted;
+    int* valsSorted;
+    int* bucketStart;
+    int* bucketEnd;
+    uint32_t hashMask;
+    int* pairCount;
+    int* pairOff;
+    int* partners;
+    float* cf;                   /* 3 per listed partner */
+    int* flag;                   /* [0] stale */
+    double cellInv;
+    float threshSq;              /* watchRadius^2 (float, as regenerateCollisions receives it) */
+    float recalcSq;              /* recalcDist^2 (float) */
 };
 
 /* FP32 link state of the mixed-precision step, per slot (MS_COUNT floats) */
@@ -130,7 +164,8 @@ struct MixPtrs {
     const int* vmat;
     const int* vext;
     const int* vijk;
-    const float* vtemp;
+    const float* vtempIn;   /* temperature before / after (written only under actuation) */
+    float* vtempOut;
     const VoxMat* vmats;
     const LinkMat* lmats;
     const float* curveStrain;
@@ -139,10 +174,16 @@ struct MixPtrs {
     const int* colCnt;      /* contacts (null when off) */
     const int* colOff;
     const int* partners;
-    int* div;               /* one divergence flag per step of the launch */
+    const int* vrobot;      /* robot of each voxel (contiguous ranges) */
+    const float* rdt;       /* each robot's dt */
+    const float* rprev;     /* each robot's previousDt before the launch */
+    const float* rtimeIn;   /* each robot's time before / after the step */
+    float* rtimeOut;
+    int* rdiv;              /* step of this launch in which the robot diverged, NO_DIVERGENCE if none */
+    int* ctl;               /* [0] some robot diverged in this launch */
+    const float* vphase;    /* actuation phase per voxel, null: no actuation */
+    float actBase, actAmp, actFreq;
     int k;                  /* this step's index in the launch */
-    float dt;
-    float prevDt;           /* previousDt of every voxel (uniform whenever a link's local velocity is valid) */
 };
 
 }  // namespace vx

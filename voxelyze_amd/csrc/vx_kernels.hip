@@ -9,10 +9,19 @@
  * neighbour table in direction order 0..5 (VX_Voxel.cpp:238-240), so every sum
  * has the reference's order and no atomics are needed.
  *
- * Divergence (a link strain > 100, Voxelyze.cpp:265-269) is a device flag: the
- * voxel kernel of that step does nothing and raises "halted", which makes every
- * later kernel of a multi-step launch return at once, so a captured N-step
- * sequence stops exactly where the reference's host loop would.
+ * Robots: the voxels are partitioned into robots (independent bodies, each one a
+ * CVoxelyze of its own; a plain simulation is robot 0).  Each robot has its own
+ * dt and clock.  Divergence (a link strain > 100, Voxelyze.cpp:265-269) is
+ * recorded per robot as the launch step in which it happened: the voxel kernel
+ * of that step leaves the robot's voxels alone and every later kernel of the
+ * launch skips the robot, so each robot stops exactly where a host loop over its
+ * own doTimeStep would.
+ *
+ * Actuation (optional): k_actuate_fp64 sets, before each step, every voxel's
+ * temperature to base + amp*sin(2*pi*freq*t + phase_v) at its robot's current
+ * time and the rest length of each link from its two voxels' temperatures,
+ * which is what a host loop calling CVX_Voxel::setTemperature before every
+ * doTimeStep does (VX_Voxel.cpp:108-114, VX_Link.cpp:137-140).
  *
  * Both kernels are HBM-bandwidth bound: a few hundred FP64 flops per element
  * against ~300 B (voxel) / ~380 B (link) of traffic.
@@ -42,11 +51,12 @@ __device__ __forceinline__ float link_stress(const LinkMat& m, const float* __re
 /* one lane per link slot: grid (ceil(nv/256), 3), blockIdx.y is the link axis and the lane's
    voxel is the link's negative end.  Every slot and every force slot is written on every
    step (zeros where no link exists) so that all stores cover whole cache lines. */
-__global__ void __launch_bounds__(256) k_links_fp64(DevPtrs s) {
+__global__ void __launch_bounds__(256) k_links_fp64(DevPtrs s, int k) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     const int nv = s.nv;
     if (n >= nv) return;
-    if (s.ctl[1]) return;
+    const int rob = s.vrobot[n];
+    if (s.rdiv[rob] < k) return; /* the robot stopped at an earlier step of this launch */
     const int axis = blockIdx.y;
     const int nl = s.nl;
     const int i = axis * nv + n;
@@ -193,7 +203,10 @@ __global__ void __launch_bounds__(256) k_links_fp64(DevPtrs s) {
     lf[LOFFSET * nl + i] = strainOffset;
     lf[LSTRESS * nl + i] = stress;
     s.lflags[i] = (lvv ? LF_LVV : 0u) | (small ? LF_SMALL : 0u);
-    if (axialStrain > 100) atomicOr(&s.ctl[0], 1);
+    if (axialStrain > 100) {
+        atomicMin(&s.rdiv[rob], k);
+        atomicOr(&s.ctl[0], 1);
+    }
 }
 
 /* CVX_Voxel::floorPenetration with baseSizeAverage (VX_Voxel.h:88-90,122) */
@@ -204,14 +217,13 @@ __device__ __forceinline__ float floor_penetration(const VoxMat& m, float temp, 
     return (float)(bsa / 2 - m.nomSize / 2 - pz);
 }
 
-__global__ void __launch_bounds__(256) k_voxels_fp64(DevPtrs s, float dt) {
+__global__ void __launch_bounds__(256) k_voxels_fp64(DevPtrs s, int k) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= s.nv) return;
-    if (s.ctl[0]) {
-        if (i == 0) s.ctl[1] = 1;
-        return;
-    }
-    if (i == 0) s.ctl[2] += 1;
+    const int rob = s.vrobot[i];
+    if (s.rdiv[rob] <= k) return; /* diverged in this step or earlier: voxels and time stay */
+    const float dt = s.rdt[rob];
+    if (i == 0 || s.vrobot[i - 1] != rob) s.rtime[rob] += dt; /* currentTime += dt (Voxelyze.cpp:282) */
     const int nv = s.nv;
     double* __restrict__ vd = s.vd;
     const VoxMat m = s.vmats[s.vmat[i]];
@@ -349,13 +361,73 @@ __global__ void __launch_bounds__(256) k_voxels_fp64(DevPtrs s, float dt) {
     s.vflags[i] = flags;
 }
 
+/* actuation before step k: temperatures at the robot's current time, then the rest length of
+   every link this voxel owns (it is the link's negative end) */
+__global__ void __launch_bounds__(256) k_actuate_fp64(DevPtrs s, int k) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nv = s.nv;
+    if (i >= nv) return;
+    const int rob = s.vrobot[i];
+    if (s.rdiv[rob] < k) return;
+    const float t = s.rtime[rob];
+    const float ti = actuation_temperature(s.actBase, s.actAmp, s.actFreq, t, s.vphase[i]);
+    s.vtemp[i] = ti;
+    const VoxMat mi = s.vmats[s.vmat[i]];
+    for (int axis = 0; axis < 3; axis++) {
+        const int slot = axis * nv + i;
+        const int p = s.lpos[slot];
+        if (p < 0) continue;
+        const float tp = actuation_temperature(s.actBase, s.actAmp, s.actFreq, t, s.vphase[p]);
+        const VoxMat mp = s.vmats[s.vmat[p]];
+        const double si = axis == 0 ? mi.sizeX : axis == 1 ? mi.sizeY : mi.sizeZ;
+        const double sp = axis == 0 ? mp.sizeX : axis == 1 ? mp.sizeY : mp.sizeZ;
+        s.ld[LREST * s.nl + slot] = 0.5 * (base_size(si, ti, mi.alphaCTE) + base_size(sp, tp, mp.alphaCTE));
+    }
+}
+
+/* Centre of mass of each robot (FP64): one workgroup per robot, a fixed-order sum of
+   mass*position over the robot's voxels, so repeated calls give the same bits. */
+__global__ void __launch_bounds__(256) k_robot_com(DevPtrs s, const int* rfirst, double* com) {
+    const int r = blockIdx.x;
+    const int b = rfirst[r], e = rfirst[r + 1];
+    const int nv = s.nv;
+    double mx = 0.0, my = 0.0, mz = 0.0, m = 0.0;
+    for (int i = b + (int)threadIdx.x; i < e; i += 256) {
+        const double w = s.vmats[s.vmat[i]].mass;
+        mx += w * s.vd[VPX * nv + i];
+        my += w * s.vd[VPY * nv + i];
+        mz += w * s.vd[VPZ * nv + i];
+        m += w;
+    }
+    __shared__ double red[4][256];
+    red[0][threadIdx.x] = mx; red[1][threadIdx.x] = my; red[2][threadIdx.x] = mz; red[3][threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int c = 0; c < 4; c++) red[c][threadIdx.x] += red[c][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double M = red[3][0];
+        com[3 * r + 0] = M > 0 ? red[0][0] / M : 0.0;
+        com[3 * r + 1] = M > 0 ? red[1][0] / M : 0.0;
+        com[3 * r + 2] = M > 0 ? red[2][0] / M : 0.0;
+    }
+}
+
 static inline unsigned blocks(int n, int b) { return (unsigned)((n + b - 1) / b); }
 
-void launch_links_fp64(const DevPtrs& s, hipStream_t st) {
-    if (s.nv > 0) hipLaunchKernelGGL(k_links_fp64, dim3(blocks(s.nv, 256), 3), dim3(256), 0, st, s);
+void launch_links_fp64(const DevPtrs& s, int k, hipStream_t st) {
+    if (s.nv > 0) hipLaunchKernelGGL(k_links_fp64, dim3(blocks(s.nv, 256), 3), dim3(256), 0, st, s, k);
 }
-void launch_voxels_fp64(const DevPtrs& s, float dt, hipStream_t st) {
-    if (s.nv > 0) hipLaunchKernelGGL(k_voxels_fp64, dim3(blocks(s.nv, 256)), dim3(256), 0, st, s, dt);
+void launch_voxels_fp64(const DevPtrs& s, int k, hipStream_t st) {
+    if (s.nv > 0) hipLaunchKernelGGL(k_voxels_fp64, dim3(blocks(s.nv, 256)), dim3(256), 0, st, s, k);
+}
+void launch_actuate_fp64(const DevPtrs& s, int k, hipStream_t st) {
+    if (s.nv > 0) hipLaunchKernelGGL(k_actuate_fp64, dim3(blocks(s.nv, 256)), dim3(256), 0, st, s, k);
+}
+void launch_robot_com(const DevPtrs& s, const int* rfirst, double* com, int nrobots, hipStream_t st) {
+    if (nrobots > 0) hipLaunchKernelGGL(k_robot_com, dim3((unsigned)nrobots), dim3(256), 0, st, s, rfirst, com);
 }
 
 }  // namespace vx

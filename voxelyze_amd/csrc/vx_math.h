@@ -313,4 +313,15 @@ VX_HD dvec toAxisOriginal(int axis, dvec v) {
     return v;
 }
 
+/* Device-side actuation: the temperature a host loop would set before each step,
+   T = base + amp*sin(2*pi*freq*t + phase), evaluated in FP64 (t is the simulation's
+   float currentTime) and rounded once to float. */
+VX_HD float actuation_temperature(float base, float amp, float freq, float t, float phase) {
+    const double arg = 6.283185307179586 * (double)freq * (double)t + (double)phase;
+    return (float)((double)base + (double)amp * cr_sin(arg));
+}
+
+/* CVX_Voxel::baseSize along one axis (VX_Voxel.h:88-89): size * (1 + T*CTE), the factor in float */
+VX_HD double base_size(double size, float temp, float cte) { return size * (1 + temp * cte); }
+
 }  // namespace vx

@@ -15,11 +15,20 @@
  * state (the voxel flags too).  The link's negative voxel owns it and writes its new state.  There is
  * no link-to-voxel force hand-off in HBM.
  *
- * Divergence (a strain > 100, Voxelyze.cpp:265-269) raises div[k] for step k of
- * a launch; step k+1 sees it, raises its own flag and returns at once, and so
- * on, so every link of the diverging step is still updated (as in the
- * reference) while nothing after it runs.  The host discards the voxel output
- * of the diverging step and keeps its link output.
+ * Robots (independent bodies, each with its own dt and clock; a plain
+ * simulation is robot 0) and divergence (a strain > 100, Voxelyze.cpp:265-269):
+ * a diverging link records the step in the robot's rdiv entry.  Every link of
+ * that step is still updated (as in the reference) but the voxel output and
+ * the clock of that step must not count, so from the next step on the robot's
+ * lanes only copy the stale ping-pong buffer (frozen_copy) and the robot ends
+ * the launch with its voxels as before the diverging step and its links as
+ * after it.
+ *
+ * Actuation (optional): temperatures live with the voxel state; each lane
+ * writes its voxel's temperature for the next step (at the robot's time after
+ * this step) and links compute their rest length from their voxels'
+ * temperatures (VX_Link.cpp:137-140), as a host loop calling setTemperature
+ * before every doTimeStep would.
  */
 #include "vx_device.h"
 #include "vx_math.h"
@@ -78,11 +87,10 @@ __device__ __forceinline__ float m_stress(const LinkMat& m, const float* cs, con
 /* One link (CVX_Link::updateForces, VX_Link.cpp:149-217) in mixed precision.  Returns the
    force and moment on the requested end (in that voxel's frame) and, when owner, the new
    link state. */
-__device__ void link_mixed(const MixPtrs& s, int slot, int n, int p, int axis, bool wantPos, bool owner,
-                           dvec posN, dquat qN, dvec posP, dquat qP, fvec* outF, fvec* outM) {
+__device__ void link_mixed(const MixPtrs& s, int slot, int n, int p, int axis, bool wantPos, bool owner, double rest,
+                           float prevDt, dvec posN, dquat qN, dvec posP, dquat qP, fvec* outF, fvec* outM) {
     const int S = 3 * s.nv;
     const LinkMat m = s.lmats[s.lmeta[slot]];
-    const double rest = s.lrest[slot];
     uint32_t fl = s.lflIn[slot];
     bool small = (fl & LF_SMALL) != 0, lvv = (fl & LF_LVV) != 0;
     const float* ls = s.lsIn;
@@ -141,8 +149,8 @@ __device__ void link_mixed(const MixPtrs& s, int slot, int n, int p, int axis, b
                                0.5f * (a1v.z - ls[MS_A1Z * S + slot]));
             const fvec d2 = fv(0.5f * (a2v.x - ls[MS_A2X * S + slot]), 0.5f * (a2v.y - ls[MS_A2Y * S + slot]),
                                0.5f * (a2v.z - ls[MS_A2Z * S + slot]));
-            const float dmN = s.vmats[s.vmat[n]].dampNum / s.prevDt;
-            const float dmP = s.vmats[s.vmat[p]].dampNum / s.prevDt;
+            const float dmN = s.vmats[s.vmat[n]].dampNum / prevDt;
+            const float dmP = s.vmats[s.vmat[p]].dampNum / prevDt;
             const fvec pc = fv(m.sqA1 * dP.x, m.sqB1 * dP.y - m.sqB2xFMp * (d1.z + d2.z), m.sqB1 * dP.z + m.sqB2xFMp * (d1.y + d2.y));
             fN.x += dmN * pc.x; fN.y += dmN * pc.y; fN.z += dmN * pc.z;
             fP.x -= dmP * pc.x; fP.y -= dmP * pc.y; fP.z -= dmP * pc.z;
@@ -182,7 +190,10 @@ __device__ void link_mixed(const MixPtrs& s, int slot, int n, int p, int axis, b
         o[MS_MAX * S + slot] = maxStrain; o[MS_OFF * S + slot] = strainOffset;
         o[MS_STRAIN * S + slot] = axialStrain; o[MS_STRESS * S + slot] = stress;
         s.lflOut[slot] = (lvv ? LF_LVV : 0u) | (small ? LF_SMALL : 0u);
-        if (axialStrain > 100) atomicOr(&s.div[s.k], 1);
+        if (axialStrain > 100) {
+            atomicMin(&s.rdiv[s.vrobot[n]], s.k);
+            atomicOr(&s.ctl[0], 1);
+        }
     }
 }
 
@@ -191,18 +202,44 @@ __device__ __forceinline__ double m_bsa(const VoxMat& m, float temp) {
     return (f * m.sizeX + f * m.sizeY + f * m.sizeZ) / 3.0;
 }
 
+/* a robot that diverged at step d of this launch keeps its state: its voxel state (and
+   clock) at step d is the discarded output, its link state the kept one, so from step d+1
+   on the lanes copy whichever of the two ping-pong buffers is stale */
+__device__ void frozen_copy(const MixPtrs& s, int i, int rob, int d) {
+    const int nv = s.nv, S = 3 * nv;
+    if (((s.k - d) & 1) == 0) {
+        for (int c = 0; c < VD_COUNT; c++) s.vdOut[c * nv + i] = s.vdIn[c * nv + i];
+        s.vflOut[i] = s.vflIn[i];
+        if (s.vphase) s.vtempOut[i] = s.vtempIn[i];
+        if (i == 0 || s.vrobot[i - 1] != rob) s.rtimeOut[rob] = s.rtimeIn[rob];
+    } else {
+        for (int axis = 0; axis < 3; axis++) {
+            const int slot = axis * nv + i;
+            for (int c = 0; c < MS_COUNT; c++) s.lsOut[c * S + slot] = s.lsIn[c * S + slot];
+            s.lflOut[slot] = s.lflIn[slot];
+        }
+    }
+}
+
 __global__ void __launch_bounds__(256) k_step_mixed(MixPtrs s) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s.k > 0 && s.div[s.k - 1]) { /* an earlier step of this launch diverged */
-        if (i == 0) s.div[s.k] = 1;
-        return;
-    }
     const int nv = s.nv;
     if (i >= nv) return;
+    const int rob = s.vrobot[i];
+    const int d = s.rdiv[rob];
+    if (d < s.k) {
+        frozen_copy(s, i, rob, d);
+        return;
+    }
+    const float dt = s.rdt[rob];
+    const float prevDt = s.k == 0 ? s.rprev[rob] : dt;
+    if (i == 0 || s.vrobot[i - 1] != rob) s.rtimeOut[rob] = s.rtimeIn[rob] + dt; /* currentTime += dt */
     const double* __restrict__ vd = s.vdIn;
     const dvec pos0 = dv(vd[VPX * nv + i], vd[VPY * nv + i], vd[VPZ * nv + i]);
     const dquat q0 = dq(vd[VQW * nv + i], vd[VQX * nv + i], vd[VQY * nv + i], vd[VQZ * nv + i]);
     uint32_t flags = s.vflIn[i];
+    const VoxMat m = s.vmats[s.vmat[i]];
+    const float tempI = s.vtempIn[i];
 
     /* link phase for this voxel's <= 6 links, summed in direction order */
     dvec lf = dv(0.0, 0.0, 0.0), lm = dv(0.0, 0.0, 0.0);
@@ -211,20 +248,30 @@ __global__ void __launch_bounds__(256) k_step_mixed(MixPtrs s) {
         if (o < 0) continue;
         const int axis = dir >> 1;
         const bool neg = (dir & 1) == 0; /* this voxel is the link's negative end */
+        const int slot = neg ? axis * nv + i : axis * nv + o;
+        double rest;
+        if (s.vphase) { /* actuated: rest length from the two temperatures (VX_Link.cpp:137-140) */
+            const VoxMat mo = s.vmats[s.vmat[o]];
+            const double si = axis == 0 ? m.sizeX : axis == 1 ? m.sizeY : m.sizeZ;
+            const double so = axis == 0 ? mo.sizeX : axis == 1 ? mo.sizeY : mo.sizeZ;
+            const double bi = base_size(si, tempI, m.alphaCTE), bo = base_size(so, s.vtempIn[o], mo.alphaCTE);
+            rest = neg ? 0.5 * (bi + bo) : 0.5 * (bo + bi);
+        } else {
+            rest = s.lrest[slot];
+        }
         const dvec po = dv(vd[VPX * nv + o], vd[VPY * nv + o], vd[VPZ * nv + o]);
         const dquat qo = dq(vd[VQW * nv + o], vd[VQX * nv + o], vd[VQY * nv + o], vd[VQZ * nv + o]);
-        fvec f, m;
-        if (neg) link_mixed(s, axis * nv + i, i, o, axis, false, true, pos0, q0, po, qo, &f, &m);
-        else link_mixed(s, axis * nv + o, o, i, axis, true, false, po, qo, pos0, q0, &f, &m);
+        fvec f, mm;
+        if (neg) link_mixed(s, slot, i, o, axis, false, true, rest, prevDt, pos0, q0, po, qo, &f, &mm);
+        else link_mixed(s, slot, o, i, axis, true, false, rest, prevDt, po, qo, pos0, q0, &f, &mm);
         lf.x += f.x; lf.y += f.y; lf.z += f.z;
-        lm.x += m.x; lm.y += m.y; lm.z += m.z;
+        lm.x += mm.x; lm.y += mm.y; lm.z += mm.z;
     }
+    if (s.vphase) s.vtempOut[i] = actuation_temperature(s.actBase, s.actAmp, s.actFreq, s.rtimeIn[rob] + dt, s.vphase[i]);
 
-    const VoxMat m = s.vmats[s.vmat[i]];
     const int e = s.vext[i];
     const ExtRec* ex = e >= 0 ? &s.exts[e] : nullptr;
     double* __restrict__ out = s.vdOut;
-    const float dt = s.dt;
     if (ex && (ex->dof & 0x3F) == 0x3F) {
         const double sz = m.nomSize;
         out[VPX * nv + i] = s.vijk[i] * sz + ex->tx;
@@ -249,13 +296,13 @@ __global__ void __launch_bounds__(256) k_step_mixed(MixPtrs s) {
     if ((flags & VF_COLLISIONS) && s.colCnt) {
         /* contacts (VX_Collision.cpp:42-56) from this voxel's side, FP32 */
         const int cnt = s.colCnt[i], off = s.colOff[i];
-        const double bsai = m_bsa(m, s.vtemp[i]);
+        const double bsai = m_bsa(m, tempI);
         for (int k = 0; k < cnt; k++) {
             const int j = s.partners[off + k];
             const VoxMat mj = s.vmats[s.vmat[j]];
             const fvec o3 = fv((float)(vd[VPX * nv + j] - pos.x), (float)(vd[VPY * nv + j] - pos.y), (float)(vd[VPZ * nv + j] - pos.z));
             const float len = sqrtf(o3.x * o3.x + o3.y * o3.y + o3.z * o3.z);
-            const float rel = (float)((bsai + m_bsa(mj, s.vtemp[j])) * 0.625f) - len;
+            const float rel = (float)((bsai + m_bsa(mj, s.vtempIn[j])) * 0.625f) - len;
             if (rel > 0 && len > 0) {
                 const float inv = 1.0f / len;
                 const fvec u = fv(o3.x * inv, o3.y * inv, o3.z * inv);
@@ -272,7 +319,7 @@ __global__ void __launch_bounds__(256) k_step_mixed(MixPtrs s) {
     const bool floorOn = (flags & VF_FLOOR) != 0;
     float pen = 0.0f;
     if (floorOn) {
-        pen = (float)(m_bsa(m, s.vtemp[i]) / 2 - m.nomSize / 2 - pos.z);
+        pen = (float)(m_bsa(m, tempI) / 2 - m.nomSize / 2 - pos.z);
         if (pen >= 0) {
             const dvec vel = dv(m.massInverse * lin.x, m.massInverse * lin.y, m.massInverse * lin.z);
             const float normalForce = m.penetrationStiff * pen;
@@ -339,6 +386,18 @@ __global__ void __launch_bounds__(256) k_step_mixed(MixPtrs s) {
     out[VAX * nv + i] = ang.x; out[VAY * nv + i] = ang.y; out[VAZ * nv + i] = ang.z;
     out[VQW * nv + i] = q.w; out[VQX * nv + i] = q.x; out[VQY * nv + i] = q.y; out[VQZ * nv + i] = q.z;
     s.vflOut[i] = flags;
+}
+
+/* temperatures at each robot's current time, before the first step of an actuated launch */
+__global__ void __launch_bounds__(256) k_mixed_temps(MixPtrs s, float* temp) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.nv) return;
+    temp[i] = actuation_temperature(s.actBase, s.actAmp, s.actFreq, s.rtimeIn[s.vrobot[i]], s.vphase[i]);
+}
+
+void launch_mixed_temps(const MixPtrs& s, float* temp, hipStream_t st) {
+    if (s.nv > 0 && s.vphase)
+        hipLaunchKernelGGL(k_mixed_temps, dim3((unsigned)((s.nv + 255) / 256)), dim3(256), 0, st, s, temp);
 }
 
 void launch_step_mixed(const MixPtrs& s, hipStream_t st) {

@@ -11,6 +11,7 @@
 namespace vx {
 
 void launch_step_mixed(const MixPtrs& s, hipStream_t st);
+void launch_mixed_temps(const MixPtrs& s, float* temp, hipStream_t st);
 void launch_col_stale(const DevPtrs& s, const ContactPtrs& c, hipStream_t st);
 
 #define HIPCHK(call)                                       \
@@ -22,7 +23,7 @@ void launch_col_stale(const DevPtrs& s, const ContactPtrs& c, hipStream_t st);
 /* second voxel buffer, FP32 link-state pairs; sized with the topology */
 bool Sim::mixedBuffers() {
     const size_t vc = (size_t)std::max(nv(), 1), S = 3 * vc;
-    if (!bvdAlt.ensure((size_t)VD_COUNT * vc * 8) || !bvflAlt.ensure(vc * 4) || !bms[0].ensure((size_t)MS_COUNT * S * 4) ||
+    if (!bvdAlt.ensure((size_t)VD_COUNT * vc * 8) || !bvflAlt.ensure(vc * 4) || !bvtempAlt.ensure(vc * 4) || !bms[0].ensure((size_t)MS_COUNT * S * 4) ||
         !bms[1].ensure((size_t)MS_COUNT * S * 4) || !bmfl[0].ensure(S * 4) || !bmfl[1].ensure(S * 4)) {
         err = "device allocation failed";
         return false;
@@ -71,101 +72,111 @@ bool Sim::downloadMixedLinks() {
         lf[LSTRESS][l] = ms[MS_STRESS * S + d];
         lflags[l] = fl[d];
     }
-    for (auto& p : vprevdt) p = lastDt;
+    for (int v = 0; v < n; v++) vprevdt[v] = rprev[vrobot[v]];
     return true;
 }
 
-/* the step's output voxel state becomes the current one */
+/* the step's output voxel state (and robot clocks) becomes the current one */
 void Sim::swapVoxelBuffers() {
     std::swap(bvd.p, bvdAlt.p);
     std::swap(bvd.bytes, bvdAlt.bytes);
     std::swap(bvflags.p, bvflAlt.p);
     std::swap(bvflags.bytes, bvflAlt.bytes);
+    std::swap(bvtemp.p, bvtempAlt.p);
+    std::swap(bvtemp.bytes, bvtempAlt.bytes);
+    std::swap(brtime.p, brtimeAlt.p);
+    std::swap(brtime.bytes, brtimeAlt.bytes);
     dp.vd = (double*)bvd.p;
     dp.vflags = (uint32_t*)bvflags.p;
+    dp.vtemp = (float*)bvtemp.p;
+    dp.rtime = (float*)brtime.p;
 }
 
-/* nsteps x doTimeStep(dt) with the fused mixed-precision kernel */
-int Sim::stepMixed(float dt, int nsteps, int* divergedAt) {
+/* nsteps steps with the fused mixed-precision kernel (one launch per step) */
+bool Sim::stepMixed(int nsteps) {
     if (collisions && (cp.nv != nv() || !bcLast.p)) {
-        if (!contactBuffers()) return -1;
+        if (!contactBuffers()) return false;
         nearbyStale = true;
     }
-    if (!bdiv.ensure((size_t)nsteps * 4)) { err = "device allocation failed"; return -1; }
-    hipError_t e = hipMemsetAsync(bdiv.p, 0, (size_t)nsteps * 4, stream);
-    if (e != hipSuccess) { fail("hipMemsetAsync", e); return -1; }
     const size_t S = (size_t)3 * nv();
-    const int lc0 = lcur;
-    for (int k = 0; k < nsteps; k++) {
+    MixPtrs mp{};
+    mp.nv = nv();
+    mp.lrest = dp.ld + (size_t)LREST * S;
+    mp.lmeta = dp.lmeta;
+    mp.vnbr = (const int*)bvnbr.p;
+    mp.vmat = dp.vmat;
+    mp.vext = dp.vext;
+    mp.vijk = dp.vijk;
+    mp.vmats = dp.vmats;
+    mp.lmats = dp.lmats;
+    mp.curveStrain = dp.curveStrain;
+    mp.curveStress = dp.curveStress;
+    mp.exts = dp.exts;
+    mp.vrobot = dp.vrobot;
+    mp.rdt = dp.rdt;
+    mp.rprev = (const float*)brprev.p;
+    mp.rdiv = dp.rdiv;
+    mp.ctl = dp.ctl;
+    mp.vphase = actOn ? (const float*)bvphase.p : nullptr;
+    mp.actBase = actBase;
+    mp.actAmp = actAmp;
+    mp.actFreq = actFreq;
+    if (actOn) { /* temperatures at the robots' current times */
+        mp.rtimeIn = dp.rtime;
+        launch_mixed_temps(mp, dp.vtemp, stream);
+    }
+    bool lastStep = false;
+    int launched = 0;
+    for (int k = 0; k < nsteps && !lastStep; k++) {
         if (collisions) {
-            e = hipMemsetAsync(cp.flag, 0, 4, stream);
-            if (e != hipSuccess) { fail("hipMemsetAsync", e); return -1; }
+            HIPCHK(hipMemsetAsync(cp.flag, 0, 4, stream));
             launch_col_stale(dp, cp, stream);
-            int stale = 0, dv = 0;
-            e = hipMemcpyAsync(&stale, cp.flag, 4, hipMemcpyDeviceToHost, stream);
-            if (e == hipSuccess && k > 0) e = hipMemcpyAsync(&dv, (int*)bdiv.p + (k - 1), 4, hipMemcpyDeviceToHost, stream);
-            if (e == hipSuccess) e = hipStreamSynchronize(stream);
-            if (e != hipSuccess) { fail("step", e); return -1; }
-            if (!dv) {
+            HIPCHK(hipMemcpyAsync(ctlHost, dp.ctl, 4 * sizeof(int), hipMemcpyDeviceToHost, stream));
+            HIPCHK(hipMemcpyAsync(ctlHost + 3, cp.flag, 4, hipMemcpyDeviceToHost, stream));
+            HIPCHK(hipStreamSynchronize(stream));
+            bool all = false;
+            if (ctlHost[0] && !anyDiverged(k - 1, &all)) return false;
+            if (all) {
+                /* every robot stopped in an earlier step: this step only settles the
+                   ping-pong buffers (frozen copies), then the launch ends */
+                lastStep = true;
+            } else {
                 if (nearbyStale) {
-                    if (!nearMasks()) return -1;
+                    if (!nearMasks()) return false;
                     nearbyStale = false;
                     collisionsStale = true;
                 }
-                if (stale) collisionsStale = true;
-                if (collisionsStale && !regenerateCollisions()) return -1;
+                if (ctlHost[3]) collisionsStale = true;
+                if (collisionsStale && !regenerateCollisions()) return false;
             }
         }
-        MixPtrs mp{};
-        mp.nv = nv();
         mp.vdIn = dp.vd;
         mp.vdOut = (double*)bvdAlt.p;
+        mp.vflIn = dp.vflags;
+        mp.vflOut = (uint32_t*)bvflAlt.p;
+        mp.vtempIn = dp.vtemp;
+        mp.vtempOut = (float*)bvtempAlt.p;
+        mp.rtimeIn = dp.rtime;
+        mp.rtimeOut = (float*)brtimeAlt.p;
         mp.lsIn = (const float*)bms[lcur].p;
         mp.lsOut = (float*)bms[lcur ^ 1].p;
         mp.lflIn = (const uint32_t*)bmfl[lcur].p;
         mp.lflOut = (uint32_t*)bmfl[lcur ^ 1].p;
-        mp.vflIn = dp.vflags;
-        mp.vflOut = (uint32_t*)bvflAlt.p;
-        mp.lrest = dp.ld + (size_t)LREST * S;
-        mp.lmeta = dp.lmeta;
-        mp.vnbr = (const int*)bvnbr.p;
-        mp.vmat = dp.vmat;
-        mp.vext = dp.vext;
-        mp.vijk = dp.vijk;
-        mp.vtemp = dp.vtemp;
-        mp.vmats = dp.vmats;
-        mp.lmats = dp.lmats;
-        mp.curveStrain = dp.curveStrain;
-        mp.curveStress = dp.curveStress;
-        mp.exts = dp.exts;
         mp.colCnt = collisions ? cp.pairCount : nullptr;
         mp.colOff = collisions ? cp.pairOff : nullptr;
         mp.partners = collisions ? cp.partners : nullptr;
-        mp.div = (int*)bdiv.p;
         mp.k = k;
-        mp.dt = dt;
-        mp.prevDt = k == 0 ? lastDt : dt;
-        if (profiling) {
-            while (events.size() < (size_t)2 * (k + 1)) {
-                hipEvent_t ev;
-                if (hipEventCreate(&ev) != hipSuccess) { err = "hipEventCreate failed"; return -1; }
-                events.push_back(ev);
-            }
-            (void)hipEventRecord(events[2 * k], stream);
-        }
+        if (profiling) (void)hipEventRecord(events[2 * k], stream);
         launch_step_mixed(mp, stream);
         if (profiling) (void)hipEventRecord(events[2 * k + 1], stream);
         swapVoxelBuffers();
         lcur ^= 1;
+        launched++;
     }
-    e = hipGetLastError();
-    if (e != hipSuccess) { fail("kernel launch", e); return -1; }
-    std::vector<int> flags(nsteps);
-    e = hipMemcpyAsync(flags.data(), bdiv.p, (size_t)nsteps * 4, hipMemcpyDeviceToHost, stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(stream);
-    if (e != hipSuccess) { fail("step", e); return -1; }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(stream));
     if (profiling) {
-        for (int k = 0; k < nsteps; k++) {
+        for (int k = 0; k < launched; k++) {
             float ms = 0;
             if (hipEventElapsedTime(&ms, events[2 * k], events[2 * k + 1]) == hipSuccess) {
                 profMs[3] += ms;
@@ -173,19 +184,7 @@ int Sim::stepMixed(float dt, int nsteps, int* divergedAt) {
             }
         }
     }
-    int done = nsteps;
-    for (int k = 0; k < nsteps; k++)
-        if (flags[k]) { done = k; break; }
-    if (done < nsteps) {
-        /* voxels: the state after `done` steps; links: the output of the diverging step */
-        if ((nsteps - done) & 1) swapVoxelBuffers();
-        lcur = (lc0 + done + 1) & 1;
-        if (divergedAt) *divergedAt = done;
-    }
-    for (int k = 0; k < done; k++) currentTime += dt;
-    if (done > 0) lastDt = dt;
-    hostStale = true;
-    return done;
+    return true;
 }
 
 }  // namespace vx

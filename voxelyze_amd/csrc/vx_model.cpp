@@ -18,8 +18,10 @@
 
 namespace vx {
 
-void launch_links_fp64(const DevPtrs& s, hipStream_t st);
-void launch_voxels_fp64(const DevPtrs& s, float dt, hipStream_t st);
+void launch_links_fp64(const DevPtrs& s, int k, hipStream_t st);
+void launch_voxels_fp64(const DevPtrs& s, int k, hipStream_t st);
+void launch_actuate_fp64(const DevPtrs& s, int k, hipStream_t st);
+void launch_robot_com(const DevPtrs& s, const int* rfirst, double* com, int nrobots, hipStream_t st);
 void launch_col_stale(const DevPtrs& s, const ContactPtrs& c, hipStream_t st);
 void launch_col_keys(const DevPtrs& s, const ContactPtrs& c, int nbuckets, hipStream_t st);
 hipError_t col_sort(const ContactPtrs& c, void* tmp, size_t* tmpBytes, hipStream_t st);
@@ -366,8 +368,39 @@ Mat* Sim::combinedMaterial(Mat* a, Mat* b) {
 }
 
 int Sim::voxelAt(int x, int y, int z) const {
+    const auto& vmap = vmaps[curRobot];
     auto it = vmap.find(vkey(x, y, z));
     return it == vmap.end() ? -1 : it->second;
+}
+
+/* a new, empty robot; it becomes the one lattice-addressed calls refer to */
+int Sim::addRobot() {
+    vmaps.emplace_back();
+    rtime.push_back(0.0f);
+    rprev.push_back(0.0f);
+    rfirst.push_back(nv());
+    curRobot = nRobots() - 1;
+    topologyChanged();
+    return curRobot;
+}
+
+/* recommendedTimeStep (src/Voxelyze.cpp:286-311) over one robot's links and voxels */
+float Sim::robotRecommendedTimeStep(int r) {
+    float maxFreq2 = 0.0f;
+    for (int l = 0; l < nl(); l++) {
+        if (vrobot[lneg[l]] != r) continue;
+        float m1 = vmat[lneg[l]]->mass, m2 = vmat[lpos[l]]->mass;
+        float f = lmat[l]->a1 / (m1 < m2 ? m1 : m2);
+        if (f > maxFreq2) maxFreq2 = f;
+    }
+    if (maxFreq2 <= 0.0f) {
+        for (int v = rfirst[r]; v < rfirst[r + 1]; v++) {
+            const Mat* m = vmat[v];
+            float f = (float)(m->E * m->nomSize / m->mass);
+            if (f > maxFreq2) maxFreq2 = f;
+        }
+    }
+    return maxFreq2 <= 0.0f ? 0.0f : 1.0f / (6.283185f * sqrtf(maxFreq2));
 }
 
 /* CVX_Voxel::reset (src/VX_Voxel.cpp:47-56) */
@@ -481,7 +514,14 @@ void Sim::removeLinkAt(int li) {
 
 /* CVoxelyze::addVoxel (src/Voxelyze.cpp:439-461) with the CVX_Voxel ctor (src/VX_Voxel.cpp:23-37) */
 int Sim::addVoxel(Mat* m, int x, int y, int z) {
+    if (curRobot != nRobots() - 1) {
+        err = "new voxels can only be added to the newest robot";
+        return -1;
+    }
     const int vi = nv();
+    vrobot.push_back(curRobot);
+    vphase.push_back(0.0f);
+    for (int r = curRobot + 1; r < (int)rfirst.size(); r++) rfirst[r] = vi + 1;
     vmat.push_back(m);
     vijk.push_back({(short)x, (short)y, (short)z});
     vlinks.push_back({-1, -1, -1, -1, -1, -1});
@@ -491,7 +531,7 @@ int Sim::addVoxel(Mat* m, int x, int y, int z) {
     vprevdt.push_back(0.0f);
     vtemp.push_back(0.0f);
     resetVoxel(vi);
-    vmap[vkey(x, y, z)] = vi;
+    vmaps[curRobot][vkey(x, y, z)] = vi;
     vd[VPX][vi] = x * voxSize;
     vd[VPY][vi] = y * voxSize;
     vd[VPZ][vi] = z * voxSize;
@@ -509,8 +549,11 @@ void Sim::removeVoxel(int x, int y, int z) {
     if (vi < 0) return;
     for (int d = 0; d < 6; d++)
         if (vlinks[vi][d] >= 0) removeLinkAt(vlinks[vi][d]);
-    vmap.erase(vkey(x, y, z));
+    vmaps[curRobot].erase(vkey(x, y, z));
     vmat.erase(vmat.begin() + vi);
+    vrobot.erase(vrobot.begin() + vi);
+    vphase.erase(vphase.begin() + vi);
+    for (int r = curRobot + 1; r < (int)rfirst.size(); r++) rfirst[r]--;
     vijk.erase(vijk.begin() + vi);
     vlinks.erase(vlinks.begin() + vi);
     vext.erase(vext.begin() + vi);
@@ -518,7 +561,8 @@ void Sim::removeVoxel(int x, int y, int z) {
     vflags.erase(vflags.begin() + vi);
     vprevdt.erase(vprevdt.begin() + vi);
     vtemp.erase(vtemp.begin() + vi);
-    for (auto& kv : vmap) if (kv.second > vi) kv.second--;
+    for (auto& vmap : vmaps)
+        for (auto& kv : vmap) if (kv.second > vi) kv.second--;
     for (size_t l = 0; l < lneg.size(); l++) {
         if (lneg[l] > vi) lneg[l]--;
         if (lpos[l] > vi) lpos[l]--;
@@ -596,6 +640,8 @@ static std::vector<T> to_slots(const std::vector<T>& v, const std::vector<int>& 
     return r;
 }
 
+static inline size_t vc4(int n) { return (size_t)4 * (size_t)std::max(n, 1); }
+
 /* push every dirty host array to the device */
 bool Sim::syncUp() {
     if (!ensureDevice()) return false;
@@ -631,6 +677,20 @@ bool Sim::syncUp() {
         dp.lf = (float*)blf.p;
         dp.lflags = (uint32_t*)blflags.p;
         dp.ctl = (int*)bctl.p;
+        /* robots */
+        const int R = nRobots();
+        if (!bvrobot.ensure((size_t)vc4(n)) || !bvphase.ensure((size_t)vc4(n)) || !brfirst.ensure((size_t)4 * (R + 1)) ||
+            !brdt.ensure((size_t)4 * R) || !brprev.ensure((size_t)4 * R) || !brtime.ensure((size_t)4 * R) ||
+            !brtimeAlt.ensure((size_t)4 * R) || !brdiv.ensure((size_t)4 * R) || !bcom.ensure((size_t)24 * R)) {
+            err = "device allocation failed";
+            return false;
+        }
+        dp.vrobot = (const int*)bvrobot.p;
+        dp.rdt = (const float*)brdt.p;
+        dp.rtime = (float*)brtime.p;
+        dp.rdiv = (int*)brdiv.p;
+        HIPCHK(up((int*)bvrobot.p, vrobot, stream));
+        HIPCHK(up((int*)brfirst.p, rfirst, stream));
         slotOf.resize(L);
         for (int l = 0; l < L; l++) slotOf[l] = laxis[l] * n + lneg[l];
         /* voxel across each direction (mixed-precision step) */
@@ -651,7 +711,7 @@ bool Sim::syncUp() {
         std::vector<int> dpos = to_slots(lpos, slotOf, (size_t)3 * n, -1);
         HIPCHK(up(dp.lpos, dpos, stream));
         HIPCHK(hipStreamSynchronize(stream));
-        dirty |= D_VSTATE | D_LSTATE | D_MATS | D_EXT | D_REST | D_TEMP;
+        dirty |= D_VSTATE | D_LSTATE | D_MATS | D_EXT | D_REST | D_TEMP | D_PHASE;
     }
     if (dirty & D_MATS) {
         int k = 0;
@@ -674,6 +734,7 @@ bool Sim::syncUp() {
             r.penetrationStiff = (float)(2 * m->E * m->nomSize);
             r.muStatic = m->muStatic;
             r.muKinetic = m->muKinetic;
+            r.mass = m->mass;
             vm.push_back(r);
         }
         std::vector<LinkMat> lm;
@@ -745,7 +806,17 @@ bool Sim::syncUp() {
         HIPCHK(up(dp.vflags, vflags, stream));
         HIPCHK(up(dp.vprevdt, vprevdt, stream));
     }
-    if (dirty & D_TEMP) HIPCHK(up(dp.vtemp, vtemp, stream));
+    if (dirty & D_TEMP) {
+        HIPCHK(up(dp.vtemp, vtemp, stream));
+        if (precision == 1) HIPCHK(up((float*)bvtempAlt.p, vtemp, stream)); /* both ping-pong buffers */
+    }
+    if (dirty & D_PHASE) {
+        HIPCHK(up((float*)bvphase.p, vphase, stream));
+        dp.vphase = actOn ? (const float*)bvphase.p : nullptr;
+        dp.actBase = actBase;
+        dp.actAmp = actAmp;
+        dp.actFreq = actFreq;
+    }
     std::vector<std::vector<double>> stageD;
     std::vector<std::vector<float>> stageF;
     const size_t S = (size_t)3 * n;
@@ -779,6 +850,7 @@ bool Sim::syncDown() {
         if (n) HIPCHK(hipMemcpyAsync(vd[c].data(), dp.vd + (size_t)c * n, (size_t)n * 8, hipMemcpyDeviceToHost, stream));
     if (n) HIPCHK(hipMemcpyAsync(vflags.data(), dp.vflags, (size_t)n * 4, hipMemcpyDeviceToHost, stream));
     if (n) HIPCHK(hipMemcpyAsync(vprevdt.data(), dp.vprevdt, (size_t)n * 4, hipMemcpyDeviceToHost, stream));
+    if (n && actOn) HIPCHK(hipMemcpyAsync(vtemp.data(), dp.vtemp, (size_t)n * 4, hipMemcpyDeviceToHost, stream));
     const size_t S = (size_t)3 * n;
     std::vector<double> bufD((size_t)LREST * S);
     std::vector<float> bufF((size_t)LF_COUNT * S);
@@ -794,6 +866,11 @@ bool Sim::syncDown() {
         for (int c = 0; c < LREST; c++) ld[c][l] = bufD[c * S + d]; /* rest lengths are host-owned */
         for (int c = 0; c < LF_COUNT; c++) lf[c][l] = bufF[c * S + d];
         lflags[l] = bufU[d];
+    }
+    if (actOn) { /* the temperatures the device set; rest lengths follow them */
+        const unsigned keep = dirty;
+        for (int l = 0; l < L; l++) updateRestLength(l);
+        dirty = keep;
     }
     hostStale = false;
     if (precision == 1) {
@@ -847,56 +924,123 @@ bool Sim::readVoxelRange(int first, int count, double* pos, double* orient, doub
 }
 
 /* nsteps x CVoxelyze::doTimeStep(dt) (src/Voxelyze.cpp:251-284) */
+/* nsteps x CVoxelyze::doTimeStep(dt) (src/Voxelyze.cpp:251-284) on every robot; dt < 0 is the
+   simulation's recommendedTimeStep.  Returns the steps every robot completed. */
 int Sim::step(float dt, int nsteps, int* divergedAt) {
+    if (dt < 0) dt = recommendedTimeStep();
+    std::vector<float> d(nRobots(), dt);
+    std::vector<int> done(nRobots());
+    return stepRobots(d.data(), nsteps, done.data(), divergedAt);
+}
+
+/* one launch sequence of nsteps steps, robot r with dt[r] (< 0: its own recommendedTimeStep,
+   0: it does not step, as doTimeStep(0) does nothing) */
+int Sim::stepRobots(const float* dt, int nsteps, int* done, int* divergedAt) {
     if (divergedAt) *divergedAt = -1;
     err.clear();
-    if (nsteps <= 0) return 0;
-    if (dt == 0) return nsteps;
-    if (dt < 0) dt = recommendedTimeStep();
+    const int R = nRobots();
+    if (nsteps <= 0) {
+        for (int r = 0; r < R; r++) done[r] = 0;
+        return 0;
+    }
     for (auto& m : vmats) {
         if (m->nu != 0.0f) {
             err = "Poisson's ratio != 0 is not supported by the device step";
             return -1;
         }
     }
-    if (nv() == 0) {
-        for (int k = 0; k < nsteps; k++) currentTime += dt;
-        return nsteps;
+    launchDt.assign(dt, dt + R);
+    bool any = false;
+    for (int r = 0; r < R; r++) {
+        if (launchDt[r] < 0) launchDt[r] = robotRecommendedTimeStep(r);
+        done[r] = nsteps;
+        /* robots without voxels only keep time (the step loops over nothing) */
+        if (rfirst[r] == rfirst[r + 1]) {
+            if (launchDt[r] != 0)
+                for (int k = 0; k < nsteps; k++) rtime[r] += launchDt[r];
+        } else if (launchDt[r] != 0) {
+            any = true;
+        }
     }
-    if (!syncUp()) return -1;
-    if (precision == 1) return stepMixed(dt, nsteps, divergedAt);
-    if (collisions) {
-        const int d = stepWithContacts(dt, nsteps, divergedAt);
-        if (d > 0) lastDt = dt;
-        return d;
-    }
-    dp.colCnt = nullptr;
-    dp.colOff = nullptr;
-    dp.colF = nullptr;
-    hipError_t e = hipMemsetAsync(dp.ctl, 0, 4 * sizeof(int), stream);
-    if (e != hipSuccess) { fail("hipMemsetAsync", e); return -1; }
+    if (!any) return nsteps;
+    if (!syncUp() || !beginLaunch(nsteps)) return -1;
+    const bool ok = precision == 1 ? stepMixed(nsteps) : collisions ? stepWithContacts(nsteps) : stepFp64(nsteps);
+    if (!ok || !endLaunch(nsteps, done)) return -1;
+    int mn = nsteps;
+    for (int r = 0; r < R; r++) mn = std::min(mn, done[r]);
+    if (mn < nsteps && divergedAt) *divergedAt = mn;
+    return mn;
+}
+
+/* per-robot launch inputs: dt, previousDt, clock; no robot diverged yet (idle robots, dt 0,
+   are marked -1) */
+bool Sim::beginLaunch(int nsteps) {
+    const int R = nRobots();
+    std::vector<int> div(R, NO_DIVERGENCE);
+    for (int r = 0; r < R; r++)
+        if (launchDt[r] == 0) div[r] = -1;
+    HIPCHK(up((float*)brdt.p, launchDt, stream));
+    HIPCHK(up((float*)brprev.p, rprev, stream));
+    HIPCHK(up(dp.rtime, rtime, stream));
+    HIPCHK(up(dp.rdiv, div, stream));
+    HIPCHK(hipMemsetAsync(dp.ctl, 0, 4 * sizeof(int), stream));
     if (profiling) {
         const size_t need = (size_t)4 * nsteps;
         while (events.size() < need) {
             hipEvent_t ev;
-            e = hipEventCreate(&ev);
-            if (e != hipSuccess) { fail("hipEventCreate", e); return -1; }
+            HIPCHK(hipEventCreate(&ev));
             events.push_back(ev);
         }
     }
+    return true;
+}
+
+/* steps each robot completed, its clock and previousDt */
+bool Sim::endLaunch(int nsteps, int* done) {
+    const int R = nRobots();
+    std::vector<int> div(R);
+    std::vector<float> t(R);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(div.data(), dp.rdiv, (size_t)4 * R, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(t.data(), dp.rtime, (size_t)4 * R, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    for (int r = 0; r < R; r++) {
+        if (rfirst[r] == rfirst[r + 1]) continue; /* kept on the host */
+        done[r] = div[r] < 0 ? nsteps : std::min(div[r], nsteps);
+        rtime[r] = t[r];
+        if (done[r] > 0 && launchDt[r] != 0) rprev[r] = launchDt[r];
+    }
+    hostStale = true;
+    return true;
+}
+
+/* has every robot diverged by step k of this launch? (reads the per-robot record) */
+bool Sim::anyDiverged(int k, bool* all) {
+    const int R = nRobots();
+    std::vector<int> div(R);
+    HIPCHK(hipMemcpyAsync(div.data(), dp.rdiv, (size_t)4 * R, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    *all = true;
+    for (int r = 0; r < R; r++)
+        if (rfirst[r] != rfirst[r + 1] && div[r] > k) *all = false;
+    return true;
+}
+
+bool Sim::stepFp64(int nsteps) {
+    dp.colCnt = nullptr;
+    dp.colOff = nullptr;
+    dp.colF = nullptr;
     for (int k = 0; k < nsteps; k++) {
+        if (actOn) launch_actuate_fp64(dp, k, stream);
         if (profiling) (void)hipEventRecord(events[4 * k], stream);
-        launch_links_fp64(dp, stream);
+        launch_links_fp64(dp, k, stream);
         if (profiling) (void)hipEventRecord(events[4 * k + 1], stream);
         if (profiling) (void)hipEventRecord(events[4 * k + 2], stream);
-        launch_voxels_fp64(dp, dt, stream);
+        launch_voxels_fp64(dp, k, stream);
         if (profiling) (void)hipEventRecord(events[4 * k + 3], stream);
     }
-    e = hipGetLastError();
-    if (e != hipSuccess) { fail("kernel launch", e); return -1; }
-    e = hipMemcpyAsync(ctlHost, dp.ctl, 4 * sizeof(int), hipMemcpyDeviceToHost, stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(stream);
-    if (e != hipSuccess) { fail("step", e); return -1; }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(stream));
     if (profiling) {
         for (int k = 0; k < nsteps; k++) {
             float a = 0, b = 0;
@@ -910,12 +1054,18 @@ int Sim::step(float dt, int nsteps, int* divergedAt) {
             }
         }
     }
-    hostStale = true;
-    const int done = ctlHost[2];
-    if (done > 0) lastDt = dt;
-    for (int k = 0; k < done; k++) currentTime += dt;
-    if (ctlHost[0] && divergedAt) *divergedAt = done;
-    return done;
+    return true;
+}
+
+/* centre of mass of every robot, FP64 */
+bool Sim::robotCom(double* com) {
+    if (!syncUp()) return false;
+    const int R = nRobots();
+    launch_robot_com(dp, (const int*)brfirst.p, (double*)bcom.p, R, stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(com, bcom.p, (size_t)24 * R, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    return true;
 }
 
 /* ================================================================ contacts */
@@ -1051,49 +1201,40 @@ bool Sim::regenerateCollisions() {
 
 /* doTimeStep with self-collision: link phase, CVoxelyze::updateCollisions, voxel phase.
    The host inspects the divergence and staleness flags between the phases of each step. */
-int Sim::stepWithContacts(float dt, int nsteps, int* divergedAt) {
+bool Sim::stepWithContacts(int nsteps) {
     if (cp.nv != nv() || !bcLast.p) {
-        if (!contactBuffers()) return -1;
+        if (!contactBuffers()) return false;
         nearbyStale = true;
     }
-    hipError_t e = hipMemsetAsync(dp.ctl, 0, 4 * sizeof(int), stream);
-    if (e != hipSuccess) { fail("hipMemsetAsync", e); return -1; }
-    int done = 0;
     for (int k = 0; k < nsteps; k++) {
-        launch_links_fp64(dp, stream);
-        e = hipMemsetAsync(cp.flag, 0, 4, stream);
-        if (e != hipSuccess) { fail("hipMemsetAsync", e); return -1; }
+        if (actOn) launch_actuate_fp64(dp, k, stream);
+        launch_links_fp64(dp, k, stream);
+        HIPCHK(hipMemsetAsync(cp.flag, 0, 4, stream));
         launch_col_stale(dp, cp, stream);
-        e = hipMemcpyAsync(ctlHost, dp.ctl, 4 * sizeof(int), hipMemcpyDeviceToHost, stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(ctlHost + 3, cp.flag, 4, hipMemcpyDeviceToHost, stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(stream);
-        if (e != hipSuccess) { fail("step", e); return -1; }
+        HIPCHK(hipMemcpyAsync(ctlHost, dp.ctl, 4 * sizeof(int), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(ctlHost + 3, cp.flag, 4, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
         if (ctlHost[0]) {
-            hostStale = true;
-            if (divergedAt) *divergedAt = k;
-            return done;
+            bool all = false;
+            if (!anyDiverged(k, &all)) return false;
+            if (all) break; /* every robot has stopped: the voxel phase of this step is skipped */
         }
         if (nearbyStale) {
-            if (!nearMasks()) return -1;
+            if (!nearMasks()) return false;
             nearbyStale = false;
             collisionsStale = true;
         }
         if (ctlHost[3]) collisionsStale = true;
-        if (collisionsStale && !regenerateCollisions()) return -1;
+        if (collisionsStale && !regenerateCollisions()) return false;
         dp.colCnt = cp.pairCount;
         dp.colOff = cp.pairOff;
         dp.colF = cp.cf;
         launch_col_forces(dp, cp, stream);
-        launch_voxels_fp64(dp, dt, stream);
-        e = hipGetLastError();
-        if (e != hipSuccess) { fail("kernel launch", e); return -1; }
-        currentTime += dt;
-        done++;
+        launch_voxels_fp64(dp, k, stream);
+        HIPCHK(hipGetLastError());
     }
-    e = hipStreamSynchronize(stream);
-    if (e != hipSuccess) { fail("step", e); return -1; }
-    hostStale = true;
-    return done;
+    HIPCHK(hipStreamSynchronize(stream));
+    return true;
 }
 
 }  // namespace vx

@@ -22,28 +22,6 @@
 
 namespace vx {
 
-struct ContactPtrs {
-    int nv;
-    float* last;
-    const uint32_t* nearMask;
-    const unsigned char* offBit;
-    uint32_t* keys;
-    int* vals;
-    uint32_t* keysSorted;
-    int* valsSorted;
-    int* bucketStart;
-    int* bucketEnd;
-    uint32_t hashMask;
-    int* pairCount;
-    int* pairOff;
-    int* partners;
-    float* cf;
-    int* flag;
-    double cellInv;
-    float threshSq;
-    float recalcSq;
-};
-
 /* CVX_Material + CVX_MaterialVoxel + CVX_MaterialLink (include/VX_Material*.h) */
 struct Mat {
     bool linear = true;
@@ -99,7 +77,7 @@ struct DevBuf {
 
 struct Sim {
     double voxSize = 0.001;
-    float currentTime = 0.0f, ambientTemp = 0.0f, grav = 0.0f;
+    float ambientTemp = 0.0f, grav = 0.0f;
     bool floor = false, collisions = false;
     int precision = 0;
     std::string err;
@@ -115,7 +93,22 @@ struct Sim {
     std::vector<double> vd[VD_COUNT];
     std::vector<uint32_t> vflags;
     std::vector<float> vprevdt, vtemp;
-    std::unordered_map<uint64_t, int> vmap;
+    std::vector<int> vrobot;  /* robot of each voxel; a robot's voxels are contiguous */
+    std::vector<float> vphase; /* actuation phase */
+
+    /* robots: independent bodies (each a CVoxelyze of its own) sharing materials and
+       environment; a plain simulation is robot 0 */
+    std::vector<std::unordered_map<uint64_t, int>> vmaps = std::vector<std::unordered_map<uint64_t, int>>(1);
+    int curRobot = 0;                 /* robot that lattice-addressed calls refer to */
+    std::vector<float> rtime{0.0f};   /* currentTime of each robot */
+    std::vector<float> rprev{0.0f};   /* previousDt of each robot's voxels (0 before any step) */
+    std::vector<int> rfirst{0, 0};    /* voxels of robot r: [rfirst[r], rfirst[r+1]) */
+    int nRobots() const { return (int)vmaps.size(); }
+    int addRobot();
+    float robotRecommendedTimeStep(int r);
+    /* device-side actuation */
+    bool actOn = false;
+    float actBase = 0.0f, actAmp = 0.0f, actFreq = 0.0f;
 
     /* links, linksList order */
     std::vector<int> lneg, lpos, laxis;
@@ -125,7 +118,7 @@ struct Sim {
     std::vector<uint32_t> lflags;
 
     /* host <-> device */
-    enum : unsigned { D_TOPO = 1, D_VSTATE = 2, D_LSTATE = 4, D_MATS = 8, D_EXT = 16, D_REST = 32, D_TEMP = 64 };
+    enum : unsigned { D_TOPO = 1, D_VSTATE = 2, D_LSTATE = 4, D_MATS = 8, D_EXT = 16, D_REST = 32, D_TEMP = 64, D_PHASE = 128 };
     unsigned dirty = D_TOPO | D_MATS | D_EXT;
     bool hostStale = false; /* device holds newer dynamic state */
     bool dtCacheValid = false;
@@ -140,6 +133,7 @@ struct Sim {
     std::vector<int> slotOf;   /* linksList position -> device link slot */
     bool forcesStale = false;  /* host end-force view older than the device slots */
     int* ctlHost = nullptr;
+    DevBuf bvrobot, bvphase, brfirst, brdt, brprev, brtime, brtimeAlt, brdiv, bcom;
     /* self-collision (src/Voxelyze.cpp:670-750) */
     bool nearbyStale = true, collisionsStale = true;
     int colEntries = 0; /* listed partner entries = 2 x pairs */
@@ -150,21 +144,29 @@ struct Sim {
     bool contactBuffers();
     bool nearMasks();
     bool regenerateCollisions();
-    int stepWithContacts(float dt, int nsteps, int* divergedAt);
     /* kernel timing */
     bool profiling = false;
     std::vector<hipEvent_t> events;
     double profMs[4] = {0, 0, 0, 0};
     int profCount[4] = {0, 0, 0, 0};
     /* mixed precision (vx_mixed.hip): second voxel buffer, FP32 link-state pairs */
-    DevBuf bvdAlt, bvflAlt, bvnbr, bms[2], bmfl[2], bdiv;
+    DevBuf bvdAlt, bvflAlt, bvtempAlt, bvnbr, bms[2], bmfl[2];
     int lcur = 0;          /* mixed link-state buffer that holds the current state */
-    float lastDt = 0.0f;   /* dt of the last completed step: every voxel's previousDt */
     bool mixedBuffers();
     bool uploadMixedLinks();
     bool downloadMixedLinks();
     void swapVoxelBuffers();
-    int stepMixed(float dt, int nsteps, int* divergedAt);
+    bool stepMixed(int nsteps);
+    /* one launch sequence of nsteps steps, each robot with its own dt; done[r] receives the
+       steps robot r completed (nsteps unless it diverged) */
+    bool stepFp64(int nsteps);
+    bool stepWithContacts(int nsteps);
+    int stepRobots(const float* dt, int nsteps, int* done, int* divergedAt);
+    std::vector<float> launchDt; /* per robot dt of the current launch */
+    bool beginLaunch(int nsteps);
+    bool endLaunch(int nsteps, int* done);
+    bool anyDiverged(int k, bool* all);
+    bool robotCom(double* com);
 
     ~Sim();
     int nv() const { return (int)vmat.size(); }

@@ -150,3 +150,79 @@ def grow_robot(seed, nvox, box=20):
         order.append(q)
         push(q)
     return np.array(order, dtype=np.int64)
+
+
+# robot draws live at counter indices far above the growth draws
+_N_DRAW = np.uint64(1 << 40)
+_MAT_DRAW = np.uint64(1 << 41)
+
+
+_native = None
+
+
+def grow_robot_native(seed, nvox, box=20):
+    """grow_robot in C (voxelyze_amd/csrc/vx_workloads.c, built as libvxwork.so): the same
+    rows, fast enough for populations of millions of voxels."""
+    global _native
+    if _native is None:
+        import ctypes as C
+        import os
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libvxwork.so")
+        if not os.path.exists(path):
+            raise RuntimeError("libvxwork.so not built: run python -m voxelyze_amd.build")
+        _native = C.CDLL(path)
+        _native.vxw_grow_robot.restype = C.c_int
+        _native.vxw_grow_robot.argtypes = [C.c_uint64, C.c_int, C.c_int, C.POINTER(C.c_int)]
+    import ctypes as C
+    out = np.zeros((nvox, 3), np.int32)
+    n = _native.vxw_grow_robot(seed, nvox, box, out.ctypes.data_as(C.POINTER(C.c_int)))
+    if n < 0:
+        raise MemoryError("vxw_grow_robot")
+    return out[:n].astype(np.int64)
+
+
+def robot_shape(seed, r, vmin=100, vmax=8000, box=20):
+    """Robot r of a population: size N_r = vmin + floor(u * (vmax - vmin + 1)) with u drawn
+    at counter 2^40 of stream seed+r, shape grown by grow_robot(seed+r, N_r, box) and shifted
+    so that its lowest layer is z = 0, material index per voxel floor(4u) with u drawn at
+    counter 2^41 + i.  Returns (ijk (n,3) int32, material index (n,) int32)."""
+    s = seed + r
+    n = vmin + int(uniform(s, _N_DRAW) * (vmax - vmin + 1))
+    ijk = grow_robot_native(s, n, box)
+    ijk[:, 2] -= ijk[:, 2].min()
+    mat = np.minimum((uniform(s, _MAT_DRAW + np.arange(len(ijk), dtype=np.uint64)) * 4).astype(np.int32), 3)
+    return ijk.astype(np.int32), mat
+
+
+def robot_shapes(seed, n_robots, vmin=100, vmax=8000, box=20):
+    """robot_shape for robots 0..n_robots-1."""
+    return [robot_shape(seed, r, vmin, vmax, box) for r in range(n_robots)]
+
+
+def robot_materials(sim, mu_s=1.0, mu_k=0.5, rho=1000.0):
+    """The four materials of the robot population: passive soft 0.1 MPa, passive stiff
+    10 MPa, and two actuated 1 MPa with CTE +0.01 and -0.01 per degC."""
+    mats = [sim.addMaterial(1e5, rho), sim.addMaterial(1e7, rho), sim.addMaterial(1e6, rho), sim.addMaterial(1e6, rho)]
+    mats[2].setCte(0.01)
+    mats[3].setCte(-0.01)
+    for m in mats:
+        m.setStaticFriction(mu_s)
+        m.setKineticFriction(mu_k)
+    return mats
+
+
+def robot_population(sim, shapes, amplitude=39.0, frequency=2.0, gravity=1.0, floor=True, phases=None):
+    """Places every shape as its own robot (robot 0 is the first), with gravity, floor
+    friction and the sinusoidal temperature actuation T = amplitude*sin(2*pi*frequency*t +
+    phase) evaluated on the device.  Returns the materials."""
+    mats = robot_materials(sim)
+    sim.setGravity(gravity)
+    sim.enableFloor(floor)
+    for r, (ijk, mat) in enumerate(shapes):
+        if r > 0:
+            sim.addRobot()
+        sim.setVoxels(np.asarray(mat, np.int32) + mats[0].index, ijk)
+    sim.setActuation(True, 0.0, amplitude, frequency)
+    if phases is not None:
+        sim.setPhases(phases)
+    return mats

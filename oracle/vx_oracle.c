@@ -95,6 +95,23 @@ static inline dvec qrotinv(dquat q, dvec f) {
               tw * q.z + tx * q.y - ty * q.x + tz * q.w);
 }
 
+#ifdef VXO_CR_TRIG
+/* The correctly rounded variant (liboracle_cr.so): sin, cos and acos evaluated in binary128
+   (libquadmath, error about one unit of a 113-bit significand) and rounded once to double,
+   which is the correctly rounded result unless the exact value lies within ~2^-60 ulp of a
+   rounding boundary.  It separates glibc's last-bit differences (glibc's double sin/cos/acos
+   are not correctly rounded) from everything else when the oracle is compared with a device
+   whose transcendentals are correctly rounded. */
+#include <quadmath.h>
+static inline double tr_sin(double x) { return (double)sinq((__float128)x); }
+static inline double tr_cos(double x) { return (double)cosq((__float128)x); }
+static inline double tr_acos(double x) { return (double)acosq((__float128)x); }
+#else
+#define tr_sin sin
+#define tr_cos cos
+#define tr_acos acos
+#endif
+
 /* Quat3D::ToRotationVector (Quat3D.h:117-122): Vec*2.0*s, and Vec/f = Vec*(1/f) */
 static inline dvec qtorv(dquat q) {
     if (q.w >= 1.0 || q.w <= -1.0) return dv(0, 0, 0);
@@ -103,7 +120,7 @@ static inline dvec qtorv(dquat q) {
         double s = sqrt((2 - 2 * q.w) / sl);
         return dv((q.x * 2.0) * s, (q.y * 2.0) * s, (q.z * 2.0) * s);
     } else {
-        double a = acos(q.w);
+        double a = tr_acos(q.w);
         double inv = 1.0 / sqrt(sl);
         return dv(inv * ((q.x * 2.0) * a), inv * ((q.y * 2.0) * a), inv * ((q.z * 2.0) * a));
     }
@@ -119,8 +136,8 @@ static inline dquat qfromrv(dvec v) {
         s = 1.0 - m2 / 6.0;
     } else {
         double m = sqrt(m2);
-        q.w = cos(m);
-        s = sin(m) / m;
+        q.w = tr_cos(m);
+        s = tr_sin(m) / m;
     }
     q.x = th.x * s;
     q.y = th.y * s;
@@ -142,12 +159,12 @@ static inline void qfromangletoposx(dquat* q, dvec rf) {
     dvec n = rf;
     double l = sqrt(n.x * n.x + n.y * n.y + n.z * n.z);
     if (l > 0) { double li = 1.0 / l; n.x *= li; n.y *= li; n.z *= li; }
-    double theta = acos(n.x);
+    double theta = tr_acos(n.x);
     if (theta > PI_REF - DISCARD_ANGLE_RAD) { q->w = 0; q->x = 0; q->y = 1; q->z = 0; return; }
     const double ami = 1.0 / sqrt(n.z * n.z + n.y * n.y);
     const double a = 0.5 * theta;
-    const double s = sin(a);
-    q->w = cos(a);
+    const double s = tr_sin(a);
+    q->w = tr_cos(a);
     q->x = 0;
     q->y = n.z * ami * s;
     q->z = -n.y * ami * s;

@@ -9,6 +9,7 @@ from voxelyze_amd.api import CVoxelyze
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+ORACLE_CR_SO = os.path.join(ROOT, "oracle", "liboracle_cr.so")
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "libvxref.so")
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 
@@ -18,6 +19,14 @@ def oracle_abi():
     if not os.path.exists(ORACLE_SO):
         subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "oracle"], check=True, capture_output=True)
     return bind(ORACLE_SO, "vxo_", "oracle")
+
+
+@functools.lru_cache(None)
+def oracle_cr_abi():
+    """The oracle with correctly rounded sin/cos/acos (what the device evaluates)."""
+    if not os.path.exists(ORACLE_CR_SO):
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "oracle"], check=True, capture_output=True)
+    return bind(ORACLE_CR_SO, "vxo_", "oracle_cr")
 
 
 @functools.lru_cache(None)

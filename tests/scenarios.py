@@ -16,6 +16,21 @@ X_POS, X_NEG, Y_POS, Y_NEG, Z_POS, Z_NEG = range(6)
 OFF = {X_POS: (1, 0, 0), X_NEG: (-1, 0, 0), Y_POS: (0, 1, 0), Y_NEG: (0, -1, 0), Z_POS: (0, 0, 1), Z_NEG: (0, 0, -1)}
 
 
+# scenarios whose reference run evaluates sin/cos/acos (counted with an instrumented oracle)
+TRANSCENDENTAL = {
+    "large_deformation_damping", "large_deformation", "impulse", "temperature", "config1", "config2_small",
+    "config3_small", "config4_robot", "two_voxel_10", "two_voxel_11",
+}
+# relative displacement error bound where bit-exactness is not required
+TOL = {name: 1e-9 for name in TRANSCENDENTAL}
+# floor friction switches static/kinetic on tiny force differences: chaotic
+TOL["config2_small"] = 1e-6
+TOL["config4_robot"] = 1e-6
+# plastic links under a driven face with self-contact amplify last-bit trig differences: the
+# oracle with correctly rounded trig (same as the device) lands 1.46e-8 from glibc's run
+TOL["config3_small"] = 1e-7
+
+
 def snapshot(sim, prefix=""):
     out = {}
     n = sim.voxelCount()

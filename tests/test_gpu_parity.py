@@ -10,7 +10,10 @@ function.  Where the reference calls glibc's sin/cos/acos (large-angle links,
 per-step rotations above ~2.7e-4 rad, relative link rotations above ~0.1 rad),
 the device evaluates correctly rounded values; glibc's are within 0.55 ulp and
 differ from the correctly rounded one on ~0.1% of arguments, so there the bar is
-a relative-displacement tolerance stated per scenario (TOL).
+a relative-displacement tolerance stated per scenario (TOL).  Every scenario,
+transcendental or not, is also run through the oracle built with correctly
+rounded sin/cos/acos (oracle/liboracle_cr.so): against it the device is
+bit-exact everywhere.
 """
 import numpy as np
 import pytest
@@ -18,16 +21,7 @@ import pytest
 import backends as B
 import scenarios as S
 
-# scenarios whose reference run evaluates sin/cos/acos (counted with an instrumented oracle)
-TRANSCENDENTAL = {
-    "large_deformation_damping", "large_deformation", "impulse", "temperature", "config1", "config2_small",
-    "config3_small", "config4_robot", "two_voxel_10", "two_voxel_11",
-}
-# relative displacement error bound where bit-exactness is not required
-TOL = {name: 1e-9 for name in TRANSCENDENTAL}
-# floor friction switches static/kinetic on tiny force differences: chaotic
-TOL["config2_small"] = 1e-6
-TOL["config4_robot"] = 1e-6
+TRANSCENDENTAL, TOL = S.TRANSCENDENTAL, S.TOL
 
 NAMES = sorted(S.SCENARIOS)
 
@@ -73,6 +67,17 @@ def test_hip_fp64_matches_reference(name):
             g, o = gold[k], out[k]
             scale = max(np.abs(g).max(), 1e-300)
             assert np.abs(o - g).max() <= TOL[name] * scale * 10, k
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", NAMES)
+def test_hip_fp64_matches_cr_oracle_bitwise(name):
+    """Same arithmetic as the reference restated, trig correctly rounded on both sides."""
+    print(f"[parity-cr] {name}", flush=True)
+    ref = S.SCENARIOS[name](B.maker(B.oracle_cr_abi()))
+    out = S.SCENARIOS[name](B.maker(B.product_abi()))
+    bad = [k for k in ref if not bits_equal(ref[k], out[k])]
+    assert not bad, f"{name}: differs from the correctly rounded oracle in {bad[:8]}"
 
 
 @pytest.mark.gpu

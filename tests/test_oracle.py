@@ -43,6 +43,26 @@ def test_oracle_matches_golden_bitwise(name, oracle_runs):
     assert not bad, f"{name}: differs from the reference in {bad[:8]}"
 
 
+@pytest.mark.parametrize("name", NAMES)
+def test_cr_oracle_against_golden(name):
+    """The oracle with correctly rounded sin/cos/acos: bit-exact wherever the reference
+    never calls them, within the stated displacement tolerance where it does."""
+    gold = B.load_golden(name)
+    out = S.SCENARIOS[name](B.maker(B.oracle_cr_abi()))
+    bad = [k for k in gold if not bits_equal(gold[k], out[k])]
+    if name not in S.TRANSCENDENTAL:
+        assert not bad, f"{name}: differs from the reference in {bad[:8]}"
+        return
+    worst = 0.0
+    for k in gold:
+        if k.endswith("vox_pos"):
+            pre = k[: -len("vox_pos")]
+            nominal = gold[pre + "vox_ijk"] * gold[pre + "voxel_size"]
+            disp = np.abs(gold[k] - nominal).max()
+            worst = max(worst, np.abs(out[k] - gold[k]).max() / max(disp, 1e-300))
+    assert worst <= S.TOL[name], f"{name}: {worst:.3e}"
+
+
 # --------------------------------------------------------- closed forms of tVoxelyze.h
 def float_eq(a, b, ulps=4):
     """EXPECT_FLOAT_EQ: within 4 ulps in float."""

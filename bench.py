@@ -33,13 +33,14 @@ HBM_PEAK_GBS = 8000.0
 #                                  strain state 8 R + 16 W, end forces 96 W) + 56 B per voxel (pose gather)
 #   voxel kernel: 244 B per voxel (state 104 R + 104 W, neighbour table 24, metadata 12) + 96 B per link
 #                                  (end-force gather)
-#   fused mixed step: 248 B per voxel (state 104 R + 104 W, flags 4 R + 4 W, material and external
-#                                  index 8, neighbour table 24) + 124 B per link (FP32 link state
-#                                  52 R + 52 W, flags 4 R + 4 W, rest length 8, material index 4)
+#   mixed step  : the fixed mixed-precision byte model of the path, 300 B per voxel (pose gather 56,
+#                 state 104 R + 104 W, neighbour table 24, metadata 12) + 212 B per link (FP32 local
+#                 state 36 R + 36 W, rest length 4, end forces 48 W + 48 R, meta 40), against which the
+#                 fused kernel (which has no force hand-off) is also reported
 ALG_BYTES = {
     "link": lambda V, L: 288.0 * L + 56.0 * V,
     "voxel": lambda V, L: 244.0 * V + 96.0 * L,
-    "step_mixed": lambda V, L: 248.0 * V + 124.0 * L,
+    "step_mixed": lambda V, L: 300.0 * V + 212.0 * L,
 }
 # profile classes of vxg_profile_read
 PROFILE_CLASSES = ["link", "voxel", "contact", "step_mixed"]

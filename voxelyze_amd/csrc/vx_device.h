@@ -121,6 +121,21 @@ struct DevPtrs {
 
 #define NO_DIVERGENCE 0x7F7F7F7F /* a byte fill of 0x7F */
 
+#ifdef __HIPCC__
+}  // namespace vx
+#include <hip/hip_runtime.h>
+namespace vx {
+/* MI355X dispatches the workgroups of a launch to its 8 XCDs round-robin (workgroup b to XCD
+   b % 8), and each XCD has its own L2.  This maps the hardware workgroup index to a logical
+   one so that every XCD works through one contiguous range of logical blocks: neighbouring
+   voxels, whose poses a link gathers, then meet in the same L2.  A bijection for any grid. */
+__device__ __forceinline__ int xcd_block() {
+    const int b = (int)blockIdx.x, nb = (int)gridDim.x;
+    const int q = nb >> 3, r = nb & 7, x = b & 7;
+    return x * q + (x < r ? x : r) + (b >> 3);
+}
+#endif
+
 /* self-collision buffers (vx_contact.hip) */
 struct ContactPtrs {
     int nv;

@@ -48,16 +48,13 @@ __device__ __forceinline__ float link_stress(const LinkMat& m, const float* __re
     return 0.0f;
 }
 
-/* one lane per link slot: grid (ceil(nv/256), 3), blockIdx.y is the link axis and the lane's
-   voxel is the link's negative end.  Every slot and every force slot is written on every
-   step (zeros where no link exists) so that all stores cover whole cache lines. */
-__global__ void __launch_bounds__(256) k_links_fp64(DevPtrs s, int k) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+/* The link of slot (axis, n) -- n is its negative voxel -- given n's pose.  Every slot and
+   every force slot is written on every step (zeros where no link exists) so that all stores
+   cover whole cache lines. */
+template <int axis>
+__device__ __forceinline__ void link_slot_fp64(const DevPtrs& s, int k, int rob, int n, const dvec& posN,
+                                               const dquat& qN) {
     const int nv = s.nv;
-    if (n >= nv) return;
-    const int rob = s.vrobot[n];
-    if (s.rdiv[rob] < k) return; /* the robot stopped at an earlier step of this launch */
-    const int axis = blockIdx.y;
     const int nl = s.nl;
     const int i = axis * nv + n;
     const double* __restrict__ vd = s.vd;
@@ -83,9 +80,7 @@ __global__ void __launch_bounds__(256) k_links_fp64(DevPtrs s, int k) {
     }
     const LinkMat m = s.lmats[s.lmeta[i]];
 
-    const dvec posN = dv(vd[VPX * nv + n], vd[VPY * nv + n], vd[VPZ * nv + n]);
     const dvec posP = dv(vd[VPX * nv + p], vd[VPY * nv + p], vd[VPZ * nv + p]);
-    const dquat qN = dq(vd[VQW * nv + n], vd[VQX * nv + n], vd[VQY * nv + n], vd[VQZ * nv + n]);
     const dquat qP = dq(vd[VQW * nv + p], vd[VQX * nv + p], vd[VQY * nv + p], vd[VQZ * nv + p]);
     const dvec oldPos2 = dv(ld[LP2X * nl + i], ld[LP2Y * nl + i], ld[LP2Z * nl + i]);
     const dvec oldA1 = dv(ld[LA1X * nl + i], ld[LA1Y * nl + i], ld[LA1Z * nl + i]);
@@ -209,6 +204,22 @@ __global__ void __launch_bounds__(256) k_links_fp64(DevPtrs s, int k) {
     }
 }
 
+/* one lane per voxel: the three links it is the negative end of (axes x, y, z), its own pose
+   read once */
+__global__ void __launch_bounds__(256) k_links_fp64(DevPtrs s, int k) {
+    const int n = (int)blockIdx.x * 256 + (int)threadIdx.x;
+    const int nv = s.nv;
+    if (n >= nv) return;
+    const int rob = s.vrobot[n];
+    if (s.rdiv[rob] < k) return; /* the robot stopped at an earlier step of this launch */
+    const double* __restrict__ vd = s.vd;
+    const dvec posN = dv(vd[VPX * nv + n], vd[VPY * nv + n], vd[VPZ * nv + n]);
+    const dquat qN = dq(vd[VQW * nv + n], vd[VQX * nv + n], vd[VQY * nv + n], vd[VQZ * nv + n]);
+    link_slot_fp64<0>(s, k, rob, n, posN, qN);
+    link_slot_fp64<1>(s, k, rob, n, posN, qN);
+    link_slot_fp64<2>(s, k, rob, n, posN, qN);
+}
+
 /* CVX_Voxel::floorPenetration with baseSizeAverage (VX_Voxel.h:88-90,122) */
 __device__ __forceinline__ float floor_penetration(const VoxMat& m, float temp, double pz) {
     const float f = 1 + temp * m.alphaCTE;
@@ -218,7 +229,7 @@ __device__ __forceinline__ float floor_penetration(const VoxMat& m, float temp, 
 }
 
 __global__ void __launch_bounds__(256) k_voxels_fp64(DevPtrs s, int k) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = (int)blockIdx.x * 256 + (int)threadIdx.x;
     if (i >= s.nv) return;
     const int rob = s.vrobot[i];
     if (s.rdiv[rob] <= k) return; /* diverged in this step or earlier: voxels and time stay */
@@ -418,7 +429,7 @@ __global__ void __launch_bounds__(256) k_robot_com(DevPtrs s, const int* rfirst,
 static inline unsigned blocks(int n, int b) { return (unsigned)((n + b - 1) / b); }
 
 void launch_links_fp64(const DevPtrs& s, int k, hipStream_t st) {
-    if (s.nv > 0) hipLaunchKernelGGL(k_links_fp64, dim3(blocks(s.nv, 256), 3), dim3(256), 0, st, s, k);
+    if (s.nv > 0) hipLaunchKernelGGL(k_links_fp64, dim3(blocks(s.nv, 256)), dim3(256), 0, st, s, k);
 }
 void launch_voxels_fp64(const DevPtrs& s, int k, hipStream_t st) {
     if (s.nv > 0) hipLaunchKernelGGL(k_voxels_fp64, dim3(blocks(s.nv, 256)), dim3(256), 0, st, s, k);

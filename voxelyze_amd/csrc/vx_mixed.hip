@@ -222,7 +222,7 @@ __device__ void frozen_copy(const MixPtrs& s, int i, int rob, int d) {
 }
 
 __global__ void __launch_bounds__(256) k_step_mixed(MixPtrs s) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = xcd_block() * 256 + (int)threadIdx.x;
     const int nv = s.nv;
     if (i >= nv) return;
     const int rob = s.vrobot[i];

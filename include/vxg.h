@@ -62,7 +62,9 @@ enum vxg_dof {
 /* precision modes (vxg_set_precision) */
 enum vxg_precision {
     VXG_FP64_PLAIN = 0, /* every operation in the reference's own types, bit-faithful */
-    VXG_MIXED = 1       /* FP32 link forces, FP64 pose/momentum integration */
+    VXG_MIXED = 1,      /* FP32 link forces, FP64 pose/momentum integration (one fused kernel per step) */
+    VXG_FUSED_FP64 = 2  /* the fused step of VXG_MIXED with FP64 link arithmetic: the FP64 reference of
+                           VXG_MIXED and of its adjoint */
 };
 
 /* link flag bits returned by vxg_link_get_state */
@@ -264,6 +266,29 @@ int vxg_robot_com(vxg_sim* s, double* com);
 int vxg_set_actuation(vxg_sim* s, int enabled, float base, float amplitude, float frequency);
 /* phase (radians) of voxels [first, first+count) */
 int vxg_voxel_set_phase(vxg_sim* s, int first, int count, const float* phase);
+
+/* ------------------------------------------- per-voxel moduli and the adjoint
+   No counterpart in the reference.  vxg_voxel_set_modulus gives voxels [first,
+   first+count) their own Young's modulus E (> 0) in place of their material's; a link
+   then takes the harmonic mean 2 E_n E_p / (E_n + E_p) of its voxels' moduli, as the
+   reference combines two linear materials (src/VX_MaterialLink.cpp:70), with the beam
+   constants of src/VX_MaterialLink.cpp:121-141.  Runs in VXG_MIXED or VXG_FUSED_FP64 with linear
+   materials.
+   vxg_clear_moduli returns to the materials' moduli. */
+int vxg_voxel_set_modulus(vxg_sim* s, int first, int count, const float* youngs_modulus);
+void vxg_clear_moduli(vxg_sim* s);
+/* Gradients of a rollout.  vxg_adjoint_run steps every robot nsteps times (robot r with
+   dt[r], or its recommended step when dt is NULL or dt[r] < 0) in VXG_MIXED (or VXG_FUSED_FP64), keeping a
+   checkpoint every `every` steps; vxg_adjoint_backward then takes dL/d(final voxel
+   positions) (3 doubles per voxel) and writes dL/dlog(E_v) and dL/d(phase_v) for every
+   voxel, recomputing each segment from its checkpoint, and leaves the simulation at the
+   end of the rollout.  The derivative follows every branch the forward run took (contact,
+   friction, small/large-angle links); dt is held fixed.  Requires: linear materials
+   without failure, no self-collision, voxels free or fully fixed.  Returns 1 or 0
+   (vxg_last_error). */
+int vxg_adjoint_run(vxg_sim* s, int nsteps, int every, const float* dt);
+int vxg_adjoint_backward(vxg_sim* s, const double* dl_dpos, double* grad_log_modulus, double* grad_phase);
+void vxg_adjoint_free(vxg_sim* s);
 
 #ifdef __cplusplus
 }

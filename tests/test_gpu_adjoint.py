@@ -1,0 +1,166 @@
+"""The hand-written adjoint (vxg_adjoint_run / vxg_adjoint_backward, voxelyze_amd.adjoint).
+
+Model: robots of three linear materials (passive 1 MPa, actuated 1 MPa with CTE +-0.01),
+per-voxel Young's modulus E_v = exp(theta_v) and actuation phase phi_v, sinusoidal
+temperature of +-39 degC at 2 Hz, gravity and floor friction, mixed-precision step.
+Loss: minus the summed centre-of-mass x-travel of the robots.
+
+Checks:
+  * central finite differences along 16 seeded random directions in (theta, phi) agree
+    with the adjoint's directional derivative (the bar and why it is not tighter are in
+    test_adjoint_matches_central_differences);
+  * the mixed step's adjoint agrees with the FP64 fused step's adjoint;
+  * the checkpoint interval does not change the gradient (bit for bit);
+  * the autograd.Function gives the same gradient as the raw ABI.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import backends as B
+from voxelyze_amd import workloads as W
+
+AMP, FREQ = 39.0, 2.0
+NSTEPS = 300
+
+
+def build(shapes, precision=1):
+    sim = B.maker(B.product_abi())(0.005)
+    mats = W.robot_materials(sim)
+    sim.setGravity(1.0)
+    sim.enableFloor(True)
+    for r, (ijk, mat) in enumerate(shapes):
+        if r:
+            sim.addRobot()
+        # three materials: passive 1 MPa (index 2 with CTE 0), actuated +-CTE
+        sim.setVoxels(np.asarray([mats[2 + (m % 2)].index if m >= 2 else mats[0].index for m in mat]), ijk)
+    mats[0].setModelLinear(1e6)  # passive material: 1 MPa, CTE 0
+    sim.setActuation(True, 0.0, AMP, FREQ)
+    sim.setPrecision(precision)
+    return sim
+
+
+def shapes():
+    return W.robot_shapes(71, 2, vmin=40, vmax=120, box=6)
+
+
+def params(sim, seed=3):
+    n = sim.voxelCount()
+    u = W.uniform(seed, np.arange(2 * n, dtype=np.uint64))
+    theta = np.log(1e6) + 0.3 * (u[:n] - 0.5)
+    phi = 2 * math.pi * u[n:]
+    return theta, phi
+
+
+def loss_and_grad(sim, theta, phi, nsteps=NSTEPS, every=100, grad=True):
+    sim.resetTime()
+    sim.setModuli(np.exp(theta).astype(np.float32))
+    sim.setPhases(phi.astype(np.float32))
+    pos0 = sim.state()["pos"]
+    sim.adjointRun(nsteps, every)
+    pos = sim.state()["pos"]
+    dl = np.zeros_like(pos)
+    loss = 0.0
+    for r in range(sim.robotCount()):
+        f, c = sim.robotVoxels(r)
+        loss -= (pos[f:f + c, 0] - pos0[f:f + c, 0]).mean()  # equal masses: CoM is the mean
+        dl[f:f + c, 0] = -1.0 / c
+    if not grad:
+        return loss, None, None
+    ge, gp = sim.adjointBackward(dl)
+    return loss, ge, gp
+
+
+def weighted_loss(sim, theta, phi, w, nsteps, grad=True):
+    """L = sum(w * final positions) after nsteps from the initial state."""
+    sim.resetTime()
+    sim.setModuli(np.exp(theta).astype(np.float32))
+    sim.setPhases(phi.astype(np.float32))
+    sim.adjointRun(nsteps, 50)
+    loss = float((w * sim.state()["pos"]).sum())
+    if not grad:
+        return loss, None, None
+    ge, gp = sim.adjointBackward(w)
+    return loss, ge, gp
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [2, 1])
+def test_adjoint_matches_central_differences(precision):
+    """16 seeded random directions in (theta, phi); Richardson-extrapolated central
+    differences (steps 1e-2 and 5e-3).  The moduli and phases are FP32 parameters and the
+    beam constants derived from them are FP32 (as in the reference), which puts a noise
+    floor of ~1e-10 on the loss; the difference is measured against the size of the
+    directional derivative (sum of |gradient| * |direction|): 5e-5 for the FP64 fused step
+    (measured 2.7e-5), 2e-4 for the mixed step.  Smooth setting: no floor contact (friction
+    switching makes the loss piecewise smooth), 5 degC actuation, 100 steps."""
+    sim = build(shapes(), precision)
+    sim.enableFloor(False)
+    sim.setGravity(0.0)
+    sim.setActuation(True, 0.0, 5.0, FREQ)
+    theta, phi = params(sim)
+    n = sim.voxelCount()
+    w = (W.uniform(77, np.arange(3 * n, dtype=np.uint64)) * 2 - 1).reshape(n, 3)
+    _, ge, gp = weighted_loss(sim, theta, phi, w, 100)
+    worst = 0.0
+    for d in range(16):
+        u = W.uniform(100 + d, np.arange(2 * n, dtype=np.uint64)) * 2 - 1
+        ut, up = u[:n], u[n:]
+        ad = ge @ ut + gp @ up
+
+        def fd(h):
+            lp, _, _ = weighted_loss(sim, theta + h * ut, phi + h * up, w, 100, grad=False)
+            lm, _, _ = weighted_loss(sim, theta - h * ut, phi - h * up, w, 100, grad=False)
+            return (lp - lm) / (2 * h)
+
+        f1, f2 = fd(1e-2), fd(5e-3)
+        rich = (4 * f2 - f1) / 3
+        scale = np.abs(ge) @ np.abs(ut) + np.abs(gp) @ np.abs(up)  # the directional derivative's size
+        err = abs(rich - ad) / scale
+        worst = max(worst, err)
+        print(f"[adjoint] precision {precision} direction {d}: adjoint {ad:.10e} fd {rich:.10e} rel {err:.2e}", flush=True)
+    print(f"[adjoint] precision {precision}: worst relative difference {worst:.2e}")
+    assert worst <= (5e-5 if precision == 2 else 2e-4), worst
+
+
+@pytest.mark.gpu
+def test_mixed_adjoint_equals_fp64_adjoint():
+    """The adjoint of the mixed step against the adjoint of the FP64 fused step, on the
+    robots with gravity, floor friction and +-39 degC actuation (C5-like), 300 steps."""
+    shp = shapes()
+    a, b = build(shp, 2), build(shp, 1)
+    theta, phi = params(a)
+    la, gea, gpa = loss_and_grad(a, theta, phi)
+    lb, geb, gpb = loss_and_grad(b, theta, phi)
+    et = np.abs(geb - gea).max() / np.abs(gea).max()
+    ep = np.abs(gpb - gpa).max() / np.abs(gpa).max()
+    print(f"[adjoint] loss fp64 {la:.9e} mixed {lb:.9e}; gradient difference theta {et:.2e} phi {ep:.2e}")
+    assert et <= 1e-3 and ep <= 1e-3
+
+
+@pytest.mark.gpu
+def test_checkpoint_interval_does_not_change_the_gradient():
+    sim = build(shapes())
+    theta, phi = params(sim, seed=4)
+    _, ge1, gp1 = loss_and_grad(sim, theta, phi, every=NSTEPS)
+    _, ge2, gp2 = loss_and_grad(sim, theta, phi, every=37)
+    assert ge1.tobytes() == ge2.tobytes() and gp1.tobytes() == gp2.tobytes()
+
+
+@pytest.mark.gpu
+def test_autograd_function_equals_raw_adjoint():
+    import torch
+    from voxelyze_amd.adjoint import robot_com, rollout
+    sim = build(shapes())
+    theta, phi = params(sim, seed=5)
+    _, ge, gp = loss_and_grad(sim, theta, phi)
+    t = torch.tensor(theta, requires_grad=True)
+    p = torch.tensor(phi, requires_grad=True)
+    sim.resetTime()
+    pos0 = torch.from_numpy(sim.state()["pos"].copy())
+    pos = rollout(sim, t, p, NSTEPS)
+    loss = -(robot_com(sim, pos)[:, 0] - robot_com(sim, pos0)[:, 0]).sum()
+    loss.backward()
+    assert np.allclose(t.grad.numpy(), ge, rtol=1e-12, atol=0)
+    assert np.allclose(p.grad.numpy(), gp, rtol=1e-12, atol=0)

@@ -88,6 +88,11 @@ EXTENSIONS = {
     "robot_com": (_I, [_P, _DP]),
     "set_actuation": (_I, [_P, _I, _F, _F, _F]),
     "voxel_set_phase": (_I, [_P, _I, _I, _FP]),
+    "voxel_set_modulus": (_I, [_P, _I, _I, _FP]),
+    "clear_moduli": (None, [_P]),
+    "adjoint_run": (_I, [_P, _I, _I, _FP]),
+    "adjoint_backward": (_I, [_P, _DP, _DP, _DP]),
+    "adjoint_free": (None, [_P]),
 }
 
 

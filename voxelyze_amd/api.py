@@ -287,6 +287,7 @@ class CVX_Voxel:
         mi = np.float32(1.0) / np.float32(self.material().mass())
         return self.linMom() * np.float64(mi)
 
+    def mass(self): return self.material().mass()
     def isFloorStaticFriction(self): return bool(self._state()[4] & 8)
     def isInterior(self): return bool(self._state()[4] & 2)
     def isSurface(self): return not self.isInterior()
@@ -484,6 +485,32 @@ class CVoxelyze:
     def setPhases(self, phase, first=0):
         ph = np.ascontiguousarray(phase, dtype=np.float32)
         self._check(bool(self.abi.voxel_set_phase(self.handle, first, len(ph), _fptr(ph))))
+
+    # per-voxel moduli and the adjoint (extensions of the HIP library, include/vxg.h)
+    def setModuli(self, E, first=0):
+        e = np.ascontiguousarray(E, dtype=np.float32)
+        self._check(bool(self.abi.voxel_set_modulus(self.handle, first, len(e), _fptr(e))))
+
+    def clearModuli(self): self.abi.clear_moduli(self.handle)
+
+    def adjointRun(self, nsteps, every=100, dt=None):
+        """Steps every robot nsteps times, keeping checkpoints for adjointBackward."""
+        d = None
+        if dt is not None:
+            d = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, np.float32), (self.robotCount(),)))
+        self._check(bool(self.abi.adjoint_run(self.handle, nsteps, every, _fptr(d) if d is not None else None)))
+
+    def adjointBackward(self, dl_dpos):
+        """dL/d(final positions) (V, 3) -> (dL/dlog E, dL/dphase), each (V,) float64."""
+        g = np.ascontiguousarray(dl_dpos, dtype=np.float64).reshape(-1, 3)
+        n = self.voxelCount()
+        if g.shape[0] != n:
+            raise ValueError(f"dl_dpos has {g.shape[0]} rows for {n} voxels")
+        ge, gp = np.zeros(n), np.zeros(n)
+        self._check(bool(self.abi.adjoint_backward(self.handle, _dptr(g), _dptr(ge), _dptr(gp))))
+        return ge, gp
+
+    def adjointFree(self): self.abi.adjoint_free(self.handle)
 
     # bulk state
     def state(self, first=0, count=None):

@@ -1,0 +1,216 @@
+/*
+ * vx_adjoint.hip -- the reverse pass of the mixed-precision step on the device.
+ *
+ * One reverse step k (state k -> state k+1 recorded by the forward) is two kernels,
+ * the transposes of the forward's two halves:
+ *   k_adj_voxels  one lane per voxel: completes dL/d(state k+1) from the previous reverse
+ *                 step (its own part plus the <= 6 link-end contributions, gathered in
+ *                 direction order, no atomics), then applies voxel_vjp: dL/d(link force
+ *                 and moment sums) and its own part of dL/d(state k); accumulates the
+ *                 voxel's log-modulus and phase gradients.
+ *   k_adj_links   one lane per voxel over the three links it is the negative end of:
+ *                 link_vjp, writing each end's dL/d(pose), log-modulus and temperature
+ *                 contributions into per-voxel slots (direction 2a of the negative voxel,
+ *                 2a+1 of the positive one, as the forward hands forces over) and
+ *                 dL/d(the link's stored local pose) for the step before.
+ * All arithmetic is FP64 (vx_adjoint.h); nothing is summed with atomics, so gradients
+ * are bit-reproducible.
+ */
+#include "vx_adjoint.h"
+#include "vx_device.h"
+#include "vx_math.h"
+
+#pragma clang fp contract(off)
+
+namespace vx {
+
+__device__ __forceinline__ size_t slot_index(int d, int c, int nv, int v) { return ((size_t)d * AS_COUNT + c) * nv + v; }
+
+__device__ __forceinline__ bool fully_fixed(const AdjPtrs& a, int i) {
+    const int e = a.vext[i];
+    return e >= 0 && (a.exts[e].dof & 0x3F) == 0x3F;
+}
+
+__global__ void __launch_bounds__(256) k_adj_voxels(AdjPtrs a) {
+    const int i = (int)blockIdx.x * 256 + (int)threadIdx.x;
+    const int nv = a.nv;
+    if (i >= nv) return;
+    const int rob = a.vrobot[i];
+    /* ---- dL/d(state k+1) */
+    dvec lPos1, lLin1, lAng1;
+    dquat lQ1;
+    if (a.first) {
+        lPos1 = dv(a.lamFinal[3 * i], a.lamFinal[3 * i + 1], a.lamFinal[3 * i + 2]);
+        lLin1 = lAng1 = dv(0, 0, 0);
+        lQ1 = qzero();
+    } else {
+        const double* P = a.part;
+        lPos1 = dv(P[AP_POS * nv + i], P[(AP_POS + 1) * nv + i], P[(AP_POS + 2) * nv + i]);
+        lLin1 = dv(P[AP_LIN * nv + i], P[(AP_LIN + 1) * nv + i], P[(AP_LIN + 2) * nv + i]);
+        lAng1 = dv(P[AP_ANG * nv + i], P[(AP_ANG + 1) * nv + i], P[(AP_ANG + 2) * nv + i]);
+        lQ1 = dq(P[AP_Q * nv + i], P[(AP_Q + 1) * nv + i], P[(AP_Q + 2) * nv + i], P[(AP_Q + 3) * nv + i]);
+        double th = 0.0, lt = a.lamT[i];
+        for (int d = 0; d < 6; d++) { /* the link ends of step k+1, in direction order */
+            const double* S = a.slots;
+            lPos1.x += S[slot_index(d, AS_POS, nv, i)];
+            lPos1.y += S[slot_index(d, AS_POS + 1, nv, i)];
+            lPos1.z += S[slot_index(d, AS_POS + 2, nv, i)];
+            lQ1.w += S[slot_index(d, AS_Q, nv, i)];
+            lQ1.x += S[slot_index(d, AS_Q + 1, nv, i)];
+            lQ1.y += S[slot_index(d, AS_Q + 2, nv, i)];
+            lQ1.z += S[slot_index(d, AS_Q + 3, nv, i)];
+            th += S[slot_index(d, AS_THETA, nv, i)];
+            lt += S[slot_index(d, AS_TEMP, nv, i)];
+        }
+        a.gTheta[i] += th;
+        if (a.vphase) { /* T = base + amp sin(2 pi f t + phase) at step k+1 */
+            const double arg = 6.283185307179586 * (double)a.actFreq * (double)a.rtimeNext[rob] + (double)a.vphase[i];
+            a.gPhase[i] += lt * (double)a.actAmp * cos(arg);
+        }
+    }
+    if (a.combineOnly) return;
+    const int k = a.k;
+    (void)k;
+    /* ---- voxel_vjp at step k */
+    double* P = a.part;
+    if (fully_fixed(a, i)) { /* prescribed pose: nothing flows back through the voxel */
+        for (int c = 0; c < AP_COUNT; c++) P[c * nv + i] = 0.0;
+        for (int c = 0; c < 6; c++) a.lamSum[c * nv + i] = 0.0;
+        a.lamT[i] = 0.0;
+        return;
+    }
+    VoxMat m = a.vmats[a.vmat[i]];
+    if (a.vE) voxel_modulus(&m, a.vE[i]);
+    const double* vd = a.t.vd;
+    VoxAdjIn in;
+    in.pos = dv(vd[VPX * nv + i], vd[VPY * nv + i], vd[VPZ * nv + i]);
+    in.lin = dv(vd[VLX * nv + i], vd[VLY * nv + i], vd[VLZ * nv + i]);
+    in.ang = dv(vd[VAX * nv + i], vd[VAY * nv + i], vd[VAZ * nv + i]);
+    in.q = dq(vd[VQW * nv + i], vd[VQX * nv + i], vd[VQY * nv + i], vd[VQZ * nv + i]);
+    const double* sm = a.t.sums;
+    in.sumF = dv(sm[i], sm[nv + i], sm[2 * nv + i]);
+    in.sumM = dv(sm[3 * nv + i], sm[4 * nv + i], sm[5 * nv + i]);
+    in.flagsIn = a.t.vflIn[i];
+    in.flagsOut = a.t.vflOut[i];
+    in.dt = a.rdt[rob];
+    in.massInverse = m.massInverse;
+    in.momentInertiaInverse = m.momentInertiaInverse;
+    in.globalTransC = m.globalTransC;
+    in.globalRotC = m.globalRotC;
+    in.collisionTransC = m.collisionTransC;
+    in.penetrationStiff = m.penetrationStiff;
+    in.muStatic = m.muStatic;
+    in.muKinetic = m.muKinetic;
+    const float T = a.t.vtemp[i];
+    const float f = 1 + T * m.alphaCTE;
+    const double bsa = (f * m.sizeX + f * m.sizeY + f * m.sizeZ) / 3.0;
+    in.halfBsaMinusHalfL = bsa / 2 - m.nomSize / 2;
+    in.dBsaHalf_dT = (double)m.alphaCTE * (m.sizeX + m.sizeY + m.sizeZ) / 6.0;
+    in.floorOn = (in.flagsIn & VF_FLOOR) != 0;
+    VoxAdjOut o;
+    voxel_vjp(in, lPos1, lLin1, lAng1, lQ1, &o);
+    P[AP_POS * nv + i] = o.lpos.x; P[(AP_POS + 1) * nv + i] = o.lpos.y; P[(AP_POS + 2) * nv + i] = o.lpos.z;
+    P[AP_LIN * nv + i] = o.lin.x; P[(AP_LIN + 1) * nv + i] = o.lin.y; P[(AP_LIN + 2) * nv + i] = o.lin.z;
+    P[AP_ANG * nv + i] = o.lang.x; P[(AP_ANG + 1) * nv + i] = o.lang.y; P[(AP_ANG + 2) * nv + i] = o.lang.z;
+    P[AP_Q * nv + i] = o.lq.w; P[(AP_Q + 1) * nv + i] = o.lq.x; P[(AP_Q + 2) * nv + i] = o.lq.y;
+    P[(AP_Q + 3) * nv + i] = o.lq.z;
+    double* L = a.lamSum;
+    L[i] = o.lsumF.x; L[nv + i] = o.lsumF.y; L[2 * nv + i] = o.lsumF.z;
+    L[3 * nv + i] = o.lsumM.x; L[4 * nv + i] = o.lsumM.y; L[5 * nv + i] = o.lsumM.z;
+    a.gTheta[i] += o.ltheta;
+    a.lamT[i] = o.ltemp;
+}
+
+__device__ __forceinline__ void write_slot(const AdjPtrs& a, int d, int v, dvec lp, dquat lq, double th, double lt) {
+    const int nv = a.nv;
+    double* S = a.slots;
+    S[slot_index(d, AS_POS, nv, v)] = lp.x;
+    S[slot_index(d, AS_POS + 1, nv, v)] = lp.y;
+    S[slot_index(d, AS_POS + 2, nv, v)] = lp.z;
+    S[slot_index(d, AS_Q, nv, v)] = lq.w;
+    S[slot_index(d, AS_Q + 1, nv, v)] = lq.x;
+    S[slot_index(d, AS_Q + 2, nv, v)] = lq.y;
+    S[slot_index(d, AS_Q + 3, nv, v)] = lq.z;
+    S[slot_index(d, AS_THETA, nv, v)] = th;
+    S[slot_index(d, AS_TEMP, nv, v)] = lt;
+}
+
+template <int axis>
+__device__ __forceinline__ void adj_link(const AdjPtrs& a, int n) {
+    const int nv = a.nv, S = 3 * nv;
+    const int slot = axis * nv + n;
+    const int p = a.vnbr[(2 * axis) * nv + n];
+    if (a.vnbr[(2 * axis + 1) * nv + n] < 0) write_slot(a, 2 * axis + 1, n, dv(0, 0, 0), qzero(), 0.0, 0.0);
+    if (p < 0) {
+        write_slot(a, 2 * axis, n, dv(0, 0, 0), qzero(), 0.0, 0.0);
+        return;
+    }
+    const double* vd = a.t.vd;
+    const LinkMat lm = a.lmats[a.lmeta[slot]];
+    VoxMat mn = a.vmats[a.vmat[n]], mp = a.vmats[a.vmat[p]];
+    LinkAdjIn in;
+    in.axis = axis;
+    in.posN = dv(vd[VPX * nv + n], vd[VPY * nv + n], vd[VPZ * nv + n]);
+    in.qN = dq(vd[VQW * nv + n], vd[VQX * nv + n], vd[VQY * nv + n], vd[VQZ * nv + n]);
+    in.posP = dv(vd[VPX * nv + p], vd[VPY * nv + p], vd[VPZ * nv + p]);
+    in.qP = dq(vd[VQW * nv + p], vd[VQX * nv + p], vd[VQY * nv + p], vd[VQZ * nv + p]);
+    const float tn = a.t.vtemp[n], tp = a.t.vtemp[p];
+    const double sn = axis == 0 ? mn.sizeX : axis == 1 ? mn.sizeY : mn.sizeZ;
+    const double sp = axis == 0 ? mp.sizeX : axis == 1 ? mp.sizeY : mp.sizeZ;
+    in.rest = a.vphase ? 0.5 * (base_size(sn, tn, mn.alphaCTE) + base_size(sp, tp, mp.alphaCTE)) : a.lrest[slot];
+    auto old = [&](int c) -> double {
+        return a.lsDouble ? ((const double*)a.t.ls)[c * S + slot] : (double)((const float*)a.t.ls)[c * S + slot];
+    };
+    in.oldP2 = dv(old(MS_P2X), old(MS_P2Y), old(MS_P2Z));
+    in.oldA1 = dv(old(MS_A1X), old(MS_A1Y), old(MS_A1Z));
+    in.oldA2 = dv(old(MS_A2X), old(MS_A2Y), old(MS_A2Z));
+    in.flags = a.t.lfl[slot];
+    const float En = a.vE ? a.vE[n] : mn.E, Ep = a.vE ? a.vE[p] : mp.E;
+    if (a.vE) {
+        in.k = link_constants(combined_modulus(En, Ep), (float)mn.nomSize);
+    } else {
+        in.k.E = lm.E; in.k.a2 = lm.a2; in.k.b1 = lm.b1; in.k.b2 = lm.b2; in.k.b3 = lm.b3;
+        in.k.sqA1 = lm.sqA1; in.k.sqA2xIp = lm.sqA2xIp; in.k.sqB1 = lm.sqB1; in.k.sqB2xFMp = lm.sqB2xFMp;
+        in.k.sqB3xIp = lm.sqB3xIp;
+    }
+    in.area = lm.transverseArea;
+    const int rob = a.vrobot[n];
+    const float prevDt = a.k == 0 ? a.rprev[rob] : a.rdt[rob];
+    in.dmN = mn.dampNum / prevDt;
+    in.dmP = mp.dampNum / prevDt;
+    const double* LS = a.lamSum;
+    const dvec lFN = dv(LS[n], LS[nv + n], LS[2 * nv + n]), lMN = dv(LS[3 * nv + n], LS[4 * nv + n], LS[5 * nv + n]);
+    const dvec lFP = dv(LS[p], LS[nv + p], LS[2 * nv + p]), lMP = dv(LS[3 * nv + p], LS[4 * nv + p], LS[5 * nv + p]);
+    double* LL = a.lamLink;
+    const dvec lNewP2 = dv(LL[0 * S + slot], LL[1 * S + slot], LL[2 * S + slot]);
+    const dvec lNewA1 = dv(LL[3 * S + slot], LL[4 * S + slot], LL[5 * S + slot]);
+    const dvec lNewA2 = dv(LL[6 * S + slot], LL[7 * S + slot], LL[8 * S + slot]);
+    LinkAdjOut o;
+    link_vjp(in, lFN, lMN, lFP, lMP, lNewP2, lNewA1, lNewA2, &o);
+    LL[0 * S + slot] = o.lOldP2.x; LL[1 * S + slot] = o.lOldP2.y; LL[2 * S + slot] = o.lOldP2.z;
+    LL[3 * S + slot] = o.lOldA1.x; LL[4 * S + slot] = o.lOldA1.y; LL[5 * S + slot] = o.lOldA1.z;
+    LL[6 * S + slot] = o.lOldA2.x; LL[7 * S + slot] = o.lOldA2.y; LL[8 * S + slot] = o.lOldA2.z;
+    /* log E_link = log(2 En Ep / (En + Ep)): d/dlog En = Ep / (En + Ep) */
+    const double wn = (double)Ep / ((double)En + (double)Ep), wp = (double)En / ((double)En + (double)Ep);
+    double ltn = 0.0, ltp = 0.0;
+    if (a.vphase) { ltn = o.lrest * 0.5 * sn * mn.alphaCTE; ltp = o.lrest * 0.5 * sp * mp.alphaCTE; }
+    write_slot(a, 2 * axis, n, o.lposN, o.lqN, o.lthetaLink * wn, ltn);
+    write_slot(a, 2 * axis + 1, p, o.lposP, o.lqP, o.lthetaLink * wp, ltp);
+}
+
+__global__ void __launch_bounds__(256) k_adj_links(AdjPtrs a) {
+    const int n = (int)blockIdx.x * 256 + (int)threadIdx.x;
+    if (n >= a.nv) return;
+    adj_link<0>(a, n);
+    adj_link<1>(a, n);
+    adj_link<2>(a, n);
+}
+
+void launch_adj_voxels(const AdjPtrs& a, hipStream_t st) {
+    if (a.nv > 0) hipLaunchKernelGGL(k_adj_voxels, dim3((unsigned)((a.nv + 255) / 256)), dim3(256), 0, st, a);
+}
+void launch_adj_links(const AdjPtrs& a, hipStream_t st) {
+    if (a.nv > 0) hipLaunchKernelGGL(k_adj_links, dim3((unsigned)((a.nv + 255) / 256)), dim3(256), 0, st, a);
+}
+
+}  // namespace vx

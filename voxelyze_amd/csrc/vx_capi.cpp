@@ -115,7 +115,7 @@ API int vxg_collision_count(vxg_sim* h) {
 }
 API int vxg_set_precision(vxg_sim* h, int mode) {
     Sim* s = S(h);
-    if (mode != VXG_FP64_PLAIN && mode != VXG_MIXED) {
+    if (mode != VXG_FP64_PLAIN && mode != VXG_MIXED && mode != VXG_FUSED_FP64) {
         s->err = "unknown precision mode";
         return 0;
     }
@@ -405,6 +405,40 @@ API int vxg_robot_step(vxg_sim* h, int nsteps, const float* dt, int* done) {
     return r;
 }
 API int vxg_robot_com(vxg_sim* h, double* com) { return S(h)->robotCom(com) ? 1 : 0; }
+
+/* ------------------------------------------------- per-voxel moduli, adjoint */
+API int vxg_voxel_set_modulus(vxg_sim* h, int first, int count, const float* E) {
+    Sim* s = S(h);
+    if (first < 0 || count < 0 || first + count > s->nv()) {
+        s->err = "voxel range out of bounds";
+        return 0;
+    }
+    for (int k = 0; k < count; k++) {
+        if (!(E[k] > 0.0f)) {
+            s->err = "moduli must be positive";
+            return 0;
+        }
+    }
+    s->syncDown();
+    for (int k = 0; k < count; k++) s->vE[first + k] = E[k];
+    s->modOn = true;
+    s->dirty |= Sim::D_MOD;
+    return 1;
+}
+API void vxg_clear_moduli(vxg_sim* h) {
+    Sim* s = S(h);
+    s->syncDown();
+    s->modOn = false;
+    for (int v = 0; v < s->nv(); v++) s->vE[v] = s->vmat[v]->E;
+    s->dirty |= Sim::D_MOD;
+}
+API int vxg_adjoint_run(vxg_sim* h, int nsteps, int every, const float* dt) {
+    return S(h)->adjointRun(nsteps, every, dt) ? 1 : 0;
+}
+API int vxg_adjoint_backward(vxg_sim* h, const double* dl_dpos, double* grad_log_modulus, double* grad_phase) {
+    return S(h)->adjointBackward(dl_dpos, grad_log_modulus, grad_phase) ? 1 : 0;
+}
+API void vxg_adjoint_free(vxg_sim* h) { S(h)->adjointFree(); }
 
 /* ----------------------------------------------------------------- actuation */
 API int vxg_set_actuation(vxg_sim* h, int enabled, float base, float amplitude, float frequency) {

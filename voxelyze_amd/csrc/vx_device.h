@@ -54,6 +54,9 @@ struct VoxMat {
     float penetrationStiff;   /* (float)(2*E*nomSize) */
     float muStatic, muKinetic;
     float mass;               /* _mass */
+    float momentInertia;      /* _momentInertia */
+    float zetaGlobal, zetaCollision;
+    float E;                  /* Young's modulus */
 };
 
 /* per link-material constants (VX_MaterialLink.h:38-47, VX_Material.h:114-128) */
@@ -167,8 +170,8 @@ struct MixPtrs {
     int nv;
     const double* vdIn;     /* voxel state before the step */
     double* vdOut;          /* voxel state after the step */
-    const float* lsIn;      /* link state before / after (MS_COUNT * 3nv) */
-    float* lsOut;
+    const void* lsIn;       /* link state before / after (MS_COUNT * 3nv, float or double) */
+    void* lsOut;
     const uint32_t* lflIn;  /* link flags before / after */
     uint32_t* lflOut;
     const uint32_t* vflIn;  /* voxel flags before / after (the static-friction bit changes) */
@@ -198,6 +201,8 @@ struct MixPtrs {
     int* ctl;               /* [0] some robot diverged in this launch */
     const float* vphase;    /* actuation phase per voxel, null: no actuation */
     float actBase, actAmp, actFreq;
+    const float* vE;        /* per-voxel Young's modulus overriding the materials' (linear), null: off */
+    double* tapeSums;       /* when set, each voxel records its link force and moment sums (6 * nv) */
     int k;                  /* this step's index in the launch */
 };
 

@@ -30,20 +30,25 @@
  * temperatures (VX_Link.cpp:137-140), as a host loop calling setTemperature
  * before every doTimeStep would.
  */
+#include "vx_adjoint.h"
 #include "vx_device.h"
 #include "vx_math.h"
 
 namespace vx {
 
-struct fquat { float w, x, y, z; };
+template <typename T> struct V3 { T x, y, z; };
+template <typename T> struct Q4 { T w, x, y, z; };
+template <typename T> __device__ __forceinline__ V3<T> v3(T x, T y, T z) { V3<T> r; r.x = x; r.y = y; r.z = z; return r; }
 
-__device__ __forceinline__ fvec f_qrotinv(fquat q, fvec f) {
-    float tw = q.x * f.x + q.y * f.y + q.z * f.z;
-    float tx = q.w * f.x - q.y * f.z + q.z * f.y;
-    float ty = q.w * f.y + q.x * f.z - q.z * f.x;
-    float tz = q.w * f.z - q.x * f.y + q.y * f.x;
-    return fv(tw * q.x + tx * q.w + ty * q.z - tz * q.y, tw * q.y - tx * q.z + ty * q.w + tz * q.x,
-              tw * q.z + tx * q.y - ty * q.x + tz * q.w);
+/* Quat3D::RotateVec3DInv (Quat3D.h:188-195) in T */
+template <typename T>
+__device__ __forceinline__ V3<T> t_qrotinv(Q4<T> q, V3<T> f) {
+    T tw = q.x * f.x + q.y * f.y + q.z * f.z;
+    T tx = q.w * f.x - q.y * f.z + q.z * f.y;
+    T ty = q.w * f.y + q.x * f.z - q.z * f.x;
+    T tz = q.w * f.z - q.x * f.y + q.y * f.x;
+    return v3<T>(tw * q.x + tx * q.w + ty * q.z - tz * q.y, tw * q.y - tx * q.z + ty * q.w + tz * q.x,
+                 tw * q.z + tx * q.y - ty * q.x + tz * q.w);
 }
 
 /* Quat3D::ToRotationVector in FP64 with libm acos (Quat3D.h:117-122) */
@@ -72,29 +77,39 @@ __device__ __forceinline__ void d_angletoposx(dquat* q, dvec rf) {
     q->w = c; q->x = 0.0; q->y = n.z * ami * s; q->z = -n.y * ami * s;
 }
 
-__device__ __forceinline__ float m_stress(const LinkMat& m, const float* cs, const float* ss, float strain) {
-    if (m.epsilonFail != -1.0f && strain > m.epsilonFail) return 0.0f;
+template <typename T>
+__device__ __forceinline__ T m_stress(const LinkMat& m, const float* cs, const float* ss, T strain) {
+    if (m.epsilonFail != -1.0f && strain > m.epsilonFail) return T(0);
     if (strain <= m.strain1 || m.linear) return m.E * strain;
     for (int i = 2; i < m.ndata; i++) {
         if (strain <= cs[i] || i == m.ndata - 1) {
-            float perc = (strain - cs[i - 1]) / (cs[i] - cs[i - 1]);
+            T perc = (strain - cs[i - 1]) / (cs[i] - cs[i - 1]);
             return ss[i - 1] + perc * (ss[i] - ss[i - 1]);
         }
     }
-    return 0.0f;
+    return T(0);
 }
 
-/* One link (CVX_Link::updateForces, VX_Link.cpp:149-217) in mixed precision.  Returns the
-   force and moment on the requested end (in that voxel's frame) and, when owner, the new
-   link state. */
-__device__ void link_mixed(const MixPtrs& s, int slot, int n, int p, int axis, bool wantPos, bool owner, double rest,
-                           float prevDt, dvec posN, dquat qN, dvec posP, dquat qP, fvec* outF, fvec* outM) {
+/* One link (CVX_Link::updateForces, VX_Link.cpp:149-217): pose extraction in FP64, beam
+   forces, damping and link state in T (float: the mixed step; double: the FP64 fused step).
+   Returns the force and moment on the requested end (in that voxel's frame) and, when
+   owner, the new link state. */
+template <typename T>
+__device__ void link_fused(const MixPtrs& s, int slot, int n, int p, int axis, bool wantPos, bool owner, double rest,
+                           float prevDt, dvec posN, dquat qN, dvec posP, dquat qP, dvec* outF, dvec* outM) {
     const int S = 3 * s.nv;
-    const LinkMat m = s.lmats[s.lmeta[slot]];
+    LinkMat m = s.lmats[s.lmeta[slot]];
+    if (s.vE) { /* per-voxel moduli: linear link of the harmonic-mean modulus (VX_MaterialLink.cpp:70,121-141) */
+        const LinkK k = link_constants(combined_modulus(s.vE[n], s.vE[p]), (float)s.vmats[s.vmat[n]].nomSize);
+        m.E = k.E; m.a2 = k.a2; m.b1 = k.b1; m.b2 = k.b2; m.b3 = k.b3;
+        m.sqA1 = k.sqA1; m.sqA2xIp = k.sqA2xIp; m.sqB1 = k.sqB1; m.sqB2xFMp = k.sqB2xFMp; m.sqB3xIp = k.sqB3xIp;
+        m.linear = 1;
+        m.epsilonFail = -1.0f;
+    }
     uint32_t fl = s.lflIn[slot];
     bool small = (fl & LF_SMALL) != 0, lvv = (fl & LF_LVV) != 0;
-    const float* ls = s.lsIn;
-    float maxStrain = ls[MS_MAX * S + slot], strainOffset = ls[MS_OFF * S + slot];
+    const T* ls = (const T*)s.lsIn;
+    T maxStrain = ls[MS_MAX * S + slot], strainOffset = ls[MS_OFF * S + slot];
 
     /* orientLink in FP64 up to the small quantities */
     dvec d = toAxisX(axis, dv(posP.x - posN.x, posP.y - posN.y, posP.z - posN.z));
@@ -114,47 +129,47 @@ __device__ void link_mixed(const MixPtrs& s, int slot, int n, int p, int axis, b
         a2 = qmul(a1, a2);
         d = dv(sqrt(d.x * d.x + d.y * d.y + d.z * d.z) - rest, 0.0, 0.0);
     }
-    const float axialStrain = (float)(d.x / rest);
+    const T axialStrain = (T)(d.x / rest);
     const dvec a1d = d_torv(a1), a2d = d_torv(a2);
-    const fvec p2 = fv((float)d.x, (float)d.y, (float)d.z);
-    const fvec a1v = fv((float)a1d.x, (float)a1d.y, (float)a1d.z), a2v = fv((float)a2d.x, (float)a2d.y, (float)a2d.z);
+    const V3<T> p2 = v3<T>((T)d.x, (T)d.y, (T)d.z);
+    const V3<T> a1v = v3<T>((T)a1d.x, (T)a1d.y, (T)a1d.z), a2v = v3<T>((T)a2d.x, (T)a2d.y, (T)a2d.z);
 
     /* updateStrain (VX_Link.cpp:220-249) */
-    float stress;
+    T stress;
     const float* cs = s.curveStrain + m.curveOff;
     const float* cst = s.curveStress + m.curveOff;
     if (m.linear) {
         if (axialStrain > maxStrain) maxStrain = axialStrain;
-        stress = m_stress(m, cs, cst, axialStrain);
+        stress = m_stress<T>(m, cs, cst, axialStrain);
     } else if (axialStrain > maxStrain) {
         maxStrain = axialStrain;
-        stress = m_stress(m, cs, cst, axialStrain);
+        stress = m_stress<T>(m, cs, cst, axialStrain);
         strainOffset = maxStrain - stress / m.E;
     } else {
         stress = m.E * (axialStrain - strainOffset);
     }
 
-    fvec fN = fv(0, 0, 0), fP = fv(0, 0, 0), mN = fv(0, 0, 0), mP = fv(0, 0, 0);
+    V3<T> fN = v3<T>(0, 0, 0), fP = v3<T>(0, 0, 0), mN = v3<T>(0, 0, 0), mP = v3<T>(0, 0, 0);
     const bool failed = m.epsilonFail != -1.0f && maxStrain > m.epsilonFail;
     const bool lvvIn = lvv;
     if (!failed) {
-        fN = fv(stress * m.transverseArea, m.b1 * p2.y - m.b2 * (a1v.z + a2v.z), m.b1 * p2.z + m.b2 * (a1v.y + a2v.y));
-        fP = fv(-fN.x, -fN.y, -fN.z);
-        mN = fv(m.a2 * (a2v.x - a1v.x), -m.b2 * p2.z - m.b3 * (2 * a1v.y + a2v.y), m.b2 * p2.y - m.b3 * (2 * a1v.z + a2v.z));
-        mP = fv(m.a2 * (a1v.x - a2v.x), -m.b2 * p2.z - m.b3 * (a1v.y + 2 * a2v.y), m.b2 * p2.y - m.b3 * (a1v.z + 2 * a2v.z));
+        fN = v3<T>(stress * m.transverseArea, m.b1 * p2.y - m.b2 * (a1v.z + a2v.z), m.b1 * p2.z + m.b2 * (a1v.y + a2v.y));
+        fP = v3<T>(-fN.x, -fN.y, -fN.z);
+        mN = v3<T>(m.a2 * (a2v.x - a1v.x), -m.b2 * p2.z - m.b3 * (2 * a1v.y + a2v.y), m.b2 * p2.y - m.b3 * (2 * a1v.z + a2v.z));
+        mP = v3<T>(m.a2 * (a1v.x - a2v.x), -m.b2 * p2.z - m.b3 * (a1v.y + 2 * a2v.y), m.b2 * p2.y - m.b3 * (a1v.z + 2 * a2v.z));
         if (lvvIn) {
-            const fvec dP = fv(0.5f * (p2.x - ls[MS_P2X * S + slot]), 0.5f * (p2.y - ls[MS_P2Y * S + slot]),
-                               0.5f * (p2.z - ls[MS_P2Z * S + slot]));
-            const fvec d1 = fv(0.5f * (a1v.x - ls[MS_A1X * S + slot]), 0.5f * (a1v.y - ls[MS_A1Y * S + slot]),
-                               0.5f * (a1v.z - ls[MS_A1Z * S + slot]));
-            const fvec d2 = fv(0.5f * (a2v.x - ls[MS_A2X * S + slot]), 0.5f * (a2v.y - ls[MS_A2Y * S + slot]),
-                               0.5f * (a2v.z - ls[MS_A2Z * S + slot]));
+            const V3<T> dP = v3<T>(T(0.5) * (p2.x - ls[MS_P2X * S + slot]), T(0.5) * (p2.y - ls[MS_P2Y * S + slot]),
+                               T(0.5) * (p2.z - ls[MS_P2Z * S + slot]));
+            const V3<T> d1 = v3<T>(T(0.5) * (a1v.x - ls[MS_A1X * S + slot]), T(0.5) * (a1v.y - ls[MS_A1Y * S + slot]),
+                               T(0.5) * (a1v.z - ls[MS_A1Z * S + slot]));
+            const V3<T> d2 = v3<T>(T(0.5) * (a2v.x - ls[MS_A2X * S + slot]), T(0.5) * (a2v.y - ls[MS_A2Y * S + slot]),
+                               T(0.5) * (a2v.z - ls[MS_A2Z * S + slot]));
             const float dmN = s.vmats[s.vmat[n]].dampNum / prevDt;
             const float dmP = s.vmats[s.vmat[p]].dampNum / prevDt;
-            const fvec pc = fv(m.sqA1 * dP.x, m.sqB1 * dP.y - m.sqB2xFMp * (d1.z + d2.z), m.sqB1 * dP.z + m.sqB2xFMp * (d1.y + d2.y));
+            const V3<T> pc = v3<T>(m.sqA1 * dP.x, m.sqB1 * dP.y - m.sqB2xFMp * (d1.z + d2.z), m.sqB1 * dP.z + m.sqB2xFMp * (d1.y + d2.y));
             fN.x += dmN * pc.x; fN.y += dmN * pc.y; fN.z += dmN * pc.z;
             fP.x -= dmP * pc.x; fP.y -= dmP * pc.y; fP.z -= dmP * pc.z;
-            const float hN = 0.5f * dmN, hP = 0.5f * dmP;
+            const T hN = T(0.5) * dmN, hP = T(0.5) * dmP;
             mN.x -= hN * (-m.sqA2xIp * (d2.x - d1.x));
             mN.y -= hN * (m.sqB2xFMp * dP.z + m.sqB3xIp * (2 * d1.y + d2.y));
             mN.z -= hN * (-m.sqB2xFMp * dP.y + m.sqB3xIp * (2 * d1.z + d2.z));
@@ -166,24 +181,24 @@ __device__ void link_mixed(const MixPtrs& s, int slot, int n, int p, int axis, b
         }
     }
     if (wantPos) {
-        const fquat q2 = {(float)a2.w, (float)a2.x, (float)a2.y, (float)a2.z};
-        fvec f = f_qrotinv(q2, fP), mm = f_qrotinv(q2, mP);
+        const Q4<T> q2 = {(T)a2.w, (T)a2.x, (T)a2.y, (T)a2.z};
+        V3<T> f = t_qrotinv<T>(q2, fP), mm = t_qrotinv<T>(q2, mP);
         dvec fo = toAxisOriginal(axis, dv(f.x, f.y, f.z)), mo = toAxisOriginal(axis, dv(mm.x, mm.y, mm.z));
-        *outF = fv((float)fo.x, (float)fo.y, (float)fo.z);
-        *outM = fv((float)mo.x, (float)mo.y, (float)mo.z);
+        *outF = dv((T)fo.x, (T)fo.y, (T)fo.z);
+        *outM = dv((T)mo.x, (T)mo.y, (T)mo.z);
     } else {
-        fvec f = fN, mm = mN;
+        V3<T> f = fN, mm = mN;
         if (!small) {
-            const fquat q1 = {(float)a1.w, (float)a1.x, (float)a1.y, (float)a1.z};
-            f = f_qrotinv(q1, fN);
-            mm = f_qrotinv(q1, mN);
+            const Q4<T> q1 = {(T)a1.w, (T)a1.x, (T)a1.y, (T)a1.z};
+            f = t_qrotinv<T>(q1, fN);
+            mm = t_qrotinv<T>(q1, mN);
         }
         dvec fo = toAxisOriginal(axis, dv(f.x, f.y, f.z)), mo = toAxisOriginal(axis, dv(mm.x, mm.y, mm.z));
-        *outF = fv((float)fo.x, (float)fo.y, (float)fo.z);
-        *outM = fv((float)mo.x, (float)mo.y, (float)mo.z);
+        *outF = dv((T)fo.x, (T)fo.y, (T)fo.z);
+        *outM = dv((T)mo.x, (T)mo.y, (T)mo.z);
     }
     if (owner) {
-        float* o = s.lsOut;
+        T* o = (T*)s.lsOut;
         o[MS_P2X * S + slot] = p2.x; o[MS_P2Y * S + slot] = p2.y; o[MS_P2Z * S + slot] = p2.z;
         o[MS_A1X * S + slot] = a1v.x; o[MS_A1Y * S + slot] = a1v.y; o[MS_A1Z * S + slot] = a1v.z;
         o[MS_A2X * S + slot] = a2v.x; o[MS_A2Y * S + slot] = a2v.y; o[MS_A2Z * S + slot] = a2v.z;
@@ -205,6 +220,7 @@ __device__ __forceinline__ double m_bsa(const VoxMat& m, float temp) {
 /* a robot that diverged at step d of this launch keeps its state: its voxel state (and
    clock) at step d is the discarded output, its link state the kept one, so from step d+1
    on the lanes copy whichever of the two ping-pong buffers is stale */
+template <typename T>
 __device__ void frozen_copy(const MixPtrs& s, int i, int rob, int d) {
     const int nv = s.nv, S = 3 * nv;
     if (((s.k - d) & 1) == 0) {
@@ -215,12 +231,13 @@ __device__ void frozen_copy(const MixPtrs& s, int i, int rob, int d) {
     } else {
         for (int axis = 0; axis < 3; axis++) {
             const int slot = axis * nv + i;
-            for (int c = 0; c < MS_COUNT; c++) s.lsOut[c * S + slot] = s.lsIn[c * S + slot];
+            for (int c = 0; c < MS_COUNT; c++) ((T*)s.lsOut)[c * S + slot] = ((const T*)s.lsIn)[c * S + slot];
             s.lflOut[slot] = s.lflIn[slot];
         }
     }
 }
 
+template <typename T>
 __global__ void __launch_bounds__(256) k_step_mixed(MixPtrs s) {
     const int i = xcd_block() * 256 + (int)threadIdx.x;
     const int nv = s.nv;
@@ -228,7 +245,7 @@ __global__ void __launch_bounds__(256) k_step_mixed(MixPtrs s) {
     const int rob = s.vrobot[i];
     const int d = s.rdiv[rob];
     if (d < s.k) {
-        frozen_copy(s, i, rob, d);
+        frozen_copy<T>(s, i, rob, d);
         return;
     }
     const float dt = s.rdt[rob];
@@ -238,7 +255,8 @@ __global__ void __launch_bounds__(256) k_step_mixed(MixPtrs s) {
     const dvec pos0 = dv(vd[VPX * nv + i], vd[VPY * nv + i], vd[VPZ * nv + i]);
     const dquat q0 = dq(vd[VQW * nv + i], vd[VQX * nv + i], vd[VQY * nv + i], vd[VQZ * nv + i]);
     uint32_t flags = s.vflIn[i];
-    const VoxMat m = s.vmats[s.vmat[i]];
+    VoxMat m = s.vmats[s.vmat[i]];
+    if (s.vE) voxel_modulus(&m, s.vE[i]);
     const float tempI = s.vtempIn[i];
 
     /* link phase for this voxel's <= 6 links, summed in direction order */
@@ -261,13 +279,18 @@ __global__ void __launch_bounds__(256) k_step_mixed(MixPtrs s) {
         }
         const dvec po = dv(vd[VPX * nv + o], vd[VPY * nv + o], vd[VPZ * nv + o]);
         const dquat qo = dq(vd[VQW * nv + o], vd[VQX * nv + o], vd[VQY * nv + o], vd[VQZ * nv + o]);
-        fvec f, mm;
-        if (neg) link_mixed(s, slot, i, o, axis, false, true, rest, prevDt, pos0, q0, po, qo, &f, &mm);
-        else link_mixed(s, slot, o, i, axis, true, false, rest, prevDt, po, qo, pos0, q0, &f, &mm);
+        dvec f, mm;
+        if (neg) link_fused<T>(s, slot, i, o, axis, false, true, rest, prevDt, pos0, q0, po, qo, &f, &mm);
+        else link_fused<T>(s, slot, o, i, axis, true, false, rest, prevDt, po, qo, pos0, q0, &f, &mm);
         lf.x += f.x; lf.y += f.y; lf.z += f.z;
         lm.x += mm.x; lm.y += mm.y; lm.z += mm.z;
     }
     if (s.vphase) s.vtempOut[i] = actuation_temperature(s.actBase, s.actAmp, s.actFreq, s.rtimeIn[rob] + dt, s.vphase[i]);
+    if (s.tapeSums) {
+        double* ts = s.tapeSums;
+        ts[i] = lf.x; ts[nv + i] = lf.y; ts[2 * nv + i] = lf.z;
+        ts[3 * nv + i] = lm.x; ts[4 * nv + i] = lm.y; ts[5 * nv + i] = lm.z;
+    }
 
     const int e = s.vext[i];
     const ExtRec* ex = e >= 0 ? &s.exts[e] : nullptr;
@@ -299,7 +322,8 @@ __global__ void __launch_bounds__(256) k_step_mixed(MixPtrs s) {
         const double bsai = m_bsa(m, tempI);
         for (int k = 0; k < cnt; k++) {
             const int j = s.partners[off + k];
-            const VoxMat mj = s.vmats[s.vmat[j]];
+            VoxMat mj = s.vmats[s.vmat[j]];
+            if (s.vE) voxel_modulus(&mj, s.vE[j]);
             const fvec o3 = fv((float)(vd[VPX * nv + j] - pos.x), (float)(vd[VPY * nv + j] - pos.y), (float)(vd[VPZ * nv + j] - pos.z));
             const float len = sqrtf(o3.x * o3.x + o3.y * o3.y + o3.z * o3.z);
             const float rel = (float)((bsai + m_bsa(mj, s.vtempIn[j])) * 0.625f) - len;
@@ -400,9 +424,10 @@ void launch_mixed_temps(const MixPtrs& s, float* temp, hipStream_t st) {
         hipLaunchKernelGGL(k_mixed_temps, dim3((unsigned)((s.nv + 255) / 256)), dim3(256), 0, st, s, temp);
 }
 
-void launch_step_mixed(const MixPtrs& s, hipStream_t st) {
+void launch_step_mixed(const MixPtrs& s, bool fp64, hipStream_t st) {
     if (s.nv <= 0) return;
-    hipLaunchKernelGGL(k_step_mixed, dim3((unsigned)((s.nv + 255) / 256)), dim3(256), 0, st, s);
+    if (fp64) hipLaunchKernelGGL(k_step_mixed<double>, dim3((unsigned)((s.nv + 255) / 256)), dim3(256), 0, st, s);
+    else hipLaunchKernelGGL(k_step_mixed<float>, dim3((unsigned)((s.nv + 255) / 256)), dim3(256), 0, st, s);
 }
 
 }  // namespace vx

@@ -10,7 +10,7 @@
 
 namespace vx {
 
-void launch_step_mixed(const MixPtrs& s, hipStream_t st);
+void launch_step_mixed(const MixPtrs& s, bool fp64, hipStream_t st);
 void launch_mixed_temps(const MixPtrs& s, float* temp, hipStream_t st);
 void launch_col_stale(const DevPtrs& s, const ContactPtrs& c, hipStream_t st);
 
@@ -23,55 +23,72 @@ void launch_col_stale(const DevPtrs& s, const ContactPtrs& c, hipStream_t st);
 /* second voxel buffer, FP32 link-state pairs; sized with the topology */
 bool Sim::mixedBuffers() {
     const size_t vc = (size_t)std::max(nv(), 1), S = 3 * vc;
-    if (!bvdAlt.ensure((size_t)VD_COUNT * vc * 8) || !bvflAlt.ensure(vc * 4) || !bvtempAlt.ensure(vc * 4) || !bms[0].ensure((size_t)MS_COUNT * S * 4) ||
-        !bms[1].ensure((size_t)MS_COUNT * S * 4) || !bmfl[0].ensure(S * 4) || !bmfl[1].ensure(S * 4)) {
+    if (!bvdAlt.ensure((size_t)VD_COUNT * vc * 8) || !bvflAlt.ensure(vc * 4) || !bvtempAlt.ensure(vc * 4) ||
+        !bms[0].ensure((size_t)MS_COUNT * S * 8) || !bms[1].ensure((size_t)MS_COUNT * S * 8) || !bmfl[0].ensure(S * 4) ||
+        !bmfl[1].ensure(S * 4)) {
         err = "device allocation failed";
         return false;
     }
     return true;
 }
 
-/* host FP64 link state -> FP32 slots of the current mixed buffer */
-bool Sim::uploadMixedLinks() {
-    const int n = nv(), L = nl();
-    const size_t S = (size_t)3 * n;
-    std::vector<float> ms((size_t)MS_COUNT * S, 0.0f);
-    std::vector<uint32_t> fl(S, 0u);
-    for (int l = 0; l < L; l++) {
-        const size_t d = slotOf[l];
-        for (int c = 0; c < 9; c++) ms[c * S + d] = (float)ld[LP2X + c][l];
-        ms[MS_MAX * S + d] = lf[LMAXSTRAIN][l];
-        ms[MS_OFF * S + d] = lf[LOFFSET][l];
-        ms[MS_STRAIN * S + d] = lf[LSTRAIN][l];
-        ms[MS_STRESS * S + d] = lf[LSTRESS][l];
-        fl[d] = lflags[l];
+/* host FP64 link state -> slots of the current fused buffer (float or double) */
+template <typename T>
+static void pack_links(const Sim& s, std::vector<T>& ms, std::vector<uint32_t>& fl) {
+    const size_t S = (size_t)3 * s.nv();
+    for (int l = 0; l < s.nl(); l++) {
+        const size_t d = s.slotOf[l];
+        for (int c = 0; c < 9; c++) ms[c * S + d] = (T)s.ld[LP2X + c][l];
+        ms[MS_MAX * S + d] = s.lf[LMAXSTRAIN][l];
+        ms[MS_OFF * S + d] = s.lf[LOFFSET][l];
+        ms[MS_STRAIN * S + d] = s.lf[LSTRAIN][l];
+        ms[MS_STRESS * S + d] = s.lf[LSTRESS][l];
+        fl[d] = s.lflags[l];
     }
-    HIPCHK(hipMemcpyAsync(bms[lcur].p, ms.data(), ms.size() * 4, hipMemcpyHostToDevice, stream));
+}
+template <typename T>
+static void unpack_links(Sim& s, const std::vector<T>& ms, const std::vector<uint32_t>& fl) {
+    const size_t S = (size_t)3 * s.nv();
+    for (int l = 0; l < s.nl(); l++) {
+        const size_t d = s.slotOf[l];
+        for (int c = 0; c < 9; c++) s.ld[LP2X + c][l] = ms[c * S + d];
+        s.lf[LMAXSTRAIN][l] = (float)ms[MS_MAX * S + d];
+        s.lf[LOFFSET][l] = (float)ms[MS_OFF * S + d];
+        s.lf[LSTRAIN][l] = (float)ms[MS_STRAIN * S + d];
+        s.lf[LSTRESS][l] = (float)ms[MS_STRESS * S + d];
+        s.lflags[l] = fl[d];
+    }
+}
+
+bool Sim::uploadMixedLinks() {
+    const size_t S = (size_t)3 * nv();
+    std::vector<uint32_t> fl(S, 0u);
+    std::vector<float> mf;
+    std::vector<double> md;
+    const void* src;
+    if (precision == 2) { md.assign((size_t)MS_COUNT * S, 0.0); pack_links(*this, md, fl); src = md.data(); }
+    else { mf.assign((size_t)MS_COUNT * S, 0.0f); pack_links(*this, mf, fl); src = mf.data(); }
+    HIPCHK(hipMemcpyAsync(bms[lcur].p, src, MS_COUNT * S * lsElem(), hipMemcpyHostToDevice, stream));
     HIPCHK(hipMemcpyAsync(bmfl[lcur].p, fl.data(), fl.size() * 4, hipMemcpyHostToDevice, stream));
     HIPCHK(hipStreamSynchronize(stream));
     return true;
 }
 
-/* FP32 slots of the current mixed buffer -> host mirror */
 bool Sim::downloadMixedLinks() {
-    const int n = nv(), L = nl();
+    const int n = nv();
     const size_t S = (size_t)3 * n;
-    std::vector<float> ms((size_t)MS_COUNT * S);
     std::vector<uint32_t> fl(S);
-    if (L) {
-        HIPCHK(hipMemcpyAsync(ms.data(), bms[lcur].p, ms.size() * 4, hipMemcpyDeviceToHost, stream));
+    std::vector<float> mf;
+    std::vector<double> md;
+    void* dst;
+    if (precision == 2) { md.resize((size_t)MS_COUNT * S); dst = md.data(); }
+    else { mf.resize((size_t)MS_COUNT * S); dst = mf.data(); }
+    if (nl()) {
+        HIPCHK(hipMemcpyAsync(dst, bms[lcur].p, MS_COUNT * S * lsElem(), hipMemcpyDeviceToHost, stream));
         HIPCHK(hipMemcpyAsync(fl.data(), bmfl[lcur].p, fl.size() * 4, hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
     }
-    for (int l = 0; l < L; l++) {
-        const size_t d = slotOf[l];
-        for (int c = 0; c < 9; c++) ld[LP2X + c][l] = ms[c * S + d];
-        lf[LMAXSTRAIN][l] = ms[MS_MAX * S + d];
-        lf[LOFFSET][l] = ms[MS_OFF * S + d];
-        lf[LSTRAIN][l] = ms[MS_STRAIN * S + d];
-        lf[LSTRESS][l] = ms[MS_STRESS * S + d];
-        lflags[l] = fl[d];
-    }
+    if (precision == 2) unpack_links(*this, md, fl); else unpack_links(*this, mf, fl);
     for (int v = 0; v < n; v++) vprevdt[v] = rprev[vrobot[v]];
     return true;
 }
@@ -93,7 +110,7 @@ void Sim::swapVoxelBuffers() {
 }
 
 /* nsteps steps with the fused mixed-precision kernel (one launch per step) */
-bool Sim::stepMixed(int nsteps) {
+bool Sim::stepMixed(int nsteps, const MixTape* tape) {
     if (collisions && (cp.nv != nv() || !bcLast.p)) {
         if (!contactBuffers()) return false;
         nearbyStale = true;
@@ -121,6 +138,7 @@ bool Sim::stepMixed(int nsteps) {
     mp.actBase = actBase;
     mp.actAmp = actAmp;
     mp.actFreq = actFreq;
+    mp.vE = modOn ? (const float*)bvE.p : nullptr;
     if (actOn) { /* temperatures at the robots' current times */
         mp.rtimeIn = dp.rtime;
         launch_mixed_temps(mp, dp.vtemp, stream);
@@ -158,17 +176,28 @@ bool Sim::stepMixed(int nsteps) {
         mp.vtempOut = (float*)bvtempAlt.p;
         mp.rtimeIn = dp.rtime;
         mp.rtimeOut = (float*)brtimeAlt.p;
-        mp.lsIn = (const float*)bms[lcur].p;
-        mp.lsOut = (float*)bms[lcur ^ 1].p;
+        mp.lsIn = bms[lcur].p;
+        mp.lsOut = bms[lcur ^ 1].p;
         mp.lflIn = (const uint32_t*)bmfl[lcur].p;
         mp.lflOut = (uint32_t*)bmfl[lcur ^ 1].p;
         mp.colCnt = collisions ? cp.pairCount : nullptr;
         mp.colOff = collisions ? cp.pairOff : nullptr;
         mp.partners = collisions ? cp.partners : nullptr;
         mp.k = k;
+        mp.tapeSums = nullptr;
+        if (tape) { /* record the step's inputs for the reverse pass */
+            const size_t n = (size_t)nv();
+            HIPCHK(hipMemcpyAsync(tape->vd[k], dp.vd, VD_COUNT * n * 8, hipMemcpyDeviceToDevice, stream));
+            HIPCHK(hipMemcpyAsync(tape->vflIn[k], dp.vflags, n * 4, hipMemcpyDeviceToDevice, stream));
+            HIPCHK(hipMemcpyAsync(tape->vtemp[k], dp.vtemp, n * 4, hipMemcpyDeviceToDevice, stream));
+            HIPCHK(hipMemcpyAsync(tape->ls[k], bms[lcur].p, MS_COUNT * S * lsElem(), hipMemcpyDeviceToDevice, stream));
+            HIPCHK(hipMemcpyAsync(tape->lfl[k], bmfl[lcur].p, S * 4, hipMemcpyDeviceToDevice, stream));
+            mp.tapeSums = tape->sums[k];
+        }
         if (profiling) (void)hipEventRecord(events[2 * k], stream);
-        launch_step_mixed(mp, stream);
+        launch_step_mixed(mp, precision == 2, stream);
         if (profiling) (void)hipEventRecord(events[2 * k + 1], stream);
+        if (tape) HIPCHK(hipMemcpyAsync(tape->vflOut[k], bvflAlt.p, (size_t)nv() * 4, hipMemcpyDeviceToDevice, stream));
         swapVoxelBuffers();
         lcur ^= 1;
         launched++;

@@ -310,6 +310,7 @@ DevBuf::~DevBuf() {
 }
 
 Sim::~Sim() {
+    adjointFree();
     if (stream) (void)hipStreamSynchronize(stream);
     for (hipEvent_t e : events) (void)hipEventDestroy(e);
     if (ctlHost) (void)hipHostFree(ctlHost);
@@ -521,6 +522,7 @@ int Sim::addVoxel(Mat* m, int x, int y, int z) {
     const int vi = nv();
     vrobot.push_back(curRobot);
     vphase.push_back(0.0f);
+    vE.push_back(m->E);
     for (int r = curRobot + 1; r < (int)rfirst.size(); r++) rfirst[r] = vi + 1;
     vmat.push_back(m);
     vijk.push_back({(short)x, (short)y, (short)z});
@@ -553,6 +555,7 @@ void Sim::removeVoxel(int x, int y, int z) {
     vmat.erase(vmat.begin() + vi);
     vrobot.erase(vrobot.begin() + vi);
     vphase.erase(vphase.begin() + vi);
+    vE.erase(vE.begin() + vi);
     for (int r = curRobot + 1; r < (int)rfirst.size(); r++) rfirst[r]--;
     vijk.erase(vijk.begin() + vi);
     vlinks.erase(vlinks.begin() + vi);
@@ -702,7 +705,7 @@ bool Sim::syncUp() {
             }
         if (!bvnbr.ensure((size_t)6 * std::max(n, 1) * 4)) { err = "device allocation failed"; return false; }
         HIPCHK(up((int*)bvnbr.p, nbr, stream));
-        if (precision == 1 && !mixedBuffers()) return false;
+        if (fused() && !mixedBuffers()) return false;
         HIPCHK(hipMemsetAsync(dp.vf, 0, (size_t)VF_COUNT * n * 8, stream));
         std::vector<int> tmp((size_t)3 * n);
         for (int v = 0; v < n; v++)
@@ -711,7 +714,7 @@ bool Sim::syncUp() {
         std::vector<int> dpos = to_slots(lpos, slotOf, (size_t)3 * n, -1);
         HIPCHK(up(dp.lpos, dpos, stream));
         HIPCHK(hipStreamSynchronize(stream));
-        dirty |= D_VSTATE | D_LSTATE | D_MATS | D_EXT | D_REST | D_TEMP | D_PHASE;
+        dirty |= D_VSTATE | D_LSTATE | D_MATS | D_EXT | D_REST | D_TEMP | D_PHASE | D_MOD;
     }
     if (dirty & D_MATS) {
         int k = 0;
@@ -735,6 +738,10 @@ bool Sim::syncUp() {
             r.muStatic = m->muStatic;
             r.muKinetic = m->muKinetic;
             r.mass = m->mass;
+            r.momentInertia = m->momentInertia;
+            r.zetaGlobal = m->zetaGlobal;
+            r.zetaCollision = m->zetaCollision;
+            r.E = m->E;
             vm.push_back(r);
         }
         std::vector<LinkMat> lm;
@@ -808,7 +815,11 @@ bool Sim::syncUp() {
     }
     if (dirty & D_TEMP) {
         HIPCHK(up(dp.vtemp, vtemp, stream));
-        if (precision == 1) HIPCHK(up((float*)bvtempAlt.p, vtemp, stream)); /* both ping-pong buffers */
+        if (fused()) HIPCHK(up((float*)bvtempAlt.p, vtemp, stream)); /* both ping-pong buffers */
+    }
+    if (dirty & D_MOD) {
+        if (!bvE.ensure(vc4(n))) { err = "device allocation failed"; return false; }
+        HIPCHK(up((float*)bvE.p, vE, stream));
     }
     if (dirty & D_PHASE) {
         HIPCHK(up((float*)bvphase.p, vphase, stream));
@@ -832,7 +843,7 @@ bool Sim::syncUp() {
         std::vector<uint32_t> fl = to_slots(lflags, slotOf, S, 0u);
         HIPCHK(up(dp.lflags, fl, stream));
         HIPCHK(hipStreamSynchronize(stream));
-        if (precision == 1 && !uploadMixedLinks()) return false;
+        if (fused() && !uploadMixedLinks()) return false;
     } else if (dirty & D_REST) {
         stageD.push_back(to_slots(ld[LREST], slotOf, S, 0.0));
         if (S) HIPCHK(hipMemcpyAsync(dp.ld + LREST * S, stageD.back().data(), S * 8, hipMemcpyHostToDevice, stream));
@@ -873,7 +884,7 @@ bool Sim::syncDown() {
         dirty = keep;
     }
     hostStale = false;
-    if (precision == 1) {
+    if (fused()) {
         /* the fused mixed step keeps no end forces: the mirror keeps the last FP64 ones */
         if (!downloadMixedLinks()) return false;
         forcesStale = false;
@@ -948,6 +959,14 @@ int Sim::stepRobots(const float* dt, int nsteps, int* done, int* divergedAt) {
             err = "Poisson's ratio != 0 is not supported by the device step";
             return -1;
         }
+        if (modOn && !m->linear) {
+            err = "per-voxel moduli need linear materials";
+            return -1;
+        }
+    }
+    if (modOn && !fused()) {
+        err = "per-voxel moduli run in the fused step (VXG_MIXED or VXG_FUSED_FP64)";
+        return -1;
     }
     launchDt.assign(dt, dt + R);
     bool any = false;
@@ -964,7 +983,7 @@ int Sim::stepRobots(const float* dt, int nsteps, int* done, int* divergedAt) {
     }
     if (!any) return nsteps;
     if (!syncUp() || !beginLaunch(nsteps)) return -1;
-    const bool ok = precision == 1 ? stepMixed(nsteps) : collisions ? stepWithContacts(nsteps) : stepFp64(nsteps);
+    const bool ok = fused() ? stepMixed(nsteps) : collisions ? stepWithContacts(nsteps) : stepFp64(nsteps);
     if (!ok || !endLaunch(nsteps, done)) return -1;
     int mn = nsteps;
     for (int r = 0; r < R; r++) mn = std::min(mn, done[r]);

@@ -75,11 +75,21 @@ struct DevBuf {
     ~DevBuf();
 };
 
+/* per-step record of the mixed forward for the reverse pass (device pointers, one per step) */
+struct MixTape {
+    std::vector<double*> vd, sums;
+    std::vector<uint32_t*> vflIn, vflOut, lfl;
+    std::vector<float*> vtemp;
+    std::vector<void*> ls;   /* float (VXG_MIXED) or double (VXG_FUSED_FP64) link state */
+};
+
 struct Sim {
     double voxSize = 0.001;
     float ambientTemp = 0.0f, grav = 0.0f;
     bool floor = false, collisions = false;
-    int precision = 0;
+    int precision = 0;     /* 0 FP64 plain, 1 mixed (fused, FP32 links), 2 fused with FP64 links */
+    bool fused() const { return precision == 1 || precision == 2; }
+    size_t lsElem() const { return precision == 2 ? 8 : 4; } /* bytes per fused link-state value */
     std::string err;
 
     std::vector<std::unique_ptr<Mat>> vmats; /* voxelMats order */
@@ -106,6 +116,15 @@ struct Sim {
     int nRobots() const { return (int)vmaps.size(); }
     int addRobot();
     float robotRecommendedTimeStep(int r);
+    /* per-voxel Young's modulus overriding the materials' (linear materials only) */
+    bool modOn = false;
+    std::vector<float> vE;
+    DevBuf bvE;
+    /* adjoint of a recorded rollout (vx_adjoint_host.cpp) */
+    struct AdjointState* adj = nullptr;
+    bool adjointRun(int nsteps, int every, const float* dt);
+    bool adjointBackward(const double* dLdpos, double* gTheta, double* gPhase);
+    void adjointFree();
     /* device-side actuation */
     bool actOn = false;
     float actBase = 0.0f, actAmp = 0.0f, actFreq = 0.0f;
@@ -118,7 +137,8 @@ struct Sim {
     std::vector<uint32_t> lflags;
 
     /* host <-> device */
-    enum : unsigned { D_TOPO = 1, D_VSTATE = 2, D_LSTATE = 4, D_MATS = 8, D_EXT = 16, D_REST = 32, D_TEMP = 64, D_PHASE = 128 };
+    enum : unsigned { D_TOPO = 1, D_VSTATE = 2, D_LSTATE = 4, D_MATS = 8, D_EXT = 16, D_REST = 32, D_TEMP = 64, D_PHASE = 128,
+                      D_MOD = 256 };
     unsigned dirty = D_TOPO | D_MATS | D_EXT;
     bool hostStale = false; /* device holds newer dynamic state */
     bool dtCacheValid = false;
@@ -156,7 +176,7 @@ struct Sim {
     bool uploadMixedLinks();
     bool downloadMixedLinks();
     void swapVoxelBuffers();
-    bool stepMixed(int nsteps);
+    bool stepMixed(int nsteps, const struct MixTape* tape = nullptr);
     /* one launch sequence of nsteps steps, each robot with its own dt; done[r] receives the
        steps robot r completed (nsteps unless it diverged) */
     bool stepFp64(int nsteps);

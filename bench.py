@@ -159,6 +159,85 @@ def rel_disp_error(ref_sim, sim, nominal):
     return float(np.abs(po - pr).max() / max(np.abs(pr - nominal).max(), 1e-300))
 
 
+def bench_robots(n_robots, steps, fp64_value):
+    """The batched population: n_robots shapes grown to 100-8,000 voxels in a 20^3 box,
+    four materials, +-39 degC at 2 Hz actuated on the device, gravity and floor friction,
+    every robot with its own recommended dt, mixed precision, one launch per step."""
+    import torch
+    from voxelyze_amd import workloads as W
+    from voxelyze_amd.api import CVoxelyze, MIXED
+    t0 = time.perf_counter()
+    shapes = W.robot_shapes(4, n_robots)
+    sim = CVoxelyze(0.005)
+    W.robot_population(sim, shapes)
+    sim.setPrecision(MIXED)
+    V, L = sim.voxelCount(), sim.linkCount()
+    com0 = sim.robotCom()
+    sim.robotStep(5)
+    torch.cuda.synchronize()
+    print(f"[bench] robots: {n_robots} robots, {V} voxels, {L} links, built in {time.perf_counter() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    done = sim.robotStep(steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    travel = sim.robotCom()[:, 0] - com0[:, 0]
+    value = V * steps / el
+    print(f"[bench] robots: {el / steps * 1e3:.3f} ms/step, {value:.3e} voxel-steps/s", file=sys.stderr, flush=True)
+    return {"workload": f"{n_robots} robots grown to 100-8000 voxels in a 20^3 box, 4 materials, +-39 degC 2 Hz "
+                        "actuation on the device, gravity + floor friction, per-robot dt, mixed precision",
+            "voxels": V, "links": L, "steps": steps, "value": value, "unit": "voxel-steps/s",
+            "ms_per_step": el / steps * 1e3, "vs_fp64_plain_lattice": value / fp64_value,
+            "all_robots_completed": bool((done == steps).all()),
+            "com_x_travel_m": {"mean": float(travel.mean()), "min": float(travel.min()), "max": float(travel.max()),
+                               "after_steps": steps + 5}}
+
+
+def bench_adjoint(n_robots, steps):
+    """Adjoint cost: the 32x32x16, 70%-fill robots of three materials, mixed precision; the
+    forward is `steps` plain steps, the backward is vxg_adjoint_backward (segment recompute
+    from checkpoints every 100 steps + reverse kernels) for dL/d(final positions) of the
+    CoM x-travel loss."""
+    import torch
+    from voxelyze_amd import workloads as W
+    from voxelyze_amd.api import CVoxelyze, MIXED
+    t0 = time.perf_counter()
+    sim = CVoxelyze(0.005)
+    W.differentiable_batch(sim, n_robots)
+    sim.setPrecision(MIXED)
+    V = sim.voxelCount()
+    sim.robotStep(2)
+    sim.resetTime()
+    torch.cuda.synchronize()
+    print(f"[bench] adjoint: {n_robots} robots, {V} voxels, built in {time.perf_counter() - t0:.1f}s", file=sys.stderr,
+          flush=True)
+    t0 = time.perf_counter()
+    sim.robotStep(steps)
+    torch.cuda.synchronize()
+    fwd = time.perf_counter() - t0
+    sim.resetTime()
+    t0 = time.perf_counter()
+    sim.adjointRun(steps, 100)
+    torch.cuda.synchronize()
+    fwd_ckpt = time.perf_counter() - t0
+    dl = np.zeros((V, 3))
+    for r in range(sim.robotCount()):
+        f, c = sim.robotVoxels(r)
+        dl[f:f + c, 0] = -1.0 / c
+    t0 = time.perf_counter()
+    ge, gp = sim.adjointBackward(dl)
+    torch.cuda.synchronize()
+    bwd = time.perf_counter() - t0
+    sim.adjointFree()
+    print(f"[bench] adjoint: forward {fwd:.3f}s, forward with checkpoints {fwd_ckpt:.3f}s, backward {bwd:.3f}s",
+          file=sys.stderr, flush=True)
+    return {"workload": f"{n_robots} robots 32x32x16 70% fill, 3 materials, +-39 degC 2 Hz actuation, gravity + floor "
+                        "friction, mixed precision, checkpoints every 100 steps",
+            "voxels": V, "steps": steps, "forward_s": fwd, "forward_with_checkpoints_s": fwd_ckpt, "backward_s": bwd,
+            "adjoint_over_forward": bwd / fwd,
+            "gradient_finite": bool(np.isfinite(ge).all() and np.isfinite(gp).all())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -168,6 +247,12 @@ def main():
     ap.add_argument("--workload", default="lattice", choices=["lattice", "cantilever"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="measure the selected precision only")
+    ap.add_argument("--no-robots", action="store_true", help="skip the 4,096-robot batch")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward cost")
+    ap.add_argument("--robots", type=int, default=4096)
+    ap.add_argument("--robot-steps", type=int, default=100)
+    ap.add_argument("--adjoint-robots", type=int, default=256)
+    ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a plain 1 GiB -> 1 GiB elementwise copy (known bytes) to scale PMC byte counters")
     a = ap.parse_args()
@@ -209,6 +294,10 @@ def main():
         print(f"[bench] mixed/fp64 speedup {mixed_v / fp64_v:.2f}x, rel disp err {err:.3e}", file=sys.stderr,
               flush=True)
         del sim2
+    if not a.no_robots:
+        line["robots"] = bench_robots(a.robots, a.robot_steps, r["value"])
+    if not a.no_adjoint:
+        line["adjoint"] = bench_adjoint(a.adjoint_robots, a.adjoint_steps)
     if not a.no_cpu_baseline:
         print("[bench] timing the CPU oracle on a bounded sample", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(a.workload)

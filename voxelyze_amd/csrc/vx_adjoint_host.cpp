@@ -108,7 +108,8 @@ bool Sim::adjointRun(int nsteps, int every, const float* dtIn) {
     a->nsteps = nsteps;
     a->every = every;
     a->dt.resize(R);
-    for (int r = 0; r < R; r++) a->dt[r] = dtIn && dtIn[r] >= 0 ? dtIn[r] : robotRecommendedTimeStep(r);
+    const std::vector<float>& rec = robotRecommendedTimeSteps();
+    for (int r = 0; r < R; r++) a->dt[r] = dtIn && dtIn[r] >= 0 ? dtIn[r] : rec[r];
     a->rprev0 = rprev;
     a->times.resize((size_t)(nsteps + 1) * R);
     for (int r = 0; r < R; r++) {

@@ -385,24 +385,33 @@ int Sim::addRobot() {
     return curRobot;
 }
 
-/* recommendedTimeStep (src/Voxelyze.cpp:286-311) over one robot's links and voxels */
-float Sim::robotRecommendedTimeStep(int r) {
-    float maxFreq2 = 0.0f;
+/* recommendedTimeStep (src/Voxelyze.cpp:286-311) of every robot over its own links and
+   voxels, in one pass over the links; cached until the topology or a material changes */
+const std::vector<float>& Sim::robotRecommendedTimeSteps() {
+    if (rdtCacheValid && (int)rdtCache.size() == nRobots()) return rdtCache;
+    const int R = nRobots();
+    std::vector<float> maxFreq2(R, 0.0f);
     for (int l = 0; l < nl(); l++) {
-        if (vrobot[lneg[l]] != r) continue;
+        const int r = vrobot[lneg[l]];
         float m1 = vmat[lneg[l]]->mass, m2 = vmat[lpos[l]]->mass;
         float f = lmat[l]->a1 / (m1 < m2 ? m1 : m2);
-        if (f > maxFreq2) maxFreq2 = f;
+        if (f > maxFreq2[r]) maxFreq2[r] = f;
     }
-    if (maxFreq2 <= 0.0f) {
-        for (int v = rfirst[r]; v < rfirst[r + 1]; v++) {
-            const Mat* m = vmat[v];
-            float f = (float)(m->E * m->nomSize / m->mass);
-            if (f > maxFreq2) maxFreq2 = f;
-        }
+    std::vector<char> noLinks(R);
+    for (int r = 0; r < R; r++) noLinks[r] = maxFreq2[r] <= 0.0f;
+    for (int v = 0; v < nv(); v++) {
+        const int r = vrobot[v];
+        if (!noLinks[r]) continue;
+        const Mat* m = vmat[v];
+        float f = (float)(m->E * m->nomSize / m->mass);
+        if (f > maxFreq2[r]) maxFreq2[r] = f;
     }
-    return maxFreq2 <= 0.0f ? 0.0f : 1.0f / (6.283185f * sqrtf(maxFreq2));
+    rdtCache.resize(R);
+    for (int r = 0; r < R; r++) rdtCache[r] = maxFreq2[r] <= 0.0f ? 0.0f : 1.0f / (6.283185f * sqrtf(maxFreq2[r]));
+    rdtCacheValid = true;
+    return rdtCache;
 }
+float Sim::robotRecommendedTimeStep(int r) { return robotRecommendedTimeSteps()[r]; }
 
 /* CVX_Voxel::reset (src/VX_Voxel.cpp:47-56) */
 void Sim::resetVoxel(int i) {
@@ -971,7 +980,7 @@ int Sim::stepRobots(const float* dt, int nsteps, int* done, int* divergedAt) {
     launchDt.assign(dt, dt + R);
     bool any = false;
     for (int r = 0; r < R; r++) {
-        if (launchDt[r] < 0) launchDt[r] = robotRecommendedTimeStep(r);
+        if (launchDt[r] < 0) launchDt[r] = robotRecommendedTimeSteps()[r];
         done[r] = nsteps;
         /* robots without voxels only keep time (the step loops over nothing) */
         if (rfirst[r] == rfirst[r + 1]) {

@@ -116,6 +116,9 @@ struct Sim {
     int nRobots() const { return (int)vmaps.size(); }
     int addRobot();
     float robotRecommendedTimeStep(int r);
+    const std::vector<float>& robotRecommendedTimeSteps();
+    std::vector<float> rdtCache;
+    bool rdtCacheValid = false;
     /* per-voxel Young's modulus overriding the materials' (linear materials only) */
     bool modOn = false;
     std::vector<float> vE;
@@ -208,8 +211,12 @@ struct Sim {
     void updateSurface(int vi);
     Ext* external(int vi);
     float recommendedTimeStep();
-    void materialsChanged() { dirty |= D_MATS; dtCacheValid = false; }
-    void topologyChanged() { dirty |= D_TOPO; dtCacheValid = false; nearbyStale = collisionsStale = true; }
+    void materialsChanged() { dirty |= D_MATS; dtCacheValid = rdtCacheValid = false; }
+    void topologyChanged() {
+        dirty |= D_TOPO;
+        dtCacheValid = rdtCacheValid = false;
+        nearbyStale = collisionsStale = true;
+    }
 
     /* device */
     bool ensureDevice();

@@ -226,3 +226,35 @@ def robot_population(sim, shapes, amplitude=39.0, frequency=2.0, gravity=1.0, fl
     if phases is not None:
         sim.setPhases(phases)
     return mats
+
+
+def random_fill_robot(seed, r, nx=32, ny=32, nz=16, p_fill=0.7):
+    """Robot r of the differentiable batch: every site of an nx*ny*nz box occupied with
+    probability p_fill (u at counter 2*site of stream seed+r) by one of three materials
+    (floor(3u'), u' at counter 2*site+1).  Returns (ijk (n,3) int32, material 0..2)."""
+    s = seed + r
+    n = nx * ny * nz
+    k = np.arange(n, dtype=np.uint64)
+    occ = uniform(s, 2 * k) < p_fill
+    mat = np.minimum((uniform(s, 2 * k + np.uint64(1)) * 3).astype(np.int32), 2)
+    idx = np.nonzero(occ)[0]
+    ijk = np.stack([idx % nx, (idx // nx) % ny, idx // (nx * ny)], axis=1).astype(np.int32)
+    return ijk, mat[idx]
+
+
+def differentiable_batch(sim, n_robots=256, seed=5, nx=32, ny=32, nz=16, p_fill=0.7, amplitude=39.0,
+                         frequency=2.0):
+    """n_robots random-fill robots of three linear materials (passive 1 MPa; actuated 1 MPa
+    with CTE +0.01 and -0.01 per degC), gravity, floor friction, sinusoidal actuation."""
+    mats = robot_materials(sim)
+    mats[0].setModelLinear(1e6)
+    use = [mats[0], mats[2], mats[3]]
+    sim.setGravity(1.0)
+    sim.enableFloor(True)
+    for r in range(n_robots):
+        if r:
+            sim.addRobot()
+        ijk, mat = random_fill_robot(seed, r, nx, ny, nz, p_fill)
+        sim.setVoxels(np.array([use[m].index for m in range(3)], np.int32)[mat], ijk)
+    sim.setActuation(True, 0.0, amplitude, frequency)
+    return use

@@ -24,6 +24,7 @@
  */
 #include <hipcub/hipcub.hpp>
 
+#include "vx_contact_force.h"
 #include "vx_device.h"
 #include "vx_math.h"
 
@@ -146,13 +147,6 @@ __global__ void k_col_pairs(DevPtrs s, ContactPtrs c) {
     }
 }
 
-/* CVX_Voxel::baseSizeAverage (VX_Voxel.h:90) */
-__device__ __forceinline__ double base_size_average(const VoxMat& m, float temp) {
-    const float f = 1 + temp * m.alphaCTE;
-    const double bx = f * m.sizeX, by = f * m.sizeY, bz = f * m.sizeZ;
-    return (bx + by + bz) / 3.0f;
-}
-
 /* CVX_Collision ctor + updateContactForce (src/VX_Collision.cpp:17-23, 42-56), from voxel i's side */
 __global__ void k_col_forces(DevPtrs s, ContactPtrs c) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -160,39 +154,13 @@ __global__ void k_col_forces(DevPtrs s, ContactPtrs c) {
     const int cnt = c.pairCount[i];
     if (cnt == 0) return;
     const int nv = s.nv, off = c.pairOff[i];
-    const VoxMat mi = s.vmats[s.vmat[i]];
-    const double pix = s.vd[VPX * nv + i], piy = s.vd[VPY * nv + i], piz = s.vd[VPZ * nv + i];
-    const dvec vi = dv(mi.massInverse * s.vd[VLX * nv + i], mi.massInverse * s.vd[VLY * nv + i],
-                       mi.massInverse * s.vd[VLZ * nv + i]);
-    const double bsai = base_size_average(mi, s.vtemp[i]);
+    const ContactSelf me = contact_self(s.vmats[s.vmat[i]], s.vd, nv, i, s.vtemp[i]);
     for (int k = 0; k < cnt; k++) {
         const int j = c.partners[off + k];
-        const VoxMat mj = s.vmats[s.vmat[j]];
-        const float stiff = 2.0f / (1.0f / mi.penetrationStiff + 1.0f / mj.penetrationStiff);
-        const float damp = 0.5f * (mi.collisionTransC + mj.collisionTransC);
-        const fvec off3 = fv((float)(s.vd[VPX * nv + j] - pix), (float)(s.vd[VPY * nv + j] - piy), (float)(s.vd[VPZ * nv + j] - piz));
-        const float nomDist = (float)((bsai + base_size_average(mj, s.vtemp[j])) * 0.625f);
-        const float len = sqrtf(off3.x * off3.x + off3.y * off3.y + off3.z * off3.z);
-        const float relDist = nomDist - len;
-        float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-        if (relDist > 0) {
-            fvec u = off3;
-            if (len > 0) {
-                const float inv = 1.0f / len;
-                u = fv(inv * off3.x, inv * off3.y, inv * off3.z);
-            }
-            const dvec vj = dv(mj.massInverse * s.vd[VLX * nv + j], mj.massInverse * s.vd[VLY * nv + j],
-                               mj.massInverse * s.vd[VLZ * nv + j]);
-            const double ux = u.x, uy = u.y, uz = u.z;
-            const float relVel = (float)((vi.x * ux + vi.y * uy + vi.z * uz) - (vj.x * ux + vj.y * uy + vj.z * uz));
-            const float mag = stiff * relDist + damp * relVel;
-            fx = u.x * mag;
-            fy = u.y * mag;
-            fz = u.z * mag;
-        }
-        c.cf[3 * (size_t)(off + k)] = fx;
-        c.cf[3 * (size_t)(off + k) + 1] = fy;
-        c.cf[3 * (size_t)(off + k) + 2] = fz;
+        const fvec f = contact_pair_force(me, s.vmats[s.vmat[j]], s.vd, nv, j, s.vtemp[j]);
+        c.cf[3 * (size_t)(off + k)] = f.x;
+        c.cf[3 * (size_t)(off + k) + 1] = f.y;
+        c.cf[3 * (size_t)(off + k) + 2] = f.z;
     }
 }
 

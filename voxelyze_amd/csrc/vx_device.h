@@ -24,10 +24,12 @@ enum {
 };
 /* Per-voxel force slots: the end vectors a link hands to each of its voxels, stored
    at the voxel and the link's direction from it: component c (0-2 force, 3-5 moment)
-   of direction d (X_POS..Z_NEG, VX_Voxel.h:39-46) of voxel v is vf[(d*6 + c)*nv + v].
+   of direction d (X_POS..Z_NEG, VX_Voxel.h:39-46) of voxel v is vf[(d*nv + v)*6 + c]
+   (the six components of one slot are contiguous: 48-byte runs for both the writer and
+   the reader, measured faster than one array per component).
    A missing link leaves its slot at +0.0, which adds nothing to the direction-ordered
    sum (a sum started at +0.0 never becomes -0.0). */
-__host__ __device__ inline int vf_index(int d, int c, int nv, int v) { return (d * 6 + c) * nv + v; }
+__host__ __device__ inline int vf_index(int d, int c, int nv, int v) { return (d * nv + v) * 6 + c; }
 #define VF_COUNT 36
 /* component order of the link FP32 block */
 enum { LSTRAIN = 0, LMAXSTRAIN, LOFFSET, LSTRESS, LF_COUNT };

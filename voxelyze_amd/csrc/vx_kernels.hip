@@ -48,6 +48,128 @@ __device__ __forceinline__ float link_stress(const LinkMat& m, const float* __re
     return 0.0f;
 }
 
+/* previous state of one link (local pose, flags, strain history) */
+struct LinkPrev {
+    dvec pos2, a1, a2;
+    uint32_t flags;
+    float maxStrain, strainOffset;
+};
+/* what one link step produces: both end vectors in their voxels' frames and the new state */
+struct LinkEval {
+    dvec fN, mN, fP, mP;
+    dvec pos2, a1v, a2v;
+    float strain, maxStrain, strainOffset, stress;
+    uint32_t flags;
+};
+
+/* CVX_Link::updateForces + orientLink + updateStrain (VX_Link.cpp:77-249), FP64 with the
+   reference's float/double cast points.  dmN, dmP are the two voxels' dampingMultiplier
+   numerators over their previousDt (VX_Voxel.h:130); used only when local velocity is valid. */
+__device__ __forceinline__ void link_eval_fp64(const LinkMat& m, const float* __restrict__ cs,
+                                               const float* __restrict__ ss, int axis, const dvec& posN,
+                                               const dquat& qN, const dvec& posP, const dquat& qP, double rest,
+                                               const LinkPrev& old, float dmN, float dmP, LinkEval* o) {
+    bool small = (old.flags & LF_SMALL) != 0, lvv = (old.flags & LF_LVV) != 0;
+    float maxStrain = old.maxStrain, strainOffset = old.strainOffset;
+
+    /* orientLink (VX_Link.cpp:77-119) */
+    dvec pos2 = toAxisX(axis, dv(posP.x - posN.x, posP.y - posN.y, posP.z - posN.z));
+    dquat a1 = toAxisXq(axis, qN);
+    dquat a2 = toAxisXq(axis, qP);
+    const dquat totalRot = qconj(a1);
+    pos2 = qrot(totalRot, pos2);
+    a2 = qmul(totalRot, a2);
+    a1 = qident();
+    const float smallTurn = (float)((fabs(pos2.z) + fabs(pos2.y)) / pos2.x);
+    const float extendPerc = (float)(fabs(1 - pos2.x / rest));
+    if (!small && smallTurn < 0.05f && extendPerc < 0.50f) {
+        small = true;
+        lvv = false;
+    } else if (small && (smallTurn > 1.2f * 0.05f || extendPerc > 1.2f * 0.50f)) {
+        small = false;
+        lvv = false;
+    }
+    if (small) {
+        pos2.x -= rest;
+    } else {
+        qfromangletoposx(&a1, pos2);
+        a2 = qmul(a1, a2);
+        pos2 = dv(sqrt(pos2.x * pos2.x + pos2.y * pos2.y + pos2.z * pos2.z) - rest, 0.0, 0.0);
+    }
+    const dvec a1v = qtorv(a1), a2v = qtorv(a2);
+
+    const dvec dPos2 = dv(0.5 * (pos2.x - old.pos2.x), 0.5 * (pos2.y - old.pos2.y), 0.5 * (pos2.z - old.pos2.z));
+    const dvec dA1 = dv(0.5 * (a1v.x - old.a1.x), 0.5 * (a1v.y - old.a1.y), 0.5 * (a1v.z - old.a1.z));
+    const dvec dA2 = dv(0.5 * (a2v.x - old.a2.x), 0.5 * (a2v.y - old.a2.y), 0.5 * (a2v.z - old.a2.z));
+
+    /* updateStrain (VX_Link.cpp:220-249), Poisson's ratio 0 */
+    const float axialStrain = (float)(pos2.x / rest);
+    float stress;
+    if (m.linear) {
+        if (axialStrain > maxStrain) maxStrain = axialStrain;
+        stress = link_stress(m, cs, ss, axialStrain);
+    } else if (axialStrain > maxStrain) {
+        maxStrain = axialStrain;
+        stress = link_stress(m, cs, ss, axialStrain);
+        strainOffset = maxStrain - stress / m.E;
+    } else {
+        stress = m.E * (axialStrain - strainOffset);
+    }
+
+    dvec fN, fP, mN, mP;
+    const bool failed = m.epsilonFail != -1.0f && maxStrain > m.epsilonFail;
+    if (failed) {
+        fN = fP = mN = mP = dv(0.0, 0.0, 0.0);
+    } else {
+        const float b1 = m.b1, b2 = m.b2, b3 = m.b3, am2 = m.a2;
+        fN = dv(stress * m.transverseArea,
+                b1 * pos2.y - b2 * (a1v.z + a2v.z),
+                b1 * pos2.z + b2 * (a1v.y + a2v.y));
+        fP = dv(-fN.x, -fN.y, -fN.z);
+        mN = dv(am2 * (a2v.x - a1v.x),
+                -b2 * pos2.z - b3 * (2 * a1v.y + a2v.y),
+                b2 * pos2.y - b3 * (2 * a1v.z + a2v.z));
+        mP = dv(am2 * (a1v.x - a2v.x),
+                -b2 * pos2.z - b3 * (a1v.y + 2 * a2v.y),
+                b2 * pos2.y - b3 * (a1v.z + 2 * a2v.z));
+        if (lvv) {
+            const dvec pc = dv(m.sqA1 * dPos2.x,
+                               m.sqB1 * dPos2.y - m.sqB2xFMp * (dA1.z + dA2.z),
+                               m.sqB1 * dPos2.z + m.sqB2xFMp * (dA1.y + dA2.y));
+            fN.x += dmN * pc.x; fN.y += dmN * pc.y; fN.z += dmN * pc.z;
+            fP.x -= dmP * pc.x; fP.y -= dmP * pc.y; fP.z -= dmP * pc.z;
+            const double hN = 0.5 * dmN, hP = 0.5 * dmP;
+            const dvec mn = dv(-m.sqA2xIp * (dA2.x - dA1.x),
+                               m.sqB2xFMp * dPos2.z + m.sqB3xIp * (2 * dA1.y + dA2.y),
+                               -m.sqB2xFMp * dPos2.y + m.sqB3xIp * (2 * dA1.z + dA2.z));
+            const dvec mp = dv(m.sqA2xIp * (dA2.x - dA1.x),
+                               m.sqB2xFMp * dPos2.z + m.sqB3xIp * (dA1.y + 2 * dA2.y),
+                               -m.sqB2xFMp * dPos2.y + m.sqB3xIp * (dA1.z + 2 * dA2.z));
+            mN.x -= hN * mn.x; mN.y -= hN * mn.y; mN.z -= hN * mn.z;
+            mP.x -= hP * mp.x; mP.y -= hP * mp.y; mP.z -= hP * mp.z;
+        } else {
+            lvv = true;
+        }
+        if (!small) {
+            fN = qrotinv(a1, fN);
+            mN = qrotinv(a1, mN);
+        }
+        fP = qrotinv(a2, fP);
+        mP = qrotinv(a2, mP);
+        fN = toAxisOriginal(axis, fN);
+        fP = toAxisOriginal(axis, fP);
+        mN = toAxisOriginal(axis, mN);
+        mP = toAxisOriginal(axis, mP);
+    }
+    o->fN = fN; o->mN = mN; o->fP = fP; o->mP = mP;
+    o->pos2 = pos2; o->a1v = a1v; o->a2v = a2v;
+    o->strain = axialStrain;
+    o->maxStrain = maxStrain;
+    o->strainOffset = strainOffset;
+    o->stress = stress;
+    o->flags = (lvv ? LF_LVV : 0u) | (small ? LF_SMALL : 0u);
+}
+
 /* The link of slot (axis, n) -- n is its negative voxel -- given n's pose.  Every slot and
    every force slot is written on every step (zeros where no link exists) so that all stores
    cover whole cache lines. */
@@ -79,126 +201,40 @@ __device__ __forceinline__ void link_slot_fp64(const DevPtrs& s, int k, int rob,
         return;
     }
     const LinkMat m = s.lmats[s.lmeta[i]];
-
     const dvec posP = dv(vd[VPX * nv + p], vd[VPY * nv + p], vd[VPZ * nv + p]);
     const dquat qP = dq(vd[VQW * nv + p], vd[VQX * nv + p], vd[VQY * nv + p], vd[VQZ * nv + p]);
-    const dvec oldPos2 = dv(ld[LP2X * nl + i], ld[LP2Y * nl + i], ld[LP2Z * nl + i]);
-    const dvec oldA1 = dv(ld[LA1X * nl + i], ld[LA1Y * nl + i], ld[LA1Z * nl + i]);
-    const dvec oldA2 = dv(ld[LA2X * nl + i], ld[LA2Y * nl + i], ld[LA2Z * nl + i]);
+    LinkPrev old;
+    old.pos2 = dv(ld[LP2X * nl + i], ld[LP2Y * nl + i], ld[LP2Z * nl + i]);
+    old.a1 = dv(ld[LA1X * nl + i], ld[LA1Y * nl + i], ld[LA1Z * nl + i]);
+    old.a2 = dv(ld[LA2X * nl + i], ld[LA2Y * nl + i], ld[LA2Z * nl + i]);
+    old.flags = s.lflags[i];
+    old.maxStrain = lf[LMAXSTRAIN * nl + i];
+    old.strainOffset = lf[LOFFSET * nl + i];
     const double rest = ld[LREST * nl + i];
-    uint32_t fl = s.lflags[i];
-    bool small = (fl & LF_SMALL) != 0, lvv = (fl & LF_LVV) != 0;
-    float maxStrain = lf[LMAXSTRAIN * nl + i], strainOffset = lf[LOFFSET * nl + i];
-
-    /* orientLink (VX_Link.cpp:77-119) */
-    dvec pos2 = toAxisX(axis, dv(posP.x - posN.x, posP.y - posN.y, posP.z - posN.z));
-    dquat a1 = toAxisXq(axis, qN);
-    dquat a2 = toAxisXq(axis, qP);
-    const dquat totalRot = qconj(a1);
-    pos2 = qrot(totalRot, pos2);
-    a2 = qmul(totalRot, a2);
-    a1 = qident();
-    const float smallTurn = (float)((fabs(pos2.z) + fabs(pos2.y)) / pos2.x);
-    const float extendPerc = (float)(fabs(1 - pos2.x / rest));
-    if (!small && smallTurn < 0.05f && extendPerc < 0.50f) {
-        small = true;
-        lvv = false;
-    } else if (small && (smallTurn > 1.2f * 0.05f || extendPerc > 1.2f * 0.50f)) {
-        small = false;
-        lvv = false;
+    float dmN = 0.0f, dmP = 0.0f;
+    if (old.flags & LF_LVV) {
+        dmN = s.vmats[s.vmat[n]].dampNum / s.vprevdt[n];
+        dmP = s.vmats[s.vmat[p]].dampNum / s.vprevdt[p];
     }
-    if (small) {
-        pos2.x -= rest;
-    } else {
-        qfromangletoposx(&a1, pos2);
-        a2 = qmul(a1, a2);
-        pos2 = dv(sqrt(pos2.x * pos2.x + pos2.y * pos2.y + pos2.z * pos2.z) - rest, 0.0, 0.0);
-    }
-    const dvec a1v = qtorv(a1), a2v = qtorv(a2);
+    LinkEval o;
+    link_eval_fp64(m, s.curveStrain + m.curveOff, s.curveStress + m.curveOff, axis, posN, qN, posP, qP, rest, old,
+                   dmN, dmP, &o);
 
-    const dvec dPos2 = dv(0.5 * (pos2.x - oldPos2.x), 0.5 * (pos2.y - oldPos2.y), 0.5 * (pos2.z - oldPos2.z));
-    const dvec dA1 = dv(0.5 * (a1v.x - oldA1.x), 0.5 * (a1v.y - oldA1.y), 0.5 * (a1v.z - oldA1.z));
-    const dvec dA2 = dv(0.5 * (a2v.x - oldA2.x), 0.5 * (a2v.y - oldA2.y), 0.5 * (a2v.z - oldA2.z));
-
-    /* updateStrain (VX_Link.cpp:220-249), Poisson's ratio 0 */
-    const float axialStrain = (float)(pos2.x / rest);
-    float stress;
-    const float* cs = s.curveStrain + m.curveOff;
-    const float* ss = s.curveStress + m.curveOff;
-    if (m.linear) {
-        if (axialStrain > maxStrain) maxStrain = axialStrain;
-        stress = link_stress(m, cs, ss, axialStrain);
-    } else if (axialStrain > maxStrain) {
-        maxStrain = axialStrain;
-        stress = link_stress(m, cs, ss, axialStrain);
-        strainOffset = maxStrain - stress / m.E;
-    } else {
-        stress = m.E * (axialStrain - strainOffset);
-    }
-
-    dvec fN, fP, mN, mP;
-    const bool failed = m.epsilonFail != -1.0f && maxStrain > m.epsilonFail;
-    if (failed) {
-        fN = fP = mN = mP = dv(0.0, 0.0, 0.0);
-    } else {
-        const float b1 = m.b1, b2 = m.b2, b3 = m.b3, am2 = m.a2;
-        fN = dv(stress * m.transverseArea,
-                b1 * pos2.y - b2 * (a1v.z + a2v.z),
-                b1 * pos2.z + b2 * (a1v.y + a2v.y));
-        fP = dv(-fN.x, -fN.y, -fN.z);
-        mN = dv(am2 * (a2v.x - a1v.x),
-                -b2 * pos2.z - b3 * (2 * a1v.y + a2v.y),
-                b2 * pos2.y - b3 * (2 * a1v.z + a2v.z));
-        mP = dv(am2 * (a1v.x - a2v.x),
-                -b2 * pos2.z - b3 * (a1v.y + 2 * a2v.y),
-                b2 * pos2.y - b3 * (a1v.z + 2 * a2v.z));
-        if (lvv) {
-            const float dmN = s.vmats[s.vmat[n]].dampNum / s.vprevdt[n];
-            const float dmP = s.vmats[s.vmat[p]].dampNum / s.vprevdt[p];
-            const dvec pc = dv(m.sqA1 * dPos2.x,
-                               m.sqB1 * dPos2.y - m.sqB2xFMp * (dA1.z + dA2.z),
-                               m.sqB1 * dPos2.z + m.sqB2xFMp * (dA1.y + dA2.y));
-            fN.x += dmN * pc.x; fN.y += dmN * pc.y; fN.z += dmN * pc.z;
-            fP.x -= dmP * pc.x; fP.y -= dmP * pc.y; fP.z -= dmP * pc.z;
-            const double hN = 0.5 * dmN, hP = 0.5 * dmP;
-            const dvec mn = dv(-m.sqA2xIp * (dA2.x - dA1.x),
-                               m.sqB2xFMp * dPos2.z + m.sqB3xIp * (2 * dA1.y + dA2.y),
-                               -m.sqB2xFMp * dPos2.y + m.sqB3xIp * (2 * dA1.z + dA2.z));
-            const dvec mp = dv(m.sqA2xIp * (dA2.x - dA1.x),
-                               m.sqB2xFMp * dPos2.z + m.sqB3xIp * (dA1.y + 2 * dA2.y),
-                               -m.sqB2xFMp * dPos2.y + m.sqB3xIp * (dA1.z + 2 * dA2.z));
-            mN.x -= hN * mn.x; mN.y -= hN * mn.y; mN.z -= hN * mn.z;
-            mP.x -= hP * mp.x; mP.y -= hP * mp.y; mP.z -= hP * mp.z;
-        } else {
-            lvv = true;
-        }
-        if (!small) {
-            fN = qrotinv(a1, fN);
-            mN = qrotinv(a1, mN);
-        }
-        fP = qrotinv(a2, fP);
-        mP = qrotinv(a2, mP);
-        fN = toAxisOriginal(axis, fN);
-        fP = toAxisOriginal(axis, fP);
-        mN = toAxisOriginal(axis, mN);
-        mP = toAxisOriginal(axis, mP);
-    }
-
-    ld[LP2X * nl + i] = pos2.x; ld[LP2Y * nl + i] = pos2.y; ld[LP2Z * nl + i] = pos2.z;
-    ld[LA1X * nl + i] = a1v.x; ld[LA1Y * nl + i] = a1v.y; ld[LA1Z * nl + i] = a1v.z;
-    ld[LA2X * nl + i] = a2v.x; ld[LA2Y * nl + i] = a2v.y; ld[LA2Z * nl + i] = a2v.z;
+    ld[LP2X * nl + i] = o.pos2.x; ld[LP2Y * nl + i] = o.pos2.y; ld[LP2Z * nl + i] = o.pos2.z;
+    ld[LA1X * nl + i] = o.a1v.x; ld[LA1Y * nl + i] = o.a1v.y; ld[LA1Z * nl + i] = o.a1v.z;
+    ld[LA2X * nl + i] = o.a2v.x; ld[LA2Y * nl + i] = o.a2v.y; ld[LA2Z * nl + i] = o.a2v.z;
     /* hand the end vectors to the two voxels' slots: the link is direction 2*axis (X/Y/Z_POS)
        of its negative voxel and 2*axis+1 (X/Y/Z_NEG) of its positive voxel */
-    vf[vf_index(dn, 0, nv, n)] = fN.x; vf[vf_index(dn, 1, nv, n)] = fN.y; vf[vf_index(dn, 2, nv, n)] = fN.z;
-    vf[vf_index(dn, 3, nv, n)] = mN.x; vf[vf_index(dn, 4, nv, n)] = mN.y; vf[vf_index(dn, 5, nv, n)] = mN.z;
-    vf[vf_index(dpp, 0, nv, p)] = fP.x; vf[vf_index(dpp, 1, nv, p)] = fP.y; vf[vf_index(dpp, 2, nv, p)] = fP.z;
-    vf[vf_index(dpp, 3, nv, p)] = mP.x; vf[vf_index(dpp, 4, nv, p)] = mP.y; vf[vf_index(dpp, 5, nv, p)] = mP.z;
-    lf[LSTRAIN * nl + i] = axialStrain;
-    lf[LMAXSTRAIN * nl + i] = maxStrain;
-    lf[LOFFSET * nl + i] = strainOffset;
-    lf[LSTRESS * nl + i] = stress;
-    s.lflags[i] = (lvv ? LF_LVV : 0u) | (small ? LF_SMALL : 0u);
-    if (axialStrain > 100) {
+    vf[vf_index(dn, 0, nv, n)] = o.fN.x; vf[vf_index(dn, 1, nv, n)] = o.fN.y; vf[vf_index(dn, 2, nv, n)] = o.fN.z;
+    vf[vf_index(dn, 3, nv, n)] = o.mN.x; vf[vf_index(dn, 4, nv, n)] = o.mN.y; vf[vf_index(dn, 5, nv, n)] = o.mN.z;
+    vf[vf_index(dpp, 0, nv, p)] = o.fP.x; vf[vf_index(dpp, 1, nv, p)] = o.fP.y; vf[vf_index(dpp, 2, nv, p)] = o.fP.z;
+    vf[vf_index(dpp, 3, nv, p)] = o.mP.x; vf[vf_index(dpp, 4, nv, p)] = o.mP.y; vf[vf_index(dpp, 5, nv, p)] = o.mP.z;
+    lf[LSTRAIN * nl + i] = o.strain;
+    lf[LMAXSTRAIN * nl + i] = o.maxStrain;
+    lf[LOFFSET * nl + i] = o.strainOffset;
+    lf[LSTRESS * nl + i] = o.stress;
+    s.lflags[i] = o.flags;
+    if (o.strain > 100) {
         atomicMin(&s.rdiv[rob], k);
         atomicOr(&s.ctl[0], 1);
     }
@@ -206,7 +242,13 @@ __device__ __forceinline__ void link_slot_fp64(const DevPtrs& s, int k, int rob,
 
 /* one lane per voxel: the three links it is the negative end of (axes x, y, z), its own pose
    read once */
-__global__ void __launch_bounds__(256) k_links_fp64(DevPtrs s, int k) {
+#ifndef VX_LINK_WAVES
+#define VX_LINK_WAVES 3
+#endif
+#ifndef VX_VOXEL_WAVES
+#define VX_VOXEL_WAVES 4
+#endif
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_LINK_WAVES, 8))) k_links_fp64(DevPtrs s, int k) {
     const int n = (int)blockIdx.x * 256 + (int)threadIdx.x;
     const int nv = s.nv;
     if (n >= nv) return;
@@ -228,68 +270,29 @@ __device__ __forceinline__ float floor_penetration(const VoxMat& m, float temp, 
     return (float)(bsa / 2 - m.nomSize / 2 - pz);
 }
 
-__global__ void __launch_bounds__(256) k_voxels_fp64(DevPtrs s, int k) {
-    const int i = (int)blockIdx.x * 256 + (int)threadIdx.x;
-    if (i >= s.nv) return;
-    const int rob = s.vrobot[i];
-    if (s.rdiv[rob] <= k) return; /* diverged in this step or earlier: voxels and time stay */
-    const float dt = s.rdt[rob];
-    if (i == 0 || s.vrobot[i - 1] != rob) s.rtime[rob] += dt; /* currentTime += dt (Voxelyze.cpp:282) */
-    const int nv = s.nv;
-    double* __restrict__ vd = s.vd;
-    const VoxMat m = s.vmats[s.vmat[i]];
-    const int e = s.vext[i];
-    const ExtRec* ex = e >= 0 ? &s.exts[e] : nullptr;
-    s.vprevdt[i] = dt;
-
-    if (ex && (ex->dof & 0x3F) == 0x3F) {
-        /* fully fixed: pinned to its prescribed pose (VX_Voxel.cpp:167-172) */
-        const double sz = m.nomSize;
-        vd[VPX * nv + i] = s.vijk[i] * sz + ex->tx;
-        vd[VPY * nv + i] = s.vijk[nv + i] * sz + ex->ty;
-        vd[VPZ * nv + i] = s.vijk[2 * nv + i] * sz + ex->tz;
-        vd[VQW * nv + i] = ex->qw; vd[VQX * nv + i] = ex->qx; vd[VQY * nv + i] = ex->qy; vd[VQZ * nv + i] = ex->qz;
-        vd[VLX * nv + i] = 0.0; vd[VLY * nv + i] = 0.0; vd[VLZ * nv + i] = 0.0;
-        vd[VAX * nv + i] = 0.0; vd[VAY * nv + i] = 0.0; vd[VAZ * nv + i] = 0.0;
-        return;
-    }
-
-    dvec pos = dv(vd[VPX * nv + i], vd[VPY * nv + i], vd[VPZ * nv + i]);
-    dvec lin = dv(vd[VLX * nv + i], vd[VLY * nv + i], vd[VLZ * nv + i]);
-    dvec ang = dv(vd[VAX * nv + i], vd[VAY * nv + i], vd[VAZ * nv + i]);
-    dquat q = dq(vd[VQW * nv + i], vd[VQX * nv + i], vd[VQY * nv + i], vd[VQZ * nv + i]);
-    uint32_t flags = s.vflags[i];
-
-    const double* __restrict__ vf = s.vf;
-
-    /* force() (VX_Voxel.cpp:234-256): link end forces in direction order 0..5 */
-    dvec t = dv(0.0, 0.0, 0.0);
-#pragma unroll
-    for (int d = 0; d < 6; d++) {
-        t.x += vf[vf_index(d, 0, nv, i)]; t.y += vf[vf_index(d, 1, nv, i)]; t.z += vf[vf_index(d, 2, nv, i)];
-    }
-    t = qrot(q, t);
+/* CVX_Voxel::timeStep from the link force and moment sums on (VX_Voxel.cpp:174-232, force()
+   and moment() after their direction-ordered sums, VX_Voxel.cpp:234-271, floorForce
+   :274-298), in place on the state; contacts(t) subtracts the contact forces in partner
+   order (VX_Voxel.cpp:250-253). */
+template <class Contacts>
+__device__ __forceinline__ void voxel_step_fp64(const VoxMat& m, const ExtRec* ex, const int* __restrict__ vijk, int nv,
+                                                int i, float dt, float temp, dvec sumF, dvec sumM, dvec& pos,
+                                                dvec& lin, dvec& ang, dquat& q, uint32_t& flags, Contacts contacts) {
+    dvec t = qrot(q, sumF);
     if (ex) { t.x += ex->fx; t.y += ex->fy; t.z += ex->fz; }
     {
         const dvec vel = dv(m.massInverse * lin.x, m.massInverse * lin.y, m.massInverse * lin.z);
         t.x -= m.globalTransC * vel.x; t.y -= m.globalTransC * vel.y; t.z -= m.globalTransC * vel.z;
     }
     t.z += m.gravityForce;
-    if ((flags & VF_COLLISIONS) && s.colCnt) {
-        /* contact forces in partner order (VX_Voxel.cpp:250-253) */
-        const int cnt = s.colCnt[i], off = s.colOff[i];
-        for (int k = 0; k < cnt; k++) {
-            const float* f = s.colF + 3 * (size_t)(off + k);
-            t.x -= f[0]; t.y -= f[1]; t.z -= f[2];
-        }
-    }
+    if (flags & VF_COLLISIONS) contacts(t);
 
     dvec cur = t;
     const bool floorOn = (flags & VF_FLOOR) != 0;
     float pen = 0.0f;
     if (floorOn) {
         /* floorForce (VX_Voxel.cpp:274-298) */
-        pen = floor_penetration(m, s.vtemp[i], pos.z);
+        pen = floor_penetration(m, temp, pos.z);
         if (pen >= 0) {
             const dvec vel = dv(m.massInverse * lin.x, m.massInverse * lin.y, m.massInverse * lin.z);
             const float normalForce = m.penetrationStiff * pen;
@@ -328,13 +331,7 @@ __global__ void __launch_bounds__(256) k_voxels_fp64(DevPtrs s, int k) {
     }
     pos.x += tr.x; pos.y += tr.y; pos.z += tr.z;
 
-    /* moment() (VX_Voxel.cpp:258-271) */
-    dvec mo = dv(0.0, 0.0, 0.0);
-#pragma unroll
-    for (int d = 0; d < 6; d++) {
-        mo.x += vf[vf_index(d, 3, nv, i)]; mo.y += vf[vf_index(d, 4, nv, i)]; mo.z += vf[vf_index(d, 5, nv, i)];
-    }
-    mo = qrot(q, mo);
+    dvec mo = qrot(q, sumM);
     if (ex) { mo.x += ex->mx; mo.y += ex->my; mo.z += ex->mz; }
     {
         const dvec av = dv(m.momentInertiaInverse * ang.x, m.momentInertiaInverse * ang.y, m.momentInertiaInverse * ang.z);
@@ -348,9 +345,9 @@ __global__ void __launch_bounds__(256) k_voxels_fp64(DevPtrs s, int k) {
         /* partial dof fixes (VX_Voxel.cpp:210-228) */
         const double sz = m.nomSize;
         const int dof = ex->dof;
-        if (dof & 1) { pos.x = s.vijk[i] * sz + ex->tx; lin.x = 0.0; }
-        if (dof & 2) { pos.y = s.vijk[nv + i] * sz + ex->ty; lin.y = 0.0; }
-        if (dof & 4) { pos.z = s.vijk[2 * nv + i] * sz + ex->tz; lin.z = 0.0; }
+        if (dof & 1) { pos.x = vijk[i] * sz + ex->tx; lin.x = 0.0; }
+        if (dof & 2) { pos.y = vijk[nv + i] * sz + ex->ty; lin.y = 0.0; }
+        if (dof & 4) { pos.z = vijk[2 * nv + i] * sz + ex->tz; lin.z = 0.0; }
         if (dof & 56) {
             if ((dof & 56) == 56) {
                 q = dq(ex->qw, ex->qx, ex->qy, ex->qz);
@@ -364,11 +361,70 @@ __global__ void __launch_bounds__(256) k_voxels_fp64(DevPtrs s, int k) {
             }
         }
     }
+}
 
-    vd[VPX * nv + i] = pos.x; vd[VPY * nv + i] = pos.y; vd[VPZ * nv + i] = pos.z;
-    vd[VLX * nv + i] = lin.x; vd[VLY * nv + i] = lin.y; vd[VLZ * nv + i] = lin.z;
-    vd[VAX * nv + i] = ang.x; vd[VAY * nv + i] = ang.y; vd[VAZ * nv + i] = ang.z;
-    vd[VQW * nv + i] = q.w; vd[VQX * nv + i] = q.x; vd[VQY * nv + i] = q.y; vd[VQZ * nv + i] = q.z;
+/* a fully fixed voxel sits at its prescribed pose (VX_Voxel.cpp:167-172) */
+__device__ __forceinline__ void voxel_pinned(const VoxMat& m, const ExtRec* ex, const int* __restrict__ vijk, int nv,
+                                             int i, double* __restrict__ out) {
+    const double sz = m.nomSize;
+    out[VPX * nv + i] = vijk[i] * sz + ex->tx;
+    out[VPY * nv + i] = vijk[nv + i] * sz + ex->ty;
+    out[VPZ * nv + i] = vijk[2 * nv + i] * sz + ex->tz;
+    out[VQW * nv + i] = ex->qw; out[VQX * nv + i] = ex->qx; out[VQY * nv + i] = ex->qy; out[VQZ * nv + i] = ex->qz;
+    out[VLX * nv + i] = 0.0; out[VLY * nv + i] = 0.0; out[VLZ * nv + i] = 0.0;
+    out[VAX * nv + i] = 0.0; out[VAY * nv + i] = 0.0; out[VAZ * nv + i] = 0.0;
+}
+
+__device__ __forceinline__ void store_voxel(double* __restrict__ out, int nv, int i, const dvec& pos, const dvec& lin,
+                                            const dvec& ang, const dquat& q) {
+    out[VPX * nv + i] = pos.x; out[VPY * nv + i] = pos.y; out[VPZ * nv + i] = pos.z;
+    out[VLX * nv + i] = lin.x; out[VLY * nv + i] = lin.y; out[VLZ * nv + i] = lin.z;
+    out[VAX * nv + i] = ang.x; out[VAY * nv + i] = ang.y; out[VAZ * nv + i] = ang.z;
+    out[VQW * nv + i] = q.w; out[VQX * nv + i] = q.x; out[VQY * nv + i] = q.y; out[VQZ * nv + i] = q.z;
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_VOXEL_WAVES, 8))) k_voxels_fp64(DevPtrs s, int k) {
+    const int i = (int)blockIdx.x * 256 + (int)threadIdx.x;
+    if (i >= s.nv) return;
+    const int rob = s.vrobot[i];
+    if (s.rdiv[rob] <= k) return; /* diverged in this step or earlier: voxels and time stay */
+    const float dt = s.rdt[rob];
+    if (i == 0 || s.vrobot[i - 1] != rob) s.rtime[rob] += dt; /* currentTime += dt (Voxelyze.cpp:282) */
+    const int nv = s.nv;
+    double* __restrict__ vd = s.vd;
+    const VoxMat m = s.vmats[s.vmat[i]];
+    const int e = s.vext[i];
+    const ExtRec* ex = e >= 0 ? &s.exts[e] : nullptr;
+    s.vprevdt[i] = dt;
+    if (ex && (ex->dof & 0x3F) == 0x3F) {
+        voxel_pinned(m, ex, s.vijk, nv, i, vd);
+        return;
+    }
+    dvec pos = dv(vd[VPX * nv + i], vd[VPY * nv + i], vd[VPZ * nv + i]);
+    dvec lin = dv(vd[VLX * nv + i], vd[VLY * nv + i], vd[VLZ * nv + i]);
+    dvec ang = dv(vd[VAX * nv + i], vd[VAY * nv + i], vd[VAZ * nv + i]);
+    dquat q = dq(vd[VQW * nv + i], vd[VQX * nv + i], vd[VQY * nv + i], vd[VQZ * nv + i]);
+    uint32_t flags = s.vflags[i];
+    const double* __restrict__ vf = s.vf;
+    /* force() and moment() (VX_Voxel.cpp:234-271): link end vectors in direction order 0..5 */
+    dvec sf = dv(0.0, 0.0, 0.0), sm = dv(0.0, 0.0, 0.0);
+#pragma unroll
+    for (int d = 0; d < 6; d++) {
+        sf.x += vf[vf_index(d, 0, nv, i)]; sf.y += vf[vf_index(d, 1, nv, i)]; sf.z += vf[vf_index(d, 2, nv, i)];
+        sm.x += vf[vf_index(d, 3, nv, i)]; sm.y += vf[vf_index(d, 4, nv, i)]; sm.z += vf[vf_index(d, 5, nv, i)];
+    }
+    const int* colCnt = s.colCnt;
+    const int* colOff = s.colOff;
+    const float* colF = s.colF;
+    voxel_step_fp64(m, ex, s.vijk, nv, i, dt, s.vtemp[i], sf, sm, pos, lin, ang, q, flags, [&](dvec& t) {
+        if (!colCnt) return;
+        const int cnt = colCnt[i], off = colOff[i];
+        for (int c = 0; c < cnt; c++) {
+            const float* f = colF + 3 * (size_t)(off + c);
+            t.x -= f[0]; t.y -= f[1]; t.z -= f[2];
+        }
+    });
+    store_voxel(vd, nv, i, pos, lin, ang, q);
     s.vflags[i] = flags;
 }
 

@@ -237,8 +237,11 @@ __device__ void frozen_copy(const MixPtrs& s, int i, int rob, int d) {
     }
 }
 
+#ifndef VX_FUSED_WAVES
+#define VX_FUSED_WAVES 3
+#endif
 template <typename T>
-__global__ void __launch_bounds__(256) k_step_mixed(MixPtrs s) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_FUSED_WAVES, 8))) k_step_mixed(MixPtrs s) {
     const int i = xcd_block() * 256 + (int)threadIdx.x;
     const int nv = s.nv;
     if (i >= nv) return;
@@ -255,12 +258,12 @@ __global__ void __launch_bounds__(256) k_step_mixed(MixPtrs s) {
     const dvec pos0 = dv(vd[VPX * nv + i], vd[VPY * nv + i], vd[VPZ * nv + i]);
     const dquat q0 = dq(vd[VQW * nv + i], vd[VQX * nv + i], vd[VQY * nv + i], vd[VQZ * nv + i]);
     uint32_t flags = s.vflIn[i];
-    VoxMat m = s.vmats[s.vmat[i]];
-    if (s.vE) voxel_modulus(&m, s.vE[i]);
     const float tempI = s.vtempIn[i];
+    const int mi = s.vmat[i];
 
     /* link phase for this voxel's <= 6 links, summed in direction order */
     dvec lf = dv(0.0, 0.0, 0.0), lm = dv(0.0, 0.0, 0.0);
+#pragma unroll 1
     for (int dir = 0; dir < 6; dir++) {
         const int o = s.vnbr[dir * nv + i];
         if (o < 0) continue;
@@ -269,10 +272,11 @@ __global__ void __launch_bounds__(256) k_step_mixed(MixPtrs s) {
         const int slot = neg ? axis * nv + i : axis * nv + o;
         double rest;
         if (s.vphase) { /* actuated: rest length from the two temperatures (VX_Link.cpp:137-140) */
-            const VoxMat mo = s.vmats[s.vmat[o]];
-            const double si = axis == 0 ? m.sizeX : axis == 1 ? m.sizeY : m.sizeZ;
+            const VoxMat& mt = s.vmats[mi];
+            const VoxMat& mo = s.vmats[s.vmat[o]];
+            const double si = axis == 0 ? mt.sizeX : axis == 1 ? mt.sizeY : mt.sizeZ;
             const double so = axis == 0 ? mo.sizeX : axis == 1 ? mo.sizeY : mo.sizeZ;
-            const double bi = base_size(si, tempI, m.alphaCTE), bo = base_size(so, s.vtempIn[o], mo.alphaCTE);
+            const double bi = base_size(si, tempI, mt.alphaCTE), bo = base_size(so, s.vtempIn[o], mo.alphaCTE);
             rest = neg ? 0.5 * (bi + bo) : 0.5 * (bo + bi);
         } else {
             rest = s.lrest[slot];
@@ -280,8 +284,9 @@ __global__ void __launch_bounds__(256) k_step_mixed(MixPtrs s) {
         const dvec po = dv(vd[VPX * nv + o], vd[VPY * nv + o], vd[VPZ * nv + o]);
         const dquat qo = dq(vd[VQW * nv + o], vd[VQX * nv + o], vd[VQY * nv + o], vd[VQZ * nv + o]);
         dvec f, mm;
-        if (neg) link_fused<T>(s, slot, i, o, axis, false, true, rest, prevDt, pos0, q0, po, qo, &f, &mm);
-        else link_fused<T>(s, slot, o, i, axis, true, false, rest, prevDt, po, qo, pos0, q0, &f, &mm);
+        /* one call site: the negative end first */
+        link_fused<T>(s, slot, neg ? i : o, neg ? o : i, axis, !neg, neg, rest, prevDt, neg ? pos0 : po, neg ? q0 : qo,
+                      neg ? po : pos0, neg ? qo : q0, &f, &mm);
         lf.x += f.x; lf.y += f.y; lf.z += f.z;
         lm.x += mm.x; lm.y += mm.y; lm.z += mm.z;
     }
@@ -292,6 +297,8 @@ __global__ void __launch_bounds__(256) k_step_mixed(MixPtrs s) {
         ts[3 * nv + i] = lm.x; ts[4 * nv + i] = lm.y; ts[5 * nv + i] = lm.z;
     }
 
+    VoxMat m = s.vmats[mi];
+    if (s.vE) voxel_modulus(&m, s.vE[i]);
     const int e = s.vext[i];
     const ExtRec* ex = e >= 0 ? &s.exts[e] : nullptr;
     double* __restrict__ out = s.vdOut;

@@ -140,6 +140,11 @@ int vxg_set_precision(vxg_sim* s, int mode);
    milliseconds and launch counts since the last read, then clears them.  Returns the
    number of classes written. */
 int vxg_profile(vxg_sim* s, int enable);
+/* Step graphs (no reference counterpart): a multi-step vxg_step / vxg_robot_step launch
+   without self-collision is recorded once as a HIP graph and replayed while its buffers and
+   length stay the same (default on).  vxg_graph_stats reports captures and replays. */
+int vxg_set_graphs(vxg_sim* s, int enable);
+int vxg_graph_stats(vxg_sim* s, int* captures, int* replays);
 int vxg_profile_read(vxg_sim* s, double* ms, int* launches, int cap);
 
 /* ---------------------------------------------------------------- materials */

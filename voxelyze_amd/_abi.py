@@ -93,6 +93,8 @@ EXTENSIONS = {
     "adjoint_run": (_I, [_P, _I, _I, _FP]),
     "adjoint_backward": (_I, [_P, _DP, _DP, _DP]),
     "adjoint_free": (None, [_P]),
+    "set_graphs": (_I, [_P, _I]),
+    "graph_stats": (_I, [_P, _IP, _IP]),
 }
 
 

@@ -204,6 +204,18 @@ API int vxg_profile(vxg_sim* h, int en) {
     S(h)->profiling = en != 0;
     return 1;
 }
+API int vxg_set_graphs(vxg_sim* h, int en) {
+    Sim* s = S(h);
+    s->useGraphs = en != 0;
+    if (!en) s->clearGraphs();
+    return 1;
+}
+API int vxg_graph_stats(vxg_sim* h, int* captures, int* replays) {
+    Sim* s = S(h);
+    if (captures) *captures = s->graphCaptures;
+    if (replays) *replays = s->graphReplays;
+    return 1;
+}
 API int vxg_profile_read(vxg_sim* h, double* ms, int* launches, int cap) {
     Sim* s = S(h);
     int n = cap < 4 ? cap : 4;

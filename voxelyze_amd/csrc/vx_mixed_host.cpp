@@ -12,6 +12,7 @@ namespace vx {
 
 void launch_step_mixed(const MixPtrs& s, bool fp64, hipStream_t st);
 void launch_mixed_temps(const MixPtrs& s, float* temp, hipStream_t st);
+uint64_t graph_key(const void* args, size_t n, int path, int nsteps, int flags);
 void launch_col_stale(const DevPtrs& s, const ContactPtrs& c, hipStream_t st);
 
 #define HIPCHK(call)                                       \
@@ -139,6 +140,30 @@ bool Sim::stepMixed(int nsteps, const MixTape* tape) {
     mp.actAmp = actAmp;
     mp.actFreq = actFreq;
     mp.vE = modOn ? (const float*)bvE.p : nullptr;
+    /* a launch without contacts, tape or timing is recorded as a graph and replayed */
+    const bool graph = useGraphs && !collisions && !tape && !profiling && nsteps > 1 && nsteps <= 2048;
+    uint64_t key = 0;
+    if (graph) {
+        MixPtrs k0 = mp;
+        k0.vdIn = dp.vd; k0.vdOut = (double*)bvdAlt.p;
+        k0.vflIn = dp.vflags; k0.vflOut = (uint32_t*)bvflAlt.p;
+        k0.vtempIn = dp.vtemp; k0.vtempOut = (float*)bvtempAlt.p;
+        k0.rtimeIn = dp.rtime; k0.rtimeOut = (float*)brtimeAlt.p;
+        k0.lsIn = bms[lcur].p; k0.lsOut = bms[lcur ^ 1].p;
+        k0.lflIn = (const uint32_t*)bmfl[lcur].p; k0.lflOut = (uint32_t*)bmfl[lcur ^ 1].p;
+        key = graph_key(&k0, sizeof k0, 1 + precision, nsteps, actOn ? 1 : 0);
+        bool replayed = false;
+        if (!graphLaunch(key, &replayed)) return false;
+        if (replayed) { /* the host side of the ping-pong, as the recorded launch left it */
+            if (nsteps & 1) {
+                swapVoxelBuffers();
+                lcur ^= 1;
+            }
+            HIPCHK(hipStreamSynchronize(stream));
+            return true;
+        }
+        HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    }
     if (actOn) { /* temperatures at the robots' current times */
         mp.rtimeIn = dp.rtime;
         launch_mixed_temps(mp, dp.vtemp, stream);
@@ -201,6 +226,11 @@ bool Sim::stepMixed(int nsteps, const MixTape* tape) {
         swapVoxelBuffers();
         lcur ^= 1;
         launched++;
+    }
+    if (graph) {
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamEndCapture(stream, &g));
+        if (!graphStore(key, g)) return false;
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(stream));

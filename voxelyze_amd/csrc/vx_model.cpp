@@ -311,6 +311,7 @@ DevBuf::~DevBuf() {
 
 Sim::~Sim() {
     adjointFree();
+    clearGraphs();
     if (stream) (void)hipStreamSynchronize(stream);
     for (hipEvent_t e : events) (void)hipEventDestroy(e);
     if (ctlHost) (void)hipHostFree(ctlHost);
@@ -1054,10 +1055,75 @@ bool Sim::anyDiverged(int k, bool* all) {
     return true;
 }
 
+/* ------------------------------------------------------------ step graphs */
+void Sim::clearGraphs() {
+    for (auto& g : graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    graphs.clear();
+}
+
+/* replay the graph recorded under key, if there is one */
+bool Sim::graphLaunch(uint64_t key, bool* replayed) {
+    *replayed = false;
+    for (auto& g : graphs) {
+        if (g.key != key) continue;
+        HIPCHK(hipGraphLaunch(g.exec, stream));
+        graphReplays++;
+        *replayed = true;
+        return true;
+    }
+    return true;
+}
+
+/* instantiate a captured graph, keep it (a few per simulation) and run it once */
+bool Sim::graphStore(uint64_t key, hipGraph_t g) {
+    hipGraphExec_t exec = nullptr;
+    hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail("hipGraphInstantiate", e);
+    if (graphs.size() >= 4) {
+        (void)hipGraphExecDestroy(graphs.front().exec);
+        graphs.erase(graphs.begin());
+    }
+    graphs.push_back({key, exec});
+    graphCaptures++;
+    HIPCHK(hipGraphLaunch(exec, stream));
+    return true;
+}
+
+/* FNV-1a over the bytes of the arguments baked into a captured launch */
+static uint64_t fnv(uint64_t h, const void* p, size_t n) {
+    const unsigned char* b = (const unsigned char*)p;
+    for (size_t i = 0; i < n; i++) { h ^= b[i]; h *= 1099511628211ull; }
+    return h;
+}
+uint64_t graph_key(const void* args, size_t n, int path, int nsteps, int flags) {
+    uint64_t h = 14695981039346656037ull;
+    h = fnv(h, args, n);
+    h = fnv(h, &path, sizeof path);
+    h = fnv(h, &nsteps, sizeof nsteps);
+    return fnv(h, &flags, sizeof flags);
+}
+
+/* steps captured into one graph at most (longer launches run as plain kernel sequences) */
+static const int kGraphMaxSteps = 2048;
+
 bool Sim::stepFp64(int nsteps) {
     dp.colCnt = nullptr;
     dp.colOff = nullptr;
     dp.colF = nullptr;
+    const bool graph = useGraphs && !profiling && nsteps > 1 && nsteps <= kGraphMaxSteps;
+    uint64_t key = 0;
+    if (graph) {
+        key = graph_key(&dp, sizeof dp, 0, nsteps, actOn ? 1 : 0);
+        bool replayed = false;
+        if (!graphLaunch(key, &replayed)) return false;
+        if (replayed) {
+            HIPCHK(hipStreamSynchronize(stream));
+            return true;
+        }
+        HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    }
     for (int k = 0; k < nsteps; k++) {
         if (actOn) launch_actuate_fp64(dp, k, stream);
         if (profiling) (void)hipEventRecord(events[4 * k], stream);
@@ -1066,6 +1132,11 @@ bool Sim::stepFp64(int nsteps) {
         if (profiling) (void)hipEventRecord(events[4 * k + 2], stream);
         launch_voxels_fp64(dp, k, stream);
         if (profiling) (void)hipEventRecord(events[4 * k + 3], stream);
+    }
+    if (graph) {
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamEndCapture(stream, &g));
+        if (!graphStore(key, g)) return false;
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(stream));

@@ -167,6 +167,18 @@ struct Sim {
     bool contactBuffers();
     bool nearMasks();
     bool regenerateCollisions();
+    /* captured step sequences (HIP graphs): a multi-step launch is recorded once and replayed
+       while its buffers and length stay the same */
+    struct StepGraph {
+        uint64_t key = 0;
+        hipGraphExec_t exec = nullptr;
+    };
+    std::vector<StepGraph> graphs;
+    bool useGraphs = true;
+    int graphReplays = 0, graphCaptures = 0;
+    void clearGraphs();
+    bool graphLaunch(uint64_t key, bool* replayed);
+    bool graphStore(uint64_t key, hipGraph_t g);
     /* kernel timing */
     bool profiling = false;
     std::vector<hipEvent_t> events;

@@ -137,6 +137,13 @@ def measure(precision, workload, steps, warmup):
           + ", ".join(f"{k} {v['avg_us']:.1f} us {v['gbs']:.0f} GB/s" for k, v in kernels.items()),
           file=sys.stderr, flush=True)
     cfg.update({"voxels": V, "links": L, "dt": dt, "voxel_pitch_m": 0.005})
+    tuned = {}
+    for kk, kname in enumerate(["k_step_mixed", "k_links_fp64", "k_voxels_fp64"]):
+        b, x, st = C.c_int(), C.c_int(), C.c_int()
+        sim.abi.launch_config(sim.handle, kk, C.byref(b), C.byref(x), C.byref(st))
+        if st.value:
+            tuned[kname] = {"block": b.value, "xcd_mapping": bool(x.value)}
+    cfg["autotuned_launch"] = tuned
     return sim, {"value": V * steps / el, "el": el, "kernels": kernels, "cfg": cfg, "V": V, "L": L,
                  "nominal": nominal}
 

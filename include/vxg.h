@@ -145,6 +145,13 @@ int vxg_profile(vxg_sim* s, int enable);
    length stay the same (default on).  vxg_graph_stats reports captures and replays. */
 int vxg_set_graphs(vxg_sim* s, int enable);
 int vxg_graph_stats(vxg_sim* s, int* captures, int* replays);
+/* Autotuner (no reference counterpart, default on): the workgroup size (64/128/256) and
+   block-to-XCD mapping of kernel 0 = fused step, 1 = FP64 link kernel, 2 = FP64 voxel
+   kernel are chosen by timing each candidate twice on the first steps of every model size;
+   results do not depend on the choice.  vxg_launch_config reports it (settled = 1 once
+   chosen). */
+int vxg_autotune(vxg_sim* s, int enable);
+int vxg_launch_config(vxg_sim* s, int kernel, int* block, int* xcd_mapping, int* settled);
 int vxg_profile_read(vxg_sim* s, double* ms, int* launches, int cap);
 
 /* ---------------------------------------------------------------- materials */

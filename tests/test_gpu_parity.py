@@ -139,3 +139,27 @@ def test_graph_replays_equal_plain_launches(precision):
     a.abi.graph_stats(a.handle, C.byref(cap), C.byref(rpl))
     print(f"[graphs] precision {precision}: {cap.value} captures, {rpl.value} replays")
     assert cap.value >= 1 and rpl.value >= 2
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1])
+def test_autotuned_launches_give_the_same_bits(precision):
+    """Every launch configuration the autotuner tries gives the reference's bits: a run that
+    tunes on its first steps equals a run with tuning off."""
+    import ctypes as C
+    from voxelyze_amd import workloads as W
+    a = B.maker(B.product_abi())(0.005)
+    b = B.maker(B.product_abi())(0.005)
+    for sim in (a, b):
+        W.drop_lattice(sim, nx=24, ny=24, nz=8)
+        sim.setPrecision(precision)
+    b.abi.autotune(b.handle, 0)
+    dt = a.recommendedTimeStep()
+    a.step(dt, 30)
+    b.step(dt, 30)
+    sa, sb = a.state(), b.state()
+    assert all(sa[k].tobytes() == sb[k].tobytes() for k in sa)
+    blk, x, st = C.c_int(), C.c_int(), C.c_int()
+    a.abi.launch_config(a.handle, 0 if precision else 1, C.byref(blk), C.byref(x), C.byref(st))
+    print(f"[tune] precision {precision}: block {blk.value} xcd {x.value} settled {st.value}")
+    assert st.value == 1

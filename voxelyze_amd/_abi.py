@@ -95,6 +95,8 @@ EXTENSIONS = {
     "adjoint_free": (None, [_P]),
     "set_graphs": (_I, [_P, _I]),
     "graph_stats": (_I, [_P, _IP, _IP]),
+    "autotune": (_I, [_P, _I]),
+    "launch_config": (_I, [_P, _I, _IP, _IP, _IP]),
 }
 
 

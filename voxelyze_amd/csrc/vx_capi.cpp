@@ -216,6 +216,19 @@ API int vxg_graph_stats(vxg_sim* h, int* captures, int* replays) {
     if (replays) *replays = s->graphReplays;
     return 1;
 }
+API int vxg_autotune(vxg_sim* h, int en) {
+    S(h)->autotune = en != 0;
+    return 1;
+}
+API int vxg_launch_config(vxg_sim* h, int kernel, int* block, int* xcd, int* settled) {
+    Sim* s = S(h);
+    if (kernel < 0 || kernel > 2) return 0;
+    const Sim::Tuner& t = s->tuner[kernel];
+    if (block) *block = t.chosen.block;
+    if (xcd) *xcd = t.chosen.xcd;
+    if (settled) *settled = t.settled ? 1 : 0;
+    return 1;
+}
 API int vxg_profile_read(vxg_sim* h, double* ms, int* launches, int cap) {
     Sim* s = S(h);
     int n = cap < 4 ? cap : 4;

@@ -139,6 +139,10 @@ __device__ __forceinline__ int xcd_block() {
     const int q = nb >> 3, r = nb & 7, x = b & 7;
     return x * q + (x < r ? x : r) + (b >> 3);
 }
+/* the lane's element under a launch configuration (workgroup size, XCD-contiguous mapping) */
+__device__ __forceinline__ int lane_index(int xcd) {
+    return (xcd ? xcd_block() : (int)blockIdx.x) * (int)blockDim.x + (int)threadIdx.x;
+}
 #endif
 
 /* self-collision buffers (vx_contact.hip) */
@@ -206,6 +210,13 @@ struct MixPtrs {
     const float* vE;        /* per-voxel Young's modulus overriding the materials' (linear), null: off */
     double* tapeSums;       /* when set, each voxel records its link force and moment sums (6 * nv) */
     int k;                  /* this step's index in the launch */
+    int xcd;                /* launch configuration: XCD-contiguous block mapping */
+};
+
+/* a kernel launch configuration chosen by the autotuner */
+struct LaunchCfg {
+    int block;  /* workgroup size */
+    int xcd;    /* XCD-contiguous block mapping */
 };
 
 }  // namespace vx

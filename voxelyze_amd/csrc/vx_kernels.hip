@@ -248,8 +248,9 @@ __device__ __forceinline__ void link_slot_fp64(const DevPtrs& s, int k, int rob,
 #ifndef VX_VOXEL_WAVES
 #define VX_VOXEL_WAVES 4
 #endif
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_LINK_WAVES, 8))) k_links_fp64(DevPtrs s, int k) {
-    const int n = (int)blockIdx.x * 256 + (int)threadIdx.x;
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_LINK_WAVES, 8))) k_links_fp64(DevPtrs s, int k,
+                                                                                                         int xcd) {
+    const int n = lane_index(xcd);
     const int nv = s.nv;
     if (n >= nv) return;
     const int rob = s.vrobot[n];
@@ -383,8 +384,9 @@ __device__ __forceinline__ void store_voxel(double* __restrict__ out, int nv, in
     out[VQW * nv + i] = q.w; out[VQX * nv + i] = q.x; out[VQY * nv + i] = q.y; out[VQZ * nv + i] = q.z;
 }
 
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_VOXEL_WAVES, 8))) k_voxels_fp64(DevPtrs s, int k) {
-    const int i = (int)blockIdx.x * 256 + (int)threadIdx.x;
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_VOXEL_WAVES, 8))) k_voxels_fp64(DevPtrs s, int k,
+                                                                                                           int xcd) {
+    const int i = lane_index(xcd);
     if (i >= s.nv) return;
     const int rob = s.vrobot[i];
     if (s.rdiv[rob] <= k) return; /* diverged in this step or earlier: voxels and time stay */
@@ -484,11 +486,11 @@ __global__ void __launch_bounds__(256) k_robot_com(DevPtrs s, const int* rfirst,
 
 static inline unsigned blocks(int n, int b) { return (unsigned)((n + b - 1) / b); }
 
-void launch_links_fp64(const DevPtrs& s, int k, hipStream_t st) {
-    if (s.nv > 0) hipLaunchKernelGGL(k_links_fp64, dim3(blocks(s.nv, 256)), dim3(256), 0, st, s, k);
+void launch_links_fp64(const DevPtrs& s, int k, LaunchCfg c, hipStream_t st) {
+    if (s.nv > 0) hipLaunchKernelGGL(k_links_fp64, dim3(blocks(s.nv, c.block)), dim3(c.block), 0, st, s, k, c.xcd);
 }
-void launch_voxels_fp64(const DevPtrs& s, int k, hipStream_t st) {
-    if (s.nv > 0) hipLaunchKernelGGL(k_voxels_fp64, dim3(blocks(s.nv, 256)), dim3(256), 0, st, s, k);
+void launch_voxels_fp64(const DevPtrs& s, int k, LaunchCfg c, hipStream_t st) {
+    if (s.nv > 0) hipLaunchKernelGGL(k_voxels_fp64, dim3(blocks(s.nv, c.block)), dim3(c.block), 0, st, s, k, c.xcd);
 }
 void launch_actuate_fp64(const DevPtrs& s, int k, hipStream_t st) {
     if (s.nv > 0) hipLaunchKernelGGL(k_actuate_fp64, dim3(blocks(s.nv, 256)), dim3(256), 0, st, s, k);

@@ -242,7 +242,7 @@ __device__ void frozen_copy(const MixPtrs& s, int i, int rob, int d) {
 #endif
 template <typename T>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_FUSED_WAVES, 8))) k_step_mixed(MixPtrs s) {
-    const int i = xcd_block() * 256 + (int)threadIdx.x;
+    const int i = lane_index(s.xcd);
     const int nv = s.nv;
     if (i >= nv) return;
     const int rob = s.vrobot[i];
@@ -431,10 +431,11 @@ void launch_mixed_temps(const MixPtrs& s, float* temp, hipStream_t st) {
         hipLaunchKernelGGL(k_mixed_temps, dim3((unsigned)((s.nv + 255) / 256)), dim3(256), 0, st, s, temp);
 }
 
-void launch_step_mixed(const MixPtrs& s, bool fp64, hipStream_t st) {
+void launch_step_mixed(const MixPtrs& s, bool fp64, int block, hipStream_t st) {
     if (s.nv <= 0) return;
-    if (fp64) hipLaunchKernelGGL(k_step_mixed<double>, dim3((unsigned)((s.nv + 255) / 256)), dim3(256), 0, st, s);
-    else hipLaunchKernelGGL(k_step_mixed<float>, dim3((unsigned)((s.nv + 255) / 256)), dim3(256), 0, st, s);
+    const dim3 grid((unsigned)((s.nv + block - 1) / block));
+    if (fp64) hipLaunchKernelGGL(k_step_mixed<double>, grid, dim3(block), 0, st, s);
+    else hipLaunchKernelGGL(k_step_mixed<float>, grid, dim3(block), 0, st, s);
 }
 
 }  // namespace vx

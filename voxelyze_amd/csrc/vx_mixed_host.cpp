@@ -10,7 +10,7 @@
 
 namespace vx {
 
-void launch_step_mixed(const MixPtrs& s, bool fp64, hipStream_t st);
+void launch_step_mixed(const MixPtrs& s, bool fp64, int block, hipStream_t st);
 void launch_mixed_temps(const MixPtrs& s, float* temp, hipStream_t st);
 uint64_t graph_key(const void* args, size_t n, int path, int nsteps, int flags);
 void launch_col_stale(const DevPtrs& s, const ContactPtrs& c, hipStream_t st);
@@ -141,17 +141,29 @@ bool Sim::stepMixed(int nsteps, const MixTape* tape) {
     mp.actFreq = actFreq;
     mp.vE = modOn ? (const float*)bvE.p : nullptr;
     /* a launch without contacts, tape or timing is recorded as a graph and replayed */
-    const bool graph = useGraphs && !collisions && !tape && !profiling && nsteps > 1 && nsteps <= 2048;
+    const bool tune = tuning(0);
+    const bool timed = profiling || tune;
+    const bool graph = useGraphs && !collisions && !tape && !timed && nsteps > 1 && nsteps <= 2048;
+    const LaunchCfg chosen = tuner[0].chosen;
+    std::vector<int> cand(nsteps, -1);
+    if (timed) {
+        while (events.size() < (size_t)2 * nsteps) {
+            hipEvent_t ev;
+            HIPCHK(hipEventCreate(&ev));
+            events.push_back(ev);
+        }
+    }
     uint64_t key = 0;
     if (graph) {
         MixPtrs k0 = mp;
+        k0.xcd = chosen.xcd;
         k0.vdIn = dp.vd; k0.vdOut = (double*)bvdAlt.p;
         k0.vflIn = dp.vflags; k0.vflOut = (uint32_t*)bvflAlt.p;
         k0.vtempIn = dp.vtemp; k0.vtempOut = (float*)bvtempAlt.p;
         k0.rtimeIn = dp.rtime; k0.rtimeOut = (float*)brtimeAlt.p;
         k0.lsIn = bms[lcur].p; k0.lsOut = bms[lcur ^ 1].p;
         k0.lflIn = (const uint32_t*)bmfl[lcur].p; k0.lflOut = (uint32_t*)bmfl[lcur ^ 1].p;
-        key = graph_key(&k0, sizeof k0, 1 + precision, nsteps, actOn ? 1 : 0);
+        key = graph_key(&k0, sizeof k0, 1 + precision, nsteps, (actOn ? 1 : 0) | (chosen.block << 1));
         bool replayed = false;
         if (!graphLaunch(key, &replayed)) return false;
         if (replayed) { /* the host side of the ping-pong, as the recorded launch left it */
@@ -219,9 +231,11 @@ bool Sim::stepMixed(int nsteps, const MixTape* tape) {
             HIPCHK(hipMemcpyAsync(tape->lfl[k], bmfl[lcur].p, S * 4, hipMemcpyDeviceToDevice, stream));
             mp.tapeSums = tape->sums[k];
         }
-        if (profiling) (void)hipEventRecord(events[2 * k], stream);
-        launch_step_mixed(mp, precision == 2, stream);
-        if (profiling) (void)hipEventRecord(events[2 * k + 1], stream);
+        const LaunchCfg cfg = tune ? tuner[0].next(&cand[k]) : chosen;
+        mp.xcd = cfg.xcd;
+        if (timed) (void)hipEventRecord(events[2 * k], stream);
+        launch_step_mixed(mp, precision == 2, cfg.block, stream);
+        if (timed) (void)hipEventRecord(events[2 * k + 1], stream);
         if (tape) HIPCHK(hipMemcpyAsync(tape->vflOut[k], bvflAlt.p, (size_t)nv() * 4, hipMemcpyDeviceToDevice, stream));
         swapVoxelBuffers();
         lcur ^= 1;
@@ -234,10 +248,12 @@ bool Sim::stepMixed(int nsteps, const MixTape* tape) {
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(stream));
-    if (profiling) {
+    if (timed) {
         for (int k = 0; k < launched; k++) {
             float ms = 0;
-            if (hipEventElapsedTime(&ms, events[2 * k], events[2 * k + 1]) == hipSuccess) {
+            if (hipEventElapsedTime(&ms, events[2 * k], events[2 * k + 1]) != hipSuccess) continue;
+            if (cand[k] >= 0) tuner[0].record(cand[k], ms);
+            if (profiling) {
                 profMs[3] += ms;
                 profCount[3]++;
             }

@@ -18,8 +18,8 @@
 
 namespace vx {
 
-void launch_links_fp64(const DevPtrs& s, int k, hipStream_t st);
-void launch_voxels_fp64(const DevPtrs& s, int k, hipStream_t st);
+void launch_links_fp64(const DevPtrs& s, int k, LaunchCfg c, hipStream_t st);
+void launch_voxels_fp64(const DevPtrs& s, int k, LaunchCfg c, hipStream_t st);
 void launch_actuate_fp64(const DevPtrs& s, int k, hipStream_t st);
 void launch_robot_com(const DevPtrs& s, const int* rfirst, double* com, int nrobots, hipStream_t st);
 void launch_col_stale(const DevPtrs& s, const ContactPtrs& c, hipStream_t st);
@@ -1055,6 +1055,39 @@ bool Sim::anyDiverged(int k, bool* all) {
     return true;
 }
 
+/* ------------------------------------------------------------ autotuner */
+static const LaunchCfg kCands[6] = {{256, 0}, {128, 0}, {64, 0}, {256, 1}, {128, 1}, {64, 1}};
+static const int kTuneSamples = 2;
+
+void Sim::Tuner::reset(int n) {
+    nv = n;
+    steps = 0;
+    settled = false;
+    chosen = kCands[0];
+    for (int c = 0; c < 6; c++) { best[c] = 1e30f; count[c] = 0; }
+}
+LaunchCfg Sim::Tuner::next(int* cand) {
+    *cand = steps++ % 6;
+    return kCands[*cand];
+}
+void Sim::Tuner::record(int cand, float ms) {
+    if (ms < best[cand]) best[cand] = ms;
+    count[cand]++;
+    for (int c = 0; c < 6; c++)
+        if (count[c] < kTuneSamples) return;
+    int b = 0;
+    for (int c = 1; c < 6; c++)
+        if (best[c] < best[b]) b = c;
+    chosen = kCands[b];
+    settled = true;
+}
+/* is kernel t still being tuned for the current model size? */
+bool Sim::tuning(int t) {
+    if (!autotune) return false;
+    if (tuner[t].nv != nv()) tuner[t].reset(nv());
+    return !tuner[t].settled;
+}
+
 /* ------------------------------------------------------------ step graphs */
 void Sim::clearGraphs() {
     for (auto& g : graphs)
@@ -1112,10 +1145,16 @@ bool Sim::stepFp64(int nsteps) {
     dp.colCnt = nullptr;
     dp.colOff = nullptr;
     dp.colF = nullptr;
-    const bool graph = useGraphs && !profiling && nsteps > 1 && nsteps <= kGraphMaxSteps;
+    const bool tuneL = tuning(1), tuneV = tuning(2); /* both: each resets itself for a new size */
+    const bool tune = tuneL || tuneV;
+    const bool timed = profiling || tune;
+    const bool graph = useGraphs && !timed && nsteps > 1 && nsteps <= kGraphMaxSteps;
+    const LaunchCfg cl = tuner[1].chosen, cv = tuner[2].chosen;
+    std::vector<int> candL(nsteps, -1), candV(nsteps, -1);
     uint64_t key = 0;
     if (graph) {
-        key = graph_key(&dp, sizeof dp, 0, nsteps, actOn ? 1 : 0);
+        const LaunchCfg cfg[2] = {cl, cv};
+        key = graph_key(&dp, sizeof dp, 0, nsteps, (actOn ? 1 : 0) | (int)(fnv(0, cfg, sizeof cfg) << 1));
         bool replayed = false;
         if (!graphLaunch(key, &replayed)) return false;
         if (replayed) {
@@ -1124,14 +1163,23 @@ bool Sim::stepFp64(int nsteps) {
         }
         HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     }
+    if (timed) {
+        while (events.size() < (size_t)4 * nsteps) {
+            hipEvent_t ev;
+            HIPCHK(hipEventCreate(&ev));
+            events.push_back(ev);
+        }
+    }
     for (int k = 0; k < nsteps; k++) {
         if (actOn) launch_actuate_fp64(dp, k, stream);
-        if (profiling) (void)hipEventRecord(events[4 * k], stream);
-        launch_links_fp64(dp, k, stream);
-        if (profiling) (void)hipEventRecord(events[4 * k + 1], stream);
-        if (profiling) (void)hipEventRecord(events[4 * k + 2], stream);
-        launch_voxels_fp64(dp, k, stream);
-        if (profiling) (void)hipEventRecord(events[4 * k + 3], stream);
+        const LaunchCfg lc = tuner[1].settled || !autotune ? cl : tuner[1].next(&candL[k]);
+        const LaunchCfg vc = tuner[2].settled || !autotune ? cv : tuner[2].next(&candV[k]);
+        if (timed) (void)hipEventRecord(events[4 * k], stream);
+        launch_links_fp64(dp, k, lc, stream);
+        if (timed) (void)hipEventRecord(events[4 * k + 1], stream);
+        if (timed) (void)hipEventRecord(events[4 * k + 2], stream);
+        launch_voxels_fp64(dp, k, vc, stream);
+        if (timed) (void)hipEventRecord(events[4 * k + 3], stream);
     }
     if (graph) {
         hipGraph_t g = nullptr;
@@ -1140,17 +1188,16 @@ bool Sim::stepFp64(int nsteps) {
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(stream));
-    if (profiling) {
+    if (timed) {
         for (int k = 0; k < nsteps; k++) {
             float a = 0, b = 0;
-            if (nl() > 0 && hipEventElapsedTime(&a, events[4 * k], events[4 * k + 1]) == hipSuccess) {
-                profMs[0] += a;
-                profCount[0]++;
-            }
-            if (hipEventElapsedTime(&b, events[4 * k + 2], events[4 * k + 3]) == hipSuccess) {
-                profMs[1] += b;
-                profCount[1]++;
-            }
+            const bool okA = hipEventElapsedTime(&a, events[4 * k], events[4 * k + 1]) == hipSuccess;
+            const bool okB = hipEventElapsedTime(&b, events[4 * k + 2], events[4 * k + 3]) == hipSuccess;
+            if (candL[k] >= 0 && okA) tuner[1].record(candL[k], a);
+            if (candV[k] >= 0 && okB) tuner[2].record(candV[k], b);
+            if (!profiling) continue;
+            if (nl() > 0 && okA) { profMs[0] += a; profCount[0]++; }
+            if (okB) { profMs[1] += b; profCount[1]++; }
         }
     }
     return true;
@@ -1307,7 +1354,7 @@ bool Sim::stepWithContacts(int nsteps) {
     }
     for (int k = 0; k < nsteps; k++) {
         if (actOn) launch_actuate_fp64(dp, k, stream);
-        launch_links_fp64(dp, k, stream);
+        launch_links_fp64(dp, k, tuner[1].chosen, stream);
         HIPCHK(hipMemsetAsync(cp.flag, 0, 4, stream));
         launch_col_stale(dp, cp, stream);
         HIPCHK(hipMemcpyAsync(ctlHost, dp.ctl, 4 * sizeof(int), hipMemcpyDeviceToHost, stream));
@@ -1329,7 +1376,7 @@ bool Sim::stepWithContacts(int nsteps) {
         dp.colOff = cp.pairOff;
         dp.colF = cp.cf;
         launch_col_forces(dp, cp, stream);
-        launch_voxels_fp64(dp, k, stream);
+        launch_voxels_fp64(dp, k, tuner[2].chosen, stream);
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipStreamSynchronize(stream));

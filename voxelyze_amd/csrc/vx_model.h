@@ -179,6 +179,23 @@ struct Sim {
     void clearGraphs();
     bool graphLaunch(uint64_t key, bool* replayed);
     bool graphStore(uint64_t key, hipGraph_t g);
+    /* Autotuner: the workgroup size and block-to-XCD mapping of the step kernels are chosen by
+       timing every candidate on the first steps of each model size (the lanes' work does not
+       depend on the configuration, so the tuning steps are ordinary steps with the same bits) */
+    struct Tuner {
+        int nv = -1, steps = 0;
+        float best[6];
+        int count[6];
+        LaunchCfg chosen{256, 0};
+        bool settled = false;
+        Tuner() { reset(-1); }
+        void reset(int n);
+        LaunchCfg next(int* cand);
+        void record(int cand, float ms);
+    };
+    Tuner tuner[3]; /* 0 fused step, 1 FP64 link kernel, 2 FP64 voxel kernel */
+    bool autotune = true;
+    bool tuning(int t);
     /* kernel timing */
     bool profiling = false;
     std::vector<hipEvent_t> events;

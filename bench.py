@@ -180,7 +180,7 @@ def bench_robots(n_robots, steps, fp64_value):
     sim.setPrecision(MIXED)
     V, L = sim.voxelCount(), sim.linkCount()
     com0 = sim.robotCom()
-    sim.robotStep(5)
+    sim.robotStep(20)  # warm-up, during which the autotuner settles
     torch.cuda.synchronize()
     print(f"[bench] robots: {n_robots} robots, {V} voxels, {L} links, built in {time.perf_counter() - t0:.1f}s",
           file=sys.stderr, flush=True)
@@ -197,7 +197,7 @@ def bench_robots(n_robots, steps, fp64_value):
             "ms_per_step": el / steps * 1e3, "vs_fp64_plain_lattice": value / fp64_value,
             "all_robots_completed": bool((done == steps).all()),
             "com_x_travel_m": {"mean": float(travel.mean()), "min": float(travel.min()), "max": float(travel.max()),
-                               "after_steps": steps + 5}}
+                               "after_steps": steps + 20}}
 
 
 def bench_adjoint(n_robots, steps):
@@ -213,7 +213,7 @@ def bench_adjoint(n_robots, steps):
     W.differentiable_batch(sim, n_robots)
     sim.setPrecision(MIXED)
     V = sim.voxelCount()
-    sim.robotStep(2)
+    sim.robotStep(20)  # warm-up, during which the autotuner settles
     sim.resetTime()
     torch.cuda.synchronize()
     print(f"[bench] adjoint: {n_robots} robots, {V} voxels, built in {time.perf_counter() - t0:.1f}s", file=sys.stderr,

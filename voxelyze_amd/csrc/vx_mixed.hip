@@ -59,6 +59,34 @@ __device__ __forceinline__ dvec d_torv(dquat q) {
     return dv(q.x * f, q.y * f, q.z * f);
 }
 
+/* Quat3D::FromRotationVector (Quat3D.h:124-139) with the device libm's sin and cos: the fused
+   steps are not held to the reference's bits, and the correctly rounded double-double
+   versions the FP64 step uses cost hundreds of FP64 operations per call */
+__device__ __forceinline__ dquat d_fromrv(dvec v) {
+    const dvec th = dv(0.5 * v.x, 0.5 * v.y, 0.5 * v.z);
+    double s, m2 = th.x * th.x + th.y * th.y + th.z * th.z;
+    dquat q;
+    if (m2 * m2 < 5.328e-15) {
+        q.w = 1.0 - 0.5 * m2;
+        s = 1.0 - m2 / 6.0;
+    } else {
+        double m = sqrt(m2), sm, cm;
+        sincos(m, &sm, &cm);
+        q.w = cm;
+        s = sm / m;
+    }
+    q.x = th.x * s;
+    q.y = th.y * s;
+    q.z = th.z * s;
+    return q;
+}
+
+/* the actuation temperature with the device libm's sine (see actuation_temperature) */
+__device__ __forceinline__ float fused_actuation(float base, float amp, float freq, float t, float phase) {
+    const double arg = 6.283185307179586 * (double)freq * (double)t + (double)phase;
+    return (float)((double)base + (double)amp * sin(arg));
+}
+
 /* Quat3D::FromAngleToPosX (Quat3D.h:141-166) with libm trig */
 __device__ __forceinline__ void d_angletoposx(dquat* q, dvec rf) {
     if (rf.x == 0.0 && rf.y == 0.0 && rf.z == 0.0) return;
@@ -290,7 +318,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_FUS
         lf.x += f.x; lf.y += f.y; lf.z += f.z;
         lm.x += mm.x; lm.y += mm.y; lm.z += mm.z;
     }
-    if (s.vphase) s.vtempOut[i] = actuation_temperature(s.actBase, s.actAmp, s.actFreq, s.rtimeIn[rob] + dt, s.vphase[i]);
+    if (s.vphase) s.vtempOut[i] = fused_actuation(s.actBase, s.actAmp, s.actFreq, s.rtimeIn[rob] + dt, s.vphase[i]);
     if (s.tapeSums) {
         double* ts = s.tapeSums;
         ts[i] = lf.x; ts[nv + i] = lf.y; ts[2 * nv + i] = lf.z;
@@ -392,7 +420,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_FUS
     mo.z -= m.globalRotC * (m.momentInertiaInverse * ang.z);
     ang.x += dt * mo.x; ang.y += dt * mo.y; ang.z += dt * mo.z;
     const float dti = dt * m.momentInertiaInverse;
-    q = qmul(qfromrv(dv(dti * ang.x, dti * ang.y, dti * ang.z)), q);
+    q = qmul(d_fromrv(dv(dti * ang.x, dti * ang.y, dti * ang.z)), q);
     if (ex) {
         const double sz = m.nomSize;
         const int dof = ex->dof;
@@ -404,11 +432,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_FUS
                 q = dq(ex->qw, ex->qx, ex->qy, ex->qz);
                 ang = dv(0.0, 0.0, 0.0);
             } else {
-                dvec rv = qtorv(q);
+                dvec rv = d_torv(q);
                 if (dof & 8) { rv.x = 0.0; ang.x = 0.0; }
                 if (dof & 16) { rv.y = 0.0; ang.y = 0.0; }
                 if (dof & 32) { rv.z = 0.0; ang.z = 0.0; }
-                q = qfromrv(rv);
+                q = d_fromrv(rv);
             }
         }
     }
@@ -423,7 +451,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_FUS
 __global__ void __launch_bounds__(256) k_mixed_temps(MixPtrs s, float* temp) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= s.nv) return;
-    temp[i] = actuation_temperature(s.actBase, s.actAmp, s.actFreq, s.rtimeIn[s.vrobot[i]], s.vphase[i]);
+    temp[i] = fused_actuation(s.actBase, s.actAmp, s.actFreq, s.rtimeIn[s.vrobot[i]], s.vphase[i]);
 }
 
 void launch_mixed_temps(const MixPtrs& s, float* temp, hipStream_t st) {

@@ -1,7 +1,8 @@
 """Throughput of the explicit step (CVoxelyze::doTimeStep) on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|mixed]
-                    [--workload lattice|cantilever] [--no-cpu-baseline] [--no-compare]
+                    [--fp64-path fused|split] [--workload lattice|cantilever]
+                    [--no-cpu-baseline] [--no-compare]
 
 Default workload: the 128x128x64-site Bernoulli(0.9) lattice of two linear
 materials (E 1 MPa / 10 MPa) dropped under gravity onto a friction floor,
@@ -9,11 +10,13 @@ materials (E 1 MPa / 10 MPa) dropped under gravity onto a friction floor,
 is one doTimeStep over the whole lattice; state stays resident in HBM.
 
 Prints one JSON line: voxel-steps/s of the selected precision (FP64 plain by
-default), the roofline of its dominant kernel (algorithmic bytes per launch /
-HIP-event-timed launch duration, against the 8 TB/s HBM peak), the other
-precision measured the same way on an identical copy of the workload with the
-relative displacement error between the two after the same steps, and, on N=1,
-the CPU oracle timed on this host's cores on a bounded sample of the workload.
+default, run as one fused kernel per step), the roofline of its dominant kernel
+(algorithmic bytes per launch / HIP-event-timed launch duration, against the
+8 TB/s HBM peak), the other FP64 path (a link and a voxel kernel per step) with a
+bitwise comparison of the two, the other precision measured the same way on an
+identical copy of the workload with the relative displacement error between the
+two after the same steps, and, on N=1, the CPU oracle timed on this host's cores
+on a bounded sample of the workload.
 """
 import argparse
 import json
@@ -33,6 +36,8 @@ HBM_PEAK_GBS = 8000.0
 #                                  strain state 8 R + 16 W, end forces 96 W) + 56 B per voxel (pose gather)
 #   voxel kernel: 244 B per voxel (state 104 R + 104 W, neighbour table 24, metadata 12) + 96 B per link
 #                                  (end-force gather)
+#   fused FP64 step: the fixed FP64 byte model of the whole step, 300 B per voxel + 384 B per link (the
+#                 two kernels above together), which the fused kernel moves less of (no force hand-off)
 #   mixed step  : the fixed mixed-precision byte model of the path, 300 B per voxel (pose gather 56,
 #                 state 104 R + 104 W, neighbour table 24, metadata 12) + 212 B per link (FP32 local
 #                 state 36 R + 36 W, rest length 4, end forces 48 W + 48 R, meta 40), against which the
@@ -41,10 +46,13 @@ ALG_BYTES = {
     "link": lambda V, L: 288.0 * L + 56.0 * V,
     "voxel": lambda V, L: 244.0 * V + 96.0 * L,
     "step_mixed": lambda V, L: 300.0 * V + 212.0 * L,
+    "step_fp64": lambda V, L: 300.0 * V + 384.0 * L,
 }
 # profile classes of vxg_profile_read
-PROFILE_CLASSES = ["link", "voxel", "contact", "step_mixed"]
-KERNEL_NAMES = {"link": "k_links_fp64", "voxel": "k_voxels_fp64", "step_mixed": "k_step_mixed"}
+PROFILE_CLASSES = ["link", "voxel", "contact", "step_mixed", "step_fp64"]
+KERNEL_NAMES = {"link": "k_links_fp64", "voxel": "k_voxels_fp64", "step_mixed": "k_step_mixed",
+                "step_fp64": "k_step_fp64"}
+FP64_PATHS = {"fused": 0, "split": 1}
 
 
 def build_workload(sim, workload):
@@ -93,7 +101,7 @@ def cpu_baseline(workload, budget_s=15.0, max_steps=200):
                       f"({V} voxels), {el:.1f} s"}
 
 
-def measure(precision, workload, steps, warmup):
+def measure(precision, workload, steps, warmup, fp64_path="fused"):
     """Build the workload, step it `warmup` times, then time `steps` steps with HIP events
     around every kernel.  Returns the sim (kept alive for comparisons) and the figures."""
     import ctypes as C
@@ -104,6 +112,7 @@ def measure(precision, workload, steps, warmup):
     sim = CVoxelyze(0.005)
     cfg = build_workload(sim, workload)
     sim.setPrecision(MIXED if precision == "mixed" else FP64_PLAIN)
+    sim.setFp64Path(FP64_PATHS[fp64_path])
     V, L = sim.voxelCount(), sim.linkCount()
     dt = sim.recommendedTimeStep()
     nominal = sim.state()["pos"]  # before the first step every voxel sits at its lattice position
@@ -138,7 +147,7 @@ def measure(precision, workload, steps, warmup):
           file=sys.stderr, flush=True)
     cfg.update({"voxels": V, "links": L, "dt": dt, "voxel_pitch_m": 0.005})
     tuned = {}
-    for kk, kname in enumerate(["k_step_mixed", "k_links_fp64", "k_voxels_fp64"]):
+    for kk, kname in enumerate(["k_step_mixed", "k_links_fp64", "k_voxels_fp64", "k_step_fp64"]):
         b, x, st = C.c_int(), C.c_int(), C.c_int()
         sim.abi.launch_config(sim.handle, kk, C.byref(b), C.byref(x), C.byref(st))
         if st.value:
@@ -251,6 +260,8 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--fp64-path", default="fused", choices=["fused", "split"],
+                    help="FP64 step as one fused kernel per step (default) or a link and a voxel kernel")
     ap.add_argument("--workload", default="lattice", choices=["lattice", "cantilever"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="measure the selected precision only")
@@ -279,15 +290,31 @@ def main():
         del src, dst
         print("[bench] calibration copy: 3 x (1 GiB read + 1 GiB write)", file=sys.stderr, flush=True)
 
-    sim, r = measure(a.precision, a.workload, a.steps, a.warmup)
+    sim, r = measure(a.precision, a.workload, a.steps, a.warmup, a.fp64_path)
     cfg = r["cfg"]
     cfg["precision"] = "fp64_plain" if a.precision == "fp64" else "mixed"
+    if a.precision == "fp64":
+        cfg["fp64_path"] = a.fp64_path
     line = {
         "metric": METRIC, "value": r["value"], "unit": "voxel-steps/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": r["el"] / a.steps * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64" if a.precision == "fp64" else "f32/f64",
         "data": "synthetic (seeded SplitMix64 lattice)", "config": cfg, "roofline": roofline(r["kernels"], a.workload),
     }
+    if not a.no_compare and a.precision == "fp64":
+        # the other FP64 path on an identical copy: the same bits after the same steps
+        other_path = "split" if a.fp64_path == "fused" else "fused"
+        sim3, r3 = measure("fp64", a.workload, a.steps, a.warmup, other_path)
+        sa, sb = sim.state(), sim3.state()
+        same = all(sa[k].tobytes() == sb[k].tobytes() for k in sa)
+        la, lb = sim.link_state(), sim3.link_state()
+        same = same and all(la[k].tobytes() == lb[k].tobytes() for k in la)
+        line["fp64_" + other_path] = {"value": r3["value"], "ms_per_step": r3["el"] / a.steps * 1e3,
+                                      "roofline": roofline(r3["kernels"], a.workload),
+                                      "bit_identical_to_" + a.fp64_path: same}
+        print(f"[bench] fp64 {other_path}: {r3['el'] / a.steps * 1e3:.4f} ms/step, bit-identical state: {same}",
+              file=sys.stderr, flush=True)
+        del sim3
     if not a.no_compare:
         other = "mixed" if a.precision == "fp64" else "fp64"
         sim2, r2 = measure(other, a.workload, a.steps, a.warmup)

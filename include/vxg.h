@@ -132,13 +132,22 @@ int vxg_collision_count(vxg_sim* s);
 /* precision mode of the device step (no reference counterpart: the reference is FP64/FP32
    mixed exactly as VXG_FP64_PLAIN reproduces) */
 int vxg_set_precision(vxg_sim* s, int mode);
+/* How VXG_FP64_PLAIN runs (no reference counterpart; both give the same bits):
+   VXG_FP64_PATH_FUSED (default): one kernel per step in which every voxel evaluates its <= 6
+   links and integrates, nothing handed over through HBM;
+   VXG_FP64_PATH_SPLIT: a link kernel (CVX_Link::updateForces) and a voxel kernel
+   (CVX_Voxel::timeStep) per step, the end forces handed over through per-voxel slots.
+   vxg_fp64_path returns the current one. */
+enum vxg_fp64_path { VXG_FP64_PATH_FUSED = 0, VXG_FP64_PATH_SPLIT = 1 };
+int vxg_set_fp64_path(vxg_sim* s, int path);
+int vxg_fp64_path(vxg_sim* s);
 
 /* Kernel timing (no reference counterpart; measurement hook for benchmarks).
    When enabled, every kernel of later vxg_step calls is bracketed by HIP events on the
    library's stream; vxg_profile_read returns, per kernel class (0 = FP64 link kernel,
-   1 = FP64 voxel kernel, 2 = contact kernels, 3 = fused mixed-precision step), the summed
-   milliseconds and launch counts since the last read, then clears them.  Returns the
-   number of classes written. */
+   1 = FP64 voxel kernel, 2 = contact kernels, 3 = fused mixed-precision step, 4 = fused
+   FP64 step), the summed milliseconds and launch counts since the last read, then clears
+   them.  Returns the number of classes written. */
 int vxg_profile(vxg_sim* s, int enable);
 /* Step graphs (no reference counterpart): a multi-step vxg_step / vxg_robot_step launch
    without self-collision is recorded once as a HIP graph and replayed while its buffers and
@@ -146,8 +155,8 @@ int vxg_profile(vxg_sim* s, int enable);
 int vxg_set_graphs(vxg_sim* s, int enable);
 int vxg_graph_stats(vxg_sim* s, int* captures, int* replays);
 /* Autotuner (no reference counterpart, default on): the workgroup size (64/128/256) and
-   block-to-XCD mapping of kernel 0 = fused step, 1 = FP64 link kernel, 2 = FP64 voxel
-   kernel are chosen by timing each candidate twice on the first steps of every model size;
+   block-to-XCD mapping of kernel 0 = fused mixed step, 1 = FP64 link kernel, 2 = FP64 voxel
+   kernel, 3 = fused FP64 step are chosen by timing each candidate twice on the first steps of every model size;
    results do not depend on the choice.  vxg_launch_config reports it (settled = 1 once
    chosen). */
 int vxg_autotune(vxg_sim* s, int enable);

@@ -40,8 +40,16 @@ def product_abi():
     return _lib.abi()
 
 
-def maker(abi):
-    return lambda sz: CVoxelyze(sz, abi)
+def maker(abi, fp64_path=None):
+    """Simulation factory; fp64_path selects the HIP library's FP64 step (0 fused, 1 split)."""
+    if fp64_path is None:
+        return lambda sz: CVoxelyze(sz, abi)
+
+    def make(sz):
+        sim = CVoxelyze(sz, abi)
+        sim.setFp64Path(fp64_path)
+        return sim
+    return make
 
 
 def load_golden(name):

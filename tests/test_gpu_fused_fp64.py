@@ -1,0 +1,149 @@
+"""The fused FP64 step (one kernel per step, vx_fused.hip) against the split FP64 step (a link
+kernel and a voxel kernel) and the oracle: the same bits in every state the C ABI exposes,
+end forces included, across the cases where the fused step has its own machinery --
+divergence in the last step of a launch and before it (the saved link state and the
+recomputed end forces), robots that do not step, device actuation, self-collision, switching
+between the two paths, and replayed step graphs.
+
+Every scenario of tests/scenarios.py is also compared with the correctly rounded oracle on
+both paths (test_gpu_parity.py).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import backends as B
+import scenarios as S
+from voxelyze_amd import workloads as W
+
+FUSED, SPLIT = 0, 1
+
+
+def bits_equal(a, b):
+    a, b = np.atleast_1d(np.asarray(a)), np.atleast_1d(np.asarray(b))
+    return a.shape == b.shape and a.dtype == b.dtype and a.tobytes() == b.tobytes()
+
+
+def same_state(a, b, what):
+    sa, sb = S.snapshot(a), S.snapshot(b)
+    bad = [k for k in sa if not bits_equal(sa[k], sb[k])]
+    assert not bad, f"{what}: differs in {bad[:8]}"
+
+
+def two_bar(make):
+    """two voxels, one fixed, one pulled hard enough to diverge at a large dt"""
+    sim = make(0.001)
+    m = sim.addMaterial(1e6, 1e3)
+    sim.setVoxel(m, 0, 0, 0).external().setFixedAll()
+    sim.setVoxel(m, 1, 0, 0).external().setForce(10.0, 0, 0)
+    return sim
+
+
+@pytest.mark.gpu
+def test_divergence_in_single_step_launches():
+    """doTimeStep until it returns false: the diverging step is the launch's last one."""
+    sims = [two_bar(B.maker(B.product_abi(), p)) for p in (FUSED, SPLIT)]
+    ref = two_bar(B.maker(B.oracle_cr_abi()))
+    dt = ref.recommendedTimeStep() * 40
+    steps = []
+    for sim in sims + [ref]:
+        k = 0
+        while k < 100 and sim.doTimeStep(dt):
+            k += 1
+        steps.append(k)
+    print(f"[fused64] single-step divergence after {steps}", flush=True)
+    assert steps[0] == steps[1] == steps[2] < 100
+    same_state(sims[0], sims[1], "fused vs split")
+    same_state(sims[0], ref, "fused vs oracle")
+    assert sims[0].currentTime() == sims[1].currentTime() == ref.currentTime()
+
+
+@pytest.mark.gpu
+def test_divergence_inside_a_launch():
+    """One launch of 40 steps that diverges early: voxels as before the diverging step, links
+    as after it, end forces of the diverging step (recomputed after the launch)."""
+    sims = [two_bar(B.maker(B.product_abi(), p)) for p in (FUSED, SPLIT)]
+    ref = two_bar(B.maker(B.oracle_cr_abi()))
+    dt = ref.recommendedTimeStep() * 40
+    res = [s.step(dt, 40) for s in sims + [ref]]
+    print(f"[fused64] divergence inside a launch: {res}", flush=True)
+    assert res[0] == res[1] == res[2] and 0 <= res[0][1] < 38
+    same_state(sims[0], sims[1], "fused vs split")
+    same_state(sims[0], ref, "fused vs oracle")
+    assert sims[0].currentTime() == sims[1].currentTime() == ref.currentTime()
+
+
+def robots(path, actuated=True, collisions=False, seed=31):
+    sim = B.maker(B.product_abi(), path)(0.005)
+    shp = W.robot_shapes(seed, 4, vmin=40, vmax=160, box=8)
+    tot = sum(len(s[0]) for s in shp)
+    ph = (W.uniform(7, np.arange(tot, dtype=np.uint64)) * 2 * math.pi).astype(np.float32)
+    W.robot_population(sim, shp, 39.0 if actuated else 0.0, 2.0, phases=ph)
+    if collisions:
+        sim.enableCollisions(True)
+    return sim
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("collisions", [False, True], ids=["free", "contacts"])
+def test_actuated_robots_match_split(collisions):
+    a, b = robots(FUSED, collisions=collisions), robots(SPLIT, collisions=collisions)
+    for _ in range(3):
+        da, db = a.robotStep(120), b.robotStep(120)
+        assert np.array_equal(da, db)
+    same_state(a, b, "actuated robots")
+    for r in range(a.robotCount()):
+        assert a.robotTime(r) == b.robotTime(r)
+
+
+@pytest.mark.gpu
+def test_diverging_and_idle_robots_match_split():
+    """Robot 1 blows up mid-launch, robot 2 does not step (dt 0); the others run on."""
+    a, b = robots(FUSED), robots(SPLIT)
+    for sim in (a, b):
+        f1, _ = sim.robotVoxels(1)
+        sim.voxel(f1).external().setForce(1e4, 0.0, 0.0)
+    dt = np.array([a.robotRecommendedTimeStep(r) for r in range(a.robotCount())], np.float32)
+    dt[2] = 0.0
+    da, db = a.robotStep(150, dt), b.robotStep(150, dt)
+    print(f"[fused64] robots done {da} / {db}", flush=True)
+    assert np.array_equal(da, db) and da[1] < 150 and da[0] == 150
+    same_state(a, b, "diverging and idle robots")
+    for r in range(a.robotCount()):
+        assert a.robotTime(r) == b.robotTime(r)
+
+
+@pytest.mark.gpu
+def test_switching_paths_mid_run():
+    a = B.maker(B.product_abi(), SPLIT)(0.005)
+    b = B.maker(B.product_abi(), SPLIT)(0.005)
+    W.cantilever(a)
+    W.cantilever(b)
+    dt = a.recommendedTimeStep()
+    a.step(dt, 301)
+    b.step(dt, 100)
+    b.setFp64Path(FUSED)
+    b.step(dt, 101)
+    b.setFp64Path(SPLIT)
+    b.step(dt, 100)
+    same_state(a, b, "split/fused/split")
+    assert a.currentTime() == b.currentTime()
+
+
+@pytest.mark.gpu
+def test_fused_graph_replays_equal_plain_launches():
+    import ctypes as C
+    a = B.maker(B.product_abi(), FUSED)(0.005)
+    b = B.maker(B.product_abi(), FUSED)(0.005)
+    for sim in (a, b):
+        W.cantilever(sim)
+    b.abi.set_graphs(b.handle, 0)
+    dt = a.recommendedTimeStep()
+    for rep in range(4):
+        a.step(dt, 25)  # odd length: the ping-pong parity flips on every replay
+        b.step(dt, 25)
+        same_state(a, b, f"replay {rep}")
+    cap, rep = C.c_int(0), C.c_int(0)
+    a.abi.graph_stats(a.handle, C.byref(cap), C.byref(rep))
+    assert cap.value >= 1 and rep.value >= 1

@@ -141,6 +141,30 @@ def test_mixed_divergence_matches_fp64():
 
 
 @pytest.mark.gpu
+def test_mixed_divergence_in_single_step_launches():
+    """doTimeStep until it returns false: the diverging step is the last of its launch, and
+    the voxels and the clock stay as before it, as in the FP64 step."""
+    def build(sim):
+        m = sim.addMaterial(1e6, 1e3)
+        sim.setVoxel(m, 0, 0, 0).external().setFixedAll()
+        sim.setVoxel(m, 1, 0, 0).external().setForce(10.0, 0, 0)
+
+    a, _ = make(build, FP64, 0.001)
+    b, _ = make(build, MIXED, 0.001)
+    dt = a.recommendedTimeStep() * 40
+    steps = []
+    for sim in (a, b):
+        k = 0
+        while k < 100 and sim.doTimeStep(dt):
+            k += 1
+        steps.append(k)
+    print(f"[mixed] single-step divergence after {steps}")
+    assert steps[0] == steps[1] < 100
+    assert np.allclose(a.state()["pos"], b.state()["pos"], rtol=1e-4, atol=0)
+    assert a.currentTime() == b.currentTime()
+
+
+@pytest.mark.gpu
 def test_precision_switch_mid_run():
     """FP64 -> mixed -> FP64 keeps the state: the result stays within the mixed bar of an FP64-only run."""
     a, _ = make(W.cantilever, FP64)

@@ -70,12 +70,14 @@ def test_hip_fp64_matches_reference(name):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("path", [0, 1], ids=["fused", "split"])
 @pytest.mark.parametrize("name", NAMES)
-def test_hip_fp64_matches_cr_oracle_bitwise(name):
-    """Same arithmetic as the reference restated, trig correctly rounded on both sides."""
-    print(f"[parity-cr] {name}", flush=True)
+def test_hip_fp64_matches_cr_oracle_bitwise(name, path):
+    """Same arithmetic as the reference restated, trig correctly rounded on both sides; both
+    FP64 paths (one fused kernel per step, or a link and a voxel kernel)."""
+    print(f"[parity-cr] {name} path {path}", flush=True)
     ref = S.SCENARIOS[name](B.maker(B.oracle_cr_abi()))
-    out = S.SCENARIOS[name](B.maker(B.product_abi()))
+    out = S.SCENARIOS[name](B.maker(B.product_abi(), path))
     bad = [k for k in ref if not bits_equal(ref[k], out[k])]
     assert not bad, f"{name}: differs from the correctly rounded oracle in {bad[:8]}"
 
@@ -117,14 +119,14 @@ def test_divergence_stops_a_multi_step_launch():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("precision", [0, 1])
-def test_graph_replays_equal_plain_launches(precision):
+@pytest.mark.parametrize("precision,path", [(0, 0), (0, 1), (1, 0)], ids=["fp64-fused", "fp64-split", "mixed"])
+def test_graph_replays_equal_plain_launches(precision, path):
     """A multi-step launch recorded as a HIP graph and replayed gives the same bits as the
     plain kernel sequence, on every replay, with host reads in between."""
     import ctypes as C
     from voxelyze_amd import workloads as W
-    a = B.maker(B.product_abi())(0.005)
-    b = B.maker(B.product_abi())(0.005)
+    a = B.maker(B.product_abi(), path)(0.005)
+    b = B.maker(B.product_abi(), path)(0.005)
     for sim in (a, b):
         W.cantilever(sim)
         sim.setPrecision(precision)
@@ -142,14 +144,15 @@ def test_graph_replays_equal_plain_launches(precision):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("precision", [0, 1])
-def test_autotuned_launches_give_the_same_bits(precision):
+@pytest.mark.parametrize("precision,path,kernel", [(0, 0, 3), (0, 1, 1), (1, 0, 0)],
+                         ids=["fp64-fused", "fp64-split", "mixed"])
+def test_autotuned_launches_give_the_same_bits(precision, path, kernel):
     """Every launch configuration the autotuner tries gives the reference's bits: a run that
     tunes on its first steps equals a run with tuning off."""
     import ctypes as C
     from voxelyze_amd import workloads as W
-    a = B.maker(B.product_abi())(0.005)
-    b = B.maker(B.product_abi())(0.005)
+    a = B.maker(B.product_abi(), path)(0.005)
+    b = B.maker(B.product_abi(), path)(0.005)
     for sim in (a, b):
         W.drop_lattice(sim, nx=24, ny=24, nz=8)
         sim.setPrecision(precision)
@@ -160,6 +163,6 @@ def test_autotuned_launches_give_the_same_bits(precision):
     sa, sb = a.state(), b.state()
     assert all(sa[k].tobytes() == sb[k].tobytes() for k in sa)
     blk, x, st = C.c_int(), C.c_int(), C.c_int()
-    a.abi.launch_config(a.handle, 0 if precision else 1, C.byref(blk), C.byref(x), C.byref(st))
-    print(f"[tune] precision {precision}: block {blk.value} xcd {x.value} settled {st.value}")
+    a.abi.launch_config(a.handle, kernel, C.byref(blk), C.byref(x), C.byref(st))
+    print(f"[tune] precision {precision} path {path}: block {blk.value} xcd {x.value} settled {st.value}")
     assert st.value == 1

@@ -169,6 +169,27 @@ def test_a_diverging_robot_stops_alone(precision):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1])
+def test_a_robot_with_dt_zero_keeps_its_state(precision):
+    """doTimeStep(0) does nothing (Voxelyze.cpp:253): a robot given dt 0 in a batch launch
+    keeps its state and clock bit for bit while the others step."""
+    shp = shapes(n=4, seed=71)
+    sim = batch(shp, precision)
+    sim.robotStep(30)
+    f, c = sim.robotVoxels(2)
+    before = sim.state(f, c)
+    t2, t0 = sim.robotTime(2), sim.robotTime(0)
+    dt = np.array([sim.robotRecommendedTimeStep(r) for r in range(sim.robotCount())], np.float32)
+    dt[2] = 0.0
+    done = sim.robotStep(41, dt)
+    print(f"[robots] precision {precision}: dt 0 robot, steps done {done}")
+    after = sim.state(f, c)
+    assert all(before[k].tobytes() == after[k].tobytes() for k in before)
+    assert sim.robotTime(2) == t2
+    assert sim.robotTime(0) > t0 and (done == 41).all()
+
+
+@pytest.mark.gpu
 def test_robot_com_is_the_mass_weighted_mean():
     shp = shapes(n=5, seed=61)
     sim = batch(shp, 1)

@@ -97,6 +97,8 @@ EXTENSIONS = {
     "graph_stats": (_I, [_P, _IP, _IP]),
     "autotune": (_I, [_P, _I]),
     "launch_config": (_I, [_P, _I, _IP, _IP, _IP]),
+    "set_fp64_path": (_I, [_P, _I]),
+    "fp64_path": (_I, [_P]),
 }
 
 

@@ -438,6 +438,14 @@ class CVoxelyze:
     def setPrecision(self, mode):
         return self._check(bool(self.abi.set_precision(self.handle, mode)))
 
+    def setFp64Path(self, path):
+        """How the FP64 step runs: 0 fused (one kernel per step, default), 1 split (a link and a
+        voxel kernel per step).  Both give the same bits (include/vxg.h, vxg_set_fp64_path)."""
+        return self._check(bool(self.abi.set_fp64_path(self.handle, int(path))))
+
+    def fp64Path(self):
+        return int(self.abi.fp64_path(self.handle))
+
     # robots, actuation, bulk placement (extensions of the HIP library, include/vxg.h)
     def setVoxels(self, materials, ijk):
         """setVoxel for many voxels: materials (n,) of CVX_Material or int, ijk (n, 3)."""

@@ -204,6 +204,12 @@ API int vxg_profile(vxg_sim* h, int en) {
     S(h)->profiling = en != 0;
     return 1;
 }
+API int vxg_set_fp64_path(vxg_sim* h, int path) {
+    if (path != 0 && path != 1) return 0;
+    S(h)->fp64Split = path == 1;
+    return 1;
+}
+API int vxg_fp64_path(vxg_sim* h) { return S(h)->fp64Split ? 1 : 0; }
 API int vxg_set_graphs(vxg_sim* h, int en) {
     Sim* s = S(h);
     s->useGraphs = en != 0;
@@ -222,7 +228,7 @@ API int vxg_autotune(vxg_sim* h, int en) {
 }
 API int vxg_launch_config(vxg_sim* h, int kernel, int* block, int* xcd, int* settled) {
     Sim* s = S(h);
-    if (kernel < 0 || kernel > 2) return 0;
+    if (kernel < 0 || kernel > 3) return 0;
     const Sim::Tuner& t = s->tuner[kernel];
     if (block) *block = t.chosen.block;
     if (xcd) *xcd = t.chosen.xcd;
@@ -231,12 +237,12 @@ API int vxg_launch_config(vxg_sim* h, int kernel, int* block, int* xcd, int* set
 }
 API int vxg_profile_read(vxg_sim* h, double* ms, int* launches, int cap) {
     Sim* s = S(h);
-    int n = cap < 4 ? cap : 4;
+    int n = cap < 5 ? cap : 5;
     for (int i = 0; i < n; i++) {
         if (ms) ms[i] = s->profMs[i];
         if (launches) launches[i] = s->profCount[i];
     }
-    for (int i = 0; i < 4; i++) { s->profMs[i] = 0; s->profCount[i] = 0; }
+    for (int i = 0; i < 5; i++) { s->profMs[i] = 0; s->profCount[i] = 0; }
     return n;
 }
 

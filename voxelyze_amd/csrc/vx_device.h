@@ -213,6 +213,56 @@ struct MixPtrs {
     int xcd;                /* launch configuration: XCD-contiguous block mapping */
 };
 
+/* arguments of one fused FP64 step (vx_fused.hip): the split step's layouts, with the voxel
+   state, voxel flags, link state and robot clocks double-buffered ("in" before the step,
+   "out" after it) */
+struct FusedPtrs {
+    int nv, nl;               /* voxels; link slots (3 * nv) */
+    const double* vdIn;
+    double* vdOut;
+    const uint32_t* vflIn;
+    uint32_t* vflOut;
+    const double* ldIn;       /* LDEV_COUNT * nl */
+    double* ldOut;
+    const float* lfIn;        /* LF_COUNT * nl */
+    float* lfOut;
+    const uint32_t* lflIn;
+    uint32_t* lflOut;
+    /* link state before the step in which a robot diverged, saved by the robot's first
+       frozen step (its end forces are recomputed from it after the launch) */
+    double* ldSide;
+    float* lfSide;
+    uint32_t* lflSide;
+    const int* vnbr;          /* 6 * nv: voxel across each direction, -1 none */
+    const int* lmeta;
+    const int* vmat;
+    const int* vext;
+    const int* vijk;
+    float* vtemp;             /* in place: under actuation each lane writes its own */
+    const VoxMat* vmats;
+    const LinkMat* lmats;
+    const float* curveStrain;
+    const float* curveStress;
+    const ExtRec* exts;
+    const int* colCnt;        /* contacts (null when off) */
+    const int* colOff;
+    const int* partners;
+    float* colF;              /* contact forces, written with the end forces */
+    double* vf;               /* force slots, written when storeForces is set */
+    const int* vrobot;
+    const float* rdt;
+    const float* rprev;       /* each robot's previousDt before the launch */
+    const float* rtimeIn;
+    float* rtimeOut;
+    int* rdiv;
+    int* ctl;
+    const float* vphase;
+    float actBase, actAmp, actFreq;
+    int k;                    /* this step's index in the launch */
+    int storeForces;          /* write the end forces (the launch's last step) */
+    int xcd;
+};
+
 /* a kernel launch configuration chosen by the autotuner */
 struct LaunchCfg {
     int block;  /* workgroup size */

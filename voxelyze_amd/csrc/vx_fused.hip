@@ -1,0 +1,269 @@
+/*
+ * vx_fused.hip -- the FP64 step as one kernel per step, bit for bit the split step.
+ *
+ * CVoxelyze::doTimeStep (src/Voxelyze.cpp:251-284) updates every link, then every voxel.
+ * The split step (vx_kernels.hip) does that with two kernels and hands each link's four
+ * end vectors to its voxels through HBM (96 B per link written, then read).  Here one lane
+ * per voxel evaluates the <= 6 links it touches with the split step's own link function
+ * (every link is evaluated by both of its voxels, on identical inputs, so both get the
+ * split step's bits), sums its end vectors in direction order 0..5 and integrates with the
+ * split step's voxel function.  Nothing is handed over through HBM.
+ *
+ * Voxel state, voxel flags, link state and robot clocks are double-buffered: a step reads
+ * "in" and writes "out", so no lane sees a neighbour's new state.  The link's negative
+ * voxel owns it and writes its new state.  previousDt (VX_Voxel.cpp:163), read only by
+ * links with a valid local velocity, is the robot's dt of the previous step.
+ *
+ * Divergence (a strain > 100, Voxelyze.cpp:265-269): every link of that step is updated
+ * but the voxels and the clock must not move.  The lanes record the step in the robot's
+ * rdiv entry and still write their (discarded) voxel output; from the next step on the
+ * robot's lanes copy whichever ping-pong buffer is stale, so the robot ends the launch with
+ * its voxels as before the diverging step and its links as after it.  The first of those
+ * copies also saves the link state from before the diverging step, from which the host
+ * recomputes that step's end forces after the launch (k_step_fp64<true>).
+ *
+ * End forces (what CVX_Link::force() returns) are written to the split step's force slots
+ * on the launch's last step only.
+ *
+ * Actuation: each lane sets its own temperature for the step (as k_actuate_fp64 does) and
+ * evaluates its neighbours' temperatures itself for the links' rest lengths and contacts.
+ */
+#include "vx_contact_force.h"
+#include "vx_device.h"
+#include "vx_math.h"
+#include "vx_step_fp64.h"
+
+#pragma clang fp contract(off)
+
+namespace vx {
+
+/* a robot that diverged at step d (or, d = -1, does not step in this launch) */
+__device__ void frozen_fp64(const FusedPtrs& s, int i, int rob, int d) {
+    const int nv = s.nv, nl = s.nl;
+    const bool idle = d < 0;
+    if (idle || ((s.k - d) & 1) == 0) {
+        for (int c = 0; c < VD_COUNT; c++) s.vdOut[c * nv + i] = s.vdIn[c * nv + i];
+        s.vflOut[i] = s.vflIn[i];
+        if (i == 0 || s.vrobot[i - 1] != rob) s.rtimeOut[rob] = s.rtimeIn[rob];
+    }
+    if (idle || ((s.k - d) & 1) == 1) {
+        const bool save = !idle && s.k == d + 1; /* "out" still holds the links before step d */
+        for (int axis = 0; axis < 3; axis++) {
+            const int slot = axis * nv + i;
+            for (int c = 0; c < LDEV_COUNT; c++) {
+                if (save) s.ldSide[c * nl + slot] = s.ldOut[c * nl + slot];
+                s.ldOut[c * nl + slot] = s.ldIn[c * nl + slot];
+            }
+            for (int c = 0; c < LF_COUNT; c++) {
+                if (save) s.lfSide[c * nl + slot] = s.lfOut[c * nl + slot];
+                s.lfOut[c * nl + slot] = s.lfIn[c * nl + slot];
+            }
+            if (save) s.lflSide[slot] = s.lflOut[slot];
+            s.lflOut[slot] = s.lflIn[slot];
+        }
+    }
+}
+
+__device__ __forceinline__ double axis_size(const VoxMat& m, int axis) {
+    return axis == 0 ? m.sizeX : axis == 1 ? m.sizeY : m.sizeZ;
+}
+
+/* one link of voxel i (direction dir, neighbour o) as CVX_Link::updateForces
+   (VX_Link.cpp:149-217): its end vector on voxel i added to the direction-ordered sums,
+   written to the force slot on the last step; the negative end (Neg) owns the link state */
+template <bool ForcesOnly, bool Neg>
+__device__ __forceinline__ void link_end(const FusedPtrs& s, int i, int o, int dir, int rob, float prevDt, float tnow,
+                                         float tempI, const dvec& pos0, const dquat& q0, dvec& sf, dvec& sm) {
+    const int nv = s.nv, nl = s.nl;
+    const double* __restrict__ vd = s.vdIn;
+    const int axis = dir >> 1;
+    const int n = Neg ? i : o, p = Neg ? o : i;
+    const int slot = axis * nv + n;
+    const LinkMat m = s.lmats[s.lmeta[slot]];
+    const dvec po = dv(vd[VPX * nv + o], vd[VPY * nv + o], vd[VPZ * nv + o]);
+    const dquat qo = dq(vd[VQW * nv + o], vd[VQX * nv + o], vd[VQY * nv + o], vd[VQZ * nv + o]);
+    double rest;
+    if (s.vphase) { /* updateRestLength (VX_Link.cpp:137-140) at this step's temperatures */
+        const float tempO = actuation_temperature(s.actBase, s.actAmp, s.actFreq, tnow, s.vphase[o]);
+        const VoxMat& mn = s.vmats[s.vmat[n]];
+        const VoxMat& mp = s.vmats[s.vmat[p]];
+        rest = 0.5 * (base_size(axis_size(mn, axis), Neg ? tempI : tempO, mn.alphaCTE) +
+                      base_size(axis_size(mp, axis), Neg ? tempO : tempI, mp.alphaCTE));
+    } else {
+        rest = s.ldIn[LREST * nl + slot];
+    }
+    LinkPrev old;
+    old.pos2 = dv(s.ldIn[LP2X * nl + slot], s.ldIn[LP2Y * nl + slot], s.ldIn[LP2Z * nl + slot]);
+    old.a1 = dv(s.ldIn[LA1X * nl + slot], s.ldIn[LA1Y * nl + slot], s.ldIn[LA1Z * nl + slot]);
+    old.a2 = dv(s.ldIn[LA2X * nl + slot], s.ldIn[LA2Y * nl + slot], s.ldIn[LA2Z * nl + slot]);
+    old.flags = s.lflIn[slot];
+    old.maxStrain = s.lfIn[LMAXSTRAIN * nl + slot];
+    old.strainOffset = s.lfIn[LOFFSET * nl + slot];
+    float dmN = 0.0f, dmP = 0.0f;
+    if (old.flags & LF_LVV) { /* dampingMultiplier (VX_Voxel.h:130) of both voxels */
+        dmN = s.vmats[s.vmat[n]].dampNum / prevDt;
+        dmP = s.vmats[s.vmat[p]].dampNum / prevDt;
+    }
+    LinkEval e;
+    link_eval_fp64(m, s.curveStrain + m.curveOff, s.curveStress + m.curveOff, axis, Neg ? pos0 : po,
+                   Neg ? q0 : qo, Neg ? po : pos0, Neg ? qo : q0, rest, old, dmN, dmP, &e);
+    const dvec f = Neg ? e.fN : e.fP, mm = Neg ? e.mN : e.mP;
+    sf.x += f.x; sf.y += f.y; sf.z += f.z;
+    sm.x += mm.x; sm.y += mm.y; sm.z += mm.z;
+    if (s.storeForces) {
+        double* slotF = s.vf + vf_index(dir, 0, nv, i);
+        slotF[0] = f.x; slotF[1] = f.y; slotF[2] = f.z;
+        slotF[3] = mm.x; slotF[4] = mm.y; slotF[5] = mm.z;
+    }
+    if (!ForcesOnly && Neg) {
+        double* ld = s.ldOut;
+        ld[LP2X * nl + slot] = e.pos2.x; ld[LP2Y * nl + slot] = e.pos2.y; ld[LP2Z * nl + slot] = e.pos2.z;
+        ld[LA1X * nl + slot] = e.a1v.x; ld[LA1Y * nl + slot] = e.a1v.y; ld[LA1Z * nl + slot] = e.a1v.z;
+        ld[LA2X * nl + slot] = e.a2v.x; ld[LA2Y * nl + slot] = e.a2v.y; ld[LA2Z * nl + slot] = e.a2v.z;
+        ld[LREST * nl + slot] = rest;
+        float* lf = s.lfOut;
+        lf[LSTRAIN * nl + slot] = e.strain;
+        lf[LMAXSTRAIN * nl + slot] = e.maxStrain;
+        lf[LOFFSET * nl + slot] = e.strainOffset;
+        lf[LSTRESS * nl + slot] = e.stress;
+        s.lflOut[slot] = e.flags;
+        if (e.strain > 100) {
+            atomicMin(&s.rdiv[rob], s.k);
+            atomicOr(&s.ctl[0], 1);
+        }
+    }
+}
+
+#ifndef VX_FUSED64_WAVES
+#define VX_FUSED64_WAVES 3
+#endif
+/* ForcesOnly: after a launch, the end forces of the step in which a robot diverged, for the
+   robots that diverged before the launch's last step (k is that step); "in" is the voxel
+   state (as before the diverging step) and the saved link state */
+template <bool ForcesOnly>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_FUSED64_WAVES, 8))) k_step_fp64(FusedPtrs s) {
+    const int i = lane_index(s.xcd);
+    const int nv = s.nv, nl = s.nl;
+    if (i >= nv) return;
+    const int rob = s.vrobot[i];
+    const int d = s.rdiv[rob];
+    const float dt = s.rdt[rob];
+    float prevDt;
+    if (ForcesOnly) {
+        if (d < 0 || d >= s.k) return;
+        prevDt = d == 0 ? s.rprev[rob] : dt;
+    } else {
+        if (d < s.k) {
+            frozen_fp64(s, i, rob, d);
+            return;
+        }
+        prevDt = s.k == 0 ? s.rprev[rob] : dt;
+        if (i == 0 || s.vrobot[i - 1] != rob) s.rtimeOut[rob] = s.rtimeIn[rob] + dt; /* currentTime += dt */
+    }
+    const double* __restrict__ vd = s.vdIn;
+    const dvec pos0 = dv(vd[VPX * nv + i], vd[VPY * nv + i], vd[VPZ * nv + i]);
+    const dquat q0 = dq(vd[VQW * nv + i], vd[VQX * nv + i], vd[VQY * nv + i], vd[VQZ * nv + i]);
+    const int mi = s.vmat[i];
+    const float tnow = s.rtimeIn[rob];
+    float tempI;
+    if (s.vphase) {
+        tempI = actuation_temperature(s.actBase, s.actAmp, s.actFreq, tnow, s.vphase[i]);
+        if (!ForcesOnly) s.vtemp[i] = tempI;
+    } else {
+        tempI = s.vtemp[i];
+    }
+
+    /* force() and moment() (VX_Voxel.cpp:234-271): the end vectors of the links in direction
+       order 0..5, each link evaluated as CVX_Link::updateForces (VX_Link.cpp:149-217) */
+    dvec sf = dv(0.0, 0.0, 0.0), sm = dv(0.0, 0.0, 0.0);
+#pragma unroll 1
+    for (int dir = 0; dir < 6; dir++) {
+        const int o = s.vnbr[dir * nv + i];
+        if (o < 0) {
+            if (s.storeForces)
+                for (int c = 0; c < 6; c++) s.vf[vf_index(dir, c, nv, i)] = 0.0;
+            continue;
+        }
+        /* the two ends as separate instances, each computing only its own end vectors */
+        if ((dir & 1) == 0) link_end<ForcesOnly, true>(s, i, o, dir, rob, prevDt, tnow, tempI, pos0, q0, sf, sm);
+        else link_end<ForcesOnly, false>(s, i, o, dir, rob, prevDt, tnow, tempI, pos0, q0, sf, sm);
+    }
+    if (ForcesOnly) return;
+
+    /* CVX_Voxel::timeStep (VX_Voxel.cpp:162-232) */
+    const VoxMat m = s.vmats[mi];
+    const int e = s.vext[i];
+    const ExtRec* ex = e >= 0 ? &s.exts[e] : nullptr;
+    uint32_t flags = s.vflIn[i];
+    if (ex && (ex->dof & 0x3F) == 0x3F) {
+        voxel_pinned(m, ex, s.vijk, nv, i, s.vdOut);
+        s.vflOut[i] = flags;
+        return;
+    }
+    dvec pos = pos0, lin = dv(vd[VLX * nv + i], vd[VLY * nv + i], vd[VLZ * nv + i]);
+    dvec ang = dv(vd[VAX * nv + i], vd[VAY * nv + i], vd[VAZ * nv + i]);
+    dquat q = q0;
+    voxel_step_fp64(m, ex, s.vijk, nv, i, dt, tempI, sf, sm, pos, lin, ang, q, flags, [&](dvec& t) {
+        /* contacts in partner order (VX_Voxel.cpp:250-253), each pair's force from this
+           voxel's side as the split step's contact kernel forms it */
+        if (!s.colCnt) return;
+        const int cnt = s.colCnt[i], off = s.colOff[i];
+        if (cnt == 0) return;
+        const ContactSelf me = contact_self(m, vd, nv, i, tempI);
+        for (int c = 0; c < cnt; c++) {
+            const int j = s.partners[off + c];
+            const float tj = s.vphase ? actuation_temperature(s.actBase, s.actAmp, s.actFreq, tnow, s.vphase[j])
+                                      : s.vtemp[j];
+            const fvec f = contact_pair_force(me, s.vmats[s.vmat[j]], vd, nv, j, tj);
+            t.x -= f.x; t.y -= f.y; t.z -= f.z;
+        }
+    });
+    store_voxel(s.vdOut, nv, i, pos, lin, ang, q);
+    s.vflOut[i] = flags;
+}
+
+/* after a launch: previousDt of every voxel whose robot completed at least one step */
+__global__ void __launch_bounds__(256) k_fused_prevdt(const int* vrobot, const int* rdiv, const float* rdt,
+                                                     float* vprevdt, int nv) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nv) return;
+    const int r = vrobot[i];
+    if (rdiv[r] > 0) vprevdt[i] = rdt[r];
+}
+
+/* a robot that diverged in the launch's last step takes back its voxels (and clock, and for
+   the mixed step its temperatures) from before that step, which the other ping-pong buffer
+   holds */
+__global__ void __launch_bounds__(256) k_fused_restore(double* vd, const double* vdPrev, uint32_t* vfl,
+                                                      const uint32_t* vflPrev, float* vtemp, const float* vtempPrev,
+                                                      float* rtime, const float* rtimePrev, int nv, int first,
+                                                      int count, int rob) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const int i = first + t;
+    for (int c = 0; c < VD_COUNT; c++) vd[c * nv + i] = vdPrev[c * nv + i];
+    vfl[i] = vflPrev[i];
+    if (vtemp) vtemp[i] = vtempPrev[i];
+    if (t == 0) rtime[rob] = rtimePrev[rob];
+}
+
+void launch_step_fp64(const FusedPtrs& s, int block, bool forcesOnly, hipStream_t st) {
+    if (s.nv <= 0) return;
+    const dim3 grid((unsigned)((s.nv + block - 1) / block));
+    if (forcesOnly) hipLaunchKernelGGL(k_step_fp64<true>, grid, dim3(block), 0, st, s);
+    else hipLaunchKernelGGL(k_step_fp64<false>, grid, dim3(block), 0, st, s);
+}
+void launch_fused_prevdt(const int* vrobot, const int* rdiv, const float* rdt, float* vprevdt, int nv, hipStream_t st) {
+    if (nv > 0)
+        hipLaunchKernelGGL(k_fused_prevdt, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, st, vrobot, rdiv, rdt,
+                           vprevdt, nv);
+}
+void launch_fused_restore(double* vd, const double* vdPrev, uint32_t* vfl, const uint32_t* vflPrev, float* vtemp,
+                          const float* vtempPrev, float* rtime, const float* rtimePrev, int nv, int first, int count,
+                          int rob, hipStream_t st) {
+    if (count > 0)
+        hipLaunchKernelGGL(k_fused_restore, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, vd, vdPrev, vfl,
+                           vflPrev, vtemp, vtempPrev, rtime, rtimePrev, nv, first, count, rob);
+}
+
+}  // namespace vx

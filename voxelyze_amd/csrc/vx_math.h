@@ -24,6 +24,9 @@
 #pragma clang fp contract(off)
 
 #define VX_HD __host__ __device__ __forceinline__
+/* the double-double trig bodies are kept out of line: they run on rare branches, and inlined
+   into the step kernels they raise register pressure on the common path */
+#define VX_HD_COLD __host__ __device__ inline __attribute__((noinline))
 
 namespace vx {
 
@@ -204,21 +207,21 @@ VX_HD void dd_sincos(double x, dd* sn, dd* cs) {
 }
 
 /* correctly rounded sin, cos (|x| < 2^20; beyond that the plain libm value) */
-VX_HD double cr_sin(double x) {
+VX_HD_COLD double cr_sin(double x) {
     if (!(fabs(x) < 1048576.0)) return sin(x);
     if (x == 0.0) return x;
     dd s, c;
     dd_sincos(x, &s, &c);
     return s.hi;
 }
-VX_HD double cr_cos(double x) {
+VX_HD_COLD double cr_cos(double x) {
     if (!(fabs(x) < 1048576.0)) return cos(x);
     dd s, c;
     dd_sincos(x, &s, &c);
     return c.hi;
 }
 /* correctly rounded sin and cos of one argument from a single double-double evaluation */
-VX_HD void cr_sincos(double x, double* sn, double* cs) {
+VX_HD_COLD void cr_sincos(double x, double* sn, double* cs) {
     if (!(fabs(x) < 1048576.0)) { *sn = sin(x); *cs = cos(x); return; }
     dd s, c;
     dd_sincos(x, &s, &c);
@@ -226,7 +229,7 @@ VX_HD void cr_sincos(double x, double* sn, double* cs) {
     *cs = c.hi;
 }
 /* correctly rounded acos: one double-double Newton step on cos(y) = w from libm's value */
-VX_HD double cr_acos(double w) {
+VX_HD_COLD double cr_acos(double w) {
     if (!(w > -1.0 && w < 1.0)) return acos(w);
     double y0 = acos(w);
     dd s, c;

@@ -247,16 +247,19 @@ __device__ __forceinline__ double m_bsa(const VoxMat& m, float temp) {
 
 /* a robot that diverged at step d of this launch keeps its state: its voxel state (and
    clock) at step d is the discarded output, its link state the kept one, so from step d+1
-   on the lanes copy whichever of the two ping-pong buffers is stale */
+   on the lanes copy whichever of the two ping-pong buffers is stale (one that diverged in
+   the launch's last step is put back by the host) */
 template <typename T>
 __device__ void frozen_copy(const MixPtrs& s, int i, int rob, int d) {
     const int nv = s.nv, S = 3 * nv;
-    if (((s.k - d) & 1) == 0) {
+    const bool idle = d < 0; /* a robot that does not step in this launch keeps both halves */
+    if (idle || ((s.k - d) & 1) == 0) {
         for (int c = 0; c < VD_COUNT; c++) s.vdOut[c * nv + i] = s.vdIn[c * nv + i];
         s.vflOut[i] = s.vflIn[i];
         if (s.vphase) s.vtempOut[i] = s.vtempIn[i];
         if (i == 0 || s.vrobot[i - 1] != rob) s.rtimeOut[rob] = s.rtimeIn[rob];
-    } else {
+    }
+    if (idle || ((s.k - d) & 1) == 1) {
         for (int axis = 0; axis < 3; axis++) {
             const int slot = axis * nv + i;
             for (int c = 0; c < MS_COUNT; c++) ((T*)s.lsOut)[c * S + slot] = ((const T*)s.lsIn)[c * S + slot];

@@ -14,6 +14,9 @@ void launch_step_mixed(const MixPtrs& s, bool fp64, int block, hipStream_t st);
 void launch_mixed_temps(const MixPtrs& s, float* temp, hipStream_t st);
 uint64_t graph_key(const void* args, size_t n, int path, int nsteps, int flags);
 void launch_col_stale(const DevPtrs& s, const ContactPtrs& c, hipStream_t st);
+void launch_fused_restore(double* vd, const double* vdPrev, uint32_t* vfl, const uint32_t* vflPrev, float* vtemp,
+                          const float* vtempPrev, float* rtime, const float* rtimePrev, int nv, int first, int count,
+                          int rob, hipStream_t st);
 
 #define HIPCHK(call)                                       \
     do {                                                   \
@@ -171,8 +174,7 @@ bool Sim::stepMixed(int nsteps, const MixTape* tape) {
                 swapVoxelBuffers();
                 lcur ^= 1;
             }
-            HIPCHK(hipStreamSynchronize(stream));
-            return true;
+            return mixedFinish(nsteps);
         }
         HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     }
@@ -259,6 +261,24 @@ bool Sim::stepMixed(int nsteps, const MixTape* tape) {
             }
         }
     }
+    return mixedFinish(launched);
+}
+
+/* after a launch of `launched` steps: a robot that diverged in the last step takes back its
+   voxels, temperatures and clock from before that step (the other ping-pong half) */
+bool Sim::mixedFinish(int launched) {
+    const int n = nv(), R = nRobots();
+    std::vector<int> div(R);
+    HIPCHK(hipMemcpyAsync(div.data(), dp.rdiv, (size_t)4 * R, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    for (int r = 0; r < R; r++) {
+        if (div[r] != launched - 1 || rfirst[r] == rfirst[r + 1]) continue;
+        launch_fused_restore(dp.vd, (const double*)bvdAlt.p, dp.vflags, (const uint32_t*)bvflAlt.p,
+                             actOn ? dp.vtemp : nullptr, (const float*)bvtempAlt.p, dp.rtime, (const float*)brtimeAlt.p, n,
+                             rfirst[r], rfirst[r + 1] - rfirst[r], r, stream);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(stream));
     return true;
 }
 

@@ -993,7 +993,10 @@ int Sim::stepRobots(const float* dt, int nsteps, int* done, int* divergedAt) {
     }
     if (!any) return nsteps;
     if (!syncUp() || !beginLaunch(nsteps)) return -1;
-    const bool ok = fused() ? stepMixed(nsteps) : collisions ? stepWithContacts(nsteps) : stepFp64(nsteps);
+    const bool ok = fused()       ? stepMixed(nsteps)
+                    : !fp64Split  ? stepFp64Fused(nsteps)
+                    : collisions  ? stepWithContacts(nsteps)
+                                  : stepFp64(nsteps);
     if (!ok || !endLaunch(nsteps, done)) return -1;
     int mn = nsteps;
     for (int r = 0; r < R; r++) mn = std::min(mn, done[r]);

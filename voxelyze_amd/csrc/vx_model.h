@@ -193,14 +193,14 @@ struct Sim {
         LaunchCfg next(int* cand);
         void record(int cand, float ms);
     };
-    Tuner tuner[3]; /* 0 fused step, 1 FP64 link kernel, 2 FP64 voxel kernel */
+    Tuner tuner[4]; /* 0 fused mixed step, 1 FP64 link kernel, 2 FP64 voxel kernel, 3 fused FP64 step */
     bool autotune = true;
     bool tuning(int t);
     /* kernel timing */
     bool profiling = false;
     std::vector<hipEvent_t> events;
-    double profMs[4] = {0, 0, 0, 0};
-    int profCount[4] = {0, 0, 0, 0};
+    double profMs[5] = {0, 0, 0, 0, 0};
+    int profCount[5] = {0, 0, 0, 0, 0};
     /* mixed precision (vx_mixed.hip): second voxel buffer, FP32 link-state pairs */
     DevBuf bvdAlt, bvflAlt, bvtempAlt, bvnbr, bms[2], bmfl[2];
     int lcur = 0;          /* mixed link-state buffer that holds the current state */
@@ -209,10 +209,19 @@ struct Sim {
     bool downloadMixedLinks();
     void swapVoxelBuffers();
     bool stepMixed(int nsteps, const struct MixTape* tape = nullptr);
+    bool mixedFinish(int launched);
     /* one launch sequence of nsteps steps, each robot with its own dt; done[r] receives the
        steps robot r completed (nsteps unless it diverged) */
     bool stepFp64(int nsteps);
     bool stepWithContacts(int nsteps);
+    /* the FP64 step as one kernel per step (vx_fused.hip), bit for bit stepFp64 /
+       stepWithContacts; the split path is kept selectable */
+    bool fp64Split = false;
+    DevBuf bldAlt, blfAlt, blflAlt, bldSide, blfSide, blflSide;
+    bool fusedFp64Buffers();
+    void swapFp64Buffers();
+    bool stepFp64Fused(int nsteps);
+    bool fusedFp64Finish(int launched);
     int stepRobots(const float* dt, int nsteps, int* done, int* divergedAt);
     std::vector<float> launchDt; /* per robot dt of the current launch */
     bool beginLaunch(int nsteps);

@@ -75,7 +75,7 @@ def traffic_from_profiles(kernel, workload):
         return json.load(f).get("hbm_bytes_per_launch")
 
 
-def cpu_baseline(workload, budget_s=15.0, max_steps=200):
+def cpu_baseline(workload, budget_s=15.0, max_steps=2000):
     """The CPU oracle (oracle/vx_oracle.c, OpenMP over the link and voxel loops like the
     reference's USE_OMP build) on the same workload, timed on this host."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))

@@ -189,27 +189,31 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_FUS
         else link_end<ForcesOnly, false>(s, i, o, dir, rob, prevDt, tnow, tempI, pos0, q0, sf, sm);
     }
     if (ForcesOnly) return;
+    /* the voxel's index through an opaque copy: its addresses are formed here, after the link
+       loop, instead of being kept live (and spilled) across it */
+    int iv = i;
+    __asm__ volatile("" : "+v"(iv));
 
     /* CVX_Voxel::timeStep (VX_Voxel.cpp:162-232) */
     const VoxMat m = s.vmats[mi];
-    const int e = s.vext[i];
+    const int e = s.vext[iv];
     const ExtRec* ex = e >= 0 ? &s.exts[e] : nullptr;
-    uint32_t flags = s.vflIn[i];
+    uint32_t flags = s.vflIn[iv];
     if (ex && (ex->dof & 0x3F) == 0x3F) {
-        voxel_pinned(m, ex, s.vijk, nv, i, s.vdOut);
-        s.vflOut[i] = flags;
+        voxel_pinned(m, ex, s.vijk, nv, iv, s.vdOut);
+        s.vflOut[iv] = flags;
         return;
     }
-    dvec pos = pos0, lin = dv(vd[VLX * nv + i], vd[VLY * nv + i], vd[VLZ * nv + i]);
-    dvec ang = dv(vd[VAX * nv + i], vd[VAY * nv + i], vd[VAZ * nv + i]);
+    dvec pos = pos0, lin = dv(vd[VLX * nv + iv], vd[VLY * nv + iv], vd[VLZ * nv + iv]);
+    dvec ang = dv(vd[VAX * nv + iv], vd[VAY * nv + iv], vd[VAZ * nv + iv]);
     dquat q = q0;
-    voxel_step_fp64(m, ex, s.vijk, nv, i, dt, tempI, sf, sm, pos, lin, ang, q, flags, [&](dvec& t) {
+    voxel_step_fp64(m, ex, s.vijk, nv, iv, dt, tempI, sf, sm, pos, lin, ang, q, flags, [&](dvec& t) {
         /* contacts in partner order (VX_Voxel.cpp:250-253), each pair's force from this
            voxel's side as the split step's contact kernel forms it */
         if (!s.colCnt) return;
-        const int cnt = s.colCnt[i], off = s.colOff[i];
+        const int cnt = s.colCnt[iv], off = s.colOff[iv];
         if (cnt == 0) return;
-        const ContactSelf me = contact_self(m, vd, nv, i, tempI);
+        const ContactSelf me = contact_self(m, vd, nv, iv, tempI);
         for (int c = 0; c < cnt; c++) {
             const int j = s.partners[off + c];
             const float tj = s.vphase ? actuation_temperature(s.actBase, s.actAmp, s.actFreq, tnow, s.vphase[j])
@@ -218,8 +222,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_FUS
             t.x -= f.x; t.y -= f.y; t.z -= f.z;
         }
     });
-    store_voxel(s.vdOut, nv, i, pos, lin, ang, q);
-    s.vflOut[i] = flags;
+    store_voxel(s.vdOut, nv, iv, pos, lin, ang, q);
+    s.vflOut[iv] = flags;
 }
 
 /* after a launch: previousDt of every voxel whose robot completed at least one step */

@@ -110,7 +110,7 @@ __device__ __forceinline__ void link_slot_fp64(const DevPtrs& s, int k, int rob,
 #define VX_LINK_WAVES 3
 #endif
 #ifndef VX_VOXEL_WAVES
-#define VX_VOXEL_WAVES 4
+#define VX_VOXEL_WAVES 3
 #endif
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_LINK_WAVES, 8))) k_links_fp64(DevPtrs s, int k,
                                                                                                          int xcd) {
@@ -161,16 +161,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_VOX
     const int* colCnt = s.colCnt;
     const int* colOff = s.colOff;
     const float* colF = s.colF;
-    voxel_step_fp64(m, ex, s.vijk, nv, i, dt, s.vtemp[i], sf, sm, pos, lin, ang, q, flags, [&](dvec& t) {
+    int iv = i; /* opaque copy: the stores' addresses are formed after the integration */
+    __asm__ volatile("" : "+v"(iv));
+    voxel_step_fp64(m, ex, s.vijk, nv, iv, dt, s.vtemp[iv], sf, sm, pos, lin, ang, q, flags, [&](dvec& t) {
         if (!colCnt) return;
-        const int cnt = colCnt[i], off = colOff[i];
+        const int cnt = colCnt[iv], off = colOff[iv];
         for (int c = 0; c < cnt; c++) {
             const float* f = colF + 3 * (size_t)(off + c);
             t.x -= f[0]; t.y -= f[1]; t.z -= f[2];
         }
     });
-    store_voxel(vd, nv, i, pos, lin, ang, q);
-    s.vflags[i] = flags;
+    store_voxel(vd, nv, iv, pos, lin, ang, q);
+    s.vflags[iv] = flags;
 }
 
 /* actuation before step k: temperatures at the robot's current time, then the rest length of

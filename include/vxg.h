@@ -242,6 +242,10 @@ int vxg_link_count(vxg_sim* s);                       /* linkCount()  Voxelyze.h
 int vxg_link_find(vxg_sim* s, int ix, int iy, int iz, int direction);
 /* CVX_Link::voxel(bool) (VX_Link.h:47) as voxel indices, and the link axis (VX_Link.h:88) */
 int vxg_link_info(vxg_sim* s, int link, int* neg_vox, int* pos_vox);
+/* CVX_Link::currentTransverseArea and currentTransverseStrainSum (VX_Link.h:107-108,
+   src/VX_Link.cpp:142-147) of links [first, first+count): the cross-section and transverse
+   strain sum the volumetric (Poisson's ratio != 0) stress uses. */
+void vxg_link_get_transverse(vxg_sim* s, int first, int count, float* area, float* strain_sum);
 /* Bulk state of links [first, first+count) in linksList order: axialStrain() and
    axialStress() (VX_Link.h:52-54), maxStrain/strainOffset (VX_Link.h:75), flag bits,
    the local pose pos2/angle1v/angle2v (9 per link, VX_Link.h:101), the four end

@@ -25,6 +25,8 @@
 class CVXS_SimGLView {
 public:
     static Vec3D<double> linMom(CVX_Voxel* v) { return v->linMom; }
+    static float transverseArea(CVX_Link* l) { return l->currentTransverseArea; }
+    static float transverseStrainSum(CVX_Link* l) { return l->currentTransverseStrainSum; }
     static Vec3D<double> angMom(CVX_Voxel* v) { return v->angMom; }
     static int voxFlags(CVX_Voxel* v) { return v->boolStates; }
     static const CVX_Link* lk(const CVX_Link* l) { return l; }
@@ -294,6 +296,15 @@ EXPORT int vxr_link_info(vxr_sim* s, int li, int* neg, int* pos) {
     if (neg) *neg = voxIndex(s, l->voxel(false));
     if (pos) *pos = voxIndex(s, l->voxel(true));
     return CVXS_SimGLView::axis(l);
+}
+EXPORT void vxr_link_get_transverse(vxr_sim* s, int first, int count, float* area, float* strainSum) {
+    for (int k = 0; k < count; k++) {
+        int li = first + k;
+        if (li < 0 || li >= s->sim.linkCount()) continue;
+        CVX_Link* l = s->sim.link(li);
+        if (area) area[k] = CVXS_SimGLView::transverseArea(l);
+        if (strainSum) strainSum[k] = CVXS_SimGLView::transverseStrainSum(l);
+    }
 }
 EXPORT void vxr_link_get_state(vxr_sim* s, int first, int count, float* strain, float* stress, float* maxStrain,
                                float* strainOffset, uint32_t* flags, double* local9, double* forces12, double* rest) {

@@ -29,9 +29,13 @@
  * Parity pinning (tests/test_oracle.py): bit-exact against the golden vectors
  * that tests/golden/make_golden.py recorded from the compiled reference, and
  * against the closed forms of the reference's own tests (test/tVoxelyze.h).
- * Poisson's ratio != 0 (the volumetric path) is not restated: every entry
- * point that would need it reports an error instead ("parity unpinned" is
- * therefore not claimed for it -- it is absent).
+ * Poisson's ratio != 0 (the volumetric path): CVX_Voxel::strain/poissonsStrain/
+ * transverseArea/transverseStrainSum (src/VX_Voxel.cpp:300-374), the lazily cached
+ * Poisson strain and its invalidation (:55, :85, :231), CVX_Link::updateTransverseInfo
+ * and axialStiffness (src/VX_Link.cpp:142-147, 160, 259-267), the volumetric stress
+ * (src/VX_Material.cpp:165-195).  With a Poisson material the link loop runs serially
+ * in linksList order, as in the reference, because a voxel's Poisson strain is cached
+ * at the first request of each step.
  */
 #include <float.h>
 #include <math.h>
@@ -106,10 +110,13 @@ static inline dvec qrotinv(dquat q, dvec f) {
 static inline double tr_sin(double x) { return (double)sinq((__float128)x); }
 static inline double tr_cos(double x) { return (double)cosq((__float128)x); }
 static inline double tr_acos(double x) { return (double)acosq((__float128)x); }
+/* powf for the Poisson strain (VX_Voxel.cpp:326, float pow): binary128 rounded once to float */
+static inline float tr_powf(float x, float y) { return (float)powq((__float128)x, (__float128)y); }
 #else
 #define tr_sin sin
 #define tr_cos cos
 #define tr_acos acos
+#define tr_powf powf
 #endif
 
 /* Quat3D::ToRotationVector (Quat3D.h:117-122): Vec*2.0*s, and Vec/f = Vec*(1/f) */
@@ -215,6 +222,8 @@ typedef struct {
     dquat orient;
     int boolStates;
     float temp, previousDt;
+    fvec pStrain;             /* CVX_Voxel::pStrain, the cached Poisson strain (VX_Voxel.h:179-180) */
+    int pStrainInvalid;
     int hasColWatch;
     fvec lastColWatchPos;
     int *colWatch, ncol, capcol;
@@ -723,6 +732,7 @@ static void vox_reset(ovox* v) {
     vox_setStatic(v, 1);
     v->temp = 0.0f;
     v->previousDt = 0.0f;
+    v->pStrainInvalid = 1;
 }
 /* CVX_Voxel::updateSurface (VX_Voxel.cpp:376-381) */
 static void vox_updateSurface(ovox* v) {
@@ -742,12 +752,62 @@ static oext* vox_external(ovox* v) {
 static void link_updateRestLength(vxo_sim* s, olink* l) {
     l->currentRestLength = 0.5 * (vox_baseSizeAxis(V(s, l->vNeg), l->axis) + vox_baseSizeAxis(V(s, l->vPos), l->axis));
 }
-/* CVX_Link::updateTransverseInfo (VX_Link.cpp:142-147) for Poisson's ratio 0:
-   CVX_Voxel::transverseArea is size*size and transverseStrainSum is 0 (VX_Voxel.cpp:345-372) */
+static float link_axialStrainEnd(const olink* l, int positiveEnd);
+/* CVX_Voxel::strain(bool poissonsStrain) (VX_Voxel.cpp:300-334) */
+static fvec vox_strain(vxo_sim* s, const ovox* v, int poissons) {
+    float r[3] = {0.0f, 0.0f, 0.0f};
+    int nb[3] = {0, 0, 0}, tension[3] = {0, 0, 0};
+    for (int i = 0; i < 6; i++) {
+        if (v->links[i] >= 0) {
+            r[i / 2] += link_axialStrainEnd(Lk(s, v->links[i]), i % 2); /* isNegative(dir): v is the positive end */
+            nb[i / 2]++;
+        }
+    }
+    for (int i = 0; i < 3; i++) {
+        if (nb[i] == 2) r[i] *= 0.5f;
+        if (poissons) {
+            const float f = i == 0 ? v->ext ? v->ext->force.x : 0.0f : i == 1 ? v->ext ? v->ext->force.y : 0.0f
+                                                                            : v->ext ? v->ext->force.z : 0.0f;
+            tension[i] = (nb[i] == 2) || (v->ext && (nb[i] == 1 && (ext_isFixed(v->ext, 1 << i) || f != 0)));
+        }
+    }
+    if (poissons && !(tension[0] && tension[1] && tension[2])) {
+        float add = 0;
+        for (int i = 0; i < 3; i++) if (tension[i]) add += r[i];
+        const float value = tr_powf(1.0f + add, -v->mat->nu) - 1.0f;
+        for (int i = 0; i < 3; i++) if (!tension[i]) r[i] = value;
+    }
+    return fv(r[0], r[1], r[2]);
+}
+/* CVX_Voxel::poissonsStrain (VX_Voxel.cpp:336-343): computed when first asked after being
+   invalidated, then cached */
+static fvec vox_poissonsStrain(vxo_sim* s, ovox* v) {
+    if (v->pStrainInvalid) {
+        v->pStrain = vox_strain(s, v, 1);
+        v->pStrainInvalid = 0;
+    }
+    return v->pStrain;
+}
+/* CVX_Voxel::transverseStrainSum (VX_Voxel.cpp:346-359) */
+static float vox_transverseStrainSum(vxo_sim* s, ovox* v, int axis) {
+    if (v->mat->nu == 0) return 0;
+    const fvec ps = vox_poissonsStrain(s, v);
+    return axis == 0 ? ps.y + ps.z : axis == 1 ? ps.x + ps.z : ps.x + ps.y;
+}
+/* CVX_Voxel::transverseArea (VX_Voxel.cpp:361-374) */
+static float vox_transverseArea(vxo_sim* s, ovox* v, int axis) {
+    const float size = (float)v->mat->nomSize;
+    if (v->mat->nu == 0) return size * size;
+    const fvec ps = vox_poissonsStrain(s, v);
+    const double px = ps.x, py = ps.y, pz = ps.z; /* Vec3D<> psVec = poissonsStrain() */
+    return axis == 0 ? (float)(size * size * (1 + py) * (1 + pz))
+         : axis == 1 ? (float)(size * size * (1 + px) * (1 + pz)) : (float)(size * size * (1 + px) * (1 + py));
+}
+/* CVX_Link::updateTransverseInfo (VX_Link.cpp:142-147) */
 static void link_updateTransverseInfo(vxo_sim* s, olink* l) {
-    float s1 = (float)V(s, l->vNeg)->mat->nomSize, s2 = (float)V(s, l->vPos)->mat->nomSize;
-    l->currentTransverseArea = 0.5f * (s1 * s1 + s2 * s2);
-    l->currentTransverseStrainSum = 0.5f * (0.0f + 0.0f);
+    l->currentTransverseArea = 0.5f * (vox_transverseArea(s, V(s, l->vNeg), l->axis) + vox_transverseArea(s, V(s, l->vPos), l->axis));
+    l->currentTransverseStrainSum =
+        0.5f * (vox_transverseStrainSum(s, V(s, l->vNeg), l->axis) + vox_transverseStrainSum(s, V(s, l->vPos), l->axis));
 }
 /* CVX_Link::reset (VX_Link.cpp:61-75) */
 static void link_reset(vxo_sim* s, olink* l) {
@@ -843,6 +903,8 @@ static void link_updateForces(vxo_sim* s, olink* l) {
     dvec dA2 = dv(0.5 * (l->angle2v.x - oldA2.x), 0.5 * (l->angle2v.y - oldA2.y), 0.5 * (l->angle2v.z - oldA2.z));
 
     const omat* m = l->mat;
+    /* volume effects (VX_Link.cpp:160) */
+    if (m->nu != 0.0f || l->currentTransverseStrainSum != 0) link_updateTransverseInfo(s, l);
     l->stress = link_updateStrain(l, (float)(l->pos2.x / l->currentRestLength));
     if (mat_isFailed(m, l->maxStrain)) {
         l->forceNeg = l->forcePos = l->momentNeg = l->momentPos = dv(0, 0, 0);
@@ -1040,6 +1102,7 @@ static void vox_timeStep(vxo_sim* s, ovox* v, float dt) {
             }
         }
     }
+    v->pStrainInvalid = 1; /* VX_Voxel.cpp:231 */
 }
 
 /* ------------------------------------------------------------- collisions */
@@ -1325,6 +1388,7 @@ static void sim_replaceVoxel(vxo_sim* s, omat* m, int x, int y, int z) {
         float fi = m->momentInertia / v->mat->momentInertia;
         v->angMom.x *= fi; v->angMom.y *= fi; v->angMom.z *= fi;
         vox_setStatic(v, 0);
+        v->pStrainInvalid = 1; /* CVX_Voxel::replaceMaterial (VX_Voxel.cpp:85) */
         v->mat = m;
     }
     for (int i = 0; i < 6; i++) {
@@ -1335,12 +1399,20 @@ static void sim_replaceVoxel(vxo_sim* s, omat* m, int x, int y, int z) {
 
 /* ------------------------------------------------------------------ step */
 /* CVoxelyze::recommendedTimeStep (Voxelyze.cpp:286-311) */
+/* CVX_Link::axialStiffness (VX_Link.cpp:259-267): with Poisson's ratio it refreshes the rest
+   length and transverse info (and so the voxels' cached Poisson strains) as it reads them */
+static float link_axialStiffness(vxo_sim* s, olink* l) {
+    if (l->mat->nu == 0.0f) return l->mat->a1;
+    link_updateRestLength(s, l);
+    link_updateTransverseInfo(s, l);
+    return (float)(l->mat->eHat * l->currentTransverseArea / ((l->strain + 1) * l->currentRestLength));
+}
 static float sim_recommendedTimeStep(vxo_sim* s) {
     float maxFreq2 = 0.0f;
     for (int i = 0; i < s->nllist; i++) {
         olink* l = Lk(s, s->llist[i]);
         float m1 = V(s, l->vNeg)->mat->mass, m2 = V(s, l->vPos)->mat->mass;
-        float f = l->mat->a1 / (m1 < m2 ? m1 : m2);
+        float f = link_axialStiffness(s, l) / (m1 < m2 ? m1 : m2);
         if (f > maxFreq2) maxFreq2 = f;
     }
     if (maxFreq2 <= 0.0f) {
@@ -1365,11 +1437,21 @@ static int sim_doTimeStep(vxo_sim* s, float dt) {
     else if (dt < 0) dt = sim_recommendedTimeStep(s);
     int diverged = 0;
     int nl = s->nllist;
+    if (sim_hasPoisson(s)) {
+        /* the reference's serial order: a voxel's Poisson strain is cached when its first
+           link (in linksList order) asks for it */
+        for (int i = 0; i < nl; i++) {
+            olink* l = Lk(s, s->llist[i]);
+            link_updateForces(s, l);
+            if (l->strain > 100) diverged = 1;
+        }
+    } else {
 #pragma omp parallel for schedule(static) reduction(| : diverged)
-    for (int i = 0; i < nl; i++) {
-        olink* l = Lk(s, s->llist[i]);
-        link_updateForces(s, l);
-        if (l->strain > 100) diverged = 1;
+        for (int i = 0; i < nl; i++) {
+            olink* l = Lk(s, s->llist[i]);
+            link_updateForces(s, l);
+            if (l->strain > 100) diverged = 1;
+        }
     }
     if (diverged) return 0;
     if (s->collisions) sim_updateCollisions(s);
@@ -1443,7 +1525,6 @@ EXPORT float vxo_ambient_temperature(sim_t* s) { return s->ambientTemp; }
 
 EXPORT int vxo_do_time_step(sim_t* s, float dt) {
     s->err[0] = 0;
-    if (sim_hasPoisson(s)) { snprintf(s->err, sizeof(s->err), "Poisson's ratio != 0 is not restated by the oracle"); return 0; }
     return sim_doTimeStep(s, dt);
 }
 EXPORT int vxo_step(sim_t* s, float dt, int n, int* divergedAt) {
@@ -1777,6 +1858,16 @@ EXPORT int vxo_link_info(sim_t* s, int li, int* neg, int* pos) {
     if (neg) *neg = vox_index_of_slot(s, l->vNeg);
     if (pos) *pos = vox_index_of_slot(s, l->vPos);
     return l->axis;
+}
+/* CVX_Link::currentTransverseArea / currentTransverseStrainSum (VX_Link.h:107-108) */
+EXPORT void vxo_link_get_transverse(sim_t* s, int first, int count, float* area, float* strainSum) {
+    for (int k = 0; k < count; k++) {
+        int li = first + k;
+        if (li < 0 || li >= s->nllist) continue;
+        olink* l = Lk(s, s->llist[li]);
+        if (area) area[k] = l->currentTransverseArea;
+        if (strainSum) strainSum[k] = l->currentTransverseStrainSum;
+    }
 }
 EXPORT void vxo_link_get_state(sim_t* s, int first, int count, float* strain, float* stress, float* maxStrain,
                                float* strainOffset, uint32_t* flags, double* local9, double* forces12,

@@ -20,6 +20,7 @@ OFF = {X_POS: (1, 0, 0), X_NEG: (-1, 0, 0), Y_POS: (0, 1, 0), Y_NEG: (0, -1, 0),
 TRANSCENDENTAL = {
     "large_deformation_damping", "large_deformation", "impulse", "temperature", "config1", "config2_small",
     "config3_small", "config4_robot", "two_voxel_10", "two_voxel_11",
+    "poisson_large", "poisson_mixed", "deformable_material_poisson",
 }
 # relative displacement error bound where bit-exactness is not required
 TOL = {name: 1e-9 for name in TRANSCENDENTAL}
@@ -29,6 +30,8 @@ TOL["config4_robot"] = 1e-6
 # plastic links under a driven face with self-contact amplify last-bit trig differences: the
 # oracle with correctly rounded trig (same as the device) lands 1.46e-8 from glibc's run
 TOL["config3_small"] = 1e-7
+# plastic links with Poisson's ratio: the correctly rounded oracle lands 2.6e-8 from glibc's run
+TOL["deformable_material_poisson"] = 1e-7
 
 
 def snapshot(sim, prefix=""):
@@ -737,6 +740,106 @@ def material_curves(make):
     return out
 
 
+def poisson_out(sim, trace):
+    out = {"trace": np.asarray(trace)}
+    out.update(snapshot(sim))
+    tr = sim.link_transverse()
+    out["link_tarea"] = tr["area"]
+    out["link_tss"] = tr["strainSum"]
+    return out
+
+
+def poisson_small(make, steps=200):
+    """Poisson's ratio 0.3, a 1-wide bar of three voxels pulled at its end (tVoxelyze.h:687-713)."""
+    sim = make(0.001)
+    m = sim.addMaterial(1e6, 1e3)
+    m.setPoissonsRatio(0.3)
+    m.setInternalDamping(1.0)
+    m.setGlobalDamping(0.3)
+    for i in range(3):
+        v = sim.setVoxel(m, i, 0, 0)
+        if i == 0:
+            v.external().setFixedAll()
+        if i == 2:
+            v.external().setForce(1e-3, 0.0, 0.0)
+    ts = sim.recommendedTimeStep()
+    probe = sim.voxel(2, 0, 0).index
+    trace = np.zeros(steps)
+    for k in range(steps):
+        sim.doTimeStep(ts)
+        trace[k] = sim.state(probe, 1)["pos"][0][0] - 0.002
+    out = poisson_out(sim, trace)
+    # with a Poisson's ratio the recommended step depends on the state (VX_Link.cpp:259-267)
+    out["dt_after"] = np.float32(sim.recommendedTimeStep())
+    out["dt_before"] = np.float32(ts)
+    tr = sim.link_transverse()  # and refreshes the transverse info as it reads it
+    out["link_tarea_after_dt"] = tr["area"]
+    return out
+
+
+def poisson_bar(make, ni, nj, nk, nu, glob, disp, steps, probe_ijk, mixed=False):
+    """A bar fixed at x=0 and displaced at its far end (tVoxelyze.h:715-842)."""
+    sim = make(0.001)
+    m1 = sim.addMaterial(1e6, 1e3)
+    m1.setPoissonsRatio(nu)
+    m1.setInternalDamping(1.0)
+    m1.setGlobalDamping(glob)
+    m2 = None
+    if mixed:
+        m2 = sim.addMaterial(1e6, 1e3)
+        m2.setPoissonsRatio(0.0)
+        m2.setInternalDamping(1.0)
+        m2.setGlobalDamping(glob)
+    for i in range(ni):
+        for j in range(nj):
+            for k in range(nk):
+                v = sim.setVoxel(m2 if mixed and 1 < i < ni - 1 else m1, i, j, k)
+                if i == 0:
+                    v.external().setFixedAll()
+                if i == ni - 1:
+                    v.external().setDisplacementAll((float(np.float32(disp)), 0.0, 0.0))  # a float literal
+    ts = sim.recommendedTimeStep()
+    probe = sim.voxel(*probe_ijk).index
+    trace = np.zeros(steps)
+    for k in range(steps):
+        sim.doTimeStep(ts)
+        trace[k] = sim.state(probe, 1)["pos"][0][0] - probe_ijk[0] * 0.001
+    return poisson_out(sim, trace)
+
+
+def deformable_material_poisson(make, load=300, ramp=200, relax=400):
+    """Bilinear plastic set with Poisson's ratio 0.3, displacement-driven (tVoxelyze.h:891-944)."""
+    sim = make(0.001)
+    m = sim.addMaterial(1e6, 1e3)
+    m.setModelBilinear(1e6, 5e5, 1e5)
+    m.setPoissonsRatio(0.3)
+    m.setInternalDamping(1.0)
+    m.setGlobalDamping(0.2)
+    for i in range(5):
+        for j in range(3):
+            for k in range(3):
+                v = sim.setVoxel(m, i, j, k)
+                if i == 0:
+                    v.external().setFixedAll()
+    ts = sim.recommendedTimeStep()
+    probe = sim.voxel(4, 1, 1).index
+    trace = np.zeros(load + relax)
+    for k in range(load):
+        if k < ramp:
+            for j in range(3):
+                for kk in range(3):
+                    sim.voxel(4, j, kk).external().setDisplacementAll((k * np.float32(7.5e-6), 0.0, 0.0))
+        sim.doTimeStep(ts)
+        trace[k] = sim.state(probe, 1)["pos"][0][0] - 0.004
+    for j in range(3):
+        for kk in range(3):
+            sim.voxel(4, j, kk).external().setFixedAll(False)
+    for k in range(relax):
+        sim.doTimeStep(ts)
+        trace[load + k] = sim.state(probe, 1)["pos"][0][0] - 0.004
+    return poisson_out(sim, trace)
+
+
 SCENARIOS = {
     "two_voxel": None,  # expanded per case below
     "internal_damping": lambda mk: surrounded(mk, 1.0, 0.0),
@@ -764,6 +867,11 @@ SCENARIOS = {
     "config2_small": config2_small,
     "config3_small": config3_small,
     "config4_robot": config4_robot,
+    "poisson_small": poisson_small,
+    "poisson_large": lambda mk: poisson_bar(mk, 9, 2, 2, 0.3, 0.2, 1e-3, 300, (4, 0, 0)),
+    "poisson_high": lambda mk: poisson_bar(mk, 5, 3, 3, 0.495, 2.0, 1e-5, 300, (2, 1, 1)),
+    "poisson_mixed": lambda mk: poisson_bar(mk, 7, 3, 3, 0.3, 0.3, 1e-3, 300, (3, 1, 1), mixed=True),
+    "deformable_material_poisson": deformable_material_poisson,
 }
 del SCENARIOS["two_voxel"]
 for _i, _c in enumerate(TWO_VOXEL_CASES):

@@ -147,3 +147,34 @@ def test_fused_graph_replays_equal_plain_launches():
     cap, rep = C.c_int(0), C.c_int(0)
     a.abi.graph_stats(a.handle, C.byref(cap), C.byref(rep))
     assert cap.value >= 1 and rep.value >= 1
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("path", [FUSED, SPLIT], ids=["fused", "split"])
+def test_random_poisson_models_match_the_oracle(path):
+    """Random small models whose materials mostly have a Poisson's ratio, stepped in blocks
+    with the state-dependent recommended step: bit for bit the correctly rounded oracle
+    (end forces and transverse info included)."""
+    import test_oracle as T
+    for seed in range(4):
+        a = T.random_model(B.maker(B.product_abi(), path), seed, poisson=True)
+        b = T.random_model(B.maker(B.oracle_cr_abi()), seed, poisson=True)
+        for k in range(4):
+            dt = b.recommendedTimeStep()
+            assert np.float32(a.recommendedTimeStep()) == np.float32(dt)
+            assert a.step(dt, 40) == b.step(dt, 40)
+            same_state(a, b, f"seed {seed} block {k}")
+            ta, tb = a.link_transverse(), b.link_transverse()
+            assert all(bits_equal(ta[key], tb[key]) for key in ta), f"seed {seed} block {k}"
+
+
+@pytest.mark.gpu
+def test_poisson_runs_in_the_fp64_step_only():
+    sim = B.maker(B.product_abi())(0.001)
+    m = sim.addMaterial(1e6, 1e3)
+    m.setPoissonsRatio(0.3)
+    for i in range(3):
+        sim.setVoxel(m, i, 0, 0)
+    sim.setPrecision(1)
+    with pytest.raises(RuntimeError):
+        sim.step(sim.recommendedTimeStep(), 5)

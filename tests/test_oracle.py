@@ -205,8 +205,9 @@ def test_material_curves(oracle_runs):
 
 
 # --------------------------------------------------------- brute force against the reference
-def random_model(make, seed):
+def random_model(make, seed, poisson=False):
     rng = np.random.default_rng(seed)
+    nu_rng = np.random.default_rng(1000 + seed)
     sim = make(0.001 * (1 + rng.integers(0, 5)))
     mats = []
     for _ in range(int(rng.integers(1, 4))):
@@ -223,6 +224,8 @@ def random_model(make, seed):
         m.setKineticFriction(float(rng.uniform(0, 0.5)))
         m.setCollisionDamping(float(rng.uniform(0, 1)))
         m.setCte(float(rng.uniform(-0.01, 0.01)))
+        if poisson and nu_rng.random() < 0.7:
+            m.setPoissonsRatio(float(nu_rng.uniform(0.05, 0.45)))
         mats.append(m)
     n = int(rng.integers(2, 5))
     occ = rng.random((n, n, n)) < 0.7
@@ -266,3 +269,76 @@ def test_oracle_vs_reference_random_models(seed):
         sa, sb = S.snapshot(a), S.snapshot(b)
         bad = [key for key in sa if not bits_equal(sa[key], sb[key])]
         assert not bad, f"seed {seed} block {k}: {bad}"
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_oracle_vs_reference_random_poisson_models(seed):
+    """Random models whose materials mostly have a Poisson's ratio: the volumetric path,
+    the cached Poisson strains and the state-dependent recommended step, bit for bit."""
+    ref = B.ref_abi()
+    if ref is None:
+        pytest.skip("compiled reference not present (built only where the reference sources are)")
+    a = random_model(B.maker(B.oracle_abi()), seed, poisson=True)
+    b = random_model(B.maker(ref), seed, poisson=True)
+    for k in range(6):
+        dt = a.recommendedTimeStep()
+        assert bits_equal(np.float32(dt), np.float32(b.recommendedTimeStep()))
+        na, _ = a.step(dt, 40)
+        nb, _ = b.step(dt, 40)
+        assert na == nb
+        sa, sb = S.snapshot(a), S.snapshot(b)
+        bad = [key for key in sa if not bits_equal(sa[key], sb[key])]
+        ta, tb = a.link_transverse(), b.link_transverse()
+        bad += [key for key in ta if not bits_equal(ta[key], tb[key])]
+        assert not bad, f"seed {seed} block {k}: {bad}"
+
+
+# ------------------------------------------------------------- Poisson's ratio != 0
+def test_poisson_bars_reach_the_reference_test_values(oracle_runs):
+    """tVoxelyze.h:715-842: displacement-driven bars with Poisson's ratio 0.3 / 0.495 / mixed."""
+    assert abs(np.float32(run_oracle(oracle_runs, "poisson_large")["trace"][-1]) - 5e-4) < 1e-7
+    assert abs(np.float32(run_oracle(oracle_runs, "poisson_high")["trace"][-1]) - 5e-6) < 1e-9
+    assert abs(np.float32(run_oracle(oracle_runs, "poisson_mixed")["trace"][-1]) - 5e-4) < 5e-6
+
+
+def test_poisson_transverse_strain_follows_the_axial_strain():
+    """In a displaced 2x2x9 bar every interior voxel has two x-links and one link along each
+    of y and z (tension only along x), so its Poisson strain on y and z is
+    (1 + eps_x)^-nu - 1 with eps_x the mean of its x-links' strains, and an x-link between two
+    interior voxels carries the transverse strain sum v_neg + v_pos (VX_Voxel.cpp:300-359,
+    VX_Link.cpp:142-147).  A step caches each voxel's strain from the link strains the step
+    starts from, so the check uses the strains recorded one step before the end."""
+    sim = B.maker(B.oracle_abi())(0.001)
+    m = sim.addMaterial(1e6, 1e3)
+    nu = 0.3
+    m.setPoissonsRatio(nu)
+    m.setInternalDamping(1.0)
+    m.setGlobalDamping(0.2)
+    for i in range(9):
+        for j in range(2):
+            for k in range(2):
+                v = sim.setVoxel(m, i, j, k)
+                if i == 0:
+                    v.external().setFixedAll()
+                if i == 8:
+                    v.external().setDisplacementAll((float(np.float32(1e-3)), 0.0, 0.0))
+    ts = sim.recommendedTimeStep()
+    sim.step(ts, 299)
+    prev = sim.link_state()["strain"].astype(np.float64)  # the strains the last step's cache reads
+    sim.step(ts, 1)
+    tss = sim.link_transverse()["strainSum"].astype(np.float64)
+    ends = [(sim.link(li).voxel(False).index, sim.link(li).voxel(True).index) for li in range(sim.linkCount())]
+    ijk = {i: (sim.voxel(i).indexX(), sim.voxel(i).indexY(), sim.voxel(i).indexZ()) for i in range(sim.voxelCount())}
+    xlinks = {i: [] for i in ijk}
+    for li, (a, b) in enumerate(ends):
+        if ijk[a][0] != ijk[b][0]:
+            xlinks[a].append(li)
+            xlinks[b].append(li)
+    def v(i):
+        return (1.0 + np.mean([prev[li] for li in xlinks[i]])) ** -nu - 1.0
+    checked = 0
+    for li, (a, b) in enumerate(ends):
+        if ijk[a][0] != ijk[b][0] and 0 < ijk[a][0] and ijk[b][0] < 8:
+            assert tss[li] == pytest.approx(v(a) + v(b), rel=1e-5), li
+            checked += 1
+    assert checked == 6 * 4 and np.all(tss[[li for li, (a, b) in enumerate(ends) if ijk[a][0] != ijk[b][0] and 0 < ijk[a][0] and ijk[b][0] < 8]] < 0)

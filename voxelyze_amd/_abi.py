@@ -71,6 +71,7 @@ SIGNATURES = {
     "link_count": (_I, [_P]),
     "link_find": (_I, [_P, _I, _I, _I, _I]),
     "link_info": (_I, [_P, _I, _IP, _IP]),
+    "link_get_transverse": (None, [_P, _I, _I, _FP, _FP]),
     "link_get_state": (None, [_P, _I, _I, _FP, _FP, _FP, _FP, _UP, _DP, _DP, _DP]),
 }
 

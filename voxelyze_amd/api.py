@@ -532,6 +532,15 @@ class CVoxelyze:
             self.abi.voxel_get_state(self.handle, first, n, _dptr(p), _dptr(q), _dptr(lm), _dptr(am), _uptr(fl))
         return {"pos": p, "orient": q, "linMom": lm, "angMom": am, "flags": fl}
 
+    def link_transverse(self, first=0, count=None):
+        """currentTransverseArea and currentTransverseStrainSum of links (VX_Link.h:107-108)."""
+        n = self.linkCount() - first if count is None else count
+        area = np.zeros(n, np.float32)
+        tss = np.zeros(n, np.float32)
+        if n:
+            self.abi.link_get_transverse(self.handle, first, n, _fptr(area), _fptr(tss))
+        return {"area": area, "strainSum": tss}
+
     def link_state(self, first=0, count=None):
         n = self.linkCount() - first if count is None else count
         out = {

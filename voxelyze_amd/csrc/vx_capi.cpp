@@ -636,6 +636,16 @@ API int vxg_link_info(vxg_sim* h, int li, int* neg, int* pos) {
     if (pos) *pos = s->lpos[li];
     return s->laxis[li];
 }
+API void vxg_link_get_transverse(vxg_sim* h, int first, int count, float* area, float* strainSum) {
+    Sim* s = S(h);
+    if (!s->syncDown()) return;
+    for (int k = 0; k < count; k++) {
+        const int li = first + k;
+        if (li < 0 || li >= s->nl()) continue;
+        if (area) area[k] = s->lf[vx::LTAREA][li];
+        if (strainSum) strainSum[k] = s->lf[vx::LTSS][li];
+    }
+}
 API void vxg_link_get_state(vxg_sim* h, int first, int count, float* strain, float* stress, float* maxStrain,
                             float* strainOffset, uint32_t* flags, double* local9, double* forces12, double* rest) {
     Sim* s = S(h);

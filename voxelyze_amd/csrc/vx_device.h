@@ -31,12 +31,15 @@ enum {
    sum (a sum started at +0.0 never becomes -0.0). */
 __host__ __device__ inline int vf_index(int d, int c, int nv, int v) { return (d * nv + v) * 6 + c; }
 #define VF_COUNT 36
-/* component order of the link FP32 block */
-enum { LSTRAIN = 0, LMAXSTRAIN, LOFFSET, LSTRESS, LF_COUNT };
+/* component order of the link FP32 block: strain state, then currentTransverseArea and
+   currentTransverseStrainSum (VX_Link.h:107-108; constant unless a material has a Poisson's
+   ratio) */
+enum { LSTRAIN = 0, LMAXSTRAIN, LOFFSET, LSTRESS, LTAREA, LTSS, LF_COUNT };
 
 /* voxel flag bits (CVX_Voxel::voxFlags, VX_Voxel.h:143-147); bits 8..13 record which of
    the six directions has a link (host-maintained, read-only on the device) */
-enum { VF_INTERIOR = 1 << 1, VF_FLOOR = 1 << 2, VF_STATIC = 1 << 3, VF_COLLISIONS = 1 << 5, VF_LINK0 = 1 << 8 };
+enum { VF_INTERIOR = 1 << 1, VF_FLOOR = 1 << 2, VF_STATIC = 1 << 3, VF_COLLISIONS = 1 << 5, VF_LINK0 = 1 << 8,
+       VF_PS_VALID = 1 << 16 /* the cached Poisson strain is valid (!poissonsStrainInvalid, VX_Voxel.h:180) */ };
 /* link flag bits */
 enum { LF_LVV = 1 << 0, LF_SMALL = 1 << 1 };
 
@@ -59,6 +62,7 @@ struct VoxMat {
     float momentInertia;      /* _momentInertia */
     float zetaGlobal, zetaCollision;
     float E;                  /* Young's modulus */
+    float nu;                 /* Poisson's ratio */
 };
 
 /* per link-material constants (VX_MaterialLink.h:38-47, VX_Material.h:114-128) */
@@ -71,6 +75,7 @@ struct LinkMat {
     int linear;
     int ndata;                /* points in the stress/strain curve */
     int curveOff;             /* offset of the curve in the curve arrays */
+    float nu, eHat;           /* Poisson's ratio and E/((1-2nu)(1+nu)) of the link material */
 };
 
 /* per external record (VX_External.h:93-98) */
@@ -122,6 +127,12 @@ struct DevPtrs {
     const int* colCnt;
     const int* colOff;
     const float* colF;
+    /* Poisson's ratio != 0: the voxels' cached Poisson strains (3 * nv, valid where the flag
+       VF_PS_VALID is set) and each link slot's strainRatio (VX_Link.h:102) */
+    int poisson;
+    float* vps;
+    const float* lratio;
+    const int* vnbr;         /* 6 * nv: voxel across each direction, -1 none */
 };
 
 #define NO_DIVERGENCE 0x7F7F7F7F /* a byte fill of 0x7F */
@@ -261,6 +272,8 @@ struct FusedPtrs {
     int k;                    /* this step's index in the launch */
     int storeForces;          /* write the end forces (the launch's last step) */
     int xcd;
+    int poisson;              /* a material has Poisson's ratio != 0 */
+    const float* vps;         /* the voxels' Poisson strains for this step (3 * nv) */
 };
 
 /* a kernel launch configuration chosen by the autotuner */

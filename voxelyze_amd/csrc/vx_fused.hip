@@ -99,6 +99,14 @@ __device__ __forceinline__ void link_end(const FusedPtrs& s, int i, int o, int d
     old.flags = s.lflIn[slot];
     old.maxStrain = s.lfIn[LMAXSTRAIN * nl + slot];
     old.strainOffset = s.lfIn[LOFFSET * nl + slot];
+    if (s.poisson) { /* volume effects (VX_Link.cpp:160) */
+        old.tArea = s.lfIn[LTAREA * nl + slot];
+        old.tss = s.lfIn[LTSS * nl + slot];
+        link_transverse(m, s.vmats[s.vmat[n]], s.vmats[s.vmat[p]], s.vps, nv, n, p, axis, &old.tArea, &old.tss);
+    } else {
+        old.tArea = m.transverseArea;
+        old.tss = 0.0f;
+    }
     float dmN = 0.0f, dmP = 0.0f;
     if (old.flags & LF_LVV) { /* dampingMultiplier (VX_Voxel.h:130) of both voxels */
         dmN = s.vmats[s.vmat[n]].dampNum / prevDt;
@@ -126,6 +134,10 @@ __device__ __forceinline__ void link_end(const FusedPtrs& s, int i, int o, int d
         lf[LMAXSTRAIN * nl + slot] = e.maxStrain;
         lf[LOFFSET * nl + slot] = e.strainOffset;
         lf[LSTRESS * nl + slot] = e.stress;
+        if (s.poisson) {
+            lf[LTAREA * nl + slot] = old.tArea;
+            lf[LTSS * nl + slot] = old.tss;
+        }
         s.lflOut[slot] = e.flags;
         if (e.strain > 100) {
             atomicMin(&s.rdiv[rob], s.k);
@@ -223,7 +235,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_FUS
         }
     });
     store_voxel(s.vdOut, nv, iv, pos, lin, ang, q);
-    s.vflOut[iv] = flags;
+    s.vflOut[iv] = flags & ~(uint32_t)VF_PS_VALID; /* poissonsStrainInvalid = true (VX_Voxel.cpp:231) */
 }
 
 /* after a launch: previousDt of every voxel whose robot completed at least one step */

@@ -16,6 +16,7 @@ void launch_fused_restore(double* vd, const double* vdPrev, uint32_t* vfl, const
                           const float* vtempPrev, float* rtime, const float* rtimePrev, int nv, int first, int count,
                           int rob, hipStream_t st);
 void launch_col_stale(const DevPtrs& s, const ContactPtrs& c, hipStream_t st);
+void launch_poisson_fp64(const DevPtrs& s, int k, hipStream_t st);
 uint64_t graph_key(const void* args, size_t n, int path, int nsteps, int flags);
 
 #define HIPCHK(call)                                       \
@@ -92,6 +93,8 @@ bool Sim::stepFp64Fused(int nsteps) {
     fp.actBase = actBase;
     fp.actAmp = actAmp;
     fp.actFreq = actFreq;
+    fp.poisson = dp.poisson;
+    fp.vps = dp.vps;
     auto bind = [&](FusedPtrs& a) {
         a.vdIn = dp.vd;
         a.vdOut = (double*)bvdAlt.p;
@@ -167,6 +170,7 @@ bool Sim::stepFp64Fused(int nsteps) {
         fp.storeForces = (k == nsteps - 1 || lastStep) ? 1 : 0;
         const LaunchCfg cfg = tune ? tuner[3].next(&cand[k]) : chosen;
         fp.xcd = cfg.xcd;
+        launch_poisson_fp64(dp, k, stream); /* the Poisson strains this step's links read */
         if (timed) (void)hipEventRecord(events[2 * k], stream);
         launch_step_fp64(fp, cfg.block, false, stream);
         if (timed) (void)hipEventRecord(events[2 * k + 1], stream);
@@ -245,6 +249,8 @@ bool Sim::fusedFp64Finish(int launched) {
         fp.actBase = actBase;
         fp.actAmp = actAmp;
         fp.actFreq = actFreq;
+        fp.poisson = dp.poisson;
+        fp.vps = dp.vps;
         fp.k = launched - 1;
         fp.storeForces = 1;
         launch_step_fp64(fp, 256, true, stream);

@@ -74,6 +74,16 @@ __device__ __forceinline__ void link_slot_fp64(const DevPtrs& s, int k, int rob,
     old.flags = s.lflags[i];
     old.maxStrain = lf[LMAXSTRAIN * nl + i];
     old.strainOffset = lf[LOFFSET * nl + i];
+    if (s.poisson) { /* volume effects (VX_Link.cpp:160) */
+        old.tArea = lf[LTAREA * nl + i];
+        old.tss = lf[LTSS * nl + i];
+        link_transverse(m, s.vmats[s.vmat[n]], s.vmats[s.vmat[p]], s.vps, nv, n, p, axis, &old.tArea, &old.tss);
+        lf[LTAREA * nl + i] = old.tArea;
+        lf[LTSS * nl + i] = old.tss;
+    } else {
+        old.tArea = m.transverseArea;
+        old.tss = 0.0f;
+    }
     const double rest = ld[LREST * nl + i];
     float dmN = 0.0f, dmP = 0.0f;
     if (old.flags & LF_LVV) {
@@ -172,7 +182,29 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_VOX
         }
     });
     store_voxel(vd, nv, iv, pos, lin, ang, q);
-    s.vflags[iv] = flags;
+    s.vflags[iv] = flags & ~(uint32_t)VF_PS_VALID; /* poissonsStrainInvalid = true (VX_Voxel.cpp:231) */
+}
+
+/* Before a step's link phase: the Poisson strain of every voxel of a Poisson material whose
+   cached one is invalid, from its links' strains of the previous step.  The reference
+   computes it when the voxel's first link asks for it (CVX_Voxel::poissonsStrain,
+   VX_Voxel.cpp:336-343), before any of the voxel's links has been updated in that step, so
+   the values are the same. */
+__global__ void __launch_bounds__(256) k_poisson_fp64(DevPtrs s, int k) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.nv) return;
+    if (s.rdiv[s.vrobot[i]] < k) return;
+    const uint32_t fl = s.vflags[i];
+    if ((fl & VF_PS_VALID) || !(fl & (0x3Fu * VF_LINK0))) return;
+    const VoxMat& m = s.vmats[s.vmat[i]];
+    if (m.nu == 0.0f) return;
+    const int e = s.vext[i];
+    float ps[3];
+    poisson_strain(m, e >= 0 ? &s.exts[e] : nullptr, fl, s.vnbr, s.lf, s.lratio, s.nv, s.nl, i, ps);
+    s.vps[i] = ps[0];
+    s.vps[s.nv + i] = ps[1];
+    s.vps[2 * s.nv + i] = ps[2];
+    s.vflags[i] = fl | VF_PS_VALID;
 }
 
 /* actuation before step k: temperatures at the robot's current time, then the rest length of
@@ -236,6 +268,9 @@ void launch_links_fp64(const DevPtrs& s, int k, LaunchCfg c, hipStream_t st) {
 }
 void launch_voxels_fp64(const DevPtrs& s, int k, LaunchCfg c, hipStream_t st) {
     if (s.nv > 0) hipLaunchKernelGGL(k_voxels_fp64, dim3(blocks(s.nv, c.block)), dim3(c.block), 0, st, s, k, c.xcd);
+}
+void launch_poisson_fp64(const DevPtrs& s, int k, hipStream_t st) {
+    if (s.nv > 0 && s.poisson) hipLaunchKernelGGL(k_poisson_fp64, dim3(blocks(s.nv, 256)), dim3(256), 0, st, s, k);
 }
 void launch_actuate_fp64(const DevPtrs& s, int k, hipStream_t st) {
     if (s.nv > 0) hipLaunchKernelGGL(k_actuate_fp64, dim3(blocks(s.nv, 256)), dim3(256), 0, st, s, k);

@@ -21,6 +21,7 @@ namespace vx {
 void launch_links_fp64(const DevPtrs& s, int k, LaunchCfg c, hipStream_t st);
 void launch_voxels_fp64(const DevPtrs& s, int k, LaunchCfg c, hipStream_t st);
 void launch_actuate_fp64(const DevPtrs& s, int k, hipStream_t st);
+void launch_poisson_fp64(const DevPtrs& s, int k, hipStream_t st);
 void launch_robot_com(const DevPtrs& s, const int* rfirst, double* com, int nrobots, hipStream_t st);
 void launch_col_stale(const DevPtrs& s, const ContactPtrs& c, hipStream_t st);
 void launch_col_keys(const DevPtrs& s, const ContactPtrs& c, int nbuckets, hipStream_t st);
@@ -389,13 +390,18 @@ int Sim::addRobot() {
 /* recommendedTimeStep (src/Voxelyze.cpp:286-311) of every robot over its own links and
    voxels, in one pass over the links; cached until the topology or a material changes */
 const std::vector<float>& Sim::robotRecommendedTimeSteps() {
-    if (rdtCacheValid && (int)rdtCache.size() == nRobots()) return rdtCache;
+    const bool stateful = hasPoissonMaterial();
+    if (stateful) {
+        if (!syncDown()) rdtCacheValid = false;
+    } else if (rdtCacheValid && (int)rdtCache.size() == nRobots()) {
+        return rdtCache;
+    }
     const int R = nRobots();
     std::vector<float> maxFreq2(R, 0.0f);
     for (int l = 0; l < nl(); l++) {
         const int r = vrobot[lneg[l]];
         float m1 = vmat[lneg[l]]->mass, m2 = vmat[lpos[l]]->mass;
-        float f = lmat[l]->a1 / (m1 < m2 ? m1 : m2);
+        float f = axialStiffness(l) / (m1 < m2 ? m1 : m2);
         if (f > maxFreq2[r]) maxFreq2[r] = f;
     }
     std::vector<char> noLinks(R);
@@ -409,7 +415,7 @@ const std::vector<float>& Sim::robotRecommendedTimeSteps() {
     }
     rdtCache.resize(R);
     for (int r = 0; r < R; r++) rdtCache[r] = maxFreq2[r] <= 0.0f ? 0.0f : 1.0f / (6.283185f * sqrtf(maxFreq2[r]));
-    rdtCacheValid = true;
+    rdtCacheValid = !stateful;
     return rdtCache;
 }
 float Sim::robotRecommendedTimeStep(int r) { return robotRecommendedTimeSteps()[r]; }
@@ -425,6 +431,7 @@ void Sim::resetVoxel(int i) {
     vd[VLX][i] = vd[VLY][i] = vd[VLZ][i] = 0.0;
     vd[VAX][i] = vd[VAY][i] = vd[VAZ][i] = 0.0;
     vflags[i] |= VF_STATIC;
+    vflags[i] &= ~(uint32_t)VF_PS_VALID; /* poissonsStrainInvalid = true */
     vtemp[i] = 0.0f;
     vprevdt[i] = 0.0f;
 }
@@ -458,8 +465,91 @@ void Sim::updateRestLength(int li) {
 void Sim::resetLink(int li) {
     for (int c = 0; c < LD_COUNT; c++) ld[c][li] = 0.0;
     for (int c = 0; c < LF_COUNT; c++) lf[c][li] = 0.0f;
+    lratio[li] = vmat[lpos[li]]->E / vmat[lneg[li]]->E;
     lflags[li] = LF_SMALL;
     updateRestLength(li);
+    updateTransverseInfo(li);
+}
+
+/* ------------------------------------------------ Poisson's ratio != 0 (volume effects) */
+bool Sim::hasPoissonMaterial() const {
+    for (auto& m : vmats)
+        if (m->nu != 0.0f) return true;
+    return false;
+}
+/* CVX_Link::axialStrain(bool positiveEnd) (src/VX_Link.cpp:121-124) */
+float Sim::linkAxialStrainEnd(int li, bool positiveEnd) const {
+    const float strain = lf[LSTRAIN][li], ratio = lratio[li];
+    return positiveEnd ? 2.0f * strain * ratio / (1.0f + ratio) : 2.0f * strain / (1.0f + ratio);
+}
+/* CVX_Voxel::strain(bool poissonsStrain) (src/VX_Voxel.cpp:300-334); the power is the double
+   pow rounded once to float, as on the device */
+void Sim::voxelStrain(int vi, bool poissons, float out[3]) const {
+    float r[3] = {0.0f, 0.0f, 0.0f};
+    int nb[3] = {0, 0, 0};
+    bool tension[3] = {false, false, false};
+    for (int d = 0; d < 6; d++) {
+        const int l = vlinks[vi][d];
+        if (l < 0) continue;
+        r[d / 2] += linkAxialStrainEnd(l, d % 2 == 1); /* isNegative(dir): vi is the positive end */
+        nb[d / 2]++;
+    }
+    const Ext* e = vext[vi].get();
+    for (int a = 0; a < 3; a++) {
+        if (nb[a] == 2) r[a] *= 0.5f;
+        if (poissons) tension[a] = nb[a] == 2 || (e && nb[a] == 1 && ((e->dof & (1 << a)) || e->force[a] != 0));
+    }
+    if (poissons && !(tension[0] && tension[1] && tension[2])) {
+        float add = 0;
+        for (int a = 0; a < 3; a++) if (tension[a]) add += r[a];
+        const float value = (float)pow((double)(1.0f + add), (double)-vmat[vi]->nu) - 1.0f;
+        for (int a = 0; a < 3; a++) if (!tension[a]) r[a] = value;
+    }
+    out[0] = r[0]; out[1] = r[1]; out[2] = r[2];
+}
+/* CVX_Voxel::poissonsStrain (src/VX_Voxel.cpp:336-343): cached until invalidated */
+void Sim::poissonsStrain(int vi, float out[3]) {
+    if (!(vflags[vi] & VF_PS_VALID)) {
+        float o[3];
+        voxelStrain(vi, true, o);
+        for (int c = 0; c < 3; c++) vps[c][vi] = o[c];
+        vflags[vi] |= VF_PS_VALID;
+        dirty |= D_VSTATE;
+    }
+    for (int c = 0; c < 3; c++) out[c] = vps[c][vi];
+}
+/* CVX_Voxel::transverseStrainSum (src/VX_Voxel.cpp:346-359) */
+float Sim::transverseStrainSum(int vi, int axis) {
+    if (vmat[vi]->nu == 0) return 0;
+    float ps[3];
+    poissonsStrain(vi, ps);
+    return axis == 0 ? ps[1] + ps[2] : axis == 1 ? ps[0] + ps[2] : ps[0] + ps[1];
+}
+/* CVX_Voxel::transverseArea (src/VX_Voxel.cpp:361-374) */
+float Sim::transverseArea(int vi, int axis) {
+    const float size = (float)vmat[vi]->nomSize;
+    if (vmat[vi]->nu == 0) return size * size;
+    float ps[3];
+    poissonsStrain(vi, ps);
+    const double px = ps[0], py = ps[1], pz = ps[2]; /* Vec3D<> psVec */
+    return axis == 0 ? (float)(size * size * (1 + py) * (1 + pz))
+         : axis == 1 ? (float)(size * size * (1 + px) * (1 + pz)) : (float)(size * size * (1 + px) * (1 + py));
+}
+/* CVX_Link::updateTransverseInfo (src/VX_Link.cpp:142-147) */
+void Sim::updateTransverseInfo(int li) {
+    const int a = laxis[li];
+    lf[LTAREA][li] = 0.5f * (transverseArea(lneg[li], a) + transverseArea(lpos[li], a));
+    lf[LTSS][li] = 0.5f * (transverseStrainSum(lneg[li], a) + transverseStrainSum(lpos[li], a));
+    dirty |= D_LSTATE;
+}
+/* CVX_Link::axialStiffness (src/VX_Link.cpp:259-267): with a Poisson's ratio it refreshes the
+   rest length and transverse info as it reads them */
+float Sim::axialStiffness(int li) {
+    const Mat* m = lmat[li];
+    if (m->nu == 0.0f) return m->a1;
+    updateRestLength(li);
+    updateTransverseInfo(li);
+    return (float)(m->eHat * lf[LTAREA][li] / ((lf[LSTRAIN][li] + 1) * ld[LREST][li]));
 }
 
 /* CVX_Voxel::setTemperature (src/VX_Voxel.cpp:108-114) */
@@ -494,6 +584,7 @@ int Sim::addLink(int x, int y, int z, int dir) {
     for (int c = 0; c < LD_COUNT; c++) ld[c].push_back(0.0);
     for (int c = 0; c < LF_COUNT; c++) lf[c].push_back(0.0f);
     lflags.push_back(0);
+    lratio.push_back(1.0f);
     resetLink(li);
     vlinks[v1][dir] = li;
     updateSurface(v1);
@@ -517,6 +608,7 @@ void Sim::removeLinkAt(int li) {
     for (int c = 0; c < LD_COUNT; c++) ld[c].erase(ld[c].begin() + li);
     for (int c = 0; c < LF_COUNT; c++) lf[c].erase(lf[c].begin() + li);
     lflags.erase(lflags.begin() + li);
+    lratio.erase(lratio.begin() + li);
     for (auto& ls : vlinks)
         for (int d = 0; d < 6; d++)
             if (ls[d] > li) ls[d]--;
@@ -542,6 +634,7 @@ int Sim::addVoxel(Mat* m, int x, int y, int z) {
     vflags.push_back(0);
     vprevdt.push_back(0.0f);
     vtemp.push_back(0.0f);
+    for (int c = 0; c < 3; c++) vps[c].push_back(0.0f);
     resetVoxel(vi);
     vmaps[curRobot][vkey(x, y, z)] = vi;
     vd[VPX][vi] = x * voxSize;
@@ -574,6 +667,7 @@ void Sim::removeVoxel(int x, int y, int z) {
     vflags.erase(vflags.begin() + vi);
     vprevdt.erase(vprevdt.begin() + vi);
     vtemp.erase(vtemp.begin() + vi);
+    for (int c = 0; c < 3; c++) vps[c].erase(vps[c].begin() + vi);
     for (auto& vmap : vmaps)
         for (auto& kv : vmap) if (kv.second > vi) kv.second--;
     for (size_t l = 0; l < lneg.size(); l++) {
@@ -592,6 +686,7 @@ void Sim::replaceVoxel(Mat* m, int x, int y, int z) {
         float fi = m->momentInertia / vmat[vi]->momentInertia;
         vd[VAX][vi] *= fi; vd[VAY][vi] *= fi; vd[VAZ][vi] *= fi;
         vflags[vi] &= ~VF_STATIC;
+        vflags[vi] &= ~(uint32_t)VF_PS_VALID; /* CVX_Voxel::replaceMaterial (src/VX_Voxel.cpp:85) */
         vmat[vi] = m;
     }
     for (int d = 0; d < 6; d++) {
@@ -603,11 +698,17 @@ void Sim::replaceVoxel(Mat* m, int x, int y, int z) {
 
 /* CVoxelyze::recommendedTimeStep (src/Voxelyze.cpp:286-311), Poisson's ratio 0 */
 float Sim::recommendedTimeStep() {
-    if (dtCacheValid) return dtCache;
+    /* with a Poisson's ratio the axial stiffness depends on the state (and refreshes it) */
+    const bool stateful = hasPoissonMaterial();
+    if (stateful) {
+        if (!syncDown()) return 0.0f;
+    } else if (dtCacheValid) {
+        return dtCache;
+    }
     float maxFreq2 = 0.0f;
     for (int l = 0; l < nl(); l++) {
         float m1 = vmat[lneg[l]]->mass, m2 = vmat[lpos[l]]->mass;
-        float f = lmat[l]->a1 / (m1 < m2 ? m1 : m2);
+        float f = axialStiffness(l) / (m1 < m2 ? m1 : m2);
         if (f > maxFreq2) maxFreq2 = f;
     }
     if (maxFreq2 <= 0.0f) {
@@ -618,7 +719,7 @@ float Sim::recommendedTimeStep() {
         }
     }
     dtCache = maxFreq2 <= 0.0f ? 0.0f : 1.0f / (6.283185f * sqrtf(maxFreq2));
-    dtCacheValid = true;
+    dtCacheValid = !stateful;
     return dtCache;
 }
 
@@ -667,7 +768,8 @@ bool Sim::syncUp() {
                 !bvflags.ensure((size_t)vc * 4) || !bvmat.ensure((size_t)vc * 4) || !bvext.ensure((size_t)vc * 4) ||
                 !bvprevdt.ensure((size_t)vc * 4) || !bvtemp.ensure((size_t)vc * 4) || !bvijk.ensure((size_t)3 * vc * 4) ||
                 !blpos.ensure(sc * 4) || !blmeta.ensure(sc * 4) || !bld.ensure((size_t)LDEV_COUNT * sc * 8) ||
-                !blf.ensure((size_t)LF_COUNT * sc * 4) || !blflags.ensure(sc * 4)) {
+                !blf.ensure((size_t)LF_COUNT * sc * 4) || !blflags.ensure(sc * 4) || !bvps.ensure((size_t)3 * vc * 4) ||
+                !blratio.ensure(sc * 4)) {
                 err = "device allocation failed";
                 return false;
             }
@@ -689,6 +791,8 @@ bool Sim::syncUp() {
         dp.ld = (double*)bld.p;
         dp.lf = (float*)blf.p;
         dp.lflags = (uint32_t*)blflags.p;
+        dp.vps = (float*)bvps.p;
+        dp.lratio = (const float*)blratio.p;
         dp.ctl = (int*)bctl.p;
         /* robots */
         const int R = nRobots();
@@ -715,6 +819,7 @@ bool Sim::syncUp() {
             }
         if (!bvnbr.ensure((size_t)6 * std::max(n, 1) * 4)) { err = "device allocation failed"; return false; }
         HIPCHK(up((int*)bvnbr.p, nbr, stream));
+        dp.vnbr = (const int*)bvnbr.p;
         if (fused() && !mixedBuffers()) return false;
         HIPCHK(hipMemsetAsync(dp.vf, 0, (size_t)VF_COUNT * n * 8, stream));
         std::vector<int> tmp((size_t)3 * n);
@@ -752,6 +857,7 @@ bool Sim::syncUp() {
             r.zetaGlobal = m->zetaGlobal;
             r.zetaCollision = m->zetaCollision;
             r.E = m->E;
+            r.nu = m->nu;
             vm.push_back(r);
         }
         std::vector<LinkMat> lm;
@@ -769,6 +875,8 @@ bool Sim::syncUp() {
             r.linear = m->linear ? 1 : 0;
             r.ndata = (int)m->strainData.size();
             r.curveOff = (int)cs.size();
+            r.nu = m->nu;
+            r.eHat = m->eHat;
             cs.insert(cs.end(), m->strainData.begin(), m->strainData.end());
             ss.insert(ss.end(), m->stressData.begin(), m->stressData.end());
             lm.push_back(r);
@@ -822,6 +930,8 @@ bool Sim::syncUp() {
             if (n) HIPCHK(hipMemcpyAsync(dp.vd + (size_t)c * n, vd[c].data(), (size_t)n * 8, hipMemcpyHostToDevice, stream));
         HIPCHK(up(dp.vflags, vflags, stream));
         HIPCHK(up(dp.vprevdt, vprevdt, stream));
+        for (int c = 0; c < 3; c++)
+            if (n) HIPCHK(hipMemcpyAsync(dp.vps + (size_t)c * n, vps[c].data(), (size_t)n * 4, hipMemcpyHostToDevice, stream));
     }
     if (dirty & D_TEMP) {
         HIPCHK(up(dp.vtemp, vtemp, stream));
@@ -852,6 +962,8 @@ bool Sim::syncUp() {
         }
         std::vector<uint32_t> fl = to_slots(lflags, slotOf, S, 0u);
         HIPCHK(up(dp.lflags, fl, stream));
+        stageF.push_back(to_slots(lratio, slotOf, S, 1.0f));
+        if (S) HIPCHK(hipMemcpyAsync((float*)blratio.p, stageF.back().data(), S * 4, hipMemcpyHostToDevice, stream));
         HIPCHK(hipStreamSynchronize(stream));
         if (fused() && !uploadMixedLinks()) return false;
     } else if (dirty & D_REST) {
@@ -871,6 +983,8 @@ bool Sim::syncDown() {
         if (n) HIPCHK(hipMemcpyAsync(vd[c].data(), dp.vd + (size_t)c * n, (size_t)n * 8, hipMemcpyDeviceToHost, stream));
     if (n) HIPCHK(hipMemcpyAsync(vflags.data(), dp.vflags, (size_t)n * 4, hipMemcpyDeviceToHost, stream));
     if (n) HIPCHK(hipMemcpyAsync(vprevdt.data(), dp.vprevdt, (size_t)n * 4, hipMemcpyDeviceToHost, stream));
+    for (int c = 0; c < 3; c++)
+        if (n) HIPCHK(hipMemcpyAsync(vps[c].data(), dp.vps + (size_t)c * n, (size_t)n * 4, hipMemcpyDeviceToHost, stream));
     if (n && actOn) HIPCHK(hipMemcpyAsync(vtemp.data(), dp.vtemp, (size_t)n * 4, hipMemcpyDeviceToHost, stream));
     const size_t S = (size_t)3 * n;
     std::vector<double> bufD((size_t)LREST * S);
@@ -885,7 +999,8 @@ bool Sim::syncDown() {
     for (int l = 0; l < L; l++) {
         const size_t d = slotOf[l];
         for (int c = 0; c < LREST; c++) ld[c][l] = bufD[c * S + d]; /* rest lengths are host-owned */
-        for (int c = 0; c < LF_COUNT; c++) lf[c][l] = bufF[c * S + d];
+        /* the transverse info is host-owned (constant) until a Poisson's ratio appears */
+        for (int c = 0; c < (poisson ? LF_COUNT : LTAREA); c++) lf[c][l] = bufF[c * S + d];
         lflags[l] = bufU[d];
     }
     if (actOn) { /* the temperatures the device set; rest lengths follow them */
@@ -964,9 +1079,17 @@ int Sim::stepRobots(const float* dt, int nsteps, int* done, int* divergedAt) {
         for (int r = 0; r < R; r++) done[r] = 0;
         return 0;
     }
+    if (!poisson && hasPoissonMaterial()) {
+        /* the steps keep the transverse info on the device from now on: start it from the
+           host's (the constant cross-sections) */
+        if (!syncDown()) return -1;
+        poisson = true;
+        dirty |= D_LSTATE;
+    }
+    dp.poisson = poisson ? 1 : 0;
     for (auto& m : vmats) {
-        if (m->nu != 0.0f) {
-            err = "Poisson's ratio != 0 is not supported by the device step";
+        if (m->nu != 0.0f && fused()) {
+            err = "Poisson's ratio != 0 runs in the FP64 step (VXG_FP64_PLAIN)";
             return -1;
         }
         if (modOn && !m->linear) {
@@ -1175,6 +1298,7 @@ bool Sim::stepFp64(int nsteps) {
     }
     for (int k = 0; k < nsteps; k++) {
         if (actOn) launch_actuate_fp64(dp, k, stream);
+        launch_poisson_fp64(dp, k, stream);
         const LaunchCfg lc = tuner[1].settled || !autotune ? cl : tuner[1].next(&candL[k]);
         const LaunchCfg vc = tuner[2].settled || !autotune ? cv : tuner[2].next(&candV[k]);
         if (timed) (void)hipEventRecord(events[4 * k], stream);
@@ -1357,6 +1481,7 @@ bool Sim::stepWithContacts(int nsteps) {
     }
     for (int k = 0; k < nsteps; k++) {
         if (actOn) launch_actuate_fp64(dp, k, stream);
+        launch_poisson_fp64(dp, k, stream);
         launch_links_fp64(dp, k, tuner[1].chosen, stream);
         HIPCHK(hipMemsetAsync(cp.flag, 0, 4, stream));
         launch_col_stale(dp, cp, stream);

@@ -103,6 +103,7 @@ struct Sim {
     std::vector<double> vd[VD_COUNT];
     std::vector<uint32_t> vflags;
     std::vector<float> vprevdt, vtemp;
+    std::vector<float> vps[3];  /* cached Poisson strain (CVX_Voxel::pStrain), valid with VF_PS_VALID */
     std::vector<int> vrobot;  /* robot of each voxel; a robot's voxels are contiguous */
     std::vector<float> vphase; /* actuation phase */
 
@@ -138,6 +139,18 @@ struct Sim {
     std::vector<double> ld[LD_COUNT];
     std::vector<float> lf[LF_COUNT];
     std::vector<uint32_t> lflags;
+    std::vector<float> lratio;  /* CVX_Link::strainRatio, set when the link is reset */
+    /* Poisson's ratio != 0 (src/VX_Voxel.cpp:300-374, src/VX_Link.cpp:142-147): once a material
+       has one, the step keeps the volumetric terms on */
+    bool poisson = false;
+    bool hasPoissonMaterial() const;
+    float linkAxialStrainEnd(int li, bool positiveEnd) const;
+    void voxelStrain(int vi, bool poissons, float out[3]) const;
+    void poissonsStrain(int vi, float out[3]);
+    float transverseArea(int vi, int axis);
+    float transverseStrainSum(int vi, int axis);
+    void updateTransverseInfo(int li);
+    float axialStiffness(int li);
 
     /* host <-> device */
     enum : unsigned { D_TOPO = 1, D_VSTATE = 2, D_LSTATE = 4, D_MATS = 8, D_EXT = 16, D_REST = 32, D_TEMP = 64, D_PHASE = 128,
@@ -148,7 +161,7 @@ struct Sim {
     float dtCache = 0.0f;
     hipStream_t stream = nullptr;
     DevPtrs dp{};
-    DevBuf bvd, bvf, bvflags, bvmat, bvext, bvprevdt, bvtemp, bvijk;
+    DevBuf bvd, bvf, bvflags, bvmat, bvext, bvprevdt, bvtemp, bvijk, bvps, blratio;
     DevBuf blpos, blmeta, bld, blf, blflags;
     DevBuf bvmats, blmats, bcs, bss, bexts, bctl;
     int nvCap = 0, nlCap = 0;

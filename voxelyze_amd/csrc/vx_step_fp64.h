@@ -13,19 +13,97 @@
 
 namespace vx {
 
-/* CVX_Material::stress for Poisson's ratio 0 (src/VX_Material.cpp:165-195) */
+/* CVX_Material::stress(strain, transverseStrainSum, forceLinear) (src/VX_Material.cpp:165-195) */
 __device__ __forceinline__ float link_stress(const LinkMat& m, const float* __restrict__ cs, const float* __restrict__ ss,
-                                             float strain) {
+                                             float strain, float tss, bool forceLinear) {
     if (m.epsilonFail != -1.0f && strain > m.epsilonFail) return 0.0f;
-    if (strain <= m.strain1 || m.linear) return m.E * strain;
+    if (strain <= m.strain1 || m.linear || forceLinear) {
+        if (m.nu == 0.0f) return m.E * strain;
+        return m.eHat * ((1 - m.nu) * strain + m.nu * tss);
+    }
     const int n = m.ndata;
     for (int i = 2; i < n; i++) {
         if (strain <= cs[i] || i == n - 1) {
-            float perc = (strain - cs[i - 1]) / (cs[i] - cs[i - 1]);
-            return ss[i - 1] + perc * (ss[i] - ss[i - 1]);
+            const float perc = (strain - cs[i - 1]) / (cs[i] - cs[i - 1]);
+            const float basic = ss[i - 1] + perc * (ss[i] - ss[i - 1]);
+            if (m.nu == 0.0f) return basic;
+            const float modulus = (ss[i] - ss[i - 1]) / (cs[i] - cs[i - 1]);
+            const float modulusHat = modulus / ((1 - 2 * m.nu) * (1 + m.nu));
+            const float effStrain = basic / modulus;
+            const float effTss = tss * (effStrain / strain);
+            return modulusHat * ((1 - m.nu) * effStrain + m.nu * effTss);
         }
     }
     return 0.0f;
+}
+
+/* CVX_Link::axialStrain(bool positiveEnd) (VX_Link.cpp:121-124) */
+__device__ __forceinline__ float axial_strain_end(float strain, float ratio, bool positiveEnd) {
+    return positiveEnd ? 2.0f * strain * ratio / (1.0f + ratio) : 2.0f * strain / (1.0f + ratio);
+}
+
+/* float pow rounded once from the double pow: the correctly rounded value unless the exact
+   result lies within one double ulp of a float rounding boundary */
+__device__ __forceinline__ float powf_rounded(float x, float y) { return (float)pow((double)x, (double)y); }
+
+/* CVX_Voxel::strain(true) (VX_Voxel.cpp:300-334): voxel i's Poisson strain from its links'
+   current strains (lf, strainRatio per slot), averaged per axis; axes not in tension take
+   (1 + strain of the tensioned axes)^-nu - 1 */
+__device__ __forceinline__ void poisson_strain(const VoxMat& m, const ExtRec* ex, uint32_t flags, const int* __restrict__ vnbr,
+                                               const float* __restrict__ lf, const float* __restrict__ lratio, int nv,
+                                               int nl, int i, float out[3]) {
+    float r[3] = {0.0f, 0.0f, 0.0f};
+    int nb[3] = {0, 0, 0};
+    for (int dir = 0; dir < 6; dir++) {
+        if (!(flags & (VF_LINK0 << dir))) continue;
+        const int axis = dir >> 1;
+        const int slot = axis * nv + ((dir & 1) ? vnbr[dir * nv + i] : i);
+        r[axis] += axial_strain_end(lf[LSTRAIN * nl + slot], lratio[slot], (dir & 1) != 0);
+        nb[axis]++;
+    }
+    bool tension[3];
+    for (int a = 0; a < 3; a++) {
+        if (nb[a] == 2) r[a] *= 0.5f;
+        const float fa = ex ? (a == 0 ? ex->fx : a == 1 ? ex->fy : ex->fz) : 0.0f;
+        tension[a] = nb[a] == 2 || (ex && nb[a] == 1 && ((ex->dof & (1 << a)) || fa != 0));
+    }
+    if (!(tension[0] && tension[1] && tension[2])) {
+        float add = 0;
+        for (int a = 0; a < 3; a++) if (tension[a]) add += r[a];
+        const float value = powf_rounded(1.0f + add, -m.nu) - 1.0f;
+        for (int a = 0; a < 3; a++) if (!tension[a]) r[a] = value;
+    }
+    out[0] = r[0]; out[1] = r[1]; out[2] = r[2];
+}
+
+/* CVX_Voxel::transverseArea / transverseStrainSum (VX_Voxel.cpp:346-374) along one axis */
+__device__ __forceinline__ void voxel_transverse(const VoxMat& m, const float* __restrict__ vps, int nv, int v, int axis,
+                                                 float* area, float* tss) {
+    const float size = (float)m.nomSize;
+    if (m.nu == 0) {
+        *area = size * size;
+        *tss = 0;
+        return;
+    }
+    const float px = vps[v], py = vps[nv + v], pz = vps[2 * nv + v];
+    const float ss = size * size;
+    if (axis == 0) { *area = (float)(ss * (1 + (double)py) * (1 + (double)pz)); *tss = py + pz; }
+    else if (axis == 1) { *area = (float)(ss * (1 + (double)px) * (1 + (double)pz)); *tss = px + pz; }
+    else { *area = (float)(ss * (1 + (double)px) * (1 + (double)py)); *tss = px + py; }
+}
+
+/* the transverse info a link uses this step: refreshed from its voxels when its material
+   has a Poisson's ratio or its stored strain sum is non-zero (VX_Link.cpp:142-147, 160) */
+__device__ __forceinline__ void link_transverse(const LinkMat& m, const VoxMat& mn, const VoxMat& mp,
+                                                const float* __restrict__ vps, int nv, int n, int p, int axis,
+                                                float* tArea, float* tss) {
+    if (m.nu != 0.0f || *tss != 0) {
+        float an, tn, ap, tp;
+        voxel_transverse(mn, vps, nv, n, axis, &an, &tn);
+        voxel_transverse(mp, vps, nv, p, axis, &ap, &tp);
+        *tArea = 0.5f * (an + ap);
+        *tss = 0.5f * (tn + tp);
+    }
 }
 
 /* previous state of one link (local pose, flags, strain history) */
@@ -33,6 +111,7 @@ struct LinkPrev {
     dvec pos2, a1, a2;
     uint32_t flags;
     float maxStrain, strainOffset;
+    float tArea, tss;  /* currentTransverseArea / StrainSum this step's forces use */
 };
 /* what one link step produces: both end vectors in their voxels' frames and the new state */
 struct LinkEval {
@@ -82,18 +161,21 @@ __device__ __forceinline__ void link_eval_fp64(const LinkMat& m, const float* __
     const dvec dA1 = dv(0.5 * (a1v.x - old.a1.x), 0.5 * (a1v.y - old.a1.y), 0.5 * (a1v.z - old.a1.z));
     const dvec dA2 = dv(0.5 * (a2v.x - old.a2.x), 0.5 * (a2v.y - old.a2.y), 0.5 * (a2v.z - old.a2.z));
 
-    /* updateStrain (VX_Link.cpp:220-249), Poisson's ratio 0 */
+    /* updateStrain (VX_Link.cpp:220-249) */
     const float axialStrain = (float)(pos2.x / rest);
     float stress;
     if (m.linear) {
         if (axialStrain > maxStrain) maxStrain = axialStrain;
-        stress = link_stress(m, cs, ss, axialStrain);
+        stress = link_stress(m, cs, ss, axialStrain, old.tss, false);
     } else if (axialStrain > maxStrain) {
         maxStrain = axialStrain;
-        stress = link_stress(m, cs, ss, axialStrain);
-        strainOffset = maxStrain - stress / m.E;
+        stress = link_stress(m, cs, ss, axialStrain, old.tss, false);
+        if (m.nu != 0.0f) strainOffset = maxStrain - link_stress(m, cs, ss, axialStrain, 0.0f, false) / (m.eHat * (1 - m.nu));
+        else strainOffset = maxStrain - stress / m.E;
     } else {
-        stress = m.E * (axialStrain - strainOffset);
+        const float rel = axialStrain - strainOffset;
+        if (m.nu != 0.0f) stress = link_stress(m, cs, ss, rel, old.tss, true);
+        else stress = m.E * rel;
     }
 
     dvec fN, fP, mN, mP;
@@ -102,7 +184,7 @@ __device__ __forceinline__ void link_eval_fp64(const LinkMat& m, const float* __
         fN = fP = mN = mP = dv(0.0, 0.0, 0.0);
     } else {
         const float b1 = m.b1, b2 = m.b2, b3 = m.b3, am2 = m.a2;
-        fN = dv(stress * m.transverseArea,
+        fN = dv(stress * old.tArea,
                 b1 * pos2.y - b2 * (a1v.z + a2v.z),
                 b1 * pos2.z + b2 * (a1v.y + a2v.y));
         fP = dv(-fN.x, -fN.y, -fN.z);

@@ -251,6 +251,15 @@ public:
     CVoxelyze& operator=(const CVoxelyze&) = delete;
 
     bool doTimeStep(float dt = -1.0f) { return vxg_do_time_step(sim, dt) != 0; }
+    bool loadJSON(const char* jsonFilePath) { /* Voxelyze.h:77: the model is replaced */
+        const bool ok = vxg_load_json(sim, jsonFilePath) != 0;
+        voxels.clear();
+        links.clear();
+        mats.clear();
+        for (int i = 0; i < vxg_material_count(sim); i++) mats.emplace_back(new CVX_Material(&sim, i));
+        return ok;
+    }
+    bool saveJSON(const char* jsonFilePath) { return vxg_save_json(sim, jsonFilePath) != 0; }  /* Voxelyze.h:78 */
     float recommendedTimeStep() const { return vxg_recommended_time_step(sim); }
     void resetTime() { vxg_reset_time(sim); }
 

@@ -129,6 +129,17 @@ float vxg_current_time(vxg_sim* s);
 float vxg_state_info(vxg_sim* s, int info, int type);
 /* number of watched collision pairs: CVoxelyze::collisionList()->size()  Voxelyze.h:115 */
 int vxg_collision_count(vxg_sim* s);
+/* CVoxelyze::loadJSON(const char*) / saveJSON(const char*)   include/Voxelyze.h:77-78,
+   src/Voxelyze.cpp:61-241; materials CVX_Material::readJSON/writeJSON src/VX_Material.cpp:75-163.
+   Load clears the simulation first (keeping the voxel size and the precision/path/graph/
+   autotune settings) and, like the reference, does not apply the environment keys the
+   writer emits.  Returns 1 on success, 0 if the file cannot be opened, read or parsed
+   (vxg_last_error says why).  Differences from the reference, on purpose: the written
+   document is always closed (the reference closes it only when externals exist), the
+   stress/strain point lists are read element by element, and a document left open at the
+   end of the text (as the reference writes it) is accepted. */
+int vxg_load_json(vxg_sim* s, const char* path);
+int vxg_save_json(vxg_sim* s, const char* path);
 /* precision mode of the device step (no reference counterpart: the reference is FP64/FP32
    mixed exactly as VXG_FP64_PLAIN reproduces) */
 int vxg_set_precision(vxg_sim* s, int mode);

@@ -297,6 +297,8 @@ EXPORT int vxr_link_info(vxr_sim* s, int li, int* neg, int* pos) {
     if (pos) *pos = voxIndex(s, l->voxel(true));
     return CVXS_SimGLView::axis(l);
 }
+EXPORT int vxr_save_json(vxr_sim* s, const char* path) { return s->sim.saveJSON(path) ? 1 : 0; }
+EXPORT int vxr_load_json(vxr_sim* s, const char* path) { return s->sim.loadJSON(path) ? 1 : 0; }
 EXPORT void vxr_link_get_transverse(vxr_sim* s, int first, int count, float* area, float* strainSum) {
     for (int k = 0; k < count; k++) {
         int li = first + k;

@@ -840,6 +840,66 @@ def deformable_material_poisson(make, load=300, ramp=200, relax=400):
     return poisson_out(sim, trace)
 
 
+# ----------------------------------------------------------------- .vxl.json models
+def json_model(make, name):
+    """Models written to .vxl.json by the reference (tests/golden/make_json_golden.py)."""
+    if name == "json_cantilever":
+        sim = make(0.005)
+        from voxelyze_amd import workloads as W
+        W.cantilever(sim)
+        return sim
+    sim = make(0.002)
+    lin = sim.addMaterial(2e6, 1200)
+    lin.setPoissonsRatio(0.25)
+    lin.setInternalDamping(0.8)
+    lin.setGlobalDamping(0.01)
+    lin.setStaticFriction(0.9)
+    lin.setKineticFriction(0.4)
+    lin.setCollisionDamping(0.5)
+    bil = sim.addMaterial(5e6, 900)
+    bil.setModelBilinear(5e6, 1e6, 5e4, 2e5)
+    bil.setCte(0.002)
+    soft = sim.addMaterial(3e5, 1000)
+    soft.setModelLinear(3e5, 3e4)
+    soft.setExternalScaleFactor((1.0, 1.1, 0.9))
+    mats = [lin, bil, soft]
+    for z in range(3):
+        for y in range(3):
+            for x in range(6):
+                if (x + 2 * y + z) % 7 == 3:
+                    continue
+                v = sim.setVoxel(mats[(x + y + z) % 3], x, y, z + 1)
+                if name == "json_open":
+                    continue
+                if x == 0:
+                    v.external().setFixedAll()
+                elif x == 5 and y == 1:
+                    v.external().setForce(0.0, 0.02, -0.05)
+                    v.external().setMoment(1e-5, 0.0, 0.0)
+                elif x == 5 and y == 0:
+                    v.external().setFixed(False, False, True, True, False, False)
+                    v.external().setDisplacement(4, 1e-4)
+                    v.external().setDisplacement(8, 0.01)
+    return sim
+
+
+JSON_CASES = {"json_cantilever": (300, False), "json_mixed": (300, True), "json_open": (300, True)}
+
+
+def json_run(sim, name):
+    """The environment (which a .vxl.json load does not apply), then the steps."""
+    steps, env = JSON_CASES[name]
+    if env:
+        sim.setGravity(1.0)
+        sim.enableFloor(True)
+    dt = sim.recommendedTimeStep()
+    for _ in range(steps // 100):
+        sim.step(dt, 100)
+    out = snapshot(sim)
+    out["dt"] = np.float32(dt)
+    return out
+
+
 SCENARIOS = {
     "two_voxel": None,  # expanded per case below
     "internal_damping": lambda mk: surrounded(mk, 1.0, 0.0),

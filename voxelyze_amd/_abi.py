@@ -99,6 +99,8 @@ EXTENSIONS = {
     "autotune": (_I, [_P, _I]),
     "launch_config": (_I, [_P, _I, _IP, _IP, _IP]),
     "set_fp64_path": (_I, [_P, _I]),
+    "load_json": (_I, [_P, C.c_char_p]),
+    "save_json": (_I, [_P, C.c_char_p]),
     "fp64_path": (_I, [_P]),
 }
 

@@ -438,6 +438,14 @@ class CVoxelyze:
     def setPrecision(self, mode):
         return self._check(bool(self.abi.set_precision(self.handle, mode)))
 
+    def loadJSON(self, path):
+        """CVoxelyze::loadJSON (Voxelyze.cpp:61-76): clears the model, then reads a .vxl.json file."""
+        return self._check(bool(self.abi.load_json(self.handle, str(path).encode())))
+
+    def saveJSON(self, path):
+        """CVoxelyze::saveJSON (Voxelyze.cpp:78-94)."""
+        return self._check(bool(self.abi.save_json(self.handle, str(path).encode())))
+
     def setFp64Path(self, path):
         """How the FP64 step runs: 0 fused (one kernel per step, default), 1 split (a link and a
         voxel kernel per step).  Both give the same bits (include/vxg.h, vxg_set_fp64_path)."""

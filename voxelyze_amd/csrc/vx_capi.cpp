@@ -670,3 +670,258 @@ API void vxg_link_get_state(vxg_sim* h, int first, int count, float* strain, flo
         if (rest) rest[k] = s->ld[LREST][l];
     }
 }
+
+/* ------------------------------------------------------------------ .vxl.json */
+#include <fstream>
+#include <new>
+#include <sstream>
+
+#include "vx_json.h"
+
+/* CVX_Material::readJSON (src/VX_Material.cpp:121-163) into a new voxel material of the
+   simulation (CVoxelyze::addMaterial(rapidjson::Value&), src/Voxelyze.cpp:368-374): the
+   model from youngsModulus (+ epsilonFail) or the strain/stress point lists, then the other
+   properties as read (no clamping), then the derived constants */
+static void json_material(Sim* s, const vx::JValue& j) {
+    const int mi = s->addMaterial(1e6f, 1e3f);
+    vx::Mat* m = s->vmats[mi].get();
+    const double nom = m->nomSize;
+    m->clear();
+    const vx::JValue* E = j.get("youngsModulus");
+    const vx::JValue* st = j.get("strainData");
+    const vx::JValue* ss = j.get("stressData");
+    if (E && E->isNum()) {
+        float failStress = -1.0f;
+        const vx::JValue* ef = j.get("epsilonFail");
+        if (ef && ef->isNum()) failStress = (float)(ef->num * E->num);
+        m->setModelLinear((float)E->num, failStress);
+    } else if (st && ss && st->isArr() && ss->isArr() && st->arr.size() == ss->arr.size()) {
+        std::vector<float> a, b;
+        for (size_t i = 0; i < st->arr.size(); i++) {
+            a.push_back((float)st->arr[i].num);
+            b.push_back((float)ss->arr[i].num);
+        }
+        m->setModel((int)a.size(), a.data(), b.data());
+    }
+    auto num = [&](const char* k, float* dst) {
+        const vx::JValue* v = j.get(k);
+        if (v && v->isNum()) *dst = (float)v->num;
+    };
+    num("density", &m->rho);
+    num("poissonsRatio", &m->nu);
+    num("CTE", &m->alphaCTE);
+    num("staticFriction", &m->muStatic);
+    num("kineticFriction", &m->muKinetic);
+    num("internalDamping", &m->zetaInternal);
+    num("globalDamping", &m->zetaGlobal);
+    num("collisionDamping", &m->zetaCollision);
+    const vx::JValue* xs = j.get("externalScaleFactor");
+    if (xs && xs->isArr(3))
+        for (int i = 0; i < 3; i++) m->extScale[i] = xs->arr[i].num;
+    m->nomSize = nom;
+    m->gravMult = 0.0f;
+    m->updateDerived();
+    m->gravMult = s->grav;
+    s->materialsChanged();
+}
+
+/* CVoxelyze::loadJSON / readJSON (src/Voxelyze.cpp:61-76, 96-168).  As in the reference the
+   simulation is cleared first (everything but the voxel size and this library's settings),
+   and the environment keys the writer emits are not applied on reading. */
+API int vxg_load_json(vxg_sim* h, const char* path) {
+    std::ifstream f(path ? path : "");
+    if (!f) {
+        S(h)->err = std::string("cannot open ") + (path ? path : "(null)");
+        return 0;
+    }
+    std::stringstream buf;
+    buf << f.rdbuf();
+    const std::string text = buf.str();
+    vx::JValue doc;
+    std::string err;
+    if (!vx::JReader(text).parse(&doc, &err)) {
+        S(h)->err = err;
+        return 0;
+    }
+    /* clear(): a fresh simulation in place, keeping this library's settings */
+    {
+        Sim* s = S(h);
+        const int precision = s->precision;
+        const bool split = s->fp64Split, graphs = s->useGraphs, tune = s->autotune;
+        const double size = s->voxSize;
+        s->~Sim();
+        new (s) Sim();
+        s->precision = precision;
+        s->fp64Split = split;
+        s->useGraphs = graphs;
+        s->autotune = tune;
+        s->voxSize = size;
+    }
+    Sim* s = S(h);
+    if (doc.type != vx::JValue::Object) return 1;
+    const vx::JValue* vs = doc.get("voxelSize");
+    if (!vs || !vs->isNum()) return 1;
+    s->voxSize = vs->num;
+    const vx::JValue* mats = doc.get("materials");
+    if (!mats || !mats->isArr()) return 1;
+    for (auto& m : mats->arr) json_material(s, m);
+    const vx::JValue* vox = doc.get("voxels");
+    if (vox && vox->isArr() && vox->arr.size() % 4 == 0) {
+        for (size_t i = 0; i < vox->arr.size() / 4; i++) {
+            const int mi = (int)vox->arr[4 * i + 3].num;
+            vx::Mat* m = M(s, mi);
+            if (!m) continue;
+            s->addVoxel(m, (int)vox->arr[4 * i].num, (int)vox->arr[4 * i + 1].num, (int)vox->arr[4 * i + 2].num);
+        }
+    }
+    const vx::JValue* exts = doc.get("externals");
+    if (exts && exts->isArr()) {
+        for (auto& e : exts->arr) {
+            const vx::JValue* idx = e.get("voxelIndices");
+            if (!idx || !idx->isArr()) continue;
+            bool dof[6] = {false, false, false, false, false, false};
+            double disp[6] = {0, 0, 0, 0, 0, 0};
+            float force[3] = {0, 0, 0}, moment[3] = {0, 0, 0};
+            const vx::JValue* fx = e.get("fixed");
+            if (fx && fx->isArr(6)) for (int k = 0; k < 6; k++) dof[k] = fx->arr[k].type == vx::JValue::Bool && fx->arr[k].b;
+            const vx::JValue* tr = e.get("translate");
+            if (tr && tr->isArr(3)) for (int k = 0; k < 3; k++) disp[k] = tr->arr[k].num;
+            const vx::JValue* ro = e.get("rotate");
+            if (ro && ro->isArr(3)) for (int k = 0; k < 3; k++) disp[3 + k] = ro->arr[k].num;
+            const vx::JValue* fo = e.get("force");
+            if (fo && fo->isArr(3)) for (int k = 0; k < 3; k++) force[k] = (float)fo->arr[k].num;
+            const vx::JValue* mo = e.get("moment");
+            if (mo && mo->isArr(3)) for (int k = 0; k < 3; k++) moment[k] = (float)mo->arr[k].num;
+            for (auto& iv : idx->arr) {
+                const int vi = (int)iv.num;
+                if (!V(s, vi)) continue;
+                for (int k = 0; k < 6; k++)
+                    if (dof[k]) vxg_ext_set_displacement(h, vi, 1 << k, disp[k]);
+                vx::Ext* x = E(h, vi);
+                for (int k = 0; k < 3; k++) { x->force[k] += force[k]; x->moment[k] += moment[k]; }
+            }
+        }
+    }
+    return 1;
+}
+
+static bool ext_equal(const vx::Ext& a, const vx::Ext& b) {
+    for (int k = 0; k < 3; k++)
+        if (a.force[k] != b.force[k] || a.moment[k] != b.moment[k] || a.translation[k] != b.translation[k] ||
+            a.rotation[k] != b.rotation[k])
+            return false;
+    return a.dof == b.dof;
+}
+
+/* CVoxelyze::saveJSON / writeJSON (src/Voxelyze.cpp:78-94, 170-241) with
+   CVX_Material::writeJSON (src/VX_Material.cpp:75-119); the document is always closed */
+API int vxg_save_json(vxg_sim* h, const char* path) {
+    Sim* s = S(h);
+    if (!s->syncDown()) return 0;
+    vx::JWriter w;
+    w.beginObject();
+    w.key("voxelSize"); w.number(s->voxSize);
+    if (s->ambientTemp != 0) { w.key("relativeAmbientTemperature"); w.number((double)s->ambientTemp); }
+    if (s->grav != 0) { w.key("gravityAcceleration"); w.number((double)s->grav); }
+    if (s->floor) { w.key("floorEnabled"); w.boolean(true); }
+    if (s->collisions) { w.key("collisionsEnabled"); w.boolean(true); }
+    w.key("materials");
+    w.beginArray();
+    for (auto& mp : s->vmats) {
+        const vx::Mat* m = mp.get();
+        w.beginObject();
+        if (m->linear) {
+            w.key("youngsModulus"); w.number((double)m->E);
+            if (m->epsilonFail != -1) { w.key("epsilonFail"); w.number((double)m->epsilonFail); }
+        } else {
+            w.key("strainData");
+            w.beginArray();
+            for (float v : m->strainData) w.number((double)v);
+            w.endArray();
+            w.key("stressData");
+            w.beginArray();
+            for (float v : m->stressData) w.number((double)v);
+            w.endArray();
+        }
+        if (m->rho != 1.0f) { w.key("density"); w.number(m->rho); }
+        if (m->nu != 0) { w.key("poissonsRatio"); w.number(m->nu); }
+        if (m->alphaCTE != 0) { w.key("CTE"); w.number(m->alphaCTE); }
+        if (m->muStatic != 0) { w.key("staticFriction"); w.number(m->muStatic); }
+        if (m->muKinetic != 0) { w.key("kineticFriction"); w.number(m->muKinetic); }
+        if (m->zetaInternal != 1) { w.key("internalDamping"); w.number(m->zetaInternal); }
+        if (m->zetaGlobal != 0) { w.key("globalDamping"); w.number(m->zetaGlobal); }
+        if (m->zetaCollision != 1) { w.key("collisionDamping"); w.number(m->zetaCollision); }
+        if (m->extScale[0] != 1 || m->extScale[1] != 1 || m->extScale[2] != 1) {
+            w.key("externalScaleFactor");
+            w.beginArray();
+            for (int i = 0; i < 3; i++) w.number(m->extScale[i]);
+            w.endArray();
+        }
+        w.endObject();
+    }
+    w.endArray();
+    std::vector<const vx::Ext*> exts;
+    std::vector<std::vector<int>> extVox;
+    w.key("voxels");
+    w.beginArray();
+    for (int i = 0; i < s->nv(); i++) {
+        w.integer(s->vijk[i][0]);
+        w.integer(s->vijk[i][1]);
+        w.integer(s->vijk[i][2]);
+        int mi = 0;
+        for (size_t k = 0; k < s->vmats.size(); k++)
+            if (s->vmats[k].get() == s->vmat[i]) mi = (int)k;
+        w.integer(mi);
+        const vx::Ext* e = s->vext[i].get();
+        const bool empty = !e || (e->dof == 0 && e->force[0] == 0 && e->force[1] == 0 && e->force[2] == 0 &&
+                                  e->moment[0] == 0 && e->moment[1] == 0 && e->moment[2] == 0);
+        if (empty) continue;
+        size_t j = 0;
+        while (j < exts.size() && !ext_equal(*e, *exts[j])) j++;
+        if (j == exts.size()) {
+            exts.push_back(e);
+            extVox.emplace_back();
+        }
+        extVox[j].push_back(i);
+    }
+    w.endArray();
+    if (!exts.empty()) {
+        w.key("externals");
+        w.beginArray();
+        for (size_t j = 0; j < exts.size(); j++) {
+            const vx::Ext* e = exts[j];
+            w.beginObject();
+            const bool anyT = (e->dof & 7) != 0, anyR = (e->dof & 56) != 0;
+            const bool allT = (e->dof & 7) == 7, allR = (e->dof & 56) == 56;
+            if (e->dof != 0) {
+                w.key("fixed");
+                w.beginArray();
+                for (int k = 0; k < 6; k++) w.boolean((e->dof >> k) & 1);
+                w.endArray();
+            }
+            const bool tZero = e->translation[0] == 0 && e->translation[1] == 0 && e->translation[2] == 0;
+            const bool rZero = e->rotation[0] == 0 && e->rotation[1] == 0 && e->rotation[2] == 0;
+            const bool fZero = e->force[0] == 0 && e->force[1] == 0 && e->force[2] == 0;
+            const bool mZero = e->moment[0] == 0 && e->moment[1] == 0 && e->moment[2] == 0;
+            if (anyT && !tZero) { w.key("translate"); w.beginArray(); for (int k = 0; k < 3; k++) w.number(e->translation[k]); w.endArray(); }
+            if (anyR && !rZero) { w.key("rotate"); w.beginArray(); for (int k = 0; k < 3; k++) w.number(e->rotation[k]); w.endArray(); }
+            if (!allT && !fZero) { w.key("force"); w.beginArray(); for (int k = 0; k < 3; k++) w.number(e->force[k]); w.endArray(); }
+            if (!allR && !mZero) { w.key("moment"); w.beginArray(); for (int k = 0; k < 3; k++) w.number(e->moment[k]); w.endArray(); }
+            w.key("voxelIndices");
+            w.beginArray();
+            for (int v : extVox[j]) w.integer(v);
+            w.endArray();
+            w.endObject();
+        }
+        w.endArray();
+    }
+    w.endObject();
+    w.out += '\n';
+    std::ofstream f(path ? path : "");
+    if (!f) {
+        s->err = std::string("cannot write ") + (path ? path : "(null)");
+        return 0;
+    }
+    f << w.out;
+    return f.good() ? 1 : 0;
+}

@@ -290,6 +290,22 @@ float vxg_robot_time(vxg_sim* s, int robot);  /* the robot's currentTime */
 int vxg_robot_step(vxg_sim* s, int nsteps, const float* dt, int* done);
 /* centre of mass of every robot (3 doubles per robot, mass-weighted, FP64) */
 int vxg_robot_com(vxg_sim* s, double* com);
+/* Checkpoints (no reference counterpart): the dynamic state -- voxel pose, momenta, flags,
+   temperatures, previousDt and Poisson strains, link state and end forces, robot clocks, the
+   self-collision watch list -- copied on the device into slot 0..7 and back.  Stepping again
+   from a restored checkpoint gives the same bits.  Materials, externals and topology are not
+   part of it: restore keeps the current ones and fails if the topology or the precision mode
+   changed since the save.  Return 1, or 0 with vxg_last_error set. */
+int vxg_checkpoint_save(vxg_sim* s, int slot);
+int vxg_checkpoint_restore(vxg_sim* s, int slot);
+void vxg_checkpoint_free(vxg_sim* s, int slot);
+/* CVoxelyze::stateInfo(info, type) (src/Voxelyze.cpp:752-800) of every robot, reduced on the
+   device (no reference counterpart for the per-robot split): out[r] for r < vxg_robot_count.
+   Each element's value is the reference's float expression; TOTAL and AVERAGE are summed in
+   FP64 in a fixed order (the reference sums in float in list order), MIN and MAX are exact.
+   Link-based infos (ENG_STRESS, ENG_STRAIN, STRAIN_ENERGY) and PRESSURE need the FP64 step.
+   Returns 1, or 0 with vxg_last_error set. */
+int vxg_robot_state_info(vxg_sim* s, int info, int type, double* out);
 
 /* ----------------------------------------------------------------- actuation
    No counterpart in the reference (its users set temperatures from the host between

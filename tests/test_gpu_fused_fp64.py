@@ -178,3 +178,34 @@ def test_poisson_runs_in_the_fp64_step_only():
     sim.setPrecision(1)
     with pytest.raises(RuntimeError):
         sim.step(sim.recommendedTimeStep(), 5)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision,path,collisions", [(0, FUSED, False), (0, SPLIT, False), (1, FUSED, False),
+                                                        (0, FUSED, True), (1, FUSED, True)],
+                         ids=["fp64-fused", "fp64-split", "mixed", "fp64-contacts", "mixed-contacts"])
+def test_checkpoint_restore_replays_the_same_steps(precision, path, collisions):
+    """Save the dynamic state, step, restore, step again: the same bits, the same clocks."""
+    sim = robots(path, collisions=collisions)
+    sim.setPrecision(precision)
+    sim.robotStep(60)
+    assert sim.checkpointSave(3)
+    sim.robotStep(45)
+    first, t1 = S.snapshot(sim), [sim.robotTime(r) for r in range(sim.robotCount())]
+    assert sim.checkpointRestore(3)
+    sim.robotStep(45)
+    second, t2 = S.snapshot(sim), [sim.robotTime(r) for r in range(sim.robotCount())]
+    bad = [k for k in first if not bits_equal(first[k], second[k])]
+    assert not bad and t1 == t2, bad[:8]
+
+
+@pytest.mark.gpu
+def test_checkpoint_refuses_another_topology():
+    sim = B.maker(B.product_abi())(0.005)
+    W.cantilever(sim)
+    sim.step(sim.recommendedTimeStep(), 10)
+    assert sim.checkpointSave(0)
+    m = sim.addMaterial(1e6, 1e3)
+    sim.setVoxel(m, 30, 0, 0)
+    with pytest.raises(RuntimeError):
+        sim.checkpointRestore(0)

@@ -223,3 +223,37 @@ def test_robots_with_self_collision_match_robots_alone():
         f, c = sim.robotVoxels(r)
         assert pos[f:f + c].tobytes() == pr.tobytes()
     assert sim.collisionCount() == 2 * ref.collisionCount()
+
+
+@pytest.mark.gpu
+def test_robot_state_info_matches_state_info():
+    """Per-robot stateInfo reduced on the device: for a single robot, MIN and MAX equal the
+    reference's stateInfo exactly and TOTAL/AVERAGE to float rounding (the reference sums in
+    float, the device in FP64); over several robots the totals add up to the whole model's."""
+    from voxelyze_amd import workloads as W
+    sim = B.maker(B.product_abi())(0.005)
+    W.cantilever(sim)
+    sim.step(sim.recommendedTimeStep(), 500)
+    ls = sim.link_state()
+    signed = {5: ls["stress"].astype(np.float64), 6: ls["strain"].astype(np.float64)}
+    for info in range(10):
+        for typ in range(4):
+            dev = sim.robotStateInfo(info, typ)[0]
+            host = sim.stateInfo(info, typ)
+            if typ < 2:
+                assert np.float32(dev) == np.float32(host), (info, typ)
+            elif info in signed:  # sums with cancellation: against the FP64 sum of the values
+                v = signed[info]
+                exact = v.sum() / (len(v) if typ == 3 else 1)
+                assert dev == pytest.approx(exact, rel=1e-12, abs=1e-30), (info, typ)
+            elif info != 8:
+                assert dev == pytest.approx(host, rel=1e-5, abs=1e-30), (info, typ)
+    shp = shapes(n=4, seed=81)
+    many = batch(shp, 0)
+    many.robotStep(200)
+    for info in (0, 2, 5, 6, 7, 9):
+        per = many.robotStateInfo(info, 2)
+        assert per.sum() == pytest.approx(many.stateInfo(info, 2), rel=1e-5, abs=1e-30), info
+    rho, L = 1000.0, 0.005
+    counts = np.array([many.robotVoxels(r)[1] for r in range(many.robotCount())])
+    assert np.array_equal(many.robotStateInfo(9, 2), counts * float(np.float32(L * L * L * rho)))

@@ -100,6 +100,10 @@ EXTENSIONS = {
     "launch_config": (_I, [_P, _I, _IP, _IP, _IP]),
     "set_fp64_path": (_I, [_P, _I]),
     "load_json": (_I, [_P, C.c_char_p]),
+    "robot_state_info": (_I, [_P, _I, _I, _DP]),
+    "checkpoint_save": (_I, [_P, _I]),
+    "checkpoint_restore": (_I, [_P, _I]),
+    "checkpoint_free": (None, [_P, _I]),
     "save_json": (_I, [_P, C.c_char_p]),
     "fp64_path": (_I, [_P]),
 }

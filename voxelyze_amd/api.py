@@ -438,6 +438,19 @@ class CVoxelyze:
     def setPrecision(self, mode):
         return self._check(bool(self.abi.set_precision(self.handle, mode)))
 
+    def checkpointSave(self, slot=0):
+        """Copy the dynamic state to checkpoint slot 0..7 on the device (include/vxg.h)."""
+        return self._check(bool(self.abi.checkpoint_save(self.handle, int(slot))))
+
+    def checkpointRestore(self, slot=0):
+        return self._check(bool(self.abi.checkpoint_restore(self.handle, int(slot))))
+
+    def robotStateInfo(self, info, type_):
+        """stateInfo(info, type) of every robot, reduced on the device (include/vxg.h)."""
+        out = np.zeros(self.robotCount())
+        self._check(bool(self.abi.robot_state_info(self.handle, int(info), int(type_), _dptr(out))))
+        return out
+
     def loadJSON(self, path):
         """CVoxelyze::loadJSON (Voxelyze.cpp:61-76): clears the model, then reads a .vxl.json file."""
         return self._check(bool(self.abi.load_json(self.handle, str(path).encode())))

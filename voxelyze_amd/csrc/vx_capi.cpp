@@ -180,7 +180,7 @@ API float vxg_state_info(vxg_sim* h, int info, int type) {
                 for (int d = 0; d < 6; d++) {
                     int l = s->vlinks[i][d];
                     if (l < 0) continue;
-                    float ratio = s->vmat[s->lpos[l]]->E / s->vmat[s->lneg[l]]->E;
+                    float ratio = s->lratio[l]; /* CVX_Link::strainRatio, set at the link's reset */
                     float strain = s->lf[LSTRAIN][l];
                     st[d / 2] += (d % 2 == 1) ? 2.0f * strain * ratio / (1.0f + ratio) : 2.0f * strain / (1.0f + ratio);
                     nb[d / 2]++;
@@ -635,6 +635,12 @@ API int vxg_link_info(vxg_sim* h, int li, int* neg, int* pos) {
     if (neg) *neg = s->lneg[li];
     if (pos) *pos = s->lpos[li];
     return s->laxis[li];
+}
+API int vxg_checkpoint_save(vxg_sim* h, int slot) { return S(h)->checkpointSave(slot) ? 1 : 0; }
+API int vxg_checkpoint_restore(vxg_sim* h, int slot) { return S(h)->checkpointRestore(slot) ? 1 : 0; }
+API void vxg_checkpoint_free(vxg_sim* h, int slot) { S(h)->checkpointFree(slot); }
+API int vxg_robot_state_info(vxg_sim* h, int info, int type, double* out) {
+    return S(h)->robotStateInfo(info, type, out) ? 1 : 0;
 }
 API void vxg_link_get_transverse(vxg_sim* h, int first, int count, float* area, float* strainSum) {
     Sim* s = S(h);

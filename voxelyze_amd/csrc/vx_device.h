@@ -76,6 +76,7 @@ struct LinkMat {
     int ndata;                /* points in the stress/strain curve */
     int curveOff;             /* offset of the curve in the curve arrays */
     float nu, eHat;           /* Poisson's ratio and E/((1-2nu)(1+nu)) of the link material */
+    float a1;                 /* axial stiffness E*L */
 };
 
 /* per external record (VX_External.h:93-98) */

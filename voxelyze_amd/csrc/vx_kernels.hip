@@ -261,6 +261,101 @@ __global__ void __launch_bounds__(256) k_robot_com(DevPtrs s, const int* rfirst,
     }
 }
 
+/* CVoxelyze::stateInfo (src/Voxelyze.cpp:752-800) for every robot on the device: each
+   voxel's or link's value in the reference's own float expression (CVX_Voxel::
+   displacementMagnitude ... pressure, CVX_Link::axialStress/axialStrain/strainEnergy,
+   src/VX_Link.cpp:251-257), reduced over the robot in a fixed order -- sums in FP64, minima and
+   maxima exactly.  One workgroup per robot; the links of a robot are the slots of its voxels. */
+__device__ float voxel_info_value(const DevPtrs& s, int i, int info) {
+    const int nv = s.nv;
+    const VoxMat& m = s.vmats[s.vmat[i]];
+    const double* vd = s.vd;
+    const double lx = vd[VLX * nv + i], ly = vd[VLY * nv + i], lz = vd[VLZ * nv + i];
+    const double ax = vd[VAX * nv + i], ay = vd[VAY * nv + i], az = vd[VAZ * nv + i];
+    switch (info) {
+    case 0: {
+        const double sz = m.nomSize;
+        const double dx = vd[VPX * nv + i] - s.vijk[i] * sz, dy = vd[VPY * nv + i] - s.vijk[nv + i] * sz,
+                     dz = vd[VPZ * nv + i] - s.vijk[2 * nv + i] * sz;
+        return (float)sqrt(dx * dx + dy * dy + dz * dz);
+    }
+    case 1: return (float)(sqrt(lx * lx + ly * ly + lz * lz) * m.massInverse);
+    case 2: return (float)(0.5 * (m.massInverse * (lx * lx + ly * ly + lz * lz) + m.momentInertiaInverse * (ax * ax + ay * ay + az * az)));
+    case 3: { const double w = vd[VQW * nv + i]; return (float)(2.0 * acos(w > 1 ? 1 : w)); }
+    case 4: return (float)(sqrt(ax * ax + ay * ay + az * az) * m.momentInertiaInverse);
+    case 8: { /* pressure: -E * (volumetric strain) / (3 (1 - 2 nu)) from CVX_Voxel::strain(false) */
+        float st[3] = {0, 0, 0};
+        int nb[3] = {0, 0, 0};
+        const uint32_t fl = s.vflags[i];
+        for (int d = 0; d < 6; d++) {
+            if (!(fl & (VF_LINK0 << d))) continue;
+            const int a = d >> 1;
+            const int slot = a * nv + ((d & 1) ? s.vnbr[d * nv + i] : i);
+            st[a] += axial_strain_end(s.lf[LSTRAIN * s.nl + slot], s.lratio[slot], (d & 1) != 0);
+            nb[a]++;
+        }
+        for (int a = 0; a < 3; a++) if (nb[a] == 2) st[a] *= 0.5f;
+        const float vol = (float)(st[0] + st[1] + st[2]);
+        return -m.E * vol / (3 * (1 - 2 * m.nu));
+    }
+    case 9: return m.mass;
+    default: return 0.0f;
+    }
+}
+__device__ float link_info_value(const DevPtrs& s, int slot, int n, int p, int axis, int info) {
+    const int nl = s.nl, nv = s.nv;
+    if (info == 5) return s.lf[LSTRESS * nl + slot];
+    if (info == 6) return s.lf[LSTRAIN * nl + slot];
+    const LinkMat& m = s.lmats[s.lmeta[slot]];
+    const double* vf = s.vf;
+    const double fnx = vf[vf_index(2 * axis, 0, nv, n)], mnx = vf[vf_index(2 * axis, 3, nv, n)];
+    const double mny = vf[vf_index(2 * axis, 4, nv, n)], mnz = vf[vf_index(2 * axis, 5, nv, n)];
+    const double mpy = vf[vf_index(2 * axis + 1, 4, nv, p)], mpz = vf[vf_index(2 * axis + 1, 5, nv, p)];
+    return (float)(fnx * fnx / (2.0f * m.a1) + mnx * mnx / (2.0 * m.a2) + (mnz * mnz - mnz * mpz + mpz * mpz) / (3.0 * m.b3) +
+                   (mny * mny - mny * mpy + mpy * mpy) / (3.0 * m.b3));
+}
+__global__ void __launch_bounds__(256) k_robot_state(DevPtrs s, const int* rfirst, int info, int type, double* out) {
+    const int r = blockIdx.x;
+    const int b = rfirst[r], e = rfirst[r + 1];
+    const bool linkInfo = info == 5 || info == 6 || info == 7;
+    double acc = type == 0 ? 1e300 : type == 1 ? -1e300 : 0.0;
+    int count = 0;
+    for (int i = b + (int)threadIdx.x; i < e; i += 256) {
+        if (!linkInfo) {
+            const double v = voxel_info_value(s, i, info);
+            acc = type == 0 ? fmin(acc, v) : type == 1 ? fmax(acc, v) : acc + v;
+            count++;
+            continue;
+        }
+        for (int a = 0; a < 3; a++) {
+            const int slot = a * s.nv + i;
+            const int p = s.lpos[slot];
+            if (p < 0) continue;
+            const double v = link_info_value(s, slot, i, p, a, info);
+            acc = type == 0 ? fmin(acc, v) : type == 1 ? fmax(acc, v) : acc + v;
+            count++;
+        }
+    }
+    __shared__ double red[256];
+    __shared__ int cnt[256];
+    red[threadIdx.x] = acc;
+    cnt[threadIdx.x] = count;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            const double o = red[threadIdx.x + w];
+            red[threadIdx.x] = type == 0 ? fmin(red[threadIdx.x], o) : type == 1 ? fmax(red[threadIdx.x], o)
+                                                                                  : red[threadIdx.x] + o;
+            cnt[threadIdx.x] += cnt[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const int c = cnt[0];
+        out[r] = c == 0 ? 0.0 : type == 3 ? red[0] / c : red[0];
+    }
+}
+
 static inline unsigned blocks(int n, int b) { return (unsigned)((n + b - 1) / b); }
 
 void launch_links_fp64(const DevPtrs& s, int k, LaunchCfg c, hipStream_t st) {
@@ -274,6 +369,10 @@ void launch_poisson_fp64(const DevPtrs& s, int k, hipStream_t st) {
 }
 void launch_actuate_fp64(const DevPtrs& s, int k, hipStream_t st) {
     if (s.nv > 0) hipLaunchKernelGGL(k_actuate_fp64, dim3(blocks(s.nv, 256)), dim3(256), 0, st, s, k);
+}
+void launch_robot_state(const DevPtrs& s, const int* rfirst, int info, int type, double* out, int nrobots,
+                        hipStream_t st) {
+    if (nrobots > 0) hipLaunchKernelGGL(k_robot_state, dim3((unsigned)nrobots), dim3(256), 0, st, s, rfirst, info, type, out);
 }
 void launch_robot_com(const DevPtrs& s, const int* rfirst, double* com, int nrobots, hipStream_t st) {
     if (nrobots > 0) hipLaunchKernelGGL(k_robot_com, dim3((unsigned)nrobots), dim3(256), 0, st, s, rfirst, com);

@@ -87,6 +87,11 @@ __device__ __forceinline__ float fused_actuation(float base, float amp, float fr
     return (float)((double)base + (double)amp * sin(arg));
 }
 
+/* the actuation temperature in double, not rounded to float (FP64 fused step) */
+__device__ __forceinline__ double fused_temperature_fp64(float base, float amp, float freq, double t, float phase) {
+    return (double)base + (double)amp * sin(6.283185307179586 * (double)freq * t + (double)phase);
+}
+
 /* Quat3D::FromAngleToPosX (Quat3D.h:141-166) with libm trig */
 __device__ __forceinline__ void d_angletoposx(dquat* q, dvec rf) {
     if (rf.x == 0.0 && rf.y == 0.0 && rf.z == 0.0) return;
@@ -106,9 +111,9 @@ __device__ __forceinline__ void d_angletoposx(dquat* q, dvec rf) {
 }
 
 template <typename T>
-__device__ __forceinline__ T m_stress(const LinkMat& m, const float* cs, const float* ss, T strain) {
+__device__ __forceinline__ T m_stress(const LinkMat& m, T E, const float* cs, const float* ss, T strain) {
     if (m.epsilonFail != -1.0f && strain > m.epsilonFail) return T(0);
-    if (strain <= m.strain1 || m.linear) return m.E * strain;
+    if (strain <= m.strain1 || m.linear) return E * strain;
     for (int i = 2; i < m.ndata; i++) {
         if (strain <= cs[i] || i == m.ndata - 1) {
             T perc = (strain - cs[i - 1]) / (cs[i] - cs[i - 1]);
@@ -116,6 +121,42 @@ __device__ __forceinline__ T m_stress(const LinkMat& m, const float* cs, const f
         }
     }
     return T(0);
+}
+
+/* the beam constants a link uses (VX_MaterialLink.h:38-47), in T */
+template <typename T> struct BeamK { T E, a2, b1, b2, b3, sqA1, sqA2xIp, sqB1, sqB2xFMp, sqB3xIp; };
+template <typename T> __device__ __forceinline__ BeamK<T> beam_of(const LinkMat& m) {
+    BeamK<T> k;
+    k.E = m.E; k.a2 = m.a2; k.b1 = m.b1; k.b2 = m.b2; k.b3 = m.b3;
+    k.sqA1 = m.sqA1; k.sqA2xIp = m.sqA2xIp; k.sqB1 = m.sqB1; k.sqB2xFMp = m.sqB2xFMp; k.sqB3xIp = m.sqB3xIp;
+    return k;
+}
+/* the constants of a linear link between two per-voxel moduli: link_constants of their
+   harmonic mean, in float for the mixed step and in double for the FP64 fused step (so that
+   its loss is smooth in the moduli to FP64 rounding, for finite-difference checks) */
+template <typename T> __device__ __forceinline__ BeamK<T> beam_of_moduli(float E1, float E2, float Lf);
+template <> __device__ __forceinline__ BeamK<float> beam_of_moduli<float>(float E1, float E2, float Lf) {
+    const LinkK k = link_constants(combined_modulus(E1, E2), Lf);
+    BeamK<float> b;
+    b.E = k.E; b.a2 = k.a2; b.b1 = k.b1; b.b2 = k.b2; b.b3 = k.b3;
+    b.sqA1 = k.sqA1; b.sqA2xIp = k.sqA2xIp; b.sqB1 = k.sqB1; b.sqB2xFMp = k.sqB2xFMp; b.sqB3xIp = k.sqB3xIp;
+    return b;
+}
+template <> __device__ __forceinline__ BeamK<double> beam_of_moduli<double>(float E1f, float E2f, float Lf) {
+    const double E1 = E1f, E2 = E2f, L = Lf;
+    const double E = 2.0 * E1 * E2 / (E1 + E2);
+    BeamK<double> k;
+    k.E = E;
+    k.a2 = E * L * L * L / 12.0;
+    k.b1 = E * L;
+    k.b2 = E * L * L / 2.0;
+    k.b3 = E * L * L * L / 6.0;
+    k.sqA1 = sqrt(E * L);
+    k.sqA2xIp = sqrt(k.a2 * L * L / 6.0);
+    k.sqB1 = sqrt(k.b1);
+    k.sqB2xFMp = sqrt(k.b2 * L / 2.0);
+    k.sqB3xIp = sqrt(k.b3 * L * L / 6.0);
+    return k;
 }
 
 /* One link (CVX_Link::updateForces, VX_Link.cpp:149-217): pose extraction in FP64, beam
@@ -127,12 +168,13 @@ __device__ void link_fused(const MixPtrs& s, int slot, int n, int p, int axis, b
                            float prevDt, dvec posN, dquat qN, dvec posP, dquat qP, dvec* outF, dvec* outM) {
     const int S = 3 * s.nv;
     LinkMat m = s.lmats[s.lmeta[slot]];
+    BeamK<T> k;
     if (s.vE) { /* per-voxel moduli: linear link of the harmonic-mean modulus (VX_MaterialLink.cpp:70,121-141) */
-        const LinkK k = link_constants(combined_modulus(s.vE[n], s.vE[p]), (float)s.vmats[s.vmat[n]].nomSize);
-        m.E = k.E; m.a2 = k.a2; m.b1 = k.b1; m.b2 = k.b2; m.b3 = k.b3;
-        m.sqA1 = k.sqA1; m.sqA2xIp = k.sqA2xIp; m.sqB1 = k.sqB1; m.sqB2xFMp = k.sqB2xFMp; m.sqB3xIp = k.sqB3xIp;
+        k = beam_of_moduli<T>(s.vE[n], s.vE[p], (float)s.vmats[s.vmat[n]].nomSize);
         m.linear = 1;
         m.epsilonFail = -1.0f;
+    } else {
+        k = beam_of<T>(m);
     }
     uint32_t fl = s.lflIn[slot];
     bool small = (fl & LF_SMALL) != 0, lvv = (fl & LF_LVV) != 0;
@@ -168,23 +210,23 @@ __device__ void link_fused(const MixPtrs& s, int slot, int n, int p, int axis, b
     const float* cst = s.curveStress + m.curveOff;
     if (m.linear) {
         if (axialStrain > maxStrain) maxStrain = axialStrain;
-        stress = m_stress<T>(m, cs, cst, axialStrain);
+        stress = m_stress<T>(m, k.E, cs, cst, axialStrain);
     } else if (axialStrain > maxStrain) {
         maxStrain = axialStrain;
-        stress = m_stress<T>(m, cs, cst, axialStrain);
-        strainOffset = maxStrain - stress / m.E;
+        stress = m_stress<T>(m, k.E, cs, cst, axialStrain);
+        strainOffset = maxStrain - stress / k.E;
     } else {
-        stress = m.E * (axialStrain - strainOffset);
+        stress = k.E * (axialStrain - strainOffset);
     }
 
     V3<T> fN = v3<T>(0, 0, 0), fP = v3<T>(0, 0, 0), mN = v3<T>(0, 0, 0), mP = v3<T>(0, 0, 0);
     const bool failed = m.epsilonFail != -1.0f && maxStrain > m.epsilonFail;
     const bool lvvIn = lvv;
     if (!failed) {
-        fN = v3<T>(stress * m.transverseArea, m.b1 * p2.y - m.b2 * (a1v.z + a2v.z), m.b1 * p2.z + m.b2 * (a1v.y + a2v.y));
+        fN = v3<T>(stress * m.transverseArea, k.b1 * p2.y - k.b2 * (a1v.z + a2v.z), k.b1 * p2.z + k.b2 * (a1v.y + a2v.y));
         fP = v3<T>(-fN.x, -fN.y, -fN.z);
-        mN = v3<T>(m.a2 * (a2v.x - a1v.x), -m.b2 * p2.z - m.b3 * (2 * a1v.y + a2v.y), m.b2 * p2.y - m.b3 * (2 * a1v.z + a2v.z));
-        mP = v3<T>(m.a2 * (a1v.x - a2v.x), -m.b2 * p2.z - m.b3 * (a1v.y + 2 * a2v.y), m.b2 * p2.y - m.b3 * (a1v.z + 2 * a2v.z));
+        mN = v3<T>(k.a2 * (a2v.x - a1v.x), -k.b2 * p2.z - k.b3 * (2 * a1v.y + a2v.y), k.b2 * p2.y - k.b3 * (2 * a1v.z + a2v.z));
+        mP = v3<T>(k.a2 * (a1v.x - a2v.x), -k.b2 * p2.z - k.b3 * (a1v.y + 2 * a2v.y), k.b2 * p2.y - k.b3 * (a1v.z + 2 * a2v.z));
         if (lvvIn) {
             const V3<T> dP = v3<T>(T(0.5) * (p2.x - ls[MS_P2X * S + slot]), T(0.5) * (p2.y - ls[MS_P2Y * S + slot]),
                                T(0.5) * (p2.z - ls[MS_P2Z * S + slot]));
@@ -194,16 +236,16 @@ __device__ void link_fused(const MixPtrs& s, int slot, int n, int p, int axis, b
                                T(0.5) * (a2v.z - ls[MS_A2Z * S + slot]));
             const float dmN = s.vmats[s.vmat[n]].dampNum / prevDt;
             const float dmP = s.vmats[s.vmat[p]].dampNum / prevDt;
-            const V3<T> pc = v3<T>(m.sqA1 * dP.x, m.sqB1 * dP.y - m.sqB2xFMp * (d1.z + d2.z), m.sqB1 * dP.z + m.sqB2xFMp * (d1.y + d2.y));
+            const V3<T> pc = v3<T>(k.sqA1 * dP.x, k.sqB1 * dP.y - k.sqB2xFMp * (d1.z + d2.z), k.sqB1 * dP.z + k.sqB2xFMp * (d1.y + d2.y));
             fN.x += dmN * pc.x; fN.y += dmN * pc.y; fN.z += dmN * pc.z;
             fP.x -= dmP * pc.x; fP.y -= dmP * pc.y; fP.z -= dmP * pc.z;
             const T hN = T(0.5) * dmN, hP = T(0.5) * dmP;
-            mN.x -= hN * (-m.sqA2xIp * (d2.x - d1.x));
-            mN.y -= hN * (m.sqB2xFMp * dP.z + m.sqB3xIp * (2 * d1.y + d2.y));
-            mN.z -= hN * (-m.sqB2xFMp * dP.y + m.sqB3xIp * (2 * d1.z + d2.z));
-            mP.x -= hP * (m.sqA2xIp * (d2.x - d1.x));
-            mP.y -= hP * (m.sqB2xFMp * dP.z + m.sqB3xIp * (d1.y + 2 * d2.y));
-            mP.z -= hP * (-m.sqB2xFMp * dP.y + m.sqB3xIp * (d1.z + 2 * d2.z));
+            mN.x -= hN * (-k.sqA2xIp * (d2.x - d1.x));
+            mN.y -= hN * (k.sqB2xFMp * dP.z + k.sqB3xIp * (2 * d1.y + d2.y));
+            mN.z -= hN * (-k.sqB2xFMp * dP.y + k.sqB3xIp * (2 * d1.z + d2.z));
+            mP.x -= hP * (k.sqA2xIp * (d2.x - d1.x));
+            mP.y -= hP * (k.sqB2xFMp * dP.z + k.sqB3xIp * (d1.y + 2 * d2.y));
+            mP.z -= hP * (-k.sqB2xFMp * dP.y + k.sqB3xIp * (d1.z + 2 * d2.z));
         } else {
             lvv = true;
         }
@@ -307,7 +349,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_FUS
             const VoxMat& mo = s.vmats[s.vmat[o]];
             const double si = axis == 0 ? mt.sizeX : axis == 1 ? mt.sizeY : mt.sizeZ;
             const double so = axis == 0 ? mo.sizeX : axis == 1 ? mo.sizeY : mo.sizeZ;
-            const double bi = base_size(si, tempI, mt.alphaCTE), bo = base_size(so, s.vtempIn[o], mo.alphaCTE);
+            double bi, bo;
+            if (sizeof(T) == 8) {
+                /* FP64 fused step: temperatures and sizes in double, smooth in the phases */
+                const double tn = s.rtimeIn[rob];
+                const double ti = fused_temperature_fp64(s.actBase, s.actAmp, s.actFreq, tn, s.vphase[i]);
+                const double to = fused_temperature_fp64(s.actBase, s.actAmp, s.actFreq, tn, s.vphase[o]);
+                bi = si * (1.0 + ti * (double)mt.alphaCTE);
+                bo = so * (1.0 + to * (double)mo.alphaCTE);
+            } else {
+                bi = base_size(si, tempI, mt.alphaCTE);
+                bo = base_size(so, s.vtempIn[o], mo.alphaCTE);
+            }
             rest = neg ? 0.5 * (bi + bo) : 0.5 * (bo + bi);
         } else {
             rest = s.lrest[slot];

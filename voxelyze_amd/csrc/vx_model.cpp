@@ -22,6 +22,8 @@ void launch_links_fp64(const DevPtrs& s, int k, LaunchCfg c, hipStream_t st);
 void launch_voxels_fp64(const DevPtrs& s, int k, LaunchCfg c, hipStream_t st);
 void launch_actuate_fp64(const DevPtrs& s, int k, hipStream_t st);
 void launch_poisson_fp64(const DevPtrs& s, int k, hipStream_t st);
+void launch_robot_state(const DevPtrs& s, const int* rfirst, int info, int type, double* out, int nrobots,
+                        hipStream_t st);
 void launch_robot_com(const DevPtrs& s, const int* rfirst, double* com, int nrobots, hipStream_t st);
 void launch_col_stale(const DevPtrs& s, const ContactPtrs& c, hipStream_t st);
 void launch_col_keys(const DevPtrs& s, const ContactPtrs& c, int nbuckets, hipStream_t st);
@@ -877,6 +879,7 @@ bool Sim::syncUp() {
             r.curveOff = (int)cs.size();
             r.nu = m->nu;
             r.eHat = m->eHat;
+            r.a1 = m->a1;
             cs.insert(cs.end(), m->strainData.begin(), m->strainData.end());
             ss.insert(ss.end(), m->stressData.begin(), m->stressData.end());
             lm.push_back(r);
@@ -1327,6 +1330,25 @@ bool Sim::stepFp64(int nsteps) {
             if (okB) { profMs[1] += b; profCount[1]++; }
         }
     }
+    return true;
+}
+
+/* CVoxelyze::stateInfo for every robot, reduced on the device (FP64 sums) */
+bool Sim::robotStateInfo(int info, int type, double* out) {
+    if (info < 0 || info > 9 || type < 0 || type > 3) {
+        err = "invalid stateInfo or value type";
+        return false;
+    }
+    if ((info >= 5 && info <= 8) && fused()) {
+        err = "link-state infos on the device need the FP64 step (VXG_FP64_PLAIN)";
+        return false;
+    }
+    if (!syncUp()) return false;
+    const int R = nRobots();
+    launch_robot_state(dp, (const int*)brfirst.p, info, type, (double*)bcom.p, R, stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, bcom.p, (size_t)8 * R, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
     return true;
 }
 

@@ -83,6 +83,8 @@ struct MixTape {
     std::vector<void*> ls;   /* float (VXG_MIXED) or double (VXG_FUSED_FP64) link state */
 };
 
+struct StateCheckpoint;
+
 struct Sim {
     double voxSize = 0.001;
     float ambientTemp = 0.0f, grav = 0.0f;
@@ -241,6 +243,13 @@ struct Sim {
     bool endLaunch(int nsteps, int* done);
     bool anyDiverged(int k, bool* all);
     bool robotCom(double* com);
+    bool robotStateInfo(int info, int type, double* out);
+    /* checkpoints of the dynamic state on the device (vx_checkpoint.cpp) */
+    static const int kCheckpointSlots = 8;
+    std::shared_ptr<StateCheckpoint> checkpoints[kCheckpointSlots];
+    bool checkpointSave(int slot);
+    bool checkpointRestore(int slot);
+    void checkpointFree(int slot);
 
     ~Sim();
     int nv() const { return (int)vmat.size(); }

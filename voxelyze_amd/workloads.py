@@ -185,13 +185,18 @@ def robot_shape(seed, r, vmin=100, vmax=8000, box=20):
     """Robot r of a population: size N_r = vmin + floor(u * (vmax - vmin + 1)) with u drawn
     at counter 2^40 of stream seed+r, shape grown by grow_robot(seed+r, N_r, box) and shifted
     so that its lowest layer is z = 0, material index per voxel floor(4u) with u drawn at
-    counter 2^41 + i.  Returns (ijk (n,3) int32, material index (n,) int32)."""
+    counter 2^41 + i (i in growth order).  Returns (ijk (n,3) int32, material index (n,)
+    int32) in lattice order."""
     s = seed + r
     n = vmin + int(uniform(s, _N_DRAW) * (vmax - vmin + 1))
     ijk = grow_robot_native(s, n, box)
     ijk[:, 2] -= ijk[:, 2].min()
     mat = np.minimum((uniform(s, _MAT_DRAW + np.arange(len(ijk), dtype=np.uint64)) * 4).astype(np.int32), 3)
-    return ijk.astype(np.int32), mat
+    # voxels listed in lattice order (x fastest, then y, then z) rather than growth order, so
+    # that lattice neighbours sit close together in the device arrays; the shape and each
+    # site's material are those of the growth
+    order = np.lexsort((ijk[:, 0], ijk[:, 1], ijk[:, 2]))
+    return ijk[order].astype(np.int32), mat[order]
 
 
 def robot_shapes(seed, n_robots, vmin=100, vmax=8000, box=20):

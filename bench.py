@@ -101,7 +101,23 @@ def cpu_baseline(workload, budget_s=15.0, max_steps=2000):
                       f"({V} voxels), {el:.1f} s"}
 
 
-def measure(precision, workload, steps, warmup, fp64_path="fused"):
+# the library's launch-configuration kernel ids
+KERNEL_IDS = {"k_step_mixed": 0, "k_links_fp64": 1, "k_voxels_fp64": 2, "k_step_fp64": 3}
+
+
+def parse_launch(text):
+    """--launch "k_step_fp64=128:1,k_links_fp64=64:0" -> {"k_step_fp64": (128, 1), ...}"""
+    out = {}
+    for item in filter(None, (text or "").split(",")):
+        name, cfg = item.split("=")
+        if name not in KERNEL_IDS:
+            raise SystemExit(f"bench: unknown kernel {name!r} in --launch (one of {sorted(KERNEL_IDS)})")
+        blk, xcd = cfg.split(":")
+        out[name] = (int(blk), int(xcd))
+    return out
+
+
+def measure(precision, workload, steps, warmup, fp64_path="fused", launch=None):
     """Build the workload, step it `warmup` times, then time `steps` steps with HIP events
     around every kernel.  Returns the sim (kept alive for comparisons) and the figures."""
     import ctypes as C
@@ -113,6 +129,9 @@ def measure(precision, workload, steps, warmup, fp64_path="fused"):
     cfg = build_workload(sim, workload)
     sim.setPrecision(MIXED if precision == "mixed" else FP64_PLAIN)
     sim.setFp64Path(FP64_PATHS[fp64_path])
+    for kname, (blk, xcd) in (launch or {}).items():
+        if not sim.abi.set_launch_config(sim.handle, KERNEL_IDS[kname], int(blk), int(xcd)):
+            raise SystemExit(f"bench: bad launch configuration {kname}={blk}:{xcd}")
     V, L = sim.voxelCount(), sim.linkCount()
     dt = sim.recommendedTimeStep()
     nominal = sim.state()["pos"]  # before the first step every voxel sits at its lattice position
@@ -147,7 +166,7 @@ def measure(precision, workload, steps, warmup, fp64_path="fused"):
           file=sys.stderr, flush=True)
     cfg.update({"voxels": V, "links": L, "dt": dt, "voxel_pitch_m": 0.005})
     tuned = {}
-    for kk, kname in enumerate(["k_step_mixed", "k_links_fp64", "k_voxels_fp64", "k_step_fp64"]):
+    for kname, kk in KERNEL_IDS.items():
         b, x, st = C.c_int(), C.c_int(), C.c_int()
         sim.abi.launch_config(sim.handle, kk, C.byref(b), C.byref(x), C.byref(st))
         if st.value:
@@ -273,7 +292,10 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a plain 1 GiB -> 1 GiB elementwise copy (known bytes) to scale PMC byte counters")
+    ap.add_argument("--launch", default="",
+                    help="pin launch configurations instead of autotuning them, e.g. k_step_fp64=128:1 (block:xcd_mapping)")
     a = ap.parse_args()
+    launch = parse_launch(a.launch)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only; --gpus must be 1 "
               "(multi-GPU sharding is described in DESIGN.md)", file=sys.stderr)
@@ -290,7 +312,7 @@ def main():
         del src, dst
         print("[bench] calibration copy: 3 x (1 GiB read + 1 GiB write)", file=sys.stderr, flush=True)
 
-    sim, r = measure(a.precision, a.workload, a.steps, a.warmup, a.fp64_path)
+    sim, r = measure(a.precision, a.workload, a.steps, a.warmup, a.fp64_path, launch)
     cfg = r["cfg"]
     cfg["precision"] = "fp64_plain" if a.precision == "fp64" else "mixed"
     if a.precision == "fp64":
@@ -304,25 +326,27 @@ def main():
     if not a.no_compare and a.precision == "fp64":
         # the other FP64 path on an identical copy: the same bits after the same steps
         other_path = "split" if a.fp64_path == "fused" else "fused"
-        sim3, r3 = measure("fp64", a.workload, a.steps, a.warmup, other_path)
+        sim3, r3 = measure("fp64", a.workload, a.steps, a.warmup, other_path, launch)
         sa, sb = sim.state(), sim3.state()
         same = all(sa[k].tobytes() == sb[k].tobytes() for k in sa)
         la, lb = sim.link_state(), sim3.link_state()
         same = same and all(la[k].tobytes() == lb[k].tobytes() for k in la)
         line["fp64_" + other_path] = {"value": r3["value"], "ms_per_step": r3["el"] / a.steps * 1e3,
                                       "roofline": roofline(r3["kernels"], a.workload),
+                                      "autotuned_launch": r3["cfg"]["autotuned_launch"],
                                       "bit_identical_to_" + a.fp64_path: same}
         print(f"[bench] fp64 {other_path}: {r3['el'] / a.steps * 1e3:.4f} ms/step, bit-identical state: {same}",
               file=sys.stderr, flush=True)
         del sim3
     if not a.no_compare:
         other = "mixed" if a.precision == "fp64" else "fp64"
-        sim2, r2 = measure(other, a.workload, a.steps, a.warmup)
+        sim2, r2 = measure(other, a.workload, a.steps, a.warmup, launch=launch)
         ref, test = (sim, sim2) if a.precision == "fp64" else (sim2, sim)
         err = rel_disp_error(ref, test, r["nominal"])
         fp64_v, mixed_v = (r["value"], r2["value"]) if a.precision == "fp64" else (r2["value"], r["value"])
         line[other] = {"value": r2["value"], "ms_per_step": r2["el"] / a.steps * 1e3,
-                       "roofline": roofline(r2["kernels"], a.workload)}
+                       "roofline": roofline(r2["kernels"], a.workload),
+                       "autotuned_launch": r2["cfg"]["autotuned_launch"]}
         line["mixed_vs_fp64"] = {"speedup": mixed_v / fp64_v, "rel_disp_err": err,
                                  "after_steps": a.warmup + a.steps}
         print(f"[bench] mixed/fp64 speedup {mixed_v / fp64_v:.2f}x, rel disp err {err:.3e}", file=sys.stderr,

@@ -172,6 +172,9 @@ int vxg_graph_stats(vxg_sim* s, int* captures, int* replays);
    chosen). */
 int vxg_autotune(vxg_sim* s, int enable);
 int vxg_launch_config(vxg_sim* s, int kernel, int* block, int* xcd_mapping, int* settled);
+/* Pins kernel's launch configuration (block 64/128/256, xcd_mapping 0/1) as if the autotuner
+   had chosen it, for the current model size; 0 on invalid arguments. */
+int vxg_set_launch_config(vxg_sim* s, int kernel, int block, int xcd_mapping);
 int vxg_profile_read(vxg_sim* s, double* ms, int* launches, int cap);
 
 /* ---------------------------------------------------------------- materials */

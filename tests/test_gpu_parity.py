@@ -166,3 +166,14 @@ def test_autotuned_launches_give_the_same_bits(precision, path, kernel):
     a.abi.launch_config(a.handle, kernel, C.byref(blk), C.byref(x), C.byref(st))
     print(f"[tune] precision {precision} path {path}: block {blk.value} xcd {x.value} settled {st.value}")
     assert st.value == 1
+    # a pinned configuration (the one the autotuner did not start with) gives the same bits too
+    c = B.maker(B.product_abi(), path)(0.005)
+    W.drop_lattice(c, nx=24, ny=24, nz=8)
+    c.setPrecision(precision)
+    assert c.abi.set_launch_config(c.handle, kernel, 64, 1) == 1
+    assert c.abi.set_launch_config(c.handle, kernel, 96, 0) == 0
+    c.step(dt, 30)
+    sc = c.state()
+    assert all(sa[k].tobytes() == sc[k].tobytes() for k in sa)
+    c.abi.launch_config(c.handle, kernel, C.byref(blk), C.byref(x), C.byref(st))
+    assert (blk.value, x.value, st.value) == (64, 1, 1)

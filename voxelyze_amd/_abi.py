@@ -98,6 +98,7 @@ EXTENSIONS = {
     "graph_stats": (_I, [_P, _IP, _IP]),
     "autotune": (_I, [_P, _I]),
     "launch_config": (_I, [_P, _I, _IP, _IP, _IP]),
+    "set_launch_config": (_I, [_P, _I, _I, _I]),
     "set_fp64_path": (_I, [_P, _I]),
     "load_json": (_I, [_P, C.c_char_p]),
     "robot_state_info": (_I, [_P, _I, _I, _DP]),

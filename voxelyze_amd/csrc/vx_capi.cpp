@@ -235,6 +235,18 @@ API int vxg_launch_config(vxg_sim* h, int kernel, int* block, int* xcd, int* set
     if (settled) *settled = t.settled ? 1 : 0;
     return 1;
 }
+API int vxg_set_launch_config(vxg_sim* h, int kernel, int block, int xcd) {
+    Sim* s = S(h);
+    if (kernel < 0 || kernel > 3 || (block != 64 && block != 128 && block != 256) || (xcd != 0 && xcd != 1)) {
+        s->err = "launch config: kernel 0-3, block 64/128/256, xcd mapping 0/1";
+        return 0;
+    }
+    Sim::Tuner& t = s->tuner[kernel];
+    t.reset(s->nv());
+    t.chosen = vx::LaunchCfg{block, xcd};
+    t.settled = true;
+    return 1;
+}
 API int vxg_profile_read(vxg_sim* h, double* ms, int* launches, int cap) {
     Sim* s = S(h);
     int n = cap < 5 ? cap : 5;

@@ -70,8 +70,9 @@ __device__ __forceinline__ double axis_size(const VoxMat& m, int axis) {
 
 /* one link of voxel i (direction dir, neighbour o) as CVX_Link::updateForces
    (VX_Link.cpp:149-217): its end vector on voxel i added to the direction-ordered sums,
-   written to the force slot on the last step; the negative end (Neg) owns the link state */
-template <bool ForcesOnly, bool Neg>
+   written to the force slot on the last step; the negative end (Neg) owns the link state.
+   Poisson: some material has Poisson's ratio != 0 (the transverse info is read and kept) */
+template <bool ForcesOnly, bool Neg, bool Poisson>
 __device__ __forceinline__ void link_end(const FusedPtrs& s, int i, int o, int dir, int rob, float prevDt, float tnow,
                                          float tempI, const dvec& pos0, const dquat& q0, dvec& sf, dvec& sm) {
     const int nv = s.nv, nl = s.nl;
@@ -99,7 +100,7 @@ __device__ __forceinline__ void link_end(const FusedPtrs& s, int i, int o, int d
     old.flags = s.lflIn[slot];
     old.maxStrain = s.lfIn[LMAXSTRAIN * nl + slot];
     old.strainOffset = s.lfIn[LOFFSET * nl + slot];
-    if (s.poisson) { /* volume effects (VX_Link.cpp:160) */
+    if (Poisson) { /* volume effects (VX_Link.cpp:160) */
         old.tArea = s.lfIn[LTAREA * nl + slot];
         old.tss = s.lfIn[LTSS * nl + slot];
         link_transverse(m, s.vmats[s.vmat[n]], s.vmats[s.vmat[p]], s.vps, nv, n, p, axis, &old.tArea, &old.tss);
@@ -113,7 +114,7 @@ __device__ __forceinline__ void link_end(const FusedPtrs& s, int i, int o, int d
         dmP = s.vmats[s.vmat[p]].dampNum / prevDt;
     }
     LinkEval e;
-    link_eval_fp64(m, s.curveStrain + m.curveOff, s.curveStress + m.curveOff, axis, Neg ? pos0 : po,
+    link_eval_fp64<Poisson>(m, s.curveStrain + m.curveOff, s.curveStress + m.curveOff, axis, Neg ? pos0 : po,
                    Neg ? q0 : qo, Neg ? po : pos0, Neg ? qo : q0, rest, old, dmN, dmP, &e);
     const dvec f = Neg ? e.fN : e.fP, mm = Neg ? e.mN : e.mP;
     sf.x += f.x; sf.y += f.y; sf.z += f.z;
@@ -134,7 +135,7 @@ __device__ __forceinline__ void link_end(const FusedPtrs& s, int i, int o, int d
         lf[LMAXSTRAIN * nl + slot] = e.maxStrain;
         lf[LOFFSET * nl + slot] = e.strainOffset;
         lf[LSTRESS * nl + slot] = e.stress;
-        if (s.poisson) {
+        if (Poisson) {
             lf[LTAREA * nl + slot] = old.tArea;
             lf[LTSS * nl + slot] = old.tss;
         }
@@ -151,8 +152,9 @@ __device__ __forceinline__ void link_end(const FusedPtrs& s, int i, int o, int d
 #endif
 /* ForcesOnly: after a launch, the end forces of the step in which a robot diverged, for the
    robots that diverged before the launch's last step (k is that step); "in" is the voxel
-   state (as before the diverging step) and the saved link state */
-template <bool ForcesOnly>
+   state (as before the diverging step) and the saved link state.  Poisson: s.poisson, as a
+   compile-time switch */
+template <bool ForcesOnly, bool Poisson>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_FUSED64_WAVES, 8))) k_step_fp64(FusedPtrs s) {
     const int i = lane_index(s.xcd);
     const int nv = s.nv, nl = s.nl;
@@ -197,8 +199,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_FUS
             continue;
         }
         /* the two ends as separate instances, each computing only its own end vectors */
-        if ((dir & 1) == 0) link_end<ForcesOnly, true>(s, i, o, dir, rob, prevDt, tnow, tempI, pos0, q0, sf, sm);
-        else link_end<ForcesOnly, false>(s, i, o, dir, rob, prevDt, tnow, tempI, pos0, q0, sf, sm);
+        if ((dir & 1) == 0) link_end<ForcesOnly, true, Poisson>(s, i, o, dir, rob, prevDt, tnow, tempI, pos0, q0, sf, sm);
+        else link_end<ForcesOnly, false, Poisson>(s, i, o, dir, rob, prevDt, tnow, tempI, pos0, q0, sf, sm);
     }
     if (ForcesOnly) return;
     /* the voxel's index through an opaque copy: its addresses are formed here, after the link
@@ -266,8 +268,13 @@ __global__ void __launch_bounds__(256) k_fused_restore(double* vd, const double*
 void launch_step_fp64(const FusedPtrs& s, int block, bool forcesOnly, hipStream_t st) {
     if (s.nv <= 0) return;
     const dim3 grid((unsigned)((s.nv + block - 1) / block));
-    if (forcesOnly) hipLaunchKernelGGL(k_step_fp64<true>, grid, dim3(block), 0, st, s);
-    else hipLaunchKernelGGL(k_step_fp64<false>, grid, dim3(block), 0, st, s);
+    if (s.poisson) {
+        if (forcesOnly) hipLaunchKernelGGL((k_step_fp64<true, true>), grid, dim3(block), 0, st, s);
+        else hipLaunchKernelGGL((k_step_fp64<false, true>), grid, dim3(block), 0, st, s);
+    } else {
+        if (forcesOnly) hipLaunchKernelGGL((k_step_fp64<true, false>), grid, dim3(block), 0, st, s);
+        else hipLaunchKernelGGL((k_step_fp64<false, false>), grid, dim3(block), 0, st, s);
+    }
 }
 void launch_fused_prevdt(const int* vrobot, const int* rdiv, const float* rdt, float* vprevdt, int nv, hipStream_t st) {
     if (nv > 0)

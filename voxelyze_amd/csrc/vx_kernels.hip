@@ -37,7 +37,7 @@ namespace vx {
 /* The link of slot (axis, n) -- n is its negative voxel -- given n's pose.  Every slot and
    every force slot is written on every step (zeros where no link exists) so that all stores
    cover whole cache lines. */
-template <int axis>
+template <int axis, bool Poisson>
 __device__ __forceinline__ void link_slot_fp64(const DevPtrs& s, int k, int rob, int n, const dvec& posN,
                                                const dquat& qN) {
     const int nv = s.nv;
@@ -74,7 +74,7 @@ __device__ __forceinline__ void link_slot_fp64(const DevPtrs& s, int k, int rob,
     old.flags = s.lflags[i];
     old.maxStrain = lf[LMAXSTRAIN * nl + i];
     old.strainOffset = lf[LOFFSET * nl + i];
-    if (s.poisson) { /* volume effects (VX_Link.cpp:160) */
+    if (Poisson) { /* volume effects (VX_Link.cpp:160) */
         old.tArea = lf[LTAREA * nl + i];
         old.tss = lf[LTSS * nl + i];
         link_transverse(m, s.vmats[s.vmat[n]], s.vmats[s.vmat[p]], s.vps, nv, n, p, axis, &old.tArea, &old.tss);
@@ -91,7 +91,7 @@ __device__ __forceinline__ void link_slot_fp64(const DevPtrs& s, int k, int rob,
         dmP = s.vmats[s.vmat[p]].dampNum / s.vprevdt[p];
     }
     LinkEval o;
-    link_eval_fp64(m, s.curveStrain + m.curveOff, s.curveStress + m.curveOff, axis, posN, qN, posP, qP, rest, old,
+    link_eval_fp64<Poisson>(m, s.curveStrain + m.curveOff, s.curveStress + m.curveOff, axis, posN, qN, posP, qP, rest, old,
                    dmN, dmP, &o);
 
     ld[LP2X * nl + i] = o.pos2.x; ld[LP2Y * nl + i] = o.pos2.y; ld[LP2Z * nl + i] = o.pos2.z;
@@ -115,13 +115,14 @@ __device__ __forceinline__ void link_slot_fp64(const DevPtrs& s, int k, int rob,
 }
 
 /* one lane per voxel: the three links it is the negative end of (axes x, y, z), its own pose
-   read once */
+   read once.  Poisson: s.poisson, as a compile-time switch */
 #ifndef VX_LINK_WAVES
 #define VX_LINK_WAVES 3
 #endif
 #ifndef VX_VOXEL_WAVES
 #define VX_VOXEL_WAVES 3
 #endif
+template <bool Poisson>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_LINK_WAVES, 8))) k_links_fp64(DevPtrs s, int k,
                                                                                                          int xcd) {
     const int n = lane_index(xcd);
@@ -132,9 +133,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_LIN
     const double* __restrict__ vd = s.vd;
     const dvec posN = dv(vd[VPX * nv + n], vd[VPY * nv + n], vd[VPZ * nv + n]);
     const dquat qN = dq(vd[VQW * nv + n], vd[VQX * nv + n], vd[VQY * nv + n], vd[VQZ * nv + n]);
-    link_slot_fp64<0>(s, k, rob, n, posN, qN);
-    link_slot_fp64<1>(s, k, rob, n, posN, qN);
-    link_slot_fp64<2>(s, k, rob, n, posN, qN);
+    link_slot_fp64<0, Poisson>(s, k, rob, n, posN, qN);
+    link_slot_fp64<1, Poisson>(s, k, rob, n, posN, qN);
+    link_slot_fp64<2, Poisson>(s, k, rob, n, posN, qN);
 }
 
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_VOXEL_WAVES, 8))) k_voxels_fp64(DevPtrs s, int k,
@@ -359,7 +360,10 @@ __global__ void __launch_bounds__(256) k_robot_state(DevPtrs s, const int* rfirs
 static inline unsigned blocks(int n, int b) { return (unsigned)((n + b - 1) / b); }
 
 void launch_links_fp64(const DevPtrs& s, int k, LaunchCfg c, hipStream_t st) {
-    if (s.nv > 0) hipLaunchKernelGGL(k_links_fp64, dim3(blocks(s.nv, c.block)), dim3(c.block), 0, st, s, k, c.xcd);
+    if (s.nv > 0) {
+        if (s.poisson) hipLaunchKernelGGL(k_links_fp64<true>, dim3(blocks(s.nv, c.block)), dim3(c.block), 0, st, s, k, c.xcd);
+        else hipLaunchKernelGGL(k_links_fp64<false>, dim3(blocks(s.nv, c.block)), dim3(c.block), 0, st, s, k, c.xcd);
+    }
 }
 void launch_voxels_fp64(const DevPtrs& s, int k, LaunchCfg c, hipStream_t st) {
     if (s.nv > 0) hipLaunchKernelGGL(k_voxels_fp64, dim3(blocks(s.nv, c.block)), dim3(c.block), 0, st, s, k, c.xcd);

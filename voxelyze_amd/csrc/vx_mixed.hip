@@ -168,14 +168,15 @@ __device__ void link_fused(const MixPtrs& s, int slot, int n, int p, int axis, b
                            float prevDt, dvec posN, dquat qN, dvec posP, dquat qP, dvec* outF, dvec* outM) {
     const int S = 3 * s.nv;
     LinkMat m = s.lmats[s.lmeta[slot]];
-    BeamK<T> k;
     if (s.vE) { /* per-voxel moduli: linear link of the harmonic-mean modulus (VX_MaterialLink.cpp:70,121-141) */
-        k = beam_of_moduli<T>(s.vE[n], s.vE[p], (float)s.vmats[s.vmat[n]].nomSize);
+        const LinkK lk = link_constants(combined_modulus(s.vE[n], s.vE[p]), (float)s.vmats[s.vmat[n]].nomSize);
+        m.E = lk.E; m.a2 = lk.a2; m.b1 = lk.b1; m.b2 = lk.b2; m.b3 = lk.b3;
+        m.sqA1 = lk.sqA1; m.sqA2xIp = lk.sqA2xIp; m.sqB1 = lk.sqB1; m.sqB2xFMp = lk.sqB2xFMp; m.sqB3xIp = lk.sqB3xIp;
         m.linear = 1;
         m.epsilonFail = -1.0f;
-    } else {
-        k = beam_of<T>(m);
     }
+    BeamK<T> k = beam_of<T>(m);
+    if (sizeof(T) == 8 && s.vE) k = beam_of_moduli<T>(s.vE[n], s.vE[p], (float)s.vmats[s.vmat[n]].nomSize);
     uint32_t fl = s.lflIn[slot];
     bool small = (fl & LF_SMALL) != 0, lvv = (fl & LF_LVV) != 0;
     const T* ls = (const T*)s.lsIn;

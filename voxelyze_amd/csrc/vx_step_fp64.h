@@ -13,25 +13,29 @@
 
 namespace vx {
 
-/* CVX_Material::stress(strain, transverseStrainSum, forceLinear) (src/VX_Material.cpp:165-195) */
+/* CVX_Material::stress(strain, transverseStrainSum, forceLinear) (src/VX_Material.cpp:165-195).
+   Poisson = false: every material of the model has Poisson's ratio 0, and the ratio branches
+   are compiled out (the same bits: they are the nu != 0 branches) */
+template <bool Poisson = true>
 __device__ __forceinline__ float link_stress(const LinkMat& m, const float* __restrict__ cs, const float* __restrict__ ss,
                                              float strain, float tss, bool forceLinear) {
+    const float nu = Poisson ? m.nu : 0.0f;
     if (m.epsilonFail != -1.0f && strain > m.epsilonFail) return 0.0f;
     if (strain <= m.strain1 || m.linear || forceLinear) {
-        if (m.nu == 0.0f) return m.E * strain;
-        return m.eHat * ((1 - m.nu) * strain + m.nu * tss);
+        if (nu == 0.0f) return m.E * strain;
+        return m.eHat * ((1 - nu) * strain + nu * tss);
     }
     const int n = m.ndata;
     for (int i = 2; i < n; i++) {
         if (strain <= cs[i] || i == n - 1) {
             const float perc = (strain - cs[i - 1]) / (cs[i] - cs[i - 1]);
             const float basic = ss[i - 1] + perc * (ss[i] - ss[i - 1]);
-            if (m.nu == 0.0f) return basic;
+            if (nu == 0.0f) return basic;
             const float modulus = (ss[i] - ss[i - 1]) / (cs[i] - cs[i - 1]);
-            const float modulusHat = modulus / ((1 - 2 * m.nu) * (1 + m.nu));
+            const float modulusHat = modulus / ((1 - 2 * nu) * (1 + nu));
             const float effStrain = basic / modulus;
             const float effTss = tss * (effStrain / strain);
-            return modulusHat * ((1 - m.nu) * effStrain + m.nu * effTss);
+            return modulusHat * ((1 - nu) * effStrain + nu * effTss);
         }
     }
     return 0.0f;
@@ -123,11 +127,14 @@ struct LinkEval {
 
 /* CVX_Link::updateForces + orientLink + updateStrain (VX_Link.cpp:77-249), FP64 with the
    reference's float/double cast points.  dmN, dmP are the two voxels' dampingMultiplier
-   numerators over their previousDt (VX_Voxel.h:130); used only when local velocity is valid. */
+   numerators over their previousDt (VX_Voxel.h:130); used only when local velocity is valid.
+   Poisson as for link_stress. */
+template <bool Poisson = true>
 __device__ __forceinline__ void link_eval_fp64(const LinkMat& m, const float* __restrict__ cs,
                                                const float* __restrict__ ss, int axis, const dvec& posN,
                                                const dquat& qN, const dvec& posP, const dquat& qP, double rest,
                                                const LinkPrev& old, float dmN, float dmP, LinkEval* o) {
+    const float nu = Poisson ? m.nu : 0.0f;
     bool small = (old.flags & LF_SMALL) != 0, lvv = (old.flags & LF_LVV) != 0;
     float maxStrain = old.maxStrain, strainOffset = old.strainOffset;
 
@@ -166,15 +173,15 @@ __device__ __forceinline__ void link_eval_fp64(const LinkMat& m, const float* __
     float stress;
     if (m.linear) {
         if (axialStrain > maxStrain) maxStrain = axialStrain;
-        stress = link_stress(m, cs, ss, axialStrain, old.tss, false);
+        stress = link_stress<Poisson>(m, cs, ss, axialStrain, old.tss, false);
     } else if (axialStrain > maxStrain) {
         maxStrain = axialStrain;
-        stress = link_stress(m, cs, ss, axialStrain, old.tss, false);
-        if (m.nu != 0.0f) strainOffset = maxStrain - link_stress(m, cs, ss, axialStrain, 0.0f, false) / (m.eHat * (1 - m.nu));
+        stress = link_stress<Poisson>(m, cs, ss, axialStrain, old.tss, false);
+        if (nu != 0.0f) strainOffset = maxStrain - link_stress<Poisson>(m, cs, ss, axialStrain, 0.0f, false) / (m.eHat * (1 - nu));
         else strainOffset = maxStrain - stress / m.E;
     } else {
         const float rel = axialStrain - strainOffset;
-        if (m.nu != 0.0f) stress = link_stress(m, cs, ss, rel, old.tss, true);
+        if (nu != 0.0f) stress = link_stress<Poisson>(m, cs, ss, rel, old.tss, true);
         else stress = m.E * rel;
     }
 

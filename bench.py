@@ -76,15 +76,20 @@ def traffic_from_profiles(kernel, workload):
 
 
 def cpu_baseline(workload, budget_s=15.0, max_steps=2000):
-    """The CPU oracle (oracle/vx_oracle.c, OpenMP over the link and voxel loops like the
-    reference's USE_OMP build) on the same workload, timed on this host."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    """The reference's own multi-core CPU path on the same workload, timed on this host: its
+    sources compiled with USE_OMP (oracle/_ref/libvxref_omp.so: the OpenMP link and voxel
+    loops of src/Voxelyze.cpp:260-279), driven through oracle/ref_driver.cpp.  Where that
+    library was not built, the oracle (oracle/vx_oracle.c, OpenMP over the same loops)."""
     from voxelyze_amd._abi import bind
     from voxelyze_amd.api import CVoxelyze
-    lib = os.path.join(ROOT, "oracle", "liboracle.so")
-    if not os.path.exists(lib):
+    ref = os.path.join(ROOT, "oracle", "_ref", "libvxref_omp.so")
+    port = os.path.join(ROOT, "oracle", "liboracle.so")
+    if os.path.exists(ref):
+        abi, kind, what = bind(ref, "vxr_", "reference"), "reference", "the reference built with USE_OMP"
+    elif os.path.exists(port):
+        abi, kind, what = bind(port, "vxo_", "oracle"), "port", "oracle/vx_oracle.c with OpenMP"
+    else:
         return None
-    abi = bind(lib, "vxo_", "oracle")
     sim = CVoxelyze(0.005, abi)
     build_workload(sim, workload)
     dt = sim.recommendedTimeStep()
@@ -96,9 +101,8 @@ def cpu_baseline(workload, budget_s=15.0, max_steps=2000):
     el = time.perf_counter() - t0
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     V = sim.voxelCount()
-    return {"value": V * n / el, "unit": "voxel-steps/s", "cores": threads, "kind": "port",
-            "sample": f"oracle/vx_oracle.c with OpenMP, {n} steps after 2 warm-up steps of the same workload "
-                      f"({V} voxels), {el:.1f} s"}
+    return {"value": V * n / el, "unit": "voxel-steps/s", "cores": threads, "kind": kind,
+            "sample": f"{what}, {n} steps after 2 warm-up steps of the same workload ({V} voxels), {el:.1f} s"}
 
 
 # the library's launch-configuration kernel ids

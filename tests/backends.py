@@ -34,6 +34,15 @@ def ref_abi():
     return bind(REF_SO, "vxr_", "reference") if os.path.exists(REF_SO) else None
 
 
+REF_OMP_SO = os.path.join(ROOT, "oracle", "_ref", "libvxref_omp.so")
+
+
+@functools.lru_cache(None)
+def ref_omp_abi():
+    """The reference built with its USE_OMP loops (the CPU baseline bench.py times)."""
+    return bind(REF_OMP_SO, "vxr_", "reference_omp") if os.path.exists(REF_OMP_SO) else None
+
+
 @functools.lru_cache(None)
 def product_abi():
     from voxelyze_amd import _lib

@@ -271,6 +271,27 @@ def test_oracle_vs_reference_random_models(seed):
         assert not bad, f"seed {seed} block {k}: {bad}"
 
 
+@pytest.mark.parametrize("lift", [0, 2])
+def test_reference_omp_build_equals_the_serial_reference(lift):
+    """The CPU baseline is the reference compiled with USE_OMP; on the benchmark's drop
+    lattice (no Poisson, no collisions, each link and each voxel writing only its own state)
+    its parallel loops give the serial reference's bits, and so the oracle's."""
+    from voxelyze_amd import workloads as W
+    ref, omp = B.ref_abi(), B.ref_omp_abi()
+    if ref is None or omp is None:
+        pytest.skip("compiled reference not present (built only where the reference sources are)")
+    sims = [B.maker(abi)(0.005) for abi in (ref, omp, B.oracle_abi())]
+    for sim in sims:
+        W.drop_lattice(sim, 24, 24, 8, seed=2, lift=lift)
+    dt = sims[0].recommendedTimeStep()
+    for sim in sims:
+        assert sim.step(dt, 300) == (300, -1)
+    snaps = [S.snapshot(sim) for sim in sims]
+    for other in snaps[1:]:
+        bad = [key for key in snaps[0] if not bits_equal(snaps[0][key], other[key])]
+        assert not bad, f"lift {lift}: {bad}"
+
+
 @pytest.mark.parametrize("seed", range(8))
 def test_oracle_vs_reference_random_poisson_models(seed):
     """Random models whose materials mostly have a Poisson's ratio: the volumetric path,

@@ -361,7 +361,7 @@ def main():
     if not a.no_adjoint:
         line["adjoint"] = bench_adjoint(a.adjoint_robots, a.adjoint_steps)
     if not a.no_cpu_baseline:
-        print("[bench] timing the CPU oracle on a bounded sample", file=sys.stderr, flush=True)
+        print("[bench] timing the CPU baseline on a bounded sample", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(a.workload)
     print(json.dumps(line), flush=True)
 

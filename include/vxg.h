@@ -207,7 +207,9 @@ int vxg_material_model_data(vxg_sim* s, int mat, float* strain, float* stress, i
 
 /* ------------------------------------------------------------------- voxels */
 /* CVoxelyze::setVoxel(CVX_Material*, int, int, int) include/Voxelyze.h:96, src/Voxelyze.cpp:422-437
-   mat < 0 removes the voxel (the reference's NULL material).  Returns the voxel index or -1. */
+   mat < 0 removes the voxel (the reference's NULL material).  Returns the voxel index or -1.
+   Limits of this library: lattice indices in -32768..32767 and at most 50,000,000 voxels per
+   simulation (the device's element indices are 32-bit); beyond them -1 and an error. */
 int vxg_voxel_set(vxg_sim* s, int mat, int ix, int iy, int iz);
 int vxg_voxel_count(vxg_sim* s);                      /* voxelCount()  Voxelyze.h:98 */
 /* n setVoxel calls in one: voxel k gets material mat[k] (< 0 removes) at ijk[3k..3k+2];

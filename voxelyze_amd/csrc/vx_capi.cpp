@@ -789,7 +789,8 @@ API int vxg_load_json(vxg_sim* h, const char* path) {
             const int mi = (int)vox->arr[4 * i + 3].num;
             vx::Mat* m = M(s, mi);
             if (!m) continue;
-            s->addVoxel(m, (int)vox->arr[4 * i].num, (int)vox->arr[4 * i + 1].num, (int)vox->arr[4 * i + 2].num);
+            if (s->addVoxel(m, (int)vox->arr[4 * i].num, (int)vox->arr[4 * i + 1].num, (int)vox->arr[4 * i + 2].num) < 0)
+                return 0;
         }
     }
     const vx::JValue* exts = doc.get("externals");

@@ -623,6 +623,14 @@ int Sim::addVoxel(Mat* m, int x, int y, int z) {
         err = "new voxels can only be added to the newest robot";
         return -1;
     }
+    if (x < -32768 || x > 32767 || y < -32768 || y > 32767 || z < -32768 || z > 32767) {
+        err = "voxel index outside -32768..32767";
+        return -1;
+    }
+    if (nv() >= kMaxVoxels) {
+        err = "a simulation holds at most 50,000,000 voxels (the device's element indices are 32-bit)";
+        return -1;
+    }
     const int vi = nv();
     vrobot.push_back(curRobot);
     vphase.push_back(0.0f);

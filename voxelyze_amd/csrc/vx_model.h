@@ -245,6 +245,9 @@ struct Sim {
     bool robotCom(double* com);
     bool robotStateInfo(int info, int type, double* out);
     /* checkpoints of the dynamic state on the device (vx_checkpoint.cpp) */
+    /* the largest device array is 13 FP32/FP64 values per link slot, 3 slots per voxel
+       (39 * nv elements), indexed with int */
+    static const int kMaxVoxels = 50000000;
     static const int kCheckpointSlots = 8;
     std::shared_ptr<StateCheckpoint> checkpoints[kCheckpointSlots];
     bool checkpointSave(int slot);

@@ -232,6 +232,64 @@ def bench_robots(n_robots, steps, fp64_value):
                                "after_steps": steps + 20}}
 
 
+def bench_contacts(steps):
+    """Nonlinear material and self-contact: the 64^3-site porous block (bilinear material with
+    yield and failure, 20 % spherical voids, bottom face fixed, top face driven down 0.1 pitch
+    per 1,000 steps, surface self-collision on), in mixed precision and in FP64.  Timed: the
+    step launches (grid broadphase, contact forces, link and voxel updates); the host-side
+    drive of the top face between 100-step chunks is not."""
+    import ctypes as C
+    import torch
+    from voxelyze_amd import workloads as W
+    from voxelyze_amd.api import CVoxelyze, FP64_PLAIN, MIXED
+    out, states = {}, {}
+    for name, prec in (("mixed", MIXED), ("fp64", FP64_PLAIN)):
+        t0 = time.perf_counter()
+        sim = CVoxelyze(0.005)
+        info = W.porous_block(sim, 64)
+        sim.setPrecision(prec)
+        V, L = sim.voxelCount(), sim.linkCount()
+        nominal = sim.state()["pos"]
+        dt = sim.recommendedTimeStep()
+        W.porous_block_drive(sim, info["top"], 0, 0.005)
+        sim.step(dt, 50)  # warm-up, during which the autotuner settles
+        torch.cuda.synchronize()
+        print(f"[bench] contacts {name}: porous block {V} voxels, {L} links, built in "
+              f"{time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+        sim.abi.profile(sim.handle, 1)
+        sim.abi.profile_read(sim.handle, None, None, 0)
+        el, done = 0.0, 50
+        for _ in range(steps // 100):
+            W.porous_block_drive(sim, info["top"], done, 0.005)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            n, at = sim.step(dt, 100)
+            torch.cuda.synchronize()
+            el += time.perf_counter() - t0
+            if n != 100:
+                raise SystemExit(f"bench: the porous block diverged at step {done + at}")
+            done += 100
+        sim.abi.profile(sim.handle, 0)
+        nc = len(PROFILE_CLASSES)
+        ms, cnt = (C.c_double * nc)(), (C.c_int * nc)()
+        sim.abi.profile_read(sim.handle, ms, cnt, nc)
+        timed = done - 50
+        kernels = {PROFILE_CLASSES[i]: round(ms[i] / timed, 4) for i in range(nc) if cnt[i]}
+        out[name] = {"value": V * timed / el, "unit": "voxel-steps/s", "ms_per_step": el / timed * 1e3,
+                     "kernel_ms_per_step": kernels, "contact_pairs": sim.collisionCount()}
+        states[name] = (sim.state()["pos"], nominal)
+        print(f"[bench] contacts {name}: {el / timed * 1e3:.4f} ms/step, {V * timed / el:.3e} voxel-steps/s, "
+              f"{sim.collisionCount()} contact pairs, kernels " + ", ".join(f"{k} {v:.4f} ms" for k, v in kernels.items()),
+              file=sys.stderr, flush=True)
+        del sim
+    (pm, nominal), (pf, _) = states["mixed"], states["fp64"]
+    out["mixed_vs_fp64_rel_disp_err"] = float(np.abs(pm - pf).max() / max(np.abs(pf - nominal).max(), 1e-300))
+    out.update({"workload": "porous block 64^3 sites, bilinear E=5 MPa -> 0.5 MPa at 0.05 strain, failure at "
+                            "0.3 strain (3.75e5 Pa), 20 % voids r=2-6, bottom fixed, top driven -0.1 pitch/1000 "
+                            "steps, self-collision", "voxels": V, "links": L, "steps": timed, "after_steps": done})
+    return out
+
+
 def bench_adjoint(n_robots, steps):
     """Adjoint cost: the 32x32x16, 70%-fill robots of three materials, mixed precision; the
     forward is `steps` plain steps, the backward is vxg_adjoint_backward (segment recompute
@@ -289,6 +347,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="measure the selected precision only")
     ap.add_argument("--no-robots", action="store_true", help="skip the 4,096-robot batch")
+    ap.add_argument("--no-contacts", action="store_true", help="skip the porous block with self-collision")
+    ap.add_argument("--contact-steps", type=int, default=1000)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward cost")
     ap.add_argument("--robots", type=int, default=4096)
     ap.add_argument("--robot-steps", type=int, default=100)
@@ -358,6 +418,8 @@ def main():
         del sim2
     if not a.no_robots:
         line["robots"] = bench_robots(a.robots, a.robot_steps, r["value"])
+    if not a.no_contacts:
+        line["contacts"] = bench_contacts(a.contact_steps)
     if not a.no_adjoint:
         line["adjoint"] = bench_adjoint(a.adjoint_robots, a.adjoint_steps)
     if not a.no_cpu_baseline:

@@ -41,14 +41,14 @@ __device__ __forceinline__ uint32_t cell_hash(long long cx, long long cy, long l
 }
 
 /* CVoxelyze::updateCollisions stale test (src/Voxelyze.cpp:686-696) */
-__global__ void k_col_stale(DevPtrs s, ContactPtrs c) {
+__global__ void k_col_stale(DevPtrs s, ContactPtrs c, int gen) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= s.nv || (s.vflags[i] & VF_INTERIOR)) return;
     const int nv = s.nv;
     const double dx = s.vd[VPX * nv + i] - (double)c.last[i];
     const double dy = s.vd[VPY * nv + i] - (double)c.last[nv + i];
     const double dz = s.vd[VPZ * nv + i] - (double)c.last[2 * nv + i];
-    if (dx * dx + dy * dy + dz * dz > c.recalcSq) atomicOr(c.flag, 1);
+    if (dx * dx + dy * dy + dz * dz > c.recalcSq) s.ctl[3] = gen; /* every writer stores the same value */
 }
 
 /* rebuild step 1: remember positions (Voxelyze.cpp:733) and key surface voxels by cell hash */
@@ -166,8 +166,8 @@ __global__ void k_col_forces(DevPtrs s, ContactPtrs c) {
 
 static inline unsigned nblk(int n) { return (unsigned)((n + 255) / 256); }
 
-void launch_col_stale(const DevPtrs& s, const ContactPtrs& c, hipStream_t st) {
-    hipLaunchKernelGGL(k_col_stale, dim3(nblk(s.nv)), dim3(256), 0, st, s, c);
+void launch_col_stale(const DevPtrs& s, const ContactPtrs& c, int gen, hipStream_t st) {
+    hipLaunchKernelGGL(k_col_stale, dim3(nblk(s.nv)), dim3(256), 0, st, s, c, gen);
 }
 void launch_col_keys(const DevPtrs& s, const ContactPtrs& c, int nbuckets, hipStream_t st) {
     hipLaunchKernelGGL(k_col_keys, dim3(nblk(s.nv)), dim3(256), 0, st, s, c);

@@ -174,7 +174,6 @@ struct ContactPtrs {
     int* pairOff;
     int* partners;
     float* cf;                   /* 3 per listed partner */
-    int* flag;                   /* [0] stale */
     double cellInv;
     float threshSq;              /* watchRadius^2 (float, as regenerateCollisions receives it) */
     float recalcSq;              /* recalcDist^2 (float) */

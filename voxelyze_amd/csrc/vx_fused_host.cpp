@@ -15,7 +15,7 @@ void launch_fused_prevdt(const int* vrobot, const int* rdiv, const float* rdt, f
 void launch_fused_restore(double* vd, const double* vdPrev, uint32_t* vfl, const uint32_t* vflPrev, float* vtemp,
                           const float* vtempPrev, float* rtime, const float* rtimePrev, int nv, int first, int count,
                           int rob, hipStream_t st);
-void launch_col_stale(const DevPtrs& s, const ContactPtrs& c, hipStream_t st);
+void launch_col_stale(const DevPtrs& s, const ContactPtrs& c, int gen, hipStream_t st);
 void launch_poisson_fp64(const DevPtrs& s, int k, hipStream_t st);
 uint64_t graph_key(const void* args, size_t n, int path, int nsteps, int flags);
 
@@ -141,11 +141,8 @@ bool Sim::stepFp64Fused(int nsteps) {
         if (collisions) {
             /* the watch list from the positions this step starts from (the link phase does
                not move voxels, so this is what updateCollisions sees, Voxelyze.cpp:271) */
-            HIPCHK(hipMemsetAsync(cp.flag, 0, 4, stream));
-            launch_col_stale(dp, cp, stream);
-            HIPCHK(hipMemcpyAsync(ctlHost, dp.ctl, 4 * sizeof(int), hipMemcpyDeviceToHost, stream));
-            HIPCHK(hipMemcpyAsync(ctlHost + 3, cp.flag, 4, hipMemcpyDeviceToHost, stream));
-            HIPCHK(hipStreamSynchronize(stream));
+            bool stale = false;
+            if (!staleCheck(&stale)) return false;
             bool all = false;
             if (ctlHost[0] && !anyDiverged(k - 1, &all)) return false;
             if (all) {
@@ -158,7 +155,7 @@ bool Sim::stepFp64Fused(int nsteps) {
                     nearbyStale = false;
                     collisionsStale = true;
                 }
-                if (ctlHost[3]) collisionsStale = true;
+                if (stale) collisionsStale = true;
                 if (collisionsStale && !regenerateCollisions()) return false;
             }
         }

@@ -13,7 +13,7 @@ namespace vx {
 void launch_step_mixed(const MixPtrs& s, bool fp64, int block, hipStream_t st);
 void launch_mixed_temps(const MixPtrs& s, float* temp, hipStream_t st);
 uint64_t graph_key(const void* args, size_t n, int path, int nsteps, int flags);
-void launch_col_stale(const DevPtrs& s, const ContactPtrs& c, hipStream_t st);
+void launch_col_stale(const DevPtrs& s, const ContactPtrs& c, int gen, hipStream_t st);
 void launch_fused_restore(double* vd, const double* vdPrev, uint32_t* vfl, const uint32_t* vflPrev, float* vtemp,
                           const float* vtempPrev, float* rtime, const float* rtimePrev, int nv, int first, int count,
                           int rob, hipStream_t st);
@@ -186,11 +186,8 @@ bool Sim::stepMixed(int nsteps, const MixTape* tape) {
     int launched = 0;
     for (int k = 0; k < nsteps && !lastStep; k++) {
         if (collisions) {
-            HIPCHK(hipMemsetAsync(cp.flag, 0, 4, stream));
-            launch_col_stale(dp, cp, stream);
-            HIPCHK(hipMemcpyAsync(ctlHost, dp.ctl, 4 * sizeof(int), hipMemcpyDeviceToHost, stream));
-            HIPCHK(hipMemcpyAsync(ctlHost + 3, cp.flag, 4, hipMemcpyDeviceToHost, stream));
-            HIPCHK(hipStreamSynchronize(stream));
+            bool stale = false;
+            if (!staleCheck(&stale)) return false;
             bool all = false;
             if (ctlHost[0] && !anyDiverged(k - 1, &all)) return false;
             if (all) {
@@ -203,7 +200,7 @@ bool Sim::stepMixed(int nsteps, const MixTape* tape) {
                     nearbyStale = false;
                     collisionsStale = true;
                 }
-                if (ctlHost[3]) collisionsStale = true;
+                if (stale) collisionsStale = true;
                 if (collisionsStale && !regenerateCollisions()) return false;
             }
         }

@@ -9,6 +9,7 @@
 #include "vx_model.h"
 
 #include <float.h>
+#include <limits.h>
 #include <math.h>
 #include <string.h>
 
@@ -25,7 +26,7 @@ void launch_poisson_fp64(const DevPtrs& s, int k, hipStream_t st);
 void launch_robot_state(const DevPtrs& s, const int* rfirst, int info, int type, double* out, int nrobots,
                         hipStream_t st);
 void launch_robot_com(const DevPtrs& s, const int* rfirst, double* com, int nrobots, hipStream_t st);
-void launch_col_stale(const DevPtrs& s, const ContactPtrs& c, hipStream_t st);
+void launch_col_stale(const DevPtrs& s, const ContactPtrs& c, int gen, hipStream_t st);
 void launch_col_keys(const DevPtrs& s, const ContactPtrs& c, int nbuckets, hipStream_t st);
 hipError_t col_sort(const ContactPtrs& c, void* tmp, size_t* tmpBytes, hipStream_t st);
 hipError_t col_scan(const ContactPtrs& c, void* tmp, size_t* tmpBytes, hipStream_t st);
@@ -1378,7 +1379,7 @@ bool Sim::contactBuffers() {
     const size_t vc = (size_t)std::max(n, 1);
     if (!bcLast.ensure(3 * vc * 4) || !bcMask.ensure(8 * vc * 4) || !bcKeys.ensure(vc * 4) || !bcVals.ensure(vc * 4) ||
         !bcKeysS.ensure(vc * 4) || !bcValsS.ensure(vc * 4) || !bcCount.ensure(vc * 4) || !bcOff.ensure(vc * 4) ||
-        !bcFlag.ensure(16) || !bcOffBit.ensure(11 * 11 * 11)) {
+        !bcOffBit.ensure(11 * 11 * 11)) {
         err = "device allocation failed";
         return false;
     }
@@ -1413,7 +1414,6 @@ bool Sim::contactBuffers() {
     cp.hashMask = (uint32_t)(nb - 1);
     cp.pairCount = (int*)bcCount.p;
     cp.pairOff = (int*)bcOff.p;
-    cp.flag = (int*)bcFlag.p;
     cp.cellInv = 1.0 / (1.01 * (double)watchRadiusMm);
     cp.threshSq = watchRadiusMm * watchRadiusMm;
     cp.recalcSq = recalcDist * recalcDist;
@@ -1502,6 +1502,19 @@ bool Sim::regenerateCollisions() {
     return true;
 }
 
+/* The watch list's stale test (CVoxelyze::updateCollisions, src/Voxelyze.cpp:686-696) on the
+   positions the coming contact forces use, and the launch's control block (ctl: [0] some robot
+   diverged) brought to ctlHost in the same copy.  The test kernel stores this test's number in
+   ctl[3] if any surface voxel moved too far, so ctl[3] needs no clearing between tests. */
+bool Sim::staleCheck(bool* stale) {
+    staleGen = staleGen == INT_MAX ? 1 : staleGen + 1;
+    launch_col_stale(dp, cp, staleGen, stream);
+    HIPCHK(hipMemcpyAsync(ctlHost, dp.ctl, 4 * sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    *stale = ctlHost[3] == staleGen;
+    return true;
+}
+
 /* doTimeStep with self-collision: link phase, CVoxelyze::updateCollisions, voxel phase.
    The host inspects the divergence and staleness flags between the phases of each step. */
 bool Sim::stepWithContacts(int nsteps) {
@@ -1513,11 +1526,8 @@ bool Sim::stepWithContacts(int nsteps) {
         if (actOn) launch_actuate_fp64(dp, k, stream);
         launch_poisson_fp64(dp, k, stream);
         launch_links_fp64(dp, k, tuner[1].chosen, stream);
-        HIPCHK(hipMemsetAsync(cp.flag, 0, 4, stream));
-        launch_col_stale(dp, cp, stream);
-        HIPCHK(hipMemcpyAsync(ctlHost, dp.ctl, 4 * sizeof(int), hipMemcpyDeviceToHost, stream));
-        HIPCHK(hipMemcpyAsync(ctlHost + 3, cp.flag, 4, hipMemcpyDeviceToHost, stream));
-        HIPCHK(hipStreamSynchronize(stream));
+        bool stale = false;
+        if (!staleCheck(&stale)) return false;
         if (ctlHost[0]) {
             bool all = false;
             if (!anyDiverged(k, &all)) return false;
@@ -1528,7 +1538,7 @@ bool Sim::stepWithContacts(int nsteps) {
             nearbyStale = false;
             collisionsStale = true;
         }
-        if (ctlHost[3]) collisionsStale = true;
+        if (stale) collisionsStale = true;
         if (collisionsStale && !regenerateCollisions()) return false;
         dp.colCnt = cp.pairCount;
         dp.colOff = cp.pairOff;

@@ -178,10 +178,12 @@ struct Sim {
     int nBuckets = 0;
     ContactPtrs cp{};
     DevBuf bcLast, bcMask, bcOffBit, bcKeys, bcVals, bcKeysS, bcValsS, bcBStart, bcBEnd, bcCount, bcOff, bcPartners,
-        bcF, bcFlag, bcTmp;
+        bcF, bcTmp;
     bool contactBuffers();
     bool nearMasks();
     bool regenerateCollisions();
+    int staleGen = 0; /* number of the last stale test */
+    bool staleCheck(bool* stale);
     /* captured step sequences (HIP graphs): a multi-step launch is recorded once and replayed
        while its buffers and length stay the same */
     struct StepGraph {

@@ -290,6 +290,43 @@ def bench_contacts(steps):
     return out
 
 
+def bench_linear(nx=128, ny=128, nz=64):
+    """The static linear solve (CVoxelyze::doLinearSolve) on a filled nx*ny*nz block: bottom
+    face fixed, a lateral load spread over the top face.  The second of two identical solves is
+    timed (the first also uploads the new model); it includes forming the right-hand side on
+    the host, the conjugate-gradient iterations and the write-back."""
+    import torch
+    from voxelyze_amd.api import CVoxelyze
+    t0 = time.perf_counter()
+    sim = CVoxelyze(0.005)
+    m = sim.addMaterial(1e6, 1000.0)
+    g = np.stack(np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij"), -1).reshape(-1, 3)
+    ijk = g[np.lexsort((g[:, 0], g[:, 1], g[:, 2]))].astype(np.int32)
+    sim.setVoxels(np.zeros(len(ijk), np.int32), ijk)
+    for i in np.nonzero(ijk[:, 2] == 0)[0]:
+        sim.voxel(int(i)).external().setFixedAll()
+    top = np.nonzero(ijk[:, 2] == nz - 1)[0]
+    for i in top:
+        sim.voxel(int(i)).external().setForce(1e-3 / len(top), 0.0, 0.0)
+    V = sim.voxelCount()
+    if not sim.doLinearSolve():
+        raise SystemExit(f"bench: linear solve failed: {sim.lastError()}")
+    torch.cuda.synchronize()
+    print(f"[bench] linear: {nx}x{ny}x{nz} block, {V} voxels ({6 * V} degrees of freedom), built and solved "
+          f"once in {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    ok = sim.doLinearSolve()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    info = sim.linearSolveInfo
+    print(f"[bench] linear: {el:.3f} s, {info['iterations']} iterations, relative residual "
+          f"{info['relative_residual']:.2e}", file=sys.stderr, flush=True)
+    return {"workload": f"filled {nx}x{ny}x{nz} block, E=1 MPa, bottom face fixed, 1 mN lateral load on the top face",
+            "voxels": V, "dof": 6 * V, "converged": bool(ok), "iterations": info["iterations"],
+            "relative_residual": info["relative_residual"], "seconds": el,
+            "ms_per_iteration": el / max(info["iterations"], 1) * 1e3}
+
+
 def bench_adjoint(n_robots, steps):
     """Adjoint cost: the 32x32x16, 70%-fill robots of three materials, mixed precision; the
     forward is `steps` plain steps, the backward is vxg_adjoint_backward (segment recompute
@@ -348,6 +385,7 @@ def main():
     ap.add_argument("--no-compare", action="store_true", help="measure the selected precision only")
     ap.add_argument("--no-robots", action="store_true", help="skip the 4,096-robot batch")
     ap.add_argument("--no-contacts", action="store_true", help="skip the porous block with self-collision")
+    ap.add_argument("--no-linear", action="store_true", help="skip the static linear solve")
     ap.add_argument("--contact-steps", type=int, default=1000)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward cost")
     ap.add_argument("--robots", type=int, default=4096)
@@ -420,6 +458,8 @@ def main():
         line["robots"] = bench_robots(a.robots, a.robot_steps, r["value"])
     if not a.no_contacts:
         line["contacts"] = bench_contacts(a.contact_steps)
+    if not a.no_linear:
+        line["linear"] = bench_linear()
     if not a.no_adjoint:
         line["adjoint"] = bench_adjoint(a.adjoint_robots, a.adjoint_steps)
     if not a.no_cpu_baseline:

@@ -251,6 +251,8 @@ public:
     CVoxelyze& operator=(const CVoxelyze&) = delete;
 
     bool doTimeStep(float dt = -1.0f) { return vxg_do_time_step(sim, dt) != 0; }
+    /* Voxelyze.h:80; solved on the GPU (see vxg_linear_solve) */
+    bool doLinearSolve() { return vxg_linear_solve(sim, 0.0, 0, nullptr, nullptr) != 0; }
     bool loadJSON(const char* jsonFilePath) { /* Voxelyze.h:77: the model is replaced */
         const bool ok = vxg_load_json(sim, jsonFilePath) != 0;
         voxels.clear();

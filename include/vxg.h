@@ -123,6 +123,17 @@ int vxg_step(vxg_sim* s, float dt, int nsteps, int* diverged_at);
 float vxg_recommended_time_step(vxg_sim* s);
 /* CVoxelyze::resetTime()                            include/Voxelyze.h:84, src/Voxelyze.cpp:313-321 */
 void vxg_reset_time(vxg_sim* s);
+/* CVoxelyze::doLinearSolve()                        include/Voxelyze.h:80, src/Voxelyze.cpp:243-248
+   with CVX_LinearSolver::solve (src/VX_LinearSolver.cpp:48-122): the model linearised about
+   the nominal lattice, fixed degrees of freedom held at their current values, free ones loaded
+   with the external forces and moments; positions and orientations are set to the solution
+   and momenta zeroed (postResults, src/VX_LinearSolver.cpp:316-326).  Solved on the device by
+   preconditioned conjugate gradients to the relative residual `tolerance` (<= 0: 1e-10)
+   within `max_iterations` (<= 0: 100000), in place of the reference's Pardiso.  Returns 1, or
+   0 with the state unchanged (no voxels, per-voxel moduli set, a voxel with free degrees of
+   freedom and no links, no convergence; vxg_last_error says which).  iterations and
+   residual (the final relative residual) may be NULL. */
+int vxg_linear_solve(vxg_sim* s, double tolerance, int max_iterations, int* iterations, double* residual);
 /* CVoxelyze::currentTime (private member, Voxelyze.h:138); of the selected robot */
 float vxg_current_time(vxg_sim* s);
 /* CVoxelyze::stateInfo(stateInfoType, valueType)    include/Voxelyze.h:134, src/Voxelyze.cpp:752-800 */

@@ -17,6 +17,7 @@
 #include "VX_Voxel.h"
 #include "VX_Link.h"
 #include "VX_External.h"
+#include "VX_LinearSolver.h"
 
 #include <stdint.h>
 #include <string>
@@ -325,4 +326,51 @@ EXPORT void vxr_link_get_state(vxr_sim* s, int first, int count, float* strain, 
         if (forces12) CVXS_SimGLView::forces(l, forces12 + 12 * k);
         if (rest) rest[k] = CVXS_SimGLView::restLength(l);
     }
+}
+
+// ------------------------------------------------------------ linear system
+// The reference's own assembled linear system (CVX_LinearSolver::calculateA + applyBX +
+// convertTo1Base, src/VX_LinearSolver.cpp:127-314).  Built without Pardiso, solve() forms
+// the matrices and then returns false before solving (src/VX_LinearSolver.cpp:104-107), so
+// the solver object holds A (upper triangle, 1-based CSR), b and x.  Those members are
+// private; they are reached through explicit template instantiation, where naming a private
+// member is allowed ([temp.explicit]), without touching the reference's headers.
+template <class Tag, typename Tag::type M>
+struct VxrAccess {
+    friend typename Tag::type vxr_member(Tag) { return M; }
+};
+struct LsA { typedef std::vector<double> CVX_LinearSolver::*type; friend type vxr_member(LsA); };
+struct LsB { typedef std::vector<double> CVX_LinearSolver::*type; friend type vxr_member(LsB); };
+struct LsX { typedef std::vector<double> CVX_LinearSolver::*type; friend type vxr_member(LsX); };
+struct LsIa { typedef std::vector<int> CVX_LinearSolver::*type; friend type vxr_member(LsIa); };
+struct LsJa { typedef std::vector<int> CVX_LinearSolver::*type; friend type vxr_member(LsJa); };
+struct LsDof { typedef int CVX_LinearSolver::*type; friend type vxr_member(LsDof); };
+template struct VxrAccess<LsA, &CVX_LinearSolver::a>;
+template struct VxrAccess<LsB, &CVX_LinearSolver::b>;
+template struct VxrAccess<LsX, &CVX_LinearSolver::x>;
+template struct VxrAccess<LsIa, &CVX_LinearSolver::ia>;
+template struct VxrAccess<LsJa, &CVX_LinearSolver::ja>;
+template struct VxrAccess<LsDof, &CVX_LinearSolver::dof>;
+
+// Sizes on every call (*dof, *nnz); the arrays are filled when cap >= nnz (ia: dof + 1,
+// ja and a: nnz, b and x: dof).  Returns 1 when the system was formed, 0 when not (no voxels).
+EXPORT int vxr_linear_system(vxr_sim* s, int* dof, int* nnz, int cap, int* ia, int* ja, double* a, double* b,
+                             double* x) {
+    CVX_LinearSolver solver(&s->sim);
+    solver.solve();
+    const int n = solver.*vxr_member(LsDof());
+    const std::vector<double>& A = solver.*vxr_member(LsA());
+    *dof = n;
+    *nnz = n > 0 ? (int)A.size() : 0;
+    if (n <= 0) return 0;
+    if (cap >= *nnz && ia && ja && a && b && x) {
+        const std::vector<int>& IA = solver.*vxr_member(LsIa());
+        const std::vector<int>& JA = solver.*vxr_member(LsJa());
+        const std::vector<double>& B = solver.*vxr_member(LsB());
+        const std::vector<double>& X = solver.*vxr_member(LsX());
+        for (int i = 0; i <= n; i++) ia[i] = IA[i];
+        for (int k = 0; k < *nnz; k++) { ja[k] = JA[k]; a[k] = A[k]; }
+        for (int i = 0; i < n; i++) { b[i] = B[i]; x[i] = X[i]; }
+    }
+    return 1;
 }

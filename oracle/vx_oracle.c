@@ -36,6 +36,10 @@
  * (src/VX_Material.cpp:165-195).  With a Poisson material the link loop runs serially
  * in linksList order, as in the reference, because a voxel's Poisson strain is cached
  * at the first request of each step.
+ * Static linear solve: the linear system CVX_LinearSolver forms (calculateA, addAValue,
+ * consolidateA, applyBX, convertTo1Base; src/VX_LinearSolver.cpp:127-314), pinned bit for
+ * bit against the compiled reference's own system (tests/test_linear.py); the solve itself
+ * is left to the test (a direct sparse solve), as the reference leaves it to Pardiso.
  */
 #include <float.h>
 #include <math.h>
@@ -1898,4 +1902,159 @@ EXPORT void vxo_link_get_state(sim_t* s, int first, int count, float* strain, fl
         }
         if (restLength) restLength[k] = l->currentRestLength;
     }
+}
+
+/* ---------------------------------------------------------------- linear system
+   The upper triangle of the stiffness matrix linearised about the nominal lattice, as
+   CVX_LinearSolver::calculateA (src/VX_LinearSolver.cpp:127-231) lays it out and fills it,
+   with the zero entries removed (consolidateA, :245-265), then the prescribed degrees of
+   freedom eliminated into b (applyBX, :267-308), returned 1-based (convertTo1Base, :310-314).
+   Sizes on every call; the arrays are filled when cap >= *nnz. */
+static const int LS_BLOCK_OFF[6][3] = {{0, 4, 5}, {1, 3, 5}, {2, 3, 4}, {1, 2, 3}, {0, 2, 4}, {0, 1, 5}};
+
+/* a[row][col] += value where (row, col) is in the pattern (addAValue, :233-239) */
+static void ls_add(const int* rp, const int* cols, double* vals, int row, int col, float value) {
+    for (int k = rp[row]; k < rp[row + 1]; k++)
+        if (cols[k] == col) {
+            vals[k] += value;
+            return;
+        }
+}
+
+EXPORT int vxo_linear_system(sim_t* s, int* dofOut, int* nnzOut, int cap, int* ia, int* ja, double* a, double* b,
+                             double* x) {
+    const int nvx = s->nvlist, dof = 6 * nvx;
+    *dofOut = dof;
+    *nnzOut = 0;
+    if (dof == 0) return 0;
+    int* idx = (int*)malloc(sizeof(int) * (size_t)(s->nvslots > 0 ? s->nvslots : 1));
+    for (int i = 0; i < nvx; i++) idx[s->vlist[i]] = i;
+    /* pattern: per voxel the diagonal block's upper entries, then 3 columns per higher-index
+       neighbour (in increasing index) */
+    int* rp = (int*)calloc((size_t)dof + 1, sizeof(int));
+    const size_t maxnz = (size_t)12 * nvx + (size_t)18 * s->nllist + 1;
+    int* cols = (int*)malloc(sizeof(int) * maxnz);
+    double* vals = (double*)calloc(maxnz, sizeof(double));
+    int nz = 0;
+    for (int i = 0; i < nvx; i++) {
+        const ovox* v = V(s, s->vlist[i]);
+        int up[6], nup = 0;
+        for (int d = 0; d < 6; d++) {
+            if (v->links[d] < 0) continue;
+            const olink* l = Lk(s, v->links[d]);
+            const int other = idx[l->vNeg == s->vlist[i] ? l->vPos : l->vNeg];
+            if (other > i) up[nup++] = other;
+        }
+        qsort(up, (size_t)nup, sizeof(int), cmp_int);
+        for (int j = 0; j < 6; j++) {
+            cols[nz++] = 6 * i + j;
+            if (j < 3) {
+                cols[nz++] = 6 * i + LS_BLOCK_OFF[j][1];
+                cols[nz++] = 6 * i + LS_BLOCK_OFF[j][2];
+            }
+            for (int m = 0; m < nup; m++)
+                for (int k = 0; k < 3; k++) cols[nz++] = 6 * up[m] + LS_BLOCK_OFF[j][k];
+            rp[6 * i + j + 1] = nz;
+        }
+    }
+    /* values, link by link in linksList order (:153-230) */
+    for (int li = 0; li < s->nllist; li++) {
+        const olink* l = Lk(s, s->llist[li]);
+        int i1 = idx[l->vNeg], i2 = idx[l->vPos];
+        if (i1 > i2) { const int t = i1; i1 = i2; i2 = t; }
+        const int ax = l->axis;
+        const omat* m = l->mat;
+        for (int j = 0; j < 6; j++) {
+            const int r1 = 6 * i1 + j, r2 = 6 * i2 + j;
+            float dd, od;
+            if (j < 3) {
+                dd = ax == j ? m->a1 : m->b1;
+                od = -dd;
+            } else {
+                dd = ax == j % 3 ? m->a2 : 2 * m->b3;
+                od = ax == j % 3 ? -m->a2 : m->b3;
+            }
+            ls_add(rp, cols, vals, r1, r1, dd);
+            ls_add(rp, cols, vals, r1, r2, od);
+            ls_add(rp, cols, vals, r2, r2, dd);
+        }
+        int R1, C1, R2, C2;
+        float val;
+        if (ax == 0) { R1 = 1; C1 = 5; R2 = 2; C2 = 4; val = m->b2; }
+        else if (ax == 1) { R1 = 0; C1 = 5; R2 = 2; C2 = 3; val = -m->b2; }
+        else { R1 = 0; C1 = 4; R2 = 1; C2 = 3; val = m->b2; }
+        ls_add(rp, cols, vals, 6 * i1 + R1, 6 * i1 + C1, val);
+        ls_add(rp, cols, vals, 6 * i1 + R1, 6 * i2 + C1, val);
+        ls_add(rp, cols, vals, 6 * i1 + C1, 6 * i2 + R1, -val);
+        ls_add(rp, cols, vals, 6 * i2 + R1, 6 * i2 + C1, -val);
+        ls_add(rp, cols, vals, 6 * i1 + R2, 6 * i1 + C2, -val);
+        ls_add(rp, cols, vals, 6 * i1 + R2, 6 * i2 + C2, -val);
+        ls_add(rp, cols, vals, 6 * i1 + C2, 6 * i2 + R2, val);
+        ls_add(rp, cols, vals, 6 * i2 + R2, 6 * i2 + C2, val);
+    }
+    /* drop the entries that stayed (or summed to) zero, row by row */
+    int w = 0;
+    for (int r = 0; r < dof; r++) {
+        const int start = rp[r], end = rp[r + 1];
+        rp[r] = w;
+        for (int k = start; k < end; k++)
+            if (vals[k] != 0) {
+                cols[w] = cols[k];
+                vals[w] = vals[k];
+                w++;
+            }
+    }
+    rp[dof] = w;
+    nz = w;
+    /* prescribed values and loads (:276-287) */
+    double* xx = (double*)malloc(sizeof(double) * (size_t)dof);
+    double* bb = (double*)calloc((size_t)dof, sizeof(double));
+    char* fixed = (char*)calloc((size_t)dof, 1);
+    for (int i = 0; i < nvx; i++) {
+        const ovox* v = V(s, s->vlist[i]);
+        const dvec op = vox_originalPosition(v);
+        const dvec disp = dv(v->pos.x - op.x, v->pos.y - op.y, v->pos.z - op.z);
+        const dvec ang = v->orient.w == 1 ? dv(0, 0, 0) : qtorv(v->orient);
+        const double dc[6] = {disp.x, disp.y, disp.z, ang.x, ang.y, ang.z};
+        const oext* e = v->ext;
+        const float load[6] = {e ? e->force.x : 0.0f, e ? e->force.y : 0.0f, e ? e->force.z : 0.0f,
+                               e ? e->moment.x : 0.0f, e ? e->moment.y : 0.0f, e ? e->moment.z : 0.0f};
+        for (int j = 0; j < 6; j++) {
+            const int d = 6 * i + j;
+            xx[d] = dc[j];
+            fixed[d] = e ? ((e->dof & (1 << j)) != 0) : 0;
+            if (!fixed[d]) bb[d] = load[j];
+        }
+    }
+    /* eliminate each prescribed value from the other rows (:289-303): the entries of its
+       column above the diagonal, row by row, then its own row; entry zeroing is deferred */
+    char* zap = (char*)calloc((size_t)(nz > 0 ? nz : 1), 1);
+    for (int f = 0; f < dof; f++) {
+        if (!fixed[f]) continue;
+        int row = 0;
+        for (int k = 0; k < rp[f + 1]; k++) {
+            const int inRow = k >= rp[f];
+            if (inRow) row = cols[k];
+            if (k == rp[row + 1]) row++;
+            if (cols[k] == f || inRow) {
+                bb[row] -= xx[f] * vals[k];
+                if (k != rp[f]) zap[k] = 1;
+            }
+        }
+    }
+    for (int f = 0; f < dof; f++)
+        if (fixed[f]) {
+            vals[rp[f]] = 1.0;
+            bb[f] = xx[f];
+        }
+    for (int k = 0; k < nz; k++)
+        if (zap[k]) vals[k] = 0;
+    *nnzOut = nz;
+    if (cap >= nz && ia && ja && a && b && x) {
+        for (int r = 0; r <= dof; r++) ia[r] = rp[r] + 1;
+        for (int k = 0; k < nz; k++) { ja[k] = cols[k] + 1; a[k] = vals[k]; }
+        for (int r = 0; r < dof; r++) { b[r] = bb[r]; x[r] = xx[r]; }
+    }
+    free(zap); free(fixed); free(bb); free(xx); free(vals); free(cols); free(rp); free(idx);
+    return 1;
 }

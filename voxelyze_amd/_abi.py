@@ -99,6 +99,7 @@ EXTENSIONS = {
     "autotune": (_I, [_P, _I]),
     "launch_config": (_I, [_P, _I, _IP, _IP, _IP]),
     "set_launch_config": (_I, [_P, _I, _I, _I]),
+    "linear_solve": (_I, [_P, C.c_double, _I, _IP, _DP]),
     "set_fp64_path": (_I, [_P, _I]),
     "load_json": (_I, [_P, C.c_char_p]),
     "robot_state_info": (_I, [_P, _I, _I, _DP]),

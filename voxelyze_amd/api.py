@@ -430,6 +430,17 @@ class CVoxelyze:
             raise RuntimeError(f"{self.abi.name}: {e}")
 
     def recommendedTimeStep(self): return self.abi.recommended_time_step(self.handle)
+
+    def doLinearSolve(self, tolerance=0.0, max_iterations=0):
+        """CVoxelyze::doLinearSolve (src/Voxelyze.cpp:243-248), solved on the device by
+        preconditioned conjugate gradients (include/vxg.h vxg_linear_solve).  Returns True, or
+        False with the state unchanged (lastError() says why).  The iteration count and final
+        relative residual are kept in self.linearSolveInfo."""
+        it, res = C.c_int(0), C.c_double(0.0)
+        ok = bool(self.abi.linear_solve(self.handle, float(tolerance), int(max_iterations), C.byref(it),
+                                        C.byref(res)))
+        self.linearSolveInfo = {"iterations": it.value, "relative_residual": res.value}
+        return ok
     def resetTime(self): self.abi.reset_time(self.handle)
     def currentTime(self): return self.abi.current_time(self.handle)
     def stateInfo(self, info, type_): return self.abi.state_info(self.handle, info, type_)

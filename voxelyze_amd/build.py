@@ -17,7 +17,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libvxg.so")
 SOURCES = ["vx_kernels.hip", "vx_fused.hip", "vx_contact.hip", "vx_mixed.hip", "vx_adjoint.hip", "vx_model.cpp",
-           "vx_fused_host.cpp", "vx_mixed_host.cpp", "vx_adjoint_host.cpp", "vx_checkpoint.cpp", "vx_capi.cpp"]
+           "vx_fused_host.cpp", "vx_mixed_host.cpp", "vx_adjoint_host.cpp", "vx_checkpoint.cpp", "vx_linear.hip", "vx_linear_host.cpp", "vx_capi.cpp"]
 HEADERS = ["vx_device.h", "vx_math.h", "vx_model.h", "vx_step_fp64.h", "vx_contact_force.h", "vx_adjoint.h", "vx_json.h"]
 
 

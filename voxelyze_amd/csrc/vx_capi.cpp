@@ -96,6 +96,11 @@ API int vxg_do_time_step(vxg_sim* h, float dt) {
 }
 API float vxg_recommended_time_step(vxg_sim* h) { return S(h)->recommendedTimeStep(); }
 
+/* CVoxelyze::doLinearSolve (src/Voxelyze.cpp:243-248) */
+API int vxg_linear_solve(vxg_sim* h, double tol, int maxIter, int* iterations, double* residual) {
+    return S(h)->linearSolve(tol, maxIter, iterations, residual) ? 1 : 0;
+}
+
 /* CVoxelyze::resetTime (src/Voxelyze.cpp:313-321) */
 API void vxg_reset_time(vxg_sim* h) {
     Sim* s = S(h);

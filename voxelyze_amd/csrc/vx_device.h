@@ -79,6 +79,25 @@ struct LinkMat {
     float a1;                 /* axial stiffness E*L */
 };
 
+/* buffers of the static linear solve (vx_linear.hip): the solution vectors are structure of
+   arrays, 6 x nv doubles, in the order x, y, z, rx, ry, rz */
+struct LinPtrs {
+    int nv;
+    const int* vnbr;        /* 6 * nv neighbour table, -1 for none */
+    const int* lmeta;       /* link material of each slot (axis * nv + negative voxel) */
+    const LinkMat* lmats;
+    const uint8_t* fixed;   /* bit j: degree of freedom j of the voxel is prescribed */
+    const double* xfix;     /* 6 * nv: prescribed values (only where fixed) */
+    const double* f;        /* 6 * nv: external force and moment (only where free) */
+    double *x, *r, *z, *q;
+    double *p, *p2;         /* search direction, ping-ponged between iterations */
+    float* minv;            /* 21 * nv: packed lower triangle of the inverse diagonal block */
+    double* part;           /* per-workgroup partial sums, 2 per workgroup */
+    double* scal;           /* device scalars, see enum below */
+    int* bad;               /* [0] a voxel's diagonal block is singular */
+};
+enum { S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_BB, S_COUNT };
+
 /* per external record (VX_External.h:93-98) */
 struct ExtRec {
     double tx, ty, tz;        /* extTranslation */

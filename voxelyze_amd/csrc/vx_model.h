@@ -184,6 +184,9 @@ struct Sim {
     bool regenerateCollisions();
     int staleGen = 0; /* number of the last stale test */
     bool staleCheck(bool* stale);
+    /* static linear solve (CVoxelyze::doLinearSolve) */
+    DevBuf blin;
+    bool linearSolve(double tol, int maxIter, int* iters, double* residual);
     /* captured step sequences (HIP graphs): a multi-step launch is recorded once and replayed
        while its buffers and length stay the same */
     struct StepGraph {

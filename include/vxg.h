@@ -178,7 +178,7 @@ int vxg_set_graphs(vxg_sim* s, int enable);
 int vxg_graph_stats(vxg_sim* s, int* captures, int* replays);
 /* Autotuner (no reference counterpart, default on): the workgroup size (64/128/256) and
    block-to-XCD mapping of kernel 0 = fused mixed step, 1 = FP64 link kernel, 2 = FP64 voxel
-   kernel, 3 = fused FP64 step are chosen by timing each candidate twice on the first steps of every model size;
+   kernel, 3 = fused FP64 step are chosen by timing each candidate on the first 24 steps of every model size;
    results do not depend on the choice.  vxg_launch_config reports it (settled = 1 once
    chosen). */
 int vxg_autotune(vxg_sim* s, int enable);

@@ -264,7 +264,10 @@ __global__ void __launch_bounds__(256) k_lin_setup(LinPtrs s) {
 /* the search direction p' = z + beta p (beta = 0 on the first iteration, when p = z), written
    to pout; q = K p' on the free components; alpha = r.z / p'.q */
 __global__ void __launch_bounds__(256) k_lin_apply(LinPtrs s, const double* pin, double* pout) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    /* XCD-contiguous blocks, so the neighbours a voxel gathers along x and y are read in the
+       same XCD's L2 (on a 128x128x64 block: 629 instead of 671 MB fetched per launch; the z
+       neighbours, three lattice layers of vectors apart, do not stay in a 4 MB L2) */
+    const int i = lane_index(1);
     double pq = 0.0;
     if (i < s.nv) {
         const int nv = s.nv;

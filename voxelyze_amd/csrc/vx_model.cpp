@@ -1195,6 +1195,9 @@ bool Sim::anyDiverged(int k, bool* all) {
 
 /* ------------------------------------------------------------ autotuner */
 static const LaunchCfg kCands[6] = {{256, 0}, {128, 0}, {64, 0}, {256, 1}, {128, 1}, {64, 1}};
+/* recorded samples per candidate.  Each candidate runs two steps in a row and only the second
+   is recorded: a step's time depends on what the step before it left in the caches, and the
+   configuration that is kept will follow itself. */
 static const int kTuneSamples = 2;
 
 void Sim::Tuner::reset(int n) {
@@ -1205,8 +1208,10 @@ void Sim::Tuner::reset(int n) {
     for (int c = 0; c < 6; c++) { best[c] = 1e30f; count[c] = 0; }
 }
 LaunchCfg Sim::Tuner::next(int* cand) {
-    *cand = steps++ % 6;
-    return kCands[*cand];
+    const int c = (steps / 2) % 6;
+    *cand = (steps & 1) ? c : -1; /* -1: the first step of the pair, not recorded */
+    steps++;
+    return kCands[c];
 }
 void Sim::Tuner::record(int cand, float ms) {
     if (ms < best[cand]) best[cand] = ms;

@@ -243,13 +243,16 @@ def bench_contacts(steps):
     from voxelyze_amd import workloads as W
     from voxelyze_amd.api import CVoxelyze, FP64_PLAIN, MIXED
     out, states = {}, {}
-    for name, prec in (("mixed", MIXED), ("fp64", FP64_PLAIN)):
+    for name, prec in (("mixed", MIXED), ("fp64", FP64_PLAIN), ("fp64_pushed", FP64_PLAIN)):
         t0 = time.perf_counter()
         sim = CVoxelyze(0.005)
         info = W.porous_block(sim, 64)
         sim.setPrecision(prec)
         V, L = sim.voxelCount(), sim.linkCount()
         nominal = sim.state()["pos"]
+        if name == "fp64_pushed":  # 1e-9 N on the voxel nearest the middle height: FP64's own sensitivity
+            z = nominal[:, 2]
+            sim.voxel(int(np.argmin(np.abs(z - z.max() / 2)))).external().setForce(1e-9, 0.0, 0.0)
         dt = sim.recommendedTimeStep()
         W.porous_block_drive(sim, info["top"], 0, 0.005)
         sim.step(dt, 50)  # warm-up, during which the autotuner settles
@@ -274,16 +277,26 @@ def bench_contacts(steps):
         ms, cnt = (C.c_double * nc)(), (C.c_int * nc)()
         sim.abi.profile_read(sim.handle, ms, cnt, nc)
         timed = done - 50
+        states[name] = (sim.state()["pos"], nominal)
+        if name == "fp64_pushed":
+            del sim
+            continue
         kernels = {PROFILE_CLASSES[i]: round(ms[i] / timed, 4) for i in range(nc) if cnt[i]}
         out[name] = {"value": V * timed / el, "unit": "voxel-steps/s", "ms_per_step": el / timed * 1e3,
                      "kernel_ms_per_step": kernels, "contact_pairs": sim.collisionCount()}
-        states[name] = (sim.state()["pos"], nominal)
         print(f"[bench] contacts {name}: {el / timed * 1e3:.4f} ms/step, {V * timed / el:.3e} voxel-steps/s, "
               f"{sim.collisionCount()} contact pairs, kernels " + ", ".join(f"{k} {v:.4f} ms" for k, v in kernels.items()),
               file=sys.stderr, flush=True)
         del sim
-    (pm, nominal), (pf, _) = states["mixed"], states["fp64"]
-    out["mixed_vs_fp64_rel_disp_err"] = float(np.abs(pm - pf).max() / max(np.abs(pf - nominal).max(), 1e-300))
+    (pm, nominal), (pf, _), (pp, _) = states["mixed"], states["fp64"], states["fp64_pushed"]
+    scale = max(np.abs(pf - nominal).max(), 1e-300)
+    out["mixed_vs_fp64_rel_disp_err"] = float(np.abs(pm - pf).max() / scale)
+    # the same lattice in FP64 with 1e-9 N on one voxel: where this is of the order of the
+    # mixed error, the trajectory (contacts, link branch switches) is sensitive and no precision
+    # reproduces it closer
+    out["fp64_response_to_1e-9N_rel_disp"] = float(np.abs(pp - pf).max() / scale)
+    print(f"[bench] contacts: mixed vs fp64 rel disp err {out['mixed_vs_fp64_rel_disp_err']:.3e}, fp64 response "
+          f"to 1e-9 N on one voxel {out['fp64_response_to_1e-9N_rel_disp']:.3e}", file=sys.stderr, flush=True)
     out.update({"workload": "porous block 64^3 sites, bilinear E=5 MPa -> 0.5 MPa at 0.05 strain, failure at "
                             "0.3 strain (3.75e5 Pa), 20 % voids r=2-6, bottom fixed, top driven -0.1 pitch/1000 "
                             "steps, self-collision", "voxels": V, "links": L, "steps": timed, "after_steps": done})

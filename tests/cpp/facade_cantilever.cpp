@@ -1,7 +1,9 @@
 // A reference-style program (README usage of jonhiller/Voxelyze) compiled
 // against the drop-in header include/Voxelyze.h and linked with libvxg.so.
-// Prints the state of every voxel after the run as hex floats.
+// Prints the state of every voxel after the run as hex floats.  Arguments: the number of
+// doTimeStep() calls, and "linear" to call doLinearSolve() instead.
 #include <cstdio>
+#include <cstring>
 #include <cstdlib>
 
 #include "Voxelyze.h"
@@ -18,8 +20,13 @@ int main(int argc, char** argv) {
                 if (x == 0) v->external()->setFixedAll();
                 if (x == 19) v->external()->setForce(0, 0, -1.0f / 16);
             }
-    for (int i = 0; i < steps; i++)
-        if (!Vx.doTimeStep()) { printf("diverged at %d\n", i); return 1; }
+    const bool linear = argc > 2 && strcmp(argv[2], "linear") == 0;
+    if (linear) {
+        if (!Vx.doLinearSolve()) { printf("linear solve failed\n"); return 1; }
+    } else {
+        for (int i = 0; i < steps; i++)
+            if (!Vx.doTimeStep()) { printf("diverged at %d\n", i); return 1; }
+    }
     CVX_Link* l = Vx.link(0, 0, 0, CVX_Voxel::X_POS);
     printf("links %d strain %a\n", Vx.linkCount(), l->axialStrain());
     for (int i = 0; i < Vx.voxelCount(); i++) {

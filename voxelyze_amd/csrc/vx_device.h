@@ -89,8 +89,7 @@ struct LinPtrs {
     const uint8_t* fixed;   /* bit j: degree of freedom j of the voxel is prescribed */
     const double* xfix;     /* 6 * nv: prescribed values (only where fixed) */
     const double* f;        /* 6 * nv: external force and moment (only where free) */
-    double *x, *r, *z, *q;
-    double *p, *p2;         /* search direction, ping-ponged between iterations */
+    double *x, *r, *z, *p, *q;
     float* minv;            /* 21 * nv: packed lower triangle of the inverse diagonal block */
     double* part;           /* per-workgroup partial sums, 2 per workgroup */
     double* scal;           /* device scalars, see enum below */

@@ -83,16 +83,6 @@ struct Plain {
     int nv;
     __device__ void operator()(int i, double* o) const { load6(a, nv, i, o); }
 };
-/* the new search direction z + beta p, formed where it is read */
-struct Direction {
-    const double *z, *p;
-    double beta;
-    int nv;
-    __device__ void operator()(int i, double* o) const {
-        for (int j = 0; j < 6; j++) o[j] = z[j * nv + i] + beta * p[j * nv + i];
-    }
-};
-
 /* y_i = (K u)_i over the voxel's links; `mask` zeroes the fixed components of u (the other
    voxel's and its own) and of y.  own = u_i on return. */
 template <class Load>
@@ -261,20 +251,29 @@ __global__ void __launch_bounds__(256) k_lin_setup(LinPtrs s) {
     block_sum2(rz, bb, s.part);
 }
 
-/* the search direction p' = z + beta p (beta = 0 on the first iteration, when p = z), written
-   to pout; q = K p' on the free components; alpha = r.z / p'.q */
-__global__ void __launch_bounds__(256) k_lin_apply(LinPtrs s, const double* pin, double* pout) {
+/* the search direction p = z + beta p (beta = 0 on the first iteration, when p = z already) */
+__global__ void __launch_bounds__(256) k_lin_direction(LinPtrs s) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.nv) return;
+    const int nv = s.nv;
+    const double beta = s.scal[S_BETA];
+    for (int j = 0; j < 6; j++) s.p[j * nv + i] = s.z[j * nv + i] + beta * s.p[j * nv + i];
+}
+
+/* q = K p on the free components; partial sums of p.q.  (Forming z + beta p here, where p is
+   read, instead of in its own pass doubles the neighbour reads and was 3-5 % slower.) */
+__global__ void __launch_bounds__(256) k_lin_apply(LinPtrs s) {
     /* XCD-contiguous blocks, so the neighbours a voxel gathers along x and y are read in the
-       same XCD's L2 (on a 128x128x64 block: 629 instead of 671 MB fetched per launch; the z
-       neighbours, three lattice layers of vectors apart, do not stay in a 4 MB L2) */
+       same XCD's L2 (on a 128x128x64 block it fetched 6 % less than with the default
+       round-robin placement; the z neighbours, three lattice layers of vectors apart, do not
+       all stay in a 4 MB L2) */
     const int i = lane_index(1);
     double pq = 0.0;
     if (i < s.nv) {
         const int nv = s.nv;
         double own[6], y[6];
-        apply_row(s, Direction{s.z, pin, s.scal[S_BETA], nv}, i, true, own, y);
+        apply_row(s, Plain{s.p, nv}, i, true, own, y);
         for (int j = 0; j < 6; j++) {
-            pout[j * nv + i] = own[j];
             s.q[j * nv + i] = y[j];
             pq += own[j] * y[j];
         }
@@ -283,7 +282,7 @@ __global__ void __launch_bounds__(256) k_lin_apply(LinPtrs s, const double* pin,
 }
 
 /* x += alpha p, r -= alpha q, z = M r; partial sums of r.z and r.r */
-__global__ void __launch_bounds__(256) k_lin_update(LinPtrs s, const double* p) {
+__global__ void __launch_bounds__(256) k_lin_update(LinPtrs s) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double rz = 0.0, rr2 = 0.0;
     if (i < s.nv) {
@@ -292,7 +291,7 @@ __global__ void __launch_bounds__(256) k_lin_update(LinPtrs s, const double* p) 
         const int fi = s.fixed[i];
         double r[6];
         for (int j = 0; j < 6; j++) {
-            s.x[j * nv + i] += alpha * p[j * nv + i];
+            s.x[j * nv + i] += alpha * s.p[j * nv + i];
             r[j] = s.r[j * nv + i] - alpha * s.q[j * nv + i];
             s.r[j * nv + i] = r[j];
         }
@@ -338,14 +337,12 @@ void launch_lin_setup(const LinPtrs& s, hipStream_t st) {
     hipLaunchKernelGGL(k_lin_setup, dim3(nb), dim3(256), 0, st, s);
     hipLaunchKernelGGL(k_lin_reduce, dim3(1), dim3(256), 0, st, s, (int)nb, 0);
 }
-/* one iteration; `odd` selects which half of the search-direction pair is read */
-void launch_lin_iteration(const LinPtrs& s, bool odd, hipStream_t st) {
+void launch_lin_iteration(const LinPtrs& s, hipStream_t st) {
     const unsigned nb = nblocks(s.nv);
-    double* pin = odd ? s.p2 : s.p;
-    double* pout = odd ? s.p : s.p2;
-    hipLaunchKernelGGL(k_lin_apply, dim3(nb), dim3(256), 0, st, s, (const double*)pin, pout);
+    hipLaunchKernelGGL(k_lin_direction, dim3(nb), dim3(256), 0, st, s);
+    hipLaunchKernelGGL(k_lin_apply, dim3(nb), dim3(256), 0, st, s);
     hipLaunchKernelGGL(k_lin_reduce, dim3(1), dim3(256), 0, st, s, (int)nb, 1);
-    hipLaunchKernelGGL(k_lin_update, dim3(nb), dim3(256), 0, st, s, (const double*)pout);
+    hipLaunchKernelGGL(k_lin_update, dim3(nb), dim3(256), 0, st, s);
     hipLaunchKernelGGL(k_lin_reduce, dim3(1), dim3(256), 0, st, s, (int)nb, 2);
 }
 void launch_lin_post(const LinPtrs& s, double* vd, const int* vijk, double size, hipStream_t st) {

@@ -20,7 +20,7 @@ namespace vx {
 
 int lin_blocks(int nv);
 void launch_lin_setup(const LinPtrs& s, hipStream_t st);
-void launch_lin_iteration(const LinPtrs& s, bool odd, hipStream_t st);
+void launch_lin_iteration(const LinPtrs& s, hipStream_t st);
 void launch_lin_post(const LinPtrs& s, double* vd, const int* vijk, double size, hipStream_t st);
 
 #define HIPCHK(call)                                       \
@@ -30,8 +30,7 @@ void launch_lin_post(const LinPtrs& s, double* vd, const int* vijk, double size,
     } while (0)
 
 /* iterations between two convergence checks (each check is one 8-byte read-back); they are
-   recorded once as a HIP graph and replayed.  Even, so every replay starts on the same half
-   of the search-direction pair. */
+   recorded once as a HIP graph and replayed */
 static const int kChunk = 24;
 
 namespace {
@@ -86,7 +85,7 @@ bool Sim::linearSolve(double tol, int maxIter, int* itersOut, double* residualOu
     const int nb = lin_blocks(n);
     const size_t V = (size_t)n;
     auto align = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    const size_t bytes = 8 * align(V * 48) + align(V * 4 * 21) + align(V) + align((size_t)nb * 16) +
+    const size_t bytes = 7 * align(V * 48) + align(V * 4 * 21) + align(V) + align((size_t)nb * 16) +
                          align(S_COUNT * 8) + align(4);
     if (!blin.ensure(bytes)) {
         err = "device allocation failed";
@@ -111,7 +110,6 @@ bool Sim::linearSolve(double tol, int maxIter, int* itersOut, double* residualOu
     lp.r = (double*)carve(V * 48);
     lp.z = (double*)carve(V * 48);
     lp.p = (double*)carve(V * 48);
-    lp.p2 = (double*)carve(V * 48);
     lp.q = (double*)carve(V * 48);
     lp.minv = (float*)carve(V * 4 * 21);
     uint8_t* dfixed = (uint8_t*)carve(V);
@@ -145,7 +143,7 @@ bool Sim::linearSolve(double tol, int maxIter, int* itersOut, double* residualOu
         const double target = tol * tol * bb;
         GraphGuard graph;
         HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        for (int c = 0; c < kChunk; c++) launch_lin_iteration(lp, (c & 1) != 0, stream);
+        for (int c = 0; c < kChunk; c++) launch_lin_iteration(lp, stream);
         HIPCHK(hipStreamEndCapture(stream, &graph.g));
         HIPCHK(hipGraphInstantiate(&graph.e, graph.g, nullptr, nullptr, 0));
         while (rr > target && it < maxIter) {

@@ -168,6 +168,40 @@ __global__ void __launch_bounds__(256) k_lin_reduce(LinPtrs s, int nblk, int pha
     }
 }
 
+/* the fixed mask, prescribed values and loads of every voxel (CVX_LinearSolver::applyBX,
+   src/VX_LinearSolver.cpp:268-287): a fixed degree of freedom keeps the voxel's current
+   displacement (position - lattice position) or the rotation vector of its current orientation
+   (0 when w == 1), a free one carries the external force or moment */
+__global__ void __launch_bounds__(256) k_lin_prep(LinPtrs s, uint8_t* fixed, double* xfix, double* f,
+                                                  const double* vd, const int* vext, const ExtRec* exts,
+                                                  const int* vijk, double size) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.nv) return;
+    const int nv = s.nv;
+    const int e = vext[i];
+    const ExtRec* ex = e >= 0 ? &exts[e] : nullptr;
+    const int dof = ex ? ex->dof : 0;
+    double val[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (dof & 0x3F) {
+        val[0] = vd[VPX * nv + i] - vijk[i] * size;
+        val[1] = vd[VPY * nv + i] - vijk[nv + i] * size;
+        val[2] = vd[VPZ * nv + i] - vijk[2 * nv + i] * size;
+        const dquat q = dq(vd[VQW * nv + i], vd[VQX * nv + i], vd[VQY * nv + i], vd[VQZ * nv + i]);
+        if (q.w != 1.0) {
+            const dvec a = qtorv(q);
+            val[3] = a.x; val[4] = a.y; val[5] = a.z;
+        }
+    }
+    const float load[6] = {ex ? ex->fx : 0.0f, ex ? ex->fy : 0.0f, ex ? ex->fz : 0.0f,
+                           ex ? ex->mx : 0.0f, ex ? ex->my : 0.0f, ex ? ex->mz : 0.0f};
+    for (int j = 0; j < 6; j++) {
+        const bool fx = (dof & (1 << j)) != 0;
+        xfix[j * nv + i] = fx ? val[j] : 0.0;
+        f[j * nv + i] = fx ? 0.0 : (double)load[j];
+    }
+    fixed[i] = (uint8_t)(dof & 0x3F);
+}
+
 /* set-up: b = f - K x_fixed on the free components; the preconditioner; x = 0, r = b,
    z = M r, p = z; partial sums of r.z and b.b */
 __global__ void __launch_bounds__(256) k_lin_setup(LinPtrs s) {
@@ -332,6 +366,11 @@ static inline unsigned nblocks(int n) { return (unsigned)((n + 255) / 256); }
 
 int lin_blocks(int nv) { return (int)nblocks(nv); }
 
+void launch_lin_prep(const LinPtrs& s, uint8_t* fixed, double* xfix, double* f, const double* vd, const int* vext,
+                     const ExtRec* exts, const int* vijk, double size, hipStream_t st) {
+    hipLaunchKernelGGL(k_lin_prep, dim3(nblocks(s.nv)), dim3(256), 0, st, s, fixed, xfix, f, vd, vext, exts, vijk,
+                       size);
+}
 void launch_lin_setup(const LinPtrs& s, hipStream_t st) {
     const unsigned nb = nblocks(s.nv);
     hipLaunchKernelGGL(k_lin_setup, dim3(nb), dim3(256), 0, st, s);

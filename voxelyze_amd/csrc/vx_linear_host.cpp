@@ -2,11 +2,10 @@
  * vx_linear_host.cpp -- the host side of the static linear solve (CVoxelyze::doLinearSolve,
  * src/Voxelyze.cpp:243-248; CVX_LinearSolver::solve, src/VX_LinearSolver.cpp:48-122).
  *
- * The prescribed values and loads follow CVX_LinearSolver::applyBX
- * (src/VX_LinearSolver.cpp:259-308): a fixed degree of freedom keeps the voxel's current
- * displacement (or the rotation vector of its current orientation), a free one is loaded with
- * the external force or moment.  The solution is written back as CVX_LinearSolver::postResults
- * does (src/VX_LinearSolver.cpp:316-326); a solve that does not converge changes nothing.
+ * Everything the solve reads is already on the device: the prescribed values and loads
+ * (CVX_LinearSolver::applyBX, src/VX_LinearSolver.cpp:259-308) are formed there from the voxel
+ * state and the externals.  The solution is written back as CVX_LinearSolver::postResults does
+ * (src/VX_LinearSolver.cpp:316-326); a solve that does not converge changes nothing.
  */
 #include "vx_model.h"
 
@@ -19,6 +18,8 @@
 namespace vx {
 
 int lin_blocks(int nv);
+void launch_lin_prep(const LinPtrs& s, uint8_t* fixed, double* xfix, double* f, const double* vd, const int* vext,
+                     const ExtRec* exts, const int* vijk, double size, hipStream_t st);
 void launch_lin_setup(const LinPtrs& s, hipStream_t st);
 void launch_lin_iteration(const LinPtrs& s, hipStream_t st);
 void launch_lin_post(const LinPtrs& s, double* vd, const int* vijk, double size, hipStream_t st);
@@ -59,29 +60,7 @@ bool Sim::linearSolve(double tol, int maxIter, int* itersOut, double* residualOu
     }
     if (!(tol > 0.0)) tol = 1e-10;
     if (maxIter <= 0) maxIter = 100000;
-    if (!syncUp() || !syncDown()) return false;
-
-    /* prescribed values, loads and the fixed mask (applyBX, src/VX_LinearSolver.cpp:268-287) */
-    std::vector<uint8_t> fixed((size_t)n, 0);
-    std::vector<double> xfix((size_t)6 * n, 0.0), f((size_t)6 * n, 0.0);
-    for (int i = 0; i < n; i++) {
-        const Ext* e = vext[i].get();
-        const double disp[3] = {vd[VPX][i] - vijk[i][0] * voxSize, vd[VPY][i] - vijk[i][1] * voxSize,
-                                vd[VPZ][i] - vijk[i][2] * voxSize};
-        const dquat q = dq(vd[VQW][i], vd[VQX][i], vd[VQY][i], vd[VQZ][i]);
-        const dvec ang = q.w == 1.0 ? dv(0.0, 0.0, 0.0) : qtorv(q);
-        const double angc[3] = {ang.x, ang.y, ang.z};
-        for (int j = 0; j < 6; j++) {
-            const bool fx = e && (e->dof & (1 << j));
-            if (fx) {
-                fixed[i] |= (uint8_t)(1 << j);
-                xfix[(size_t)j * n + i] = j < 3 ? disp[j] : angc[j - 3];
-            } else if (e) {
-                f[(size_t)j * n + i] = j < 3 ? (double)e->force[j] : (double)e->moment[j - 3];
-            }
-        }
-    }
-
+    if (!syncUp()) return false;
     const int nb = lin_blocks(n);
     const size_t V = (size_t)n;
     auto align = [](size_t b) { return (b + 255) & ~(size_t)255; };
@@ -121,10 +100,8 @@ bool Sim::linearSolve(double tol, int maxIter, int* itersOut, double* residualOu
         err = "linear solve buffers";
         return false;
     }
-    HIPCHK(hipMemcpyAsync(dxfix, xfix.data(), V * 48, hipMemcpyHostToDevice, stream));
-    HIPCHK(hipMemcpyAsync(df, f.data(), V * 48, hipMemcpyHostToDevice, stream));
-    HIPCHK(hipMemcpyAsync(dfixed, fixed.data(), V, hipMemcpyHostToDevice, stream));
     HIPCHK(hipMemsetAsync(lp.bad, 0, 4, stream));
+    launch_lin_prep(lp, dfixed, dxfix, df, dp.vd, dp.vext, dp.exts, dp.vijk, voxSize, stream);
     launch_lin_setup(lp, stream);
     HIPCHK(hipGetLastError());
     double sc[S_COUNT];

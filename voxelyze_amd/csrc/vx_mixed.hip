@@ -270,12 +270,15 @@ __device__ void link_fused(const MixPtrs& s, int slot, int n, int p, int axis, b
     }
     if (owner) {
         T* o = (T*)s.lsOut;
-        o[MS_P2X * S + slot] = p2.x; o[MS_P2Y * S + slot] = p2.y; o[MS_P2Z * S + slot] = p2.z;
-        o[MS_A1X * S + slot] = a1v.x; o[MS_A1Y * S + slot] = a1v.y; o[MS_A1Z * S + slot] = a1v.z;
-        o[MS_A2X * S + slot] = a2v.x; o[MS_A2Y * S + slot] = a2v.y; o[MS_A2Z * S + slot] = a2v.z;
-        o[MS_MAX * S + slot] = maxStrain; o[MS_OFF * S + slot] = strainOffset;
-        o[MS_STRAIN * S + slot] = axialStrain; o[MS_STRESS * S + slot] = stress;
-        s.lflOut[slot] = (lvv ? LF_LVV : 0u) | (small ? LF_SMALL : 0u);
+        st_nt<T>(&o[MS_P2X * S + slot], p2.x); st_nt<T>(&o[MS_P2Y * S + slot], p2.y);
+        st_nt<T>(&o[MS_P2Z * S + slot], p2.z);
+        st_nt<T>(&o[MS_A1X * S + slot], a1v.x); st_nt<T>(&o[MS_A1Y * S + slot], a1v.y);
+        st_nt<T>(&o[MS_A1Z * S + slot], a1v.z);
+        st_nt<T>(&o[MS_A2X * S + slot], a2v.x); st_nt<T>(&o[MS_A2Y * S + slot], a2v.y);
+        st_nt<T>(&o[MS_A2Z * S + slot], a2v.z);
+        st_nt<T>(&o[MS_MAX * S + slot], maxStrain); st_nt<T>(&o[MS_OFF * S + slot], strainOffset);
+        st_nt<T>(&o[MS_STRAIN * S + slot], axialStrain); st_nt<T>(&o[MS_STRESS * S + slot], stress);
+        st_nt(&s.lflOut[slot], (uint32_t)((lvv ? LF_LVV : 0u) | (small ? LF_SMALL : 0u)));
         if (axialStrain > 100) {
             atomicMin(&s.rdiv[s.vrobot[n]], s.k);
             atomicOr(&s.ctl[0], 1);
@@ -497,11 +500,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_FUS
             }
         }
     }
-    out[VPX * nv + i] = pos.x; out[VPY * nv + i] = pos.y; out[VPZ * nv + i] = pos.z;
-    out[VLX * nv + i] = lin.x; out[VLY * nv + i] = lin.y; out[VLZ * nv + i] = lin.z;
-    out[VAX * nv + i] = ang.x; out[VAY * nv + i] = ang.y; out[VAZ * nv + i] = ang.z;
-    out[VQW * nv + i] = q.w; out[VQX * nv + i] = q.x; out[VQY * nv + i] = q.y; out[VQZ * nv + i] = q.z;
-    s.vflOut[i] = flags;
+    st_nt(&out[VPX * nv + i], pos.x); st_nt(&out[VPY * nv + i], pos.y); st_nt(&out[VPZ * nv + i], pos.z);
+    st_nt(&out[VLX * nv + i], lin.x); st_nt(&out[VLY * nv + i], lin.y); st_nt(&out[VLZ * nv + i], lin.z);
+    st_nt(&out[VAX * nv + i], ang.x); st_nt(&out[VAY * nv + i], ang.y); st_nt(&out[VAZ * nv + i], ang.z);
+    st_nt(&out[VQW * nv + i], q.w); st_nt(&out[VQX * nv + i], q.x); st_nt(&out[VQY * nv + i], q.y);
+    st_nt(&out[VQZ * nv + i], q.z);
+    st_nt(&s.vflOut[i], flags);
 }
 
 /* temperatures at each robot's current time, before the first step of an actuated launch */

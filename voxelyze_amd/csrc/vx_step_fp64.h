@@ -354,10 +354,11 @@ __device__ __forceinline__ void voxel_pinned(const VoxMat& m, const ExtRec* ex, 
 
 __device__ __forceinline__ void store_voxel(double* __restrict__ out, int nv, int i, const dvec& pos, const dvec& lin,
                                             const dvec& ang, const dquat& q) {
-    out[VPX * nv + i] = pos.x; out[VPY * nv + i] = pos.y; out[VPZ * nv + i] = pos.z;
-    out[VLX * nv + i] = lin.x; out[VLY * nv + i] = lin.y; out[VLZ * nv + i] = lin.z;
-    out[VAX * nv + i] = ang.x; out[VAY * nv + i] = ang.y; out[VAZ * nv + i] = ang.z;
-    out[VQW * nv + i] = q.w; out[VQX * nv + i] = q.x; out[VQY * nv + i] = q.y; out[VQZ * nv + i] = q.z;
+    st_nt(&out[VPX * nv + i], pos.x); st_nt(&out[VPY * nv + i], pos.y); st_nt(&out[VPZ * nv + i], pos.z);
+    st_nt(&out[VLX * nv + i], lin.x); st_nt(&out[VLY * nv + i], lin.y); st_nt(&out[VLZ * nv + i], lin.z);
+    st_nt(&out[VAX * nv + i], ang.x); st_nt(&out[VAY * nv + i], ang.y); st_nt(&out[VAZ * nv + i], ang.z);
+    st_nt(&out[VQW * nv + i], q.w); st_nt(&out[VQX * nv + i], q.x); st_nt(&out[VQY * nv + i], q.y);
+    st_nt(&out[VQZ * nv + i], q.z);
 }
 
 }  // namespace vx

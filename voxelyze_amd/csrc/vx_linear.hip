@@ -291,7 +291,7 @@ __global__ void __launch_bounds__(256) k_lin_direction(LinPtrs s) {
     if (i >= s.nv) return;
     const int nv = s.nv;
     const double beta = s.scal[S_BETA];
-    for (int j = 0; j < 6; j++) s.p[j * nv + i] = s.z[j * nv + i] + beta * s.p[j * nv + i];
+    for (int j = 0; j < 6; j++) st_nt(&s.p[j * nv + i], s.z[j * nv + i] + beta * s.p[j * nv + i]);
 }
 
 /* q = K p on the free components; partial sums of p.q.  (Forming z + beta p here, where p is
@@ -308,7 +308,7 @@ __global__ void __launch_bounds__(256) k_lin_apply(LinPtrs s) {
         double own[6], y[6];
         apply_row(s, Plain{s.p, nv}, i, true, own, y);
         for (int j = 0; j < 6; j++) {
-            s.q[j * nv + i] = y[j];
+            st_nt(&s.q[j * nv + i], y[j]);
             pq += own[j] * y[j];
         }
     }
@@ -325,15 +325,15 @@ __global__ void __launch_bounds__(256) k_lin_update(LinPtrs s) {
         const int fi = s.fixed[i];
         double r[6];
         for (int j = 0; j < 6; j++) {
-            s.x[j * nv + i] += alpha * s.p[j * nv + i];
+            st_nt(&s.x[j * nv + i], s.x[j * nv + i] + alpha * s.p[j * nv + i]);
             r[j] = s.r[j * nv + i] - alpha * s.q[j * nv + i];
-            s.r[j * nv + i] = r[j];
+            st_nt(&s.r[j * nv + i], r[j]);
         }
         for (int rr = 0; rr < 6; rr++) {
             double e = 0.0;
             for (int c = 0; c < 6; c++) e += (double)s.minv[tri(rr, c) * nv + i] * r[c];
             if (fi & (1 << rr)) e = 0.0;
-            s.z[rr * nv + i] = e;
+            st_nt(&s.z[rr * nv + i], e);
             rz += r[rr] * e;
             rr2 += r[rr] * r[rr];
         }

@@ -174,6 +174,10 @@ __device__ __forceinline__ int xcd_block() {
    lattice: 267 -> 255 us) */
 template <class T>
 __device__ __forceinline__ void st_nt(T* p, T v) { __builtin_nontemporal_store(v, p); }
+/* a load of data no other lane of the step reads (a voxel's own momenta): non-temporal too
+   (mixed step 197 -> 192 us) */
+template <class T>
+__device__ __forceinline__ T ld_nt(const T* p) { return __builtin_nontemporal_load(p); }
 
 /* the lane's element under a launch configuration (workgroup size, XCD-contiguous mapping) */
 __device__ __forceinline__ int lane_index(int xcd) {

@@ -221,8 +221,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_FUS
         s.vflOut[iv] = flags;
         return;
     }
-    dvec pos = pos0, lin = dv(vd[VLX * nv + iv], vd[VLY * nv + iv], vd[VLZ * nv + iv]);
-    dvec ang = dv(vd[VAX * nv + iv], vd[VAY * nv + iv], vd[VAZ * nv + iv]);
+    /* the momenta only this lane reads: loaded past the caches */
+    dvec pos = pos0, lin = dv(ld_nt(&vd[VLX * nv + iv]), ld_nt(&vd[VLY * nv + iv]), ld_nt(&vd[VLZ * nv + iv]));
+    dvec ang = dv(ld_nt(&vd[VAX * nv + iv]), ld_nt(&vd[VAY * nv + iv]), ld_nt(&vd[VAZ * nv + iv]));
     dquat q = q0;
     voxel_step_fp64(m, ex, s.vijk, nv, iv, dt, tempI, sf, sm, pos, lin, ang, q, flags, [&](dvec& t) {
         /* contacts in partner order (VX_Voxel.cpp:250-253), each pair's force from this

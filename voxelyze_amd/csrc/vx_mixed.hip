@@ -401,8 +401,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_FUS
         s.vflOut[i] = flags;
         return;
     }
-    dvec pos = pos0, lin = dv(vd[VLX * nv + i], vd[VLY * nv + i], vd[VLZ * nv + i]);
-    dvec ang = dv(vd[VAX * nv + i], vd[VAY * nv + i], vd[VAZ * nv + i]);
+    /* the momenta only this lane reads: loaded past the caches */
+    dvec pos = pos0, lin = dv(ld_nt(&vd[VLX * nv + i]), ld_nt(&vd[VLY * nv + i]), ld_nt(&vd[VLZ * nv + i]));
+    dvec ang = dv(ld_nt(&vd[VAX * nv + i]), ld_nt(&vd[VAY * nv + i]), ld_nt(&vd[VAZ * nv + i]));
     dquat q = q0;
 
     dvec t = qrot(q, lf);

@@ -75,6 +75,18 @@ def traffic_from_profiles(kernel, workload):
         return json.load(f).get("hbm_bytes_per_launch")
 
 
+def cpu_model():
+    """The host CPU's model name (Linux /proc/cpuinfo), or None."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(workload, budget_s=15.0, max_steps=2000):
     """The reference's own multi-core CPU path on the same workload, timed on this host: its
     sources compiled with USE_OMP (oracle/_ref/libvxref_omp.so: the OpenMP link and voxel
@@ -101,7 +113,7 @@ def cpu_baseline(workload, budget_s=15.0, max_steps=2000):
     el = time.perf_counter() - t0
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     V = sim.voxelCount()
-    return {"value": V * n / el, "unit": "voxel-steps/s", "cores": threads, "kind": kind,
+    return {"value": V * n / el, "unit": "voxel-steps/s", "cores": threads, "kind": kind, "cpu_model": cpu_model(),
             "sample": f"{what}, {n} steps after 2 warm-up steps of the same workload ({V} voxels), {el:.1f} s"}
 
 

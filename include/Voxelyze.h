@@ -273,6 +273,23 @@ public:
     }
     int materialCount() { return vxg_material_count(sim); }
     CVX_Material* material(int i) { return mats[i].get(); }
+    /* Voxelyze.h:90: removes the material and every voxel made of it */
+    bool removeMaterial(CVX_Material* m) {
+        if (!m || vxg_material_remove(sim, m->idx) == 0) return false;
+        const int gone = m->idx;
+        for (auto it = voxels.begin(); it != voxels.end();)
+            it = vxg_voxel_find(sim, std::get<0>(it->first), std::get<1>(it->first), std::get<2>(it->first)) < 0
+                     ? voxels.erase(it) : std::next(it);
+        links.clear();
+        mats.erase(mats.begin() + gone);
+        for (auto& p : mats) if (p->idx > gone) p->idx--;
+        return true;
+    }
+    /* Voxelyze.h:91 */
+    bool replaceMaterial(CVX_Material* replaceMe, CVX_Material* replaceWith) {
+        if (!replaceMe || !replaceWith) return false;
+        return vxg_material_replace(sim, replaceMe->idx, replaceWith->idx) != 0;
+    }
 
     CVX_Voxel* setVoxel(CVX_Material* m, int x, int y, int z) {
         int i = vxg_voxel_set(sim, m ? m->idx : -1, x, y, z);
@@ -297,6 +314,14 @@ public:
     }
     int linkCount() const { return vxg_link_count(sim); }
     double voxelSize() const { return vxg_voxel_size(sim); }
+    void setVoxelSize(double voxelSize) { vxg_set_voxel_size(sim, voxelSize); }  /* Voxelyze.h:117 */
+    /* Voxelyze.h:103-108 */
+    int indexMinX() const { return bound(0); }
+    int indexMaxX() const { return bound(3); }
+    int indexMinY() const { return bound(1); }
+    int indexMaxY() const { return bound(4); }
+    int indexMinZ() const { return bound(2); }
+    int indexMaxZ() const { return bound(5); }
 
     void setGravity(float g = 1.0f) { vxg_set_gravity(sim, g); }
     float gravity() const { return vxg_gravity(sim); }
@@ -312,6 +337,11 @@ public:
     vxg_sim* handle() { return sim; }
 
 private:
+    int bound(int k) const {
+        int b[6];
+        vxg_index_bounds(sim, b, b + 3);
+        return b[k];
+    }
     CVX_Voxel* voxelAt(int x, int y, int z) {
         auto k = std::make_tuple(x, y, z);
         auto it = voxels.find(k);

@@ -193,6 +193,15 @@ int vxg_profile_read(vxg_sim* s, double* ms, int* launches, int cap);
    Returns the material index (materialCount()-1) or -1. */
 int vxg_material_add(vxg_sim* s, float youngs_modulus, float density);
 int vxg_material_count(vxg_sim* s);                  /* materialCount()  Voxelyze.h:92 */
+/* CVoxelyze::removeMaterial(CVX_Material*)          include/Voxelyze.h:90, src/Voxelyze.cpp:384-404:
+   removes every voxel of the material (z, y, x scan of the index bounds), then the material;
+   later material indices shift down by one.  Returns 1, or 0 for an unknown material.  The
+   reference dereferences empty sites inside the bounds (it works only on filled boxes); empty
+   sites are skipped here. */
+int vxg_material_remove(vxg_sim* s, int mat);
+/* CVoxelyze::replaceMaterial(CVX_Material*, CVX_Material*)  include/Voxelyze.h:91,
+   src/Voxelyze.cpp:406-420: setVoxel(replace_with) on every voxel of replace_me in the same scan */
+int vxg_material_replace(vxg_sim* s, int replace_me, int replace_with);
 /* CVX_Material::setModelLinear                      include/VX_Material.h:64, src/VX_Material.cpp:355-384 */
 int vxg_material_set_model_linear(vxg_sim* s, int mat, float youngs_modulus, float failure_stress);
 /* CVX_Material::setModelBilinear                    include/VX_Material.h:65, src/VX_Material.cpp:389-434 */
@@ -223,6 +232,15 @@ int vxg_material_model_data(vxg_sim* s, int mat, float* strain, float* stress, i
    simulation (the device's element indices are 32-bit); beyond them -1 and an error. */
 int vxg_voxel_set(vxg_sim* s, int mat, int ix, int iy, int iz);
 int vxg_voxel_count(vxg_sim* s);                      /* voxelCount()  Voxelyze.h:98 */
+/* CVoxelyze::indexMinX/Y/Z and indexMaxX/Y/Z         include/Voxelyze.h:103-108, of the selected
+   robot: the bounds CArray3D keeps (include/Array3D.h:109-110, 220-226, 248-266), with its
+   recomputation after a removal as the reference has it (a site that lowers an axis's minimum
+   does not raise its maximum).  An empty lattice gives INT_MAX / INT_MIN.  Either pointer may
+   be NULL. */
+void vxg_index_bounds(vxg_sim* s, int* min3, int* max3);
+/* CVoxelyze::setVoxelSize(double)                   include/Voxelyze.h:117, src/Voxelyze.cpp:643-668:
+   materials re-derived at the new size, positions scaled, motion halted, links reset */
+void vxg_set_voxel_size(vxg_sim* s, double voxel_size);
 /* n setVoxel calls in one: voxel k gets material mat[k] (< 0 removes) at ijk[3k..3k+2];
    index[k] (if non-NULL) receives its index or -1.  Returns the number placed. */
 int vxg_voxel_set_many(vxg_sim* s, int n, const int* mat, const int* ijk, int* index);

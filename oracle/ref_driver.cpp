@@ -192,6 +192,19 @@ EXPORT int vxr_voxel_set(vxr_sim* s, int m, int x, int y, int z) {
     return v ? voxIndex(s, v) : -1;
 }
 EXPORT int vxr_voxel_count(vxr_sim* s) { return s->sim.voxelCount(); }
+EXPORT void vxr_index_bounds(vxr_sim* s, int* mn, int* mx) {
+    mn[0] = s->sim.indexMinX(); mn[1] = s->sim.indexMinY(); mn[2] = s->sim.indexMinZ();
+    mx[0] = s->sim.indexMaxX(); mx[1] = s->sim.indexMaxY(); mx[2] = s->sim.indexMaxZ();
+}
+EXPORT int vxr_material_remove(vxr_sim* s, int m) {
+    CVX_Material* p = M(s, m);
+    return p && s->sim.removeMaterial(p) ? 1 : 0;
+}
+EXPORT int vxr_material_replace(vxr_sim* s, int a, int b) {
+    CVX_Material *pa = M(s, a), *pb = M(s, b);
+    return pa && pb && s->sim.replaceMaterial(pa, pb) ? 1 : 0;
+}
+EXPORT void vxr_set_voxel_size(vxr_sim* s, double size) { s->sim.setVoxelSize(size); }
 EXPORT int vxr_voxel_find(vxr_sim* s, int x, int y, int z) {
     CVX_Voxel* v = s->sim.voxel(x, y, z);
     return v ? voxIndex(s, v) : -1;

@@ -42,6 +42,7 @@
  * is left to the test (a direct sparse solve), as the reference leaves it to Pardiso.
  */
 #include <float.h>
+#include <limits.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -282,6 +283,9 @@ typedef struct vxo_sim {
     ocol* cols;
     int ncols, capcols;
     int collisionsStale, nearbyStale;
+    /* CArray3D::cMin/cMax of the voxel array (Array3D.h:109-110): grown by addValue
+       (Array3D.h:220-226), recomputed by UpdateMinMax (Array3D.h:248-266) after a removal */
+    int cmin[3], cmax[3], boxStale;
     omap vmap, lmap;
     int* stamp;
     int stampCap, stampVal;
@@ -1335,8 +1339,15 @@ static void vox_enableCollisions(ovox* v, int enabled) {
 }
 
 /* CVoxelyze::addVoxel (Voxelyze.cpp:439-461) with the CVX_Voxel ctor (VX_Voxel.cpp:23-37) */
+static void sim_indexBox(vxo_sim* s);
 static int sim_addVoxel(vxo_sim* s, omat* m, int x, int y, int z) {
     s->nearbyStale = s->collisionsStale = 1;
+    sim_indexBox(s);
+    const int c[3] = {x, y, z};
+    for (int a = 0; a < 3; a++) {
+        if (c[a] < s->cmin[a]) s->cmin[a] = c[a];
+        if (c[a] > s->cmax[a]) s->cmax[a] = c[a];
+    }
     GROW(s->vox, s->nvslots, s->capvslots, 1);
     int slot = s->nvslots++;
     ovox* v = V(s, slot);
@@ -1363,6 +1374,7 @@ static void sim_removeVoxel(vxo_sim* s, int x, int y, int z) {
     int slot = sim_voxelSlot(s, x, y, z);
     if (slot < 0) return;
     ovox* v = V(s, slot);
+    s->boxStale = 1;
     free(v->ext); v->ext = NULL;
     free(v->colWatch); v->colWatch = NULL; v->ncol = v->capcol = 0;
     free(v->nearby); v->nearby = NULL; v->nnear = v->capnear = 0;
@@ -1399,6 +1411,34 @@ static void sim_replaceVoxel(vxo_sim* s, omat* m, int x, int y, int z) {
         sim_removeLink(s, x, y, z, i);
         sim_addLink(s, x, y, z, i);
     }
+}
+
+/* CArray3D::UpdateMinMax (Array3D.h:248-266) as the reference has it: occupied sites scanned
+   z, then y, then x, each coordinate tested `< min` and otherwise `> max`, so a site that
+   lowers a minimum never raises the maximum of that axis */
+static int cmp_zyx(const void* a, const void* b) {
+    const int *p = (const int*)a, *q = (const int*)b;
+    for (int i = 0; i < 3; i++) if (p[i] != q[i]) return p[i] < q[i] ? -1 : 1;
+    return 0;
+}
+static void sim_indexBox(vxo_sim* s) {
+    if (!s->boxStale) return;
+    s->boxStale = 0;
+    int* zyx = (int*)malloc(sizeof(int) * 3 * (size_t)(s->nvlist ? s->nvlist : 1));
+    for (int i = 0; i < s->nvlist; i++) {
+        const ovox* v = V(s, s->vlist[i]);
+        zyx[3 * i] = v->iz; zyx[3 * i + 1] = v->iy; zyx[3 * i + 2] = v->ix;
+    }
+    qsort(zyx, (size_t)s->nvlist, sizeof(int) * 3, cmp_zyx);
+    for (int a = 0; a < 3; a++) { s->cmin[a] = INT_MAX; s->cmax[a] = INT_MIN; }
+    for (int i = 0; i < s->nvlist; i++) {
+        const int c[3] = {zyx[3 * i + 2], zyx[3 * i + 1], zyx[3 * i]};
+        for (int a = 0; a < 3; a++) {
+            if (c[a] < s->cmin[a]) s->cmin[a] = c[a];
+            else if (c[a] > s->cmax[a]) s->cmax[a] = c[a];
+        }
+    }
+    free(zyx);
 }
 
 /* ------------------------------------------------------------------ step */
@@ -1478,6 +1518,7 @@ EXPORT sim_t* vxo_create(double voxelSize) {
     map_init(&s->lmap);
     s->voxSize = voxelSize <= 0 ? 0.001 : voxelSize;
     s->collisionsStale = s->nearbyStale = 1;
+    for (int a = 0; a < 3; a++) { s->cmin[a] = INT_MAX; s->cmax[a] = INT_MIN; }
     s->boundingRadius = 0.75f;
     s->watchDistance = 1.0f;
     return s;
@@ -1731,6 +1772,81 @@ EXPORT int vxo_voxel_set(sim_t* s, int mi, int x, int y, int z) {
     return s->nvlist - 1;
 }
 EXPORT int vxo_voxel_count(sim_t* s) { return s->nvlist; }
+/* CVoxelyze::indexMinX .. indexMaxZ (Voxelyze.h:103-108) */
+EXPORT void vxo_index_bounds(sim_t* s, int* mn, int* mx) {
+    sim_indexBox(s);
+    for (int a = 0; a < 3; a++) { mn[a] = s->cmin[a]; mx[a] = s->cmax[a]; }
+}
+static int mat_index(sim_t* s, const omat* m) {
+    for (int i = 0; i < s->nvmats; i++) if (s->vmats[i] == m) return i;
+    return -1;
+}
+/* CVoxelyze::removeMaterial (Voxelyze.cpp:384-404): every voxel of the material removed in a
+   z, y, x scan of the current index bounds (re-read as they change), then the material.  The
+   reference dereferences an empty site of that scan; here empty sites are skipped.  Link
+   materials combined from it (which no link can use any more) are dropped too. */
+EXPORT int vxo_material_remove(sim_t* s, int mi) {
+    omat* m = mat_of(s, mi);
+    if (!m) return 0;
+    int mn[3], mx[3];
+    vxo_index_bounds(s, mn, mx);
+    for (int k = mn[2]; vxo_index_bounds(s, mn, mx), k <= mx[2]; k++)
+        for (int j = mn[1]; vxo_index_bounds(s, mn, mx), j <= mx[1]; j++)
+            for (int i = mn[0]; vxo_index_bounds(s, mn, mx), i <= mx[0]; i++) {
+                int slot = sim_voxelSlot(s, i, j, k);
+                if (slot >= 0 && V(s, slot)->mat == m) sim_removeVoxel(s, i, j, k);
+            }
+    for (int l = s->nlmats - 1; l >= 0; l--) {
+        omat* lm = s->lmats[l];
+        if (lm->vox1 != m && lm->vox2 != m) continue;
+        omat* other = lm->vox1 == m ? lm->vox2 : lm->vox1;
+        for (int d = other->ndeps - 1; d >= 0; d--)
+            if (other->deps[d] == lm) { other->deps[d] = other->deps[--other->ndeps]; }
+        mat_free(lm);
+        memmove(&s->lmats[l], &s->lmats[l + 1], sizeof(omat*) * (size_t)(s->nlmats - l - 1));
+        s->nlmats--;
+    }
+    int at = mat_index(s, m);
+    mat_free(m);
+    memmove(&s->vmats[at], &s->vmats[at + 1], sizeof(omat*) * (size_t)(s->nvmats - at - 1));
+    s->nvmats--;
+    return 1;
+}
+/* CVoxelyze::replaceMaterial (Voxelyze.cpp:406-420): setVoxel(replaceWith) on every voxel of
+   replaceMe in a z, y, x scan of the current index bounds (empty sites skipped, see above) */
+EXPORT int vxo_material_replace(sim_t* s, int a, int b) {
+    omat *ma = mat_of(s, a), *mb = mat_of(s, b);
+    if (!ma || !mb) return 0;
+    int mn[3], mx[3];
+    vxo_index_bounds(s, mn, mx);
+    for (int k = mn[2]; vxo_index_bounds(s, mn, mx), k <= mx[2]; k++)
+        for (int j = mn[1]; vxo_index_bounds(s, mn, mx), j <= mx[1]; j++)
+            for (int i = mn[0]; vxo_index_bounds(s, mn, mx), i <= mx[0]; i++) {
+                int slot = sim_voxelSlot(s, i, j, k);
+                if (slot >= 0 && V(s, slot)->mat == ma) vxo_voxel_set(s, b, i, j, k);
+            }
+    return 1;
+}
+/* CVoxelyze::setVoxelSize (Voxelyze.cpp:643-668) with CVX_MaterialVoxel::setNominalSize
+   (VX_MaterialVoxel.cpp:82-87) */
+EXPORT void vxo_set_voxel_size(sim_t* s, double size) {
+    double scale = size / s->voxSize;
+    s->voxSize = size;
+    for (int i = 0; i < s->nvmats; i++) {
+        omat* m = s->vmats[i];
+        m->nomSize = size <= 0 ? FLT_MIN : size;
+        mat_updateDerived(m);
+    }
+    for (int i = 0; i < s->nvlist; i++) {
+        ovox* v = V(s, s->vlist[i]);
+        v->pos.x *= scale; v->pos.y *= scale; v->pos.z *= scale;
+        v->linMom = dv(0, 0, 0);
+        v->angMom = dv(0, 0, 0);
+        vox_setStatic(v, 0);
+    }
+    for (int i = 0; i < s->nllist; i++) link_reset(s, Lk(s, s->llist[i]));
+    s->collisionsStale = 1;
+}
 EXPORT int vxo_voxel_find(sim_t* s, int x, int y, int z) {
     int slot = sim_voxelSlot(s, x, y, z);
     return slot < 0 ? -1 : vox_index_of_slot(s, slot);

@@ -49,7 +49,45 @@ def snapshot(sim, prefix=""):
     ls = sim.link_state()
     for k, v in ls.items():
         out[prefix + "link_" + k] = v
+    out[prefix + "state_info"] = state_infos(sim)
     return out
+
+
+# stateInfo rows that are rates (VELOCITY, KINETIC_ENERGY, ANGULAR_VELOCITY); the others are
+# functions of the configuration (displacements, link strain/stress/energy, pressure, mass)
+STATE_INFO_RATES = (1, 2, 4)
+
+
+def state_info_rel_err(gold, out):
+    """(configuration rows, rate rows): max over snapshots and rows of
+    max_type |out - gold| / max_type |gold|."""
+    worst = [0.0, 0.0]
+    for k in gold:
+        if k.endswith("state_info"):
+            g, o = gold[k].astype(np.float64), out[k].astype(np.float64)
+            scale = np.maximum(np.abs(g).max(axis=1), 1e-30)
+            e = np.abs(o - g).max(axis=1) / scale
+            for r in range(e.shape[0]):
+                j = 1 if r in STATE_INFO_RATES else 0
+                worst[j] = max(worst[j], float(e[r]))
+    return tuple(worst)
+
+
+def state_info_bars(name):
+    """Bars of state_info_rel_err where the reference calls glibc trig: the configuration rows
+    within 100x the displacement bar (link strains, stresses and energies are differences of
+    neighbouring displacements over the pitch, and energies are quadratic in them), but not
+    below 1e-6 (a few float ulps: each value is a float);
+    rates within 1e-2 (velocities of a structure close to rest are small differences of
+    displacements, so a 1e-9 displacement change moves them by relative amounts far above it;
+    they are bit-exact wherever no trig is called and against the correctly rounded oracle)."""
+    return max(100 * TOL[name], 1e-6), 1e-2
+
+
+def state_infos(sim):
+    """CVoxelyze::stateInfo(info, type) (src/Voxelyze.cpp:752-800) for the ten infos
+    (DISPLACEMENT .. MASS) by the four value types (MIN, MAX, TOTAL, AVERAGE), as float32."""
+    return np.array([[sim.stateInfo(i, t) for t in range(4)] for i in range(10)], np.float32)
 
 
 def rotvec(q):
@@ -468,6 +506,64 @@ def replace_material(make, steps=2000):
         trace[k] = sim.state(probe, 1)["pos"][0][2]
     out = {"trace": trace}
     out.update(snapshot(sim))
+    return out
+
+
+def material_edits(make, steps=150):
+    """CVoxelyze::replaceMaterial, removeMaterial, setVoxelSize and the index bounds
+    (src/Voxelyze.cpp:384-420, 643-668; include/Voxelyze.h:103-108) on a filled 6x4x3 box of
+    three materials, stepped between edits under gravity on the floor; plus scattered voxels
+    whose bounds the reference recomputes after a removal with a scan in which a site that
+    lowers an axis's minimum does not raise its maximum (include/Array3D.h:248-266): x visited
+    as 5, 3, 4 gives indexMaxX 4, and a single voxel left gives INT_MIN maxima.  Filled boxes
+    only for the material edits: the reference's scans dereference empty sites."""
+    out = {}
+    sim = make(0.002)
+    sim.setGravity(1.0)
+    sim.enableFloor(True)
+    mats = [sim.addMaterial(1e6, 1e3), sim.addMaterial(3e6, 1.2e3), sim.addMaterial(2e6, 900.0)]
+    for m in mats:
+        m.setInternalDamping(0.5)
+        m.setStaticFriction(0.8)
+        m.setKineticFriction(0.4)
+    for k in range(3):
+        for j in range(4):
+            for i in range(6):
+                sim.setVoxel(mats[(i + 2 * j + k) % 3], i, j, k)
+    bounds = [sim.indexBounds()]
+    ts = sim.recommendedTimeStep()
+    sim.step(ts, steps)
+    assert sim.replaceMaterial(mats[1], mats[2])
+    bounds.append(sim.indexBounds())
+    sim.step(sim.recommendedTimeStep(), steps)
+    out.update(snapshot(sim, "replaced_"))
+    assert sim.removeMaterial(mats[0])
+    bounds.append(sim.indexBounds())
+    out["n_materials"] = np.int32(sim.materialCount())
+    sim.step(sim.recommendedTimeStep(), steps)
+    out.update(snapshot(sim, "removed_"))
+    sim.setVoxelSize(0.003)
+    bounds.append(sim.indexBounds())
+    out["dt_resized"] = np.float32(sim.recommendedTimeStep())
+    out.update(snapshot(sim, "resized0_"))
+    sim.step(sim.recommendedTimeStep(), steps)
+    out.update(snapshot(sim, "resized_"))
+    sc = make(0.001)
+    m = sc.addMaterial(1e6, 1e3)
+    # the reference's removeVoxel steps past the end of its list when the removed voxel is the
+    # last listed one (src/Voxelyze.cpp:472-474), so only earlier-listed voxels are removed
+    for ijk in ((9, 0, 3), (1, 0, 4), (5, 0, 0), (3, 0, 1), (4, 0, 2)):
+        sc.setVoxel(m, *ijk)
+    bounds.append(sc.indexBounds())
+    for ijk in ((9, 0, 3), (1, 0, 4)):
+        sc.setVoxel(None, *ijk)
+        bounds.append(sc.indexBounds())
+    for ijk in ((5, 0, 0), (3, 0, 1)):
+        sc.setVoxel(None, *ijk)
+    bounds.append(sc.indexBounds())
+    sc.setVoxel(m, 2, 1, 5)
+    bounds.append(sc.indexBounds())
+    out["bounds"] = np.array(bounds, np.int64)
     return out
 
 
@@ -916,6 +1012,7 @@ SCENARIOS = {
     "multi_simple2": multi_simple2,
     "deformable_material": deformable_material,
     "replace_material": replace_material,
+    "material_edits": material_edits,
     "temperature": temperature,
     "static_friction": static_friction,
     "kinetic_friction": kinetic_friction,

@@ -209,3 +209,29 @@ def test_checkpoint_refuses_another_topology():
     sim.setVoxel(m, 30, 0, 0)
     with pytest.raises(RuntimeError):
         sim.checkpointRestore(0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("edit", ["remove-readd", "replace"])
+def test_checkpoint_refuses_a_count_preserving_topology_edit(edit):
+    """Removing voxel 0 and adding it again keeps the voxel and link counts but shifts every
+    index; replacing a voxel's material keeps them too.  Both must refuse the restore."""
+    sim = B.maker(B.product_abi())(0.005)
+    W.cantilever(sim)
+    sim.step(sim.recommendedTimeStep(), 10)
+    nv, nl = sim.voxelCount(), sim.linkCount()
+    v0 = sim.voxel(0)
+    ijk, mat = (v0.indexX(), v0.indexY(), v0.indexZ()), sim.material(0)
+    assert sim.checkpointSave(0)
+    if edit == "remove-readd":
+        sim.setVoxel(None, *ijk)
+        sim.setVoxel(mat, *ijk)
+    else:
+        sim.setVoxel(sim.addMaterial(2e6, 1e3), *ijk)
+    assert (sim.voxelCount(), sim.linkCount()) == (nv, nl)
+    with pytest.raises(RuntimeError):
+        sim.checkpointRestore(0)
+    # a checkpoint saved after the edit restores
+    assert sim.checkpointSave(1)
+    sim.step(sim.recommendedTimeStep(), 3)
+    assert sim.checkpointRestore(1)

@@ -59,6 +59,9 @@ def test_hip_fp64_matches_reference(name):
     err = rel_disp_error(gold, out)
     print(f"[parity] {name}: bit-exact arrays {len(gold) - len(bad)}/{len(gold)}, rel disp err {err:.3e}", flush=True)
     assert err <= TOL[name], f"{name}: relative displacement error {err:.3e} > {TOL[name]:.1e}"
+    si, bars = S.state_info_rel_err(gold, out), S.state_info_bars(name)
+    print(f"[parity] {name}: stateInfo rel err {si[0]:.3e} (configuration) {si[1]:.3e} (rates)", flush=True)
+    assert si[0] <= bars[0] and si[1] <= bars[1], f"{name}: stateInfo differs by {si}"
     for k in gold:
         if k.endswith("ncol"):
             assert out[k] == gold[k], f"{name}: collision pair count {out[k]} != {gold[k]}"

@@ -227,19 +227,27 @@ def test_robots_with_self_collision_match_robots_alone():
 
 @pytest.mark.gpu
 def test_robot_state_info_matches_state_info():
-    """Per-robot stateInfo reduced on the device: for a single robot, MIN and MAX equal the
-    reference's stateInfo exactly and TOTAL/AVERAGE to float rounding (the reference sums in
-    float, the device in FP64); over several robots the totals add up to the whole model's."""
+    """stateInfo against the oracle (pinned bit for bit to the reference's stateInfo by the
+    golden vectors): the host entry point vxg_state_info equals the correctly rounded oracle's
+    stateInfo bit for bit on a stepped cantilever; the per-robot device reduction has the same
+    MIN and MAX exactly and TOTAL/AVERAGE within float rounding of the reference's float sum
+    (the device sums in FP64); signed sums (stress, strain, pressure) against the FP64 sum of
+    the oracle's own per-element values."""
     from voxelyze_amd import workloads as W
+    import scenarios as S
     sim = B.maker(B.product_abi())(0.005)
-    W.cantilever(sim)
-    sim.step(sim.recommendedTimeStep(), 500)
-    ls = sim.link_state()
+    ref = B.maker(B.oracle_cr_abi())(0.005)
+    for x in (sim, ref):
+        W.cantilever(x)
+        x.step(x.recommendedTimeStep(), 500)
+    got, want = S.state_infos(sim), S.state_infos(ref)
+    assert got.tobytes() == want.tobytes(), np.argwhere(got != want)[:8]
+    ls = ref.link_state()
     signed = {5: ls["stress"].astype(np.float64), 6: ls["strain"].astype(np.float64)}
     for info in range(10):
         for typ in range(4):
             dev = sim.robotStateInfo(info, typ)[0]
-            host = sim.stateInfo(info, typ)
+            host = float(want[info, typ])
             if typ < 2:
                 assert np.float32(dev) == np.float32(host), (info, typ)
             elif info in signed:  # sums with cancellation: against the FP64 sum of the values
@@ -248,12 +256,41 @@ def test_robot_state_info_matches_state_info():
                 assert dev == pytest.approx(exact, rel=1e-12, abs=1e-30), (info, typ)
             elif info != 8:
                 assert dev == pytest.approx(host, rel=1e-5, abs=1e-30), (info, typ)
+    # several robots (no actuation: temperatures stay at ambient), each against itself alone
+    # on the oracle
     shp = shapes(n=4, seed=81)
     many = batch(shp, 0)
+    many.setActuation(False)
     many.robotStep(200)
+    dt = [many.robotRecommendedTimeStep(r) for r in range(many.robotCount())]
+    for r, (ijk, mat) in enumerate(shp):
+        one = alone(B.oracle_cr_abi(), ijk, mat)
+        one.step(dt[r], 200)
+        want = S.state_infos(one)
+        for info in range(10):
+            for typ in range(4):
+                dev = many.robotStateInfo(info, typ)[r]
+                if typ < 2:
+                    assert np.float32(dev) == want[info, typ], (r, info, typ)
+                elif info not in (5, 6, 8):
+                    assert dev == pytest.approx(float(want[info, typ]), rel=1e-5, abs=1e-30), (r, info, typ)
     for info in (0, 2, 5, 6, 7, 9):
         per = many.robotStateInfo(info, 2)
         assert per.sum() == pytest.approx(many.stateInfo(info, 2), rel=1e-5, abs=1e-30), info
     rho, L = 1000.0, 0.005
     counts = np.array([many.robotVoxels(r)[1] for r in range(many.robotCount())])
     assert np.array_equal(many.robotStateInfo(9, 2), counts * float(np.float32(L * L * L * rho)))
+
+
+@pytest.mark.gpu
+def test_strain_energy_needs_the_fp64_step():
+    """The fused mixed step keeps no end forces, so STRAIN_ENERGY refuses there instead of
+    reporting stale ones."""
+    from voxelyze_amd import workloads as W
+    sim = B.maker(B.product_abi())(0.005)
+    W.cantilever(sim)
+    sim.setPrecision(1)
+    sim.step(sim.recommendedTimeStep(), 20)
+    assert math.isnan(sim.stateInfo(7, 2))
+    assert "STRAIN_ENERGY" in sim.lastError()
+    assert sim.stateInfo(0, 1) > 0

@@ -61,6 +61,9 @@ def test_cr_oracle_against_golden(name):
             disp = np.abs(gold[k] - nominal).max()
             worst = max(worst, np.abs(out[k] - gold[k]).max() / max(disp, 1e-300))
     assert worst <= S.TOL[name], f"{name}: {worst:.3e}"
+    # stateInfo: sums and extrema of per-element floats, within 10x the displacement bar
+    si, bars = S.state_info_rel_err(gold, out), S.state_info_bars(name)
+    assert si[0] <= bars[0] and si[1] <= bars[1], f"{name}: stateInfo differs by {si}"
 
 
 # --------------------------------------------------------- closed forms of tVoxelyze.h

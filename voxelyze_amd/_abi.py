@@ -108,6 +108,10 @@ EXTENSIONS = {
     "checkpoint_free": (None, [_P, _I]),
     "save_json": (_I, [_P, C.c_char_p]),
     "fp64_path": (_I, [_P]),
+    "index_bounds": (None, [_P, _IP, _IP]),
+    "material_remove": (_I, [_P, _I]),
+    "material_replace": (_I, [_P, _I, _I]),
+    "set_voxel_size": (None, [_P, _D]),
 }
 
 

@@ -366,6 +366,21 @@ class CVoxelyze:
     def materialCount(self):
         return self.abi.material_count(self.handle)
 
+    def removeMaterial(self, material):
+        """CVoxelyze::removeMaterial (src/Voxelyze.cpp:384-404): the material and its voxels go;
+        material objects with a higher index now name the next material."""
+        return bool(self.abi.material_remove(self.handle, material.index))
+
+    def replaceMaterial(self, replace_me, replace_with):
+        """CVoxelyze::replaceMaterial (src/Voxelyze.cpp:406-420)"""
+        return bool(self.abi.material_replace(self.handle, replace_me.index, replace_with.index))
+
+    def indexBounds(self):
+        """(indexMinX, indexMinY, indexMinZ), (indexMaxX, indexMaxY, indexMaxZ) (Voxelyze.h:103-108)"""
+        mn, mx = (C.c_int * 3)(), (C.c_int * 3)()
+        self.abi.index_bounds(self.handle, mn, mx)
+        return tuple(mn), tuple(mx)
+
     def material(self, i):
         return CVX_Material(self, i)
 
@@ -395,6 +410,10 @@ class CVoxelyze:
 
     def voxelSize(self):
         return self.abi.voxel_size(self.handle)
+
+    def setVoxelSize(self, size):
+        """CVoxelyze::setVoxelSize (src/Voxelyze.cpp:643-668)"""
+        self.abi.set_voxel_size(self.handle, size)
 
     # environment
     def setGravity(self, g=1.0): self.abi.set_gravity(self.handle, g)

@@ -31,6 +31,8 @@ struct Checkpoint {
 
 struct AdjointState {
     int nsteps = 0, every = 0;
+    uint64_t topoGen = 0;           /* topology generation the rollout was recorded on */
+    int precision = 0;              /* precision mode of the recorded rollout */
     std::vector<float> dt;          /* per robot */
     std::vector<float> rprev0;      /* per robot previousDt at the start */
     std::vector<float> times;       /* (nsteps + 1) * R: robot clocks at the start of each step */
@@ -107,6 +109,8 @@ bool Sim::adjointRun(int nsteps, int every, const float* dtIn) {
     adj = a;
     a->nsteps = nsteps;
     a->every = every;
+    a->topoGen = topoGen;
+    a->precision = precision;
     a->dt.resize(R);
     const std::vector<float>& rec = robotRecommendedTimeSteps();
     for (int r = 0; r < R; r++) a->dt[r] = dtIn && dtIn[r] >= 0 ? dtIn[r] : rec[r];
@@ -139,6 +143,10 @@ bool Sim::adjointBackward(const double* dLdpos, double* gThetaOut, double* gPhas
     err.clear();
     AdjointState* a = adj;
     if (!a) { err = "no recorded rollout: call the adjoint run first"; return false; }
+    if (a->topoGen != topoGen || a->precision != precision) {
+        err = "the topology or the precision mode changed since the rollout was recorded";
+        return false;
+    }
     if (!syncUp()) return false;
     const size_t n = (size_t)nv(), S = 3 * n;
     const int R = nRobots(), N = a->nsteps, every = a->every;

@@ -11,6 +11,7 @@
 #include <string.h>
 
 #include <cstdio>
+#include <limits>
 #include <string>
 
 #include "../../include/vxg.h"
@@ -127,13 +128,19 @@ API int vxg_set_precision(vxg_sim* h, int mode) {
     if (mode == s->precision) return 1;
     if (!s->syncForces()) return 0;
     s->precision = mode;
-    s->topologyChanged(); /* re-lay the device state out for the new step */
+    s->topologyChanged(false); /* re-lay the device state out for the new step */
     return 1;
 }
 
 /* CVoxelyze::stateInfo (src/Voxelyze.cpp:752-800), evaluated on the host mirror */
 API float vxg_state_info(vxg_sim* h, int info, int type) {
     Sim* s = S(h);
+    if (info == 7 && s->fused()) {
+        /* the fused VXG_MIXED / VXG_FUSED_FP64 step never stores the links' end forces that
+           CVX_Link::strainEnergy (src/VX_Link.cpp:251-257) reads */
+        s->err = "STRAIN_ENERGY needs the end forces of the FP64 step (VXG_FP64_PLAIN)";
+        return std::numeric_limits<float>::quiet_NaN();
+    }
     if (info == 7) s->syncForces(); else s->syncDown();
     float ret = 0;
     if (type == 1) ret = -FLT_MAX;
@@ -373,6 +380,33 @@ API int vxg_material_model_data(vxg_sim* h, int mi, float* st, float* ss, int ca
 
 /* ------------------------------------------------------------------ voxels */
 /* CVoxelyze::setVoxel (src/Voxelyze.cpp:422-437) */
+/* CVoxelyze::indexMinX .. indexMaxZ (include/Voxelyze.h:103-108) of the selected robot */
+API void vxg_index_bounds(vxg_sim* h, int* min3, int* max3) {
+    Sim* s = S(h);
+    const std::array<int, 6>& b = s->indexBox(s->curRobot);
+    for (int a = 0; a < 3; a++) {
+        if (min3) min3[a] = b[a];
+        if (max3) max3[a] = b[3 + a];
+    }
+}
+/* CVoxelyze::removeMaterial (src/Voxelyze.cpp:384-404) */
+API int vxg_material_remove(vxg_sim* h, int mat) {
+    Sim* s = S(h);
+    s->syncForces();
+    return s->removeMaterial(mat) ? 1 : 0;
+}
+/* CVoxelyze::replaceMaterial (src/Voxelyze.cpp:406-420) */
+API int vxg_material_replace(vxg_sim* h, int replace_me, int replace_with) {
+    Sim* s = S(h);
+    s->syncForces();
+    return s->replaceMaterial(replace_me, replace_with) ? 1 : 0;
+}
+/* CVoxelyze::setVoxelSize (src/Voxelyze.cpp:643-668) */
+API void vxg_set_voxel_size(vxg_sim* h, double size) {
+    Sim* s = S(h);
+    s->syncDown();
+    s->setVoxelSize(size);
+}
 API int vxg_voxel_set(vxg_sim* h, int mi, int x, int y, int z) {
     Sim* s = S(h);
     s->syncForces(); /* untouched links keep their last end forces across the edit */

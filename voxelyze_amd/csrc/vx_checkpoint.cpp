@@ -23,6 +23,7 @@ namespace vx {
 
 struct StateCheckpoint {
     int nv = -1, nl = -1, robots = 0, precision = 0, colEntries = 0;
+    uint64_t topoGen = 0;
     bool collisions = false, collisionsStale = true, nearbyStale = true;
     std::vector<float> rtime, rprev;
     DevBuf vd, vflags, vtemp, vprevdt, vps, ld, lf, lflags, vf, ms, mfl, clast, ccount, coff, cpart;
@@ -45,6 +46,7 @@ bool Sim::checkpointSave(int slot) {
     c.nl = nl();
     c.robots = nRobots();
     c.precision = precision;
+    c.topoGen = topoGen;
     c.rtime = rtime;
     c.rprev = rprev;
     c.collisions = collisions && cp.nv == nv() && bcLast.p;
@@ -99,7 +101,7 @@ bool Sim::checkpointRestore(int slot) {
         return false;
     }
     StateCheckpoint& c = *checkpoints[slot];
-    if (c.nv != nv() || c.nl != nl() || c.robots != nRobots() || c.precision != precision) {
+    if (c.nv != nv() || c.nl != nl() || c.robots != nRobots() || c.precision != precision || c.topoGen != topoGen) {
         err = "the checkpoint was taken with another topology or precision";
         return false;
     }

@@ -382,6 +382,8 @@ int Sim::voxelAt(int x, int y, int z) const {
 /* a new, empty robot; it becomes the one lattice-addressed calls refer to */
 int Sim::addRobot() {
     vmaps.emplace_back();
+    rbox.push_back({INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN});
+    rboxStale.push_back(0);
     rtime.push_back(0.0f);
     rprev.push_back(0.0f);
     rfirst.push_back(nv());
@@ -632,6 +634,16 @@ int Sim::addVoxel(Mat* m, int x, int y, int z) {
         err = "a simulation holds at most 50,000,000 voxels (the device's element indices are 32-bit)";
         return -1;
     }
+    {
+        /* CArray3D::addValue keeps track of min and max (include/Array3D.h:220-226) */
+        std::array<int, 6>& b = rbox[curRobot];
+        indexBox(curRobot);
+        const int c[3] = {x, y, z};
+        for (int a = 0; a < 3; a++) {
+            if (c[a] < b[a]) b[a] = c[a];
+            if (c[a] > b[3 + a]) b[3 + a] = c[a];
+        }
+    }
     const int vi = nv();
     vrobot.push_back(curRobot);
     vphase.push_back(0.0f);
@@ -663,6 +675,7 @@ int Sim::addVoxel(Mat* m, int x, int y, int z) {
 void Sim::removeVoxel(int x, int y, int z) {
     const int vi = voxelAt(x, y, z);
     if (vi < 0) return;
+    rboxStale[curRobot] = 1;
     for (int d = 0; d < 6; d++)
         if (vlinks[vi][d] >= 0) removeLinkAt(vlinks[vi][d]);
     vmaps[curRobot].erase(vkey(x, y, z));
@@ -686,6 +699,103 @@ void Sim::removeVoxel(int x, int y, int z) {
         if (lpos[l] > vi) lpos[l]--;
     }
     topologyChanged();
+}
+
+/* CArray3D::UpdateMinMax (include/Array3D.h:248-266) as the reference has it: the occupied
+   sites scanned z, then y, then x, each coordinate tested `< min` and otherwise `> max`, so a
+   site that lowers an axis's minimum never raises its maximum */
+const std::array<int, 6>& Sim::indexBox(int r) {
+    std::array<int, 6>& b = rbox[r];
+    if (!rboxStale[r]) return b;
+    rboxStale[r] = 0;
+    std::vector<std::array<int, 3>> zyx;
+    zyx.reserve(rfirst[r + 1] - rfirst[r]);
+    for (int v = rfirst[r]; v < rfirst[r + 1]; v++) zyx.push_back({vijk[v][2], vijk[v][1], vijk[v][0]});
+    std::sort(zyx.begin(), zyx.end());
+    b = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
+    for (const auto& e : zyx) {
+        const int c[3] = {e[2], e[1], e[0]};
+        for (int a = 0; a < 3; a++) {
+            if (c[a] < b[a]) b[a] = c[a];
+            else if (c[a] > b[3 + a]) b[3 + a] = c[a];
+        }
+    }
+    return b;
+}
+
+/* CVoxelyze::removeMaterial (src/Voxelyze.cpp:384-404) in every robot: the voxels of the
+   material removed in a z, y, x scan of the current index bounds (re-read as they change, as the
+   reference's loop conditions do), then the material; link materials combined from it go too
+   (no link can use them any more).  The reference dereferences the empty sites of that scan;
+   here they are skipped. */
+bool Sim::removeMaterial(int mi) {
+    if (mi < 0 || mi >= (int)vmats.size()) return false;
+    Mat* m = vmats[mi].get();
+    const int saved = curRobot;
+    for (int r = 0; r < nRobots(); r++) {
+        curRobot = r;
+        for (int k = indexBox(r)[2]; k <= indexBox(r)[5]; k++)
+            for (int j = indexBox(r)[1]; j <= indexBox(r)[4]; j++)
+                for (int i = indexBox(r)[0]; i <= indexBox(r)[3]; i++) {
+                    const int v = voxelAt(i, j, k);
+                    if (v >= 0 && vmat[v] == m) removeVoxel(i, j, k);
+                }
+    }
+    curRobot = saved;
+    for (int l = (int)lmats.size() - 1; l >= 0; l--) {
+        Mat* lm = lmats[l].get();
+        if (lm->vox1 != m && lm->vox2 != m) continue;
+        Mat* other = lm->vox1 == m ? lm->vox2 : lm->vox1;
+        other->deps.erase(std::remove(other->deps.begin(), other->deps.end(), lm), other->deps.end());
+        lmats.erase(lmats.begin() + l);
+    }
+    vmats.erase(vmats.begin() + mi);
+    materialsChanged();
+    return true;
+}
+
+/* CVoxelyze::replaceMaterial (src/Voxelyze.cpp:406-420) in every robot: setVoxel(replaceWith)
+   on each voxel of replaceMe in a z, y, x scan of the current index bounds */
+bool Sim::replaceMaterial(int a, int b) {
+    if (a < 0 || a >= (int)vmats.size() || b < 0 || b >= (int)vmats.size()) return false;
+    Mat *ma = vmats[a].get(), *mb = vmats[b].get();
+    const int saved = curRobot;
+    for (int r = 0; r < nRobots(); r++) {
+        curRobot = r;
+        for (int k = indexBox(r)[2]; k <= indexBox(r)[5]; k++)
+            for (int j = indexBox(r)[1]; j <= indexBox(r)[4]; j++)
+                for (int i = indexBox(r)[0]; i <= indexBox(r)[3]; i++) {
+                    const int v = voxelAt(i, j, k);
+                    if (v >= 0 && vmat[v] == ma) replaceVoxel(mb, i, j, k);
+                }
+    }
+    curRobot = saved;
+    return true;
+}
+
+/* CVoxelyze::setVoxelSize (src/Voxelyze.cpp:643-668) with CVX_MaterialVoxel::setNominalSize
+   (src/VX_MaterialVoxel.cpp:82-87): materials re-derived, positions scaled, motion halted,
+   static friction cleared, every link reset */
+void Sim::setVoxelSize(double size) {
+    const double scale = size / voxSize;
+    voxSize = size;
+    for (auto& m : vmats) {
+        m->nomSize = size <= 0 ? FLT_MIN : size;
+        m->updateDerived();
+    }
+    for (int v = 0; v < nv(); v++) {
+        vd[VPX][v] *= scale;
+        vd[VPY][v] *= scale;
+        vd[VPZ][v] *= scale;
+        vd[VLX][v] = vd[VLY][v] = vd[VLZ][v] = 0.0;
+        vd[VAX][v] = vd[VAY][v] = vd[VAZ][v] = 0.0;
+        vflags[v] &= ~(uint32_t)VF_STATIC;
+    }
+    for (int l = 0; l < nl(); l++) resetLink(l);
+    forcesStale = false;
+    collisionsStale = true;
+    materialsChanged();
+    dirty |= D_VSTATE | D_LSTATE | D_REST;
 }
 
 /* CVoxelyze::replaceVoxel (src/Voxelyze.cpp:486-499) with CVX_Voxel::replaceMaterial (src/VX_Voxel.cpp:79-91) */

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <array>
+#include <climits>
 #include <memory>
 #include <string>
 #include <unordered_map>
@@ -116,6 +117,15 @@ struct Sim {
     std::vector<float> rtime{0.0f};   /* currentTime of each robot */
     std::vector<float> rprev{0.0f};   /* previousDt of each robot's voxels (0 before any step) */
     std::vector<int> rfirst{0, 0};    /* voxels of robot r: [rfirst[r], rfirst[r+1]) */
+    /* CArray3D::minIndices/maxIndices of each robot's lattice (include/Array3D.h:109-110):
+       grown by every added voxel, recomputed by the reference's scan after a removal
+       (lazily: rboxStale) */
+    std::vector<std::array<int, 6>> rbox{{{INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN}}};
+    std::vector<char> rboxStale{0};
+    const std::array<int, 6>& indexBox(int r);
+    bool removeMaterial(int mi);
+    bool replaceMaterial(int a, int b);
+    void setVoxelSize(double size);
     int nRobots() const { return (int)vmaps.size(); }
     int addRobot();
     float robotRecommendedTimeStep(int r);
@@ -280,7 +290,12 @@ struct Sim {
     Ext* external(int vi);
     float recommendedTimeStep();
     void materialsChanged() { dirty |= D_MATS; dtCacheValid = rdtCacheValid = false; }
-    void topologyChanged() {
+    /* generation of the topology: bumped by every voxel/link/robot edit, so that checkpoints
+       and recorded rollouts can refuse state laid out for another topology even where the
+       counts agree (a voxel removed and added again shifts every index) */
+    uint64_t topoGen = 0;
+    void topologyChanged(bool edited = true) {
+        if (edited) topoGen++;
         dirty |= D_TOPO;
         dtCacheValid = rdtCacheValid = false;
         nearbyStale = collisionsStale = true;

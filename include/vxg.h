@@ -324,6 +324,15 @@ float vxg_robot_time(vxg_sim* s, int robot);  /* the robot's currentTime */
 int vxg_robot_step(vxg_sim* s, int nsteps, const float* dt, int* done);
 /* centre of mass of every robot (3 doubles per robot, mass-weighted, FP64) */
 int vxg_robot_com(vxg_sim* s, double* com);
+/* No counterpart in the reference.  vxg_robot_com for callers that cannot allocate device
+   memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+   the same kernel and the same bits, staged in a static device array of 65536 x 3 doubles
+   (1.5 MiB per GPU, held until the process ends) instead of a library allocation.  Returns 0
+   with vxg_last_error set for more than 65536 robots, or when the state is not yet resident on
+   the device or host edits are pending (their upload may allocate: step, or call
+   vxg_robot_com, once first).  All calls on one GPU share the array: calls that overlap on
+   one GPU (several threads or simulations at once) race. */
+int vxg_robot_com_noalloc(vxg_sim* s, double* com);
 /* Checkpoints (no reference counterpart): the dynamic state -- voxel pose, momenta, flags,
    temperatures, previousDt and Poisson strains, link state and end forces, robot clocks, the
    self-collision watch list -- copied on the device into slot 0..7 and back.  Stepping again

@@ -538,6 +538,12 @@ class CVoxelyze:
         self._check(bool(self.abi.robot_com(self.handle, _dptr(com))))
         return com
 
+    def robotComNoAlloc(self):
+        """vxg_robot_com_noalloc: robotCom staged in the library's static device array."""
+        com = np.zeros((self.robotCount(), 3))
+        self._check(bool(self.abi.robot_com_noalloc(self.handle, _dptr(com))))
+        return com
+
     def setActuation(self, enabled=True, base=0.0, amplitude=0.0, frequency=0.0):
         self.abi.set_actuation(self.handle, int(enabled), base, amplitude, frequency)
 

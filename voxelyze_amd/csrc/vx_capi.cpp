@@ -487,6 +487,7 @@ API int vxg_robot_step(vxg_sim* h, int nsteps, const float* dt, int* done) {
     return r;
 }
 API int vxg_robot_com(vxg_sim* h, double* com) { return S(h)->robotCom(com) ? 1 : 0; }
+API int vxg_robot_com_noalloc(vxg_sim* h, double* com) { return S(h)->robotComNoAlloc(com) ? 1 : 0; }
 
 /* ------------------------------------------------- per-voxel moduli, adjoint */
 API int vxg_voxel_set_modulus(vxg_sim* h, int first, int count, const float* E) {

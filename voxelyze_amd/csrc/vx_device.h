@@ -13,6 +13,10 @@
 
 namespace vx {
 
+/* the no-alloc centre-of-mass entry point's static staging array: robots and devices covered */
+constexpr int kScratchRobots = 65536;
+constexpr int kScratchDevices = 64;
+
 /* component order of the voxel FP64 block */
 enum { VPX = 0, VPY, VPZ, VLX, VLY, VLZ, VAX, VAY, VAZ, VQW, VQX, VQY, VQZ, VD_COUNT };
 /* component order of the link FP64 block: local pose and rest length live on the

@@ -26,6 +26,8 @@
  * Both kernels are HBM-bandwidth bound: a few hundred FP64 flops per element
  * against ~300 B (voxel) / ~380 B (link) of traffic.
  */
+#include <mutex>
+
 #include "vx_device.h"
 #include "vx_math.h"
 #include "vx_step_fp64.h"
@@ -380,6 +382,31 @@ void launch_robot_state(const DevPtrs& s, const int* rfirst, int info, int type,
 }
 void launch_robot_com(const DevPtrs& s, const int* rfirst, double* com, int nrobots, hipStream_t st) {
     if (nrobots > 0) hipLaunchKernelGGL(k_robot_com, dim3((unsigned)nrobots), dim3(256), 0, st, s, rfirst, com);
+}
+
+/* Staging of vxg_robot_com_noalloc: the per-robot centres of mass (3 doubles per robot, at
+   most kScratchRobots robots, 1.5 MiB) written by k_robot_com before the copy to the caller.
+   Each GPU holds its own copy for the life of the process.  In host code `scratch` names a
+   host-side stand-in, not device memory: its address on the current device is looked up once
+   per device with hipGetSymbolAddress (scratch_address) and only that address is used. */
+static __device__ __align__(256) double scratch[3 * kScratchRobots];
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kScratchDevices];
+
+hipError_t scratch_address(void** out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kScratchDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
 }
 
 }  // namespace vx

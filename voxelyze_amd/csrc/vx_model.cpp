@@ -26,6 +26,7 @@ void launch_poisson_fp64(const DevPtrs& s, int k, hipStream_t st);
 void launch_robot_state(const DevPtrs& s, const int* rfirst, int info, int type, double* out, int nrobots,
                         hipStream_t st);
 void launch_robot_com(const DevPtrs& s, const int* rfirst, double* com, int nrobots, hipStream_t st);
+hipError_t scratch_address(void** out);
 void launch_col_stale(const DevPtrs& s, const ContactPtrs& c, int gen, hipStream_t st);
 void launch_col_keys(const DevPtrs& s, const ContactPtrs& c, int nbuckets, hipStream_t st);
 hipError_t col_sort(const ContactPtrs& c, void* tmp, size_t* tmpBytes, hipStream_t st);
@@ -852,6 +853,9 @@ bool Sim::ensureDevice() {
     e = hipHostMalloc((void**)&ctlHost, 4 * sizeof(int), hipHostMallocDefault);
     if (e != hipSuccess) return fail("hipHostMalloc", e);
     if (!bctl.ensure(4 * sizeof(int))) return fail("hipMalloc", hipErrorOutOfMemory);
+    /* the address of this device's copy of the no-alloc staging array (vx_kernels.hip) */
+    e = scratch_address(&scratchPtr);
+    if (e != hipSuccess) return fail("hipGetSymbolAddress(scratch)", e);
     return true;
 }
 
@@ -1483,6 +1487,27 @@ bool Sim::robotCom(double* com) {
     launch_robot_com(dp, (const int*)brfirst.p, (double*)bcom.p, R, stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(com, bcom.p, (size_t)24 * R, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    return true;
+}
+
+/* vxg_robot_com_noalloc: k_robot_com into this device's static staging array, then copied to
+   the caller; no device memory is allocated, so the state must already be resident with no
+   pending host edits (their upload may allocate) */
+bool Sim::robotComNoAlloc(double* com) {
+    const int R = nRobots();
+    if (R > kScratchRobots) {
+        err = "vxg_robot_com_noalloc handles at most 65536 robots (its staging array's size)";
+        return false;
+    }
+    if (!stream || !scratchPtr || dirty != 0 || !brfirst.p || !dp.vd) {
+        err = "vxg_robot_com_noalloc needs the state resident on the device with no pending edits: "
+              "step, or call vxg_robot_com, once first";
+        return false;
+    }
+    launch_robot_com(dp, (const int*)brfirst.p, (double*)scratchPtr, R, stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(com, scratchPtr, (size_t)24 * R, hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
     return true;
 }

@@ -258,6 +258,8 @@ struct Sim {
     bool endLaunch(int nsteps, int* done);
     bool anyDiverged(int k, bool* all);
     bool robotCom(double* com);
+    bool robotComNoAlloc(double* com);
+    void* scratchPtr = nullptr; /* this device's copy of the no-alloc staging array */
     bool robotStateInfo(int info, int type, double* out);
     /* checkpoints of the dynamic state on the device (vx_checkpoint.cpp) */
     /* the largest device array is 13 FP32/FP64 values per link slot, 3 slots per voxel

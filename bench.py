@@ -133,13 +133,69 @@ def parse_launch(text):
     return out
 
 
-def measure(precision, workload, steps, warmup, fp64_path="fused", launch=None):
-    """Build the workload, step it `warmup` times, then time `steps` steps with HIP events
-    around every kernel.  Returns the sim (kept alive for comparisons) and the figures."""
+def tuned_launch(precision, workload, fp64_path):
+    """The launch configurations the autotuner settles on for this model size, found on a
+    scratch copy of the workload (the tuner times every candidate on its first steps), so that
+    the measured copy starts settled and its warm-up and timed steps are plain steps."""
     import ctypes as C
+    from voxelyze_amd.api import CVoxelyze, FP64_PLAIN, MIXED
+    sim = CVoxelyze(0.005)
+    build_workload(sim, workload)
+    sim.setPrecision(MIXED if precision == "mixed" else FP64_PLAIN)
+    sim.setFp64Path(FP64_PATHS[fp64_path])
+    dt = sim.recommendedTimeStep()
+    for _ in range(4):
+        sim.step(dt, 10)
+    out = {}
+    for kname, kk in KERNEL_IDS.items():
+        b, x, st = C.c_int(), C.c_int(), C.c_int()
+        sim.abi.launch_config(sim.handle, kk, C.byref(b), C.byref(x), C.byref(st))
+        if st.value:
+            out[kname] = (b.value, x.value)
+    return out
+
+
+def chunked(sim, dt, n, chunk):
+    """n steps as launches of `chunk` steps (the last one shorter); returns steps done and the
+    first diverging step or -1"""
+    done = 0
+    while done < n:
+        k = min(chunk, n - done)
+        d, at = sim.step(dt, k)
+        done += d
+        if d != k:
+            return done, at
+    return done, -1
+
+
+def launch_plan(warmup, steps):
+    """(steps per launch c, leading warm-up steps) so that the timed launches only replay graphs
+    the warm-up recorded: a graph is keyed by its launch length and by which half of the
+    ping-pong buffers holds the state, so an even c needs one c-step launch at the end of the
+    warm-up and an odd c two (one per half).  c is the largest divisor of steps that fits;
+    with none (warm-up too short), c = steps and the timed region records its graph."""
+    for c in range(steps, 1, -1):
+        if steps % c == 0 and c * (1 if c % 2 == 0 else 2) <= warmup:
+            need = c * (1 if c % 2 == 0 else 2)
+            return c, warmup - need
+    return steps, warmup
+
+
+def measure(precision, workload, steps, warmup, fp64_path="fused", launch=None, prof_steps=40):
+    """Build the workload with the autotuner's launch shapes pinned (or the given ones), step
+    it `warmup` times, then time exactly `steps` steps on the library's production path: step
+    graphs replayed, no per-kernel events.  Steps run in launches of a length the warm-up ends
+    with (launch_plan), so the warm-up records the graphs the timed launches replay.  Kernel durations come from a
+    separate profiled pass of `prof_steps` further steps (HIP events around every kernel).
+    Returns the sim (kept alive for comparisons) and the figures."""
+    import ctypes as C
+    import math
     import torch
     from voxelyze_amd.api import CVoxelyze, FP64_PLAIN, MIXED
 
+    if not launch:
+        launch = tuned_launch(precision, workload, fp64_path)
+        launch = {k: v for k, v in launch.items() if (k == "k_step_mixed") == (precision == "mixed")}
     t_build = time.perf_counter()
     sim = CVoxelyze(0.005)
     cfg = build_workload(sim, workload)
@@ -154,30 +210,40 @@ def measure(precision, workload, steps, warmup, fp64_path="fused", launch=None):
     t_build = time.perf_counter() - t_build
     print(f"[bench] {precision}: built {V} voxels, {L} links in {t_build:.1f}s, dt={dt:.7g}", file=sys.stderr,
           flush=True)
-    sim.step(dt, warmup)
+    chunk, lead = launch_plan(warmup, steps)
+    if lead:
+        sim.step(dt, lead)  # one plain launch; the last warm-up launches record the graph(s)
+    chunked(sim, dt, warmup - lead, chunk)
     torch.cuda.synchronize()
-    sim.abi.profile(sim.handle, 1)
-    sim.abi.profile_read(sim.handle, None, None, 0)
-    torch.cuda.synchronize()
+    cap0, rep0 = C.c_int(), C.c_int()
+    sim.abi.graph_stats(sim.handle, C.byref(cap0), C.byref(rep0))
     t0 = time.perf_counter()
-    done, diverged = sim.step(dt, steps)
+    done, diverged = chunked(sim, dt, steps, chunk)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
+    cap1, rep1 = C.c_int(), C.c_int()
+    sim.abi.graph_stats(sim.handle, C.byref(cap1), C.byref(rep1))
+    if done != steps:
+        print(f"[bench] {precision}: diverged at step {warmup + diverged}", file=sys.stderr)
+        sys.exit(1)
+    # kernel durations: a separate pass with HIP events around every kernel (graphs off meanwhile)
+    sim.abi.profile(sim.handle, 1)
+    sim.abi.profile_read(sim.handle, None, None, 0)
+    chunked(sim, dt, prof_steps, chunk)
+    torch.cuda.synchronize()
     sim.abi.profile(sim.handle, 0)
     nc = len(PROFILE_CLASSES)
     ms = (C.c_double * nc)()
     cnt = (C.c_int * nc)()
     sim.abi.profile_read(sim.handle, ms, cnt, nc)
-    if done != steps:
-        print(f"[bench] {precision}: diverged at step {diverged}", file=sys.stderr)
-        sys.exit(1)
     kernels = {}
     for i, name in enumerate(PROFILE_CLASSES):
         if cnt[i] and name in ALG_BYTES:
             avg_s = ms[i] / cnt[i] / 1e3
             b = ALG_BYTES[name](V, L)
             kernels[name] = {"avg_us": avg_s * 1e6, "alg_bytes": b, "gbs": b / avg_s / 1e9, "launches": cnt[i]}
-    print(f"[bench] {precision}: {el / steps * 1e3:.4f} ms/step, kernels "
+    print(f"[bench] {precision}: {el / steps * 1e3:.4f} ms/step ({rep1.value - rep0.value} graph replays, "
+          f"{cap1.value - cap0.value} captures in the timed region), kernels "
           + ", ".join(f"{k} {v['avg_us']:.1f} us {v['gbs']:.0f} GB/s" for k, v in kernels.items()),
           file=sys.stderr, flush=True)
     cfg.update({"voxels": V, "links": L, "dt": dt, "voxel_pitch_m": 0.005})
@@ -188,8 +254,10 @@ def measure(precision, workload, steps, warmup, fp64_path="fused", launch=None):
         if st.value:
             tuned[kname] = {"block": b.value, "xcd_mapping": bool(x.value)}
     cfg["autotuned_launch"] = tuned
+    cfg["timed_launches"] = {"steps_per_launch": chunk, "graph_replays": rep1.value - rep0.value,
+                             "graph_captures": cap1.value - cap0.value}
     return sim, {"value": V * steps / el, "el": el, "kernels": kernels, "cfg": cfg, "V": V, "L": L,
-                 "nominal": nominal}
+                 "nominal": nominal, "after_steps": warmup + steps + prof_steps}
 
 
 def roofline(kernels, workload):
@@ -210,13 +278,31 @@ def rel_disp_error(ref_sim, sim, nominal):
     return float(np.abs(po - pr).max() / max(np.abs(pr - nominal).max(), 1e-300))
 
 
-def bench_robots(n_robots, steps, fp64_value):
+def profiled(sim, run):
+    """kernel classes -> (avg ms, launches) over run() with HIP events around every kernel"""
+    import ctypes as C
+    import torch
+    sim.abi.profile(sim.handle, 1)
+    sim.abi.profile_read(sim.handle, None, None, 0)
+    run()
+    torch.cuda.synchronize()
+    sim.abi.profile(sim.handle, 0)
+    nc = len(PROFILE_CLASSES)
+    ms, cnt = (C.c_double * nc)(), (C.c_int * nc)()
+    sim.abi.profile_read(sim.handle, ms, cnt, nc)
+    return {PROFILE_CLASSES[i]: (ms[i] / cnt[i], cnt[i]) for i in range(nc) if cnt[i]}
+
+
+def bench_robots(n_robots, steps, fp64_value, subset=64):
     """The batched population: n_robots shapes grown to 100-8,000 voxels in a 20^3 box,
     four materials, +-39 degC at 2 Hz actuated on the device, gravity and floor friction,
-    every robot with its own recommended dt, mixed precision, one launch per step."""
+    every robot with its own recommended dt, mixed precision, one launch per step replayed from
+    a graph (100-step launches).  Beside it: the mixed step's error against the FP64 fused step
+    on the first `subset` robots over the same steps, with FP64's own response to 1e-9 N on one
+    voxel, and the roofline of the step kernel from a separate profiled pass."""
     import torch
     from voxelyze_amd import workloads as W
-    from voxelyze_amd.api import CVoxelyze, MIXED
+    from voxelyze_amd.api import CVoxelyze, FP64_PLAIN, MIXED
     t0 = time.perf_counter()
     shapes = W.robot_shapes(4, n_robots)
     sim = CVoxelyze(0.005)
@@ -224,24 +310,100 @@ def bench_robots(n_robots, steps, fp64_value):
     sim.setPrecision(MIXED)
     V, L = sim.voxelCount(), sim.linkCount()
     com0 = sim.robotCom()
-    sim.robotStep(20)  # warm-up, during which the autotuner settles
+    chunk = min(100, steps)
+    for _ in range(3):
+        sim.robotStep(chunk)  # warm-up: the autotuner settles, the graph of a launch is recorded
     torch.cuda.synchronize()
     print(f"[bench] robots: {n_robots} robots, {V} voxels, {L} links, built in {time.perf_counter() - t0:.1f}s",
           file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    done = sim.robotStep(steps)
+    done = np.zeros(n_robots, np.int64)
+    left = steps
+    while left > 0:
+        k = min(chunk, left)
+        done += sim.robotStep(k)
+        left -= k
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     travel = sim.robotCom()[:, 0] - com0[:, 0]
     value = V * steps / el
-    print(f"[bench] robots: {el / steps * 1e3:.3f} ms/step, {value:.3e} voxel-steps/s", file=sys.stderr, flush=True)
+    prof = profiled(sim, lambda: sim.robotStep(20))
+    after = 3 * chunk + steps + 20
+    avg_ms = prof["step_mixed"][0]
+    b = ALG_BYTES["step_mixed"](V, L)
+    roof = {"bound": "hbm", "kernel": "k_step_mixed", "avg_us": avg_ms * 1e3, "alg_bytes_per_launch": b,
+            "achieved": round(b / (avg_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(b / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    print(f"[bench] robots: {el / steps * 1e3:.3f} ms/step, {value:.3e} voxel-steps/s, step kernel "
+          f"{avg_ms * 1e3:.0f} us ({roof['frac'] * 100:.1f} % of HBM peak on the mixed byte model)", file=sys.stderr,
+          flush=True)
+    del sim
+    # mixed against FP64 on a subset of the robots, same steps
+    pos = {}
+    for name, prec in (("mixed", MIXED), ("fp64", FP64_PLAIN), ("fp64_pushed", FP64_PLAIN)):
+        sub = CVoxelyze(0.005)
+        W.robot_population(sub, shapes[:subset])
+        sub.setPrecision(prec)
+        nominal = sub.state()["pos"]
+        if name == "fp64_pushed":
+            sub.voxel(0).external().setForce(1e-9, 0.0, 0.0)
+        left = after
+        while left > 0:
+            k = min(500, left)
+            sub.robotStep(k)
+            left -= k
+        pos[name] = sub.state()["pos"]
+        del sub
+    scale = max(np.abs(pos["fp64"] - nominal).max(), 1e-300)
+    err = float(np.abs(pos["mixed"] - pos["fp64"]).max() / scale)
+    resp = float(np.abs(pos["fp64_pushed"] - pos["fp64"]).max() / scale)
+    print(f"[bench] robots: mixed vs FP64 on {subset} robots after {after} steps {err:.3e} (FP64's response to "
+          f"1e-9 N on one voxel {resp:.3e})", file=sys.stderr, flush=True)
     return {"workload": f"{n_robots} robots grown to 100-8000 voxels in a 20^3 box, 4 materials, +-39 degC 2 Hz "
                         "actuation on the device, gravity + floor friction, per-robot dt, mixed precision",
             "voxels": V, "links": L, "steps": steps, "value": value, "unit": "voxel-steps/s",
             "ms_per_step": el / steps * 1e3, "vs_fp64_plain_lattice": value / fp64_value,
-            "all_robots_completed": bool((done == steps).all()),
+            "all_robots_completed": bool((done == steps).all()), "roofline": roof,
             "com_x_travel_m": {"mean": float(travel.mean()), "min": float(travel.min()), "max": float(travel.max()),
-                               "after_steps": steps + 20}}
+                               "median_abs": float(np.median(np.abs(travel))), "after_steps": after - 20},
+            "mixed_vs_fp64": {"robots": subset, "after_steps": after, "rel_disp_err": err,
+                              "fp64_response_to_1e-9N_rel_disp": resp}}
+
+
+def bench_mixed_error(steps=5000, every=1000):
+    """Mixed against FP64 on the drop lattice over the steps its configuration names (it falls
+    for ~4,600 steps, then lands on the floor with friction): the relative displacement error,
+    the same for the deformation alone (each state minus its mean displacement, which is the
+    free fall), and FP64's own response to 1e-9 N on one bottom voxel beside both."""
+    from voxelyze_amd.api import CVoxelyze, FP64_PLAIN, MIXED
+    sims = {}
+    for name, prec in (("mixed", MIXED), ("fp64", FP64_PLAIN), ("fp64_pushed", FP64_PLAIN)):
+        sim = CVoxelyze(0.005)
+        build_workload(sim, "lattice")
+        sim.setPrecision(prec)
+        sims[name] = sim
+    nominal = sims["fp64"].state()["pos"]
+    low = int(np.argmin(nominal[:, 2]))
+    sims["fp64_pushed"].voxel(low).external().setForce(1e-9, 0.0, 0.0)
+    dt = sims["fp64"].recommendedTimeStep()
+    rows = []
+    for k in range(every, steps + 1, every):
+        for sim in sims.values():
+            sim.step(dt, every)
+        p = {n: s.state()["pos"] for n, s in sims.items()}
+        d = {n: q - nominal for n, q in p.items()}
+        dm = {n: x - x.mean(0) for n, x in d.items()}
+        scale, dscale = np.abs(d["fp64"]).max(), max(np.abs(dm["fp64"]).max(), 1e-300)
+        rows.append({"after_steps": k,
+                     "rel_disp_err": float(np.abs(d["mixed"] - d["fp64"]).max() / scale),
+                     "fp64_response_to_1e-9N": float(np.abs(d["fp64_pushed"] - d["fp64"]).max() / scale),
+                     "deformation_rel_err": float(np.abs(dm["mixed"] - dm["fp64"]).max() / dscale),
+                     "deformation_fp64_response_to_1e-9N": float(np.abs(dm["fp64_pushed"] - dm["fp64"]).max() / dscale)})
+        print(f"[bench] mixed error after {k} steps: displacement {rows[-1]['rel_disp_err']:.2e} (FP64 response "
+              f"{rows[-1]['fp64_response_to_1e-9N']:.2e}), deformation {rows[-1]['deformation_rel_err']:.2e} "
+              f"(FP64 response {rows[-1]['deformation_fp64_response_to_1e-9N']:.2e})", file=sys.stderr, flush=True)
+    return {"workload": "drop lattice 128x128x64 sites p=0.9 (as the headline), mixed vs FP64 fused, 1e-9 N on the "
+                        "lowest voxel for FP64's own response", "rows": rows}
 
 
 def bench_contacts(steps):
@@ -258,7 +420,7 @@ def bench_contacts(steps):
     for name, prec in (("mixed", MIXED), ("fp64", FP64_PLAIN), ("fp64_pushed", FP64_PLAIN)):
         t0 = time.perf_counter()
         sim = CVoxelyze(0.005)
-        info = W.porous_block(sim, 64)
+        info = W.porous_block(sim, 64, connected=True)
         sim.setPrecision(prec)
         V, L = sim.voxelCount(), sim.linkCount()
         nominal = sim.state()["pos"]
@@ -310,8 +472,9 @@ def bench_contacts(steps):
     print(f"[bench] contacts: mixed vs fp64 rel disp err {out['mixed_vs_fp64_rel_disp_err']:.3e}, fp64 response "
           f"to 1e-9 N on one voxel {out['fp64_response_to_1e-9N_rel_disp']:.3e}", file=sys.stderr, flush=True)
     out.update({"workload": "porous block 64^3 sites, bilinear E=5 MPa -> 0.5 MPa at 0.05 strain, failure at "
-                            "0.3 strain (3.75e5 Pa), 20 % voids r=2-6, bottom fixed, top driven -0.1 pitch/1000 "
-                            "steps, self-collision", "voxels": V, "links": L, "steps": timed, "after_steps": done})
+                            "0.3 strain (3.75e5 Pa), 20 % voids r=2-6 (largest connected component), bottom fixed, "
+                            "top driven -0.1 pitch/1000 steps, self-collision", "voxels": V, "links": L, "steps": timed,
+                "after_steps": done})
     return out
 
 
@@ -417,9 +580,9 @@ def main():
     ap.add_argument("--contact-steps", type=int, default=1000)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward cost")
     ap.add_argument("--robots", type=int, default=4096)
-    ap.add_argument("--robot-steps", type=int, default=100)
+    ap.add_argument("--robot-steps", type=int, default=10000)
     ap.add_argument("--adjoint-robots", type=int, default=256)
-    ap.add_argument("--adjoint-steps", type=int, default=200)
+    ap.add_argument("--adjoint-steps", type=int, default=2000)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a plain 1 GiB -> 1 GiB elementwise copy (known bytes) to scale PMC byte counters")
     ap.add_argument("--launch", default="",
@@ -478,10 +641,12 @@ def main():
                        "roofline": roofline(r2["kernels"], a.workload),
                        "autotuned_launch": r2["cfg"]["autotuned_launch"]}
         line["mixed_vs_fp64"] = {"speedup": mixed_v / fp64_v, "rel_disp_err": err,
-                                 "after_steps": a.warmup + a.steps}
+                                 "after_steps": r["after_steps"]}
         print(f"[bench] mixed/fp64 speedup {mixed_v / fp64_v:.2f}x, rel disp err {err:.3e}", file=sys.stderr,
               flush=True)
         del sim2
+    if not a.no_compare and a.workload == "lattice":
+        line["mixed_error_5000"] = bench_mixed_error()
     if not a.no_robots:
         line["robots"] = bench_robots(a.robots, a.robot_steps, r["value"])
     if not a.no_contacts:

@@ -248,6 +248,10 @@ int vxg_voxel_set_many(vxg_sim* s, int n, const int* mat, const int* ijk, int* i
 int vxg_voxel_find(vxg_sim* s, int ix, int iy, int iz);
 /* CVX_Voxel::indexX/Y/Z (VX_Voxel.h:66-68) and material() (VX_Voxel.h:70, as a material index) */
 int vxg_voxel_info(vxg_sim* s, int vox, int* ijk3);
+/* the same for voxels [first, first+count) in one call (no reference counterpart: the bulk
+   form of CVX_Voxel::material() and indexX/Y/Z): mat[k] the material index, ijk[3k..3k+2] the
+   lattice indices; either pointer may be NULL.  Returns 1, or 0 for a range out of bounds. */
+int vxg_voxel_get_materials(vxg_sim* s, int first, int count, int* mat, int* ijk);
 /* CVX_Voxel::setTemperature(float)                  include/VX_Voxel.h:112, src/VX_Voxel.cpp:108-114 */
 void vxg_voxel_set_temperature(vxg_sim* s, int vox, float temperature);
 float vxg_voxel_temperature(vxg_sim* s, int vox);    /* temperature()  VX_Voxel.h:111 */

@@ -24,8 +24,8 @@ TRANSCENDENTAL = {
 }
 # relative displacement error bound where bit-exactness is not required
 TOL = {name: 1e-9 for name in TRANSCENDENTAL}
-# floor friction switches static/kinetic on tiny force differences: chaotic
-TOL["config2_small"] = 1e-6
+# floor friction switches static/kinetic on tiny force differences: chaotic (config2_small, the
+# drop lattice, is bit-exact here and held to the default bar)
 TOL["config4_robot"] = 1e-6
 # plastic links under a driven face with self-contact amplify last-bit trig differences: the
 # oracle with correctly rounded trig (same as the device) lands 1.46e-8 from glibc's run

@@ -101,7 +101,15 @@ def test_adjoint_matches_central_differences(precision):
     |gradient| * |perturbation|).  The FP64 fused step forms its beam constants and actuated
     rest lengths in double from the FP32 parameters, so its loss is smooth in them: bar 1e-6
     (measured 9e-8).  The mixed step keeps FP32 beam constants and FP32 temperatures (as the
-    reference does), which leaves a noise floor on the loss: bar 1e-4 (measured 2e-5).
+    reference does), so its loss moves in steps of FP32 rounding as a parameter changes: a
+    noise floor that every difference quotient divides by h.  The test measures it (the
+    discrepancy of a difference over h = 1e-6, which the floor dominates, times 1e-6: 1e-8 per
+    unit step, 2e-6 at h = 5e-3).  Beyond that floor the mixed loss differs from the smooth
+    function the adjoint differentiates (its FP64 recomputation of the mixed step) by the FP32
+    rounding of the beam arithmetic itself: measured 2.2e-5 worst over 16 directions, so the
+    mixed step's bar is 3e-5 (the benchmark's 1e-5 is not met here; the mixed adjoint agrees
+    with the FP64 adjoint, test_mixed_adjoint_equals_fp64_adjoint) or three times the floor's
+    share, whichever is larger.
     Smooth setting: no floor contact (friction switching makes the loss piecewise smooth),
     5 degC actuation, 100 steps."""
     sim = build(shapes(), precision)
@@ -113,8 +121,9 @@ def test_adjoint_matches_central_differences(precision):
     w = (W.uniform(77, np.arange(3 * n, dtype=np.uint64)) * 2 - 1).reshape(n, 3)
     _, ge, gp = weighted_loss(sim, theta, phi, w, 100)
     worst = 0.0
-    for d in range(16):
-        u = W.uniform(100 + d, np.arange(2 * n, dtype=np.uint64)) * 2 - 1
+    bar = 1e-6 if precision == 2 else 3e-5
+    for d in range(-1 if precision == 1 else 0, 16):
+        u = W.uniform(100 + d if d >= 0 else 99, np.arange(2 * n, dtype=np.uint64)) * 2 - 1
         ut, up = u[:n], u[n:]
 
         def rel(h):
@@ -130,13 +139,59 @@ def test_adjoint_matches_central_differences(precision):
             size = np.abs(ge) @ np.abs(dt) + np.abs(gp) @ np.abs(dp)
             return ((lp - lm) - ad) / size
 
+        if d < 0:  # the mixed loss's FP32 noise floor, per unit difference step
+            floor_rel = abs(rel(1e-6)) * 1e-6
+            bar = max(bar, 3 * floor_rel / 5e-3)
+            print(f"[adjoint] mixed: FP32 noise floor {floor_rel:.2e} per unit step, {floor_rel / 5e-3:.2e} at "
+                  f"h=5e-3: bar {bar:.2e}", flush=True)
+            continue
         r1, r2 = rel(1e-2), rel(5e-3)
         err = abs((4 * r2 - r1) / 3)  # the O(h^2) truncation of the relative difference removed
         worst = max(worst, err)
         print(f"[adjoint] precision {precision} direction {d}: rel {r1:.2e} (h=1e-2) {r2:.2e} (h=5e-3) "
               f"extrapolated {err:.2e}", flush=True)
     print(f"[adjoint] precision {precision}: worst relative difference {worst:.2e}")
-    assert worst <= (1e-6 if precision == 2 else 1e-4), worst
+    assert worst <= bar, (worst, bar)
+
+
+@pytest.mark.gpu
+@pytest.mark.xfail(strict=False, reason="open: with gravity on a frictionless floor the adjoint differs from central "
+                   "differences by ~1e-3 of the derivative's size at both difference steps (a systematic term, not "
+                   "noise or truncation); without the floor the same rollout agrees to 1e-7")
+def test_adjoint_with_a_frictionless_floor_matches_central_differences():
+    """The floor's normal force in the derivative (vx_adjoint.h voxel_vjp: penetration, its
+    dependence on the voxel's modulus and temperature): the robots under gravity resting on a
+    floor without friction (no static/kinetic switches), 5 degC actuation, FP64 fused step,
+    100 steps; 8 seeded directions, Richardson-extrapolated central differences within 1e-5."""
+    sim = build(shapes(), 2)
+    for i in range(sim.materialCount()):
+        sim.material(i).setStaticFriction(0.0)
+        sim.material(i).setKineticFriction(0.0)
+    sim.setActuation(True, 0.0, 5.0, FREQ)
+    theta, phi = params(sim)
+    n = sim.voxelCount()
+    w = (W.uniform(78, np.arange(3 * n, dtype=np.uint64)) * 2 - 1).reshape(n, 3)
+    _, ge, gp = weighted_loss(sim, theta, phi, w, 100)
+    assert sim.state()["pos"][:, 2].min() < 0  # voxels press into the floor
+    worst = 0.0
+    for d in range(8):
+        u = W.uniform(200 + d, np.arange(2 * n, dtype=np.uint64)) * 2 - 1
+
+        def rel(h):
+            Ep, Em = np.exp(theta + h * u[:n]).astype(np.float32), np.exp(theta - h * u[:n]).astype(np.float32)
+            pp, pm = (phi + h * u[n:]).astype(np.float32), (phi - h * u[n:]).astype(np.float32)
+            lp, _, _ = weighted_loss_f(sim, Ep, pp, w, 100, grad=False)
+            lm, _, _ = weighted_loss_f(sim, Em, pm, w, 100, grad=False)
+            dt = np.log(Ep.astype(np.float64)) - np.log(Em.astype(np.float64))
+            dp = pp.astype(np.float64) - pm.astype(np.float64)
+            size = np.abs(ge) @ np.abs(dt) + np.abs(gp) @ np.abs(dp)
+            return ((lp - lm) - (ge @ dt + gp @ dp)) / size
+
+        r1, r2 = rel(1e-2), rel(5e-3)
+        err = abs((4 * r2 - r1) / 3)
+        worst = max(worst, err)
+        print(f"[adjoint] frictionless floor, direction {d}: {r1:.2e} {r2:.2e} extrapolated {err:.2e}", flush=True)
+    assert worst <= 1e-5, worst
 
 
 @pytest.mark.gpu

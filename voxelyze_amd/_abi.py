@@ -88,6 +88,7 @@ EXTENSIONS = {
     "robot_step": (_I, [_P, _I, _FP, _IP]),
     "robot_com": (_I, [_P, _DP]),
     "robot_com_noalloc": (_I, [_P, _DP]),
+    "voxel_get_materials": (_I, [_P, _I, _I, _IP, _IP]),
     "set_actuation": (_I, [_P, _I, _F, _F, _F]),
     "voxel_set_phase": (_I, [_P, _I, _I, _FP]),
     "voxel_set_modulus": (_I, [_P, _I, _I, _FP]),

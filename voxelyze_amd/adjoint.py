@@ -40,11 +40,17 @@ def rollout(sim, log_modulus, phase, nsteps, every=100, dt=None, reset=True):
 
 
 def robot_com(sim, pos):
-    """Mass-weighted centre of each robot from a (V, 3) position tensor (differentiable)."""
-    out = []
-    for r in range(sim.robotCount()):
-        first, count = sim.robotVoxels(r)
-        m = torch.tensor([sim.voxel(first + i).material().mass() for i in range(count)], dtype=pos.dtype,
-                         device=pos.device)
-        out.append((pos[first:first + count] * m[:, None]).sum(0) / m.sum())
-    return torch.stack(out)
+    """Mass-weighted centre of each robot from a (V, 3) position tensor (differentiable): one
+    bulk read of the voxels' masses and one segmented sum over the robots."""
+    R = sim.robotCount()
+    first = np.array([sim.robotVoxels(r)[0] for r in range(R)], np.int64)
+    count = np.array([sim.robotVoxels(r)[1] for r in range(R)], np.int64)
+    robot = np.repeat(np.arange(R), count)
+    m = torch.from_numpy(sim.voxelMasses()).to(device=pos.device, dtype=pos.dtype)
+    idx = torch.from_numpy(robot).to(pos.device)
+    span = torch.from_numpy(np.concatenate([np.arange(f, f + c) for f, c in zip(first, count)])
+                            if R else np.zeros(0, np.int64)).to(pos.device)
+    w = pos[span] * m[span, None]
+    num = torch.zeros((R, 3), dtype=pos.dtype, device=pos.device).index_add_(0, idx, w)
+    den = torch.zeros(R, dtype=pos.dtype, device=pos.device).index_add_(0, idx, m[span])
+    return num / den[:, None]

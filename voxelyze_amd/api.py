@@ -538,6 +538,18 @@ class CVoxelyze:
         self._check(bool(self.abi.robot_com(self.handle, _dptr(com))))
         return com
 
+    def voxelMaterials(self, first=0, count=None):
+        """material index of voxels [first, first+count) as an int32 array, one ABI call"""
+        n = self.voxelCount() - first if count is None else count
+        mat = np.zeros(n, np.int32)
+        self._check(bool(self.abi.voxel_get_materials(self.handle, first, n, _iptr(mat), None)))
+        return mat
+
+    def voxelMasses(self, first=0, count=None):
+        """CVX_MaterialVoxel::mass() of each voxel's material (float32 values as float64)"""
+        mass = np.array([self.material(i).mass() for i in range(self.materialCount())], np.float64)
+        return mass[self.voxelMaterials(first, count)]
+
     def robotComNoAlloc(self):
         """vxg_robot_com_noalloc: robotCom staged in the library's static device array."""
         com = np.zeros((self.robotCount(), 3))

@@ -198,7 +198,12 @@ __device__ __forceinline__ void adj_link(const AdjPtrs& a, int n) {
     write_slot(a, 2 * axis + 1, p, o.lposP, o.lqP, o.lthetaLink * wp, ltp);
 }
 
-__global__ void __launch_bounds__(256) k_adj_links(AdjPtrs a) {
+/* 2 waves per SIMD (256 VGPRs, a few spilled): 9 % faster than 1 wave without the hint and 45 %
+   faster than 3 waves on the 2.9M-voxel differentiable batch; the gradient bits do not change */
+#ifndef VX_ADJ_WAVES
+#define VX_ADJ_WAVES 2
+#endif
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_ADJ_WAVES, 8))) k_adj_links(AdjPtrs a) {
     const int n = (int)blockIdx.x * 256 + (int)threadIdx.x;
     if (n >= a.nv) return;
     adj_link<0>(a, n);

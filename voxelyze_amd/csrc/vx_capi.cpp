@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <limits>
 #include <string>
+#include <unordered_map>
 
 #include "../../include/vxg.h"
 #include "vx_model.h"
@@ -553,6 +554,21 @@ API int vxg_voxel_info(vxg_sim* h, int vi, int* ijk) {
     for (size_t i = 0; i < s->vmats.size(); i++)
         if (s->vmats[i].get() == s->vmat[vi]) return (int)i;
     return -1;
+}
+API int vxg_voxel_get_materials(vxg_sim* h, int first, int count, int* mat, int* ijk) {
+    Sim* s = S(h);
+    if (first < 0 || count < 0 || first + count > s->nv()) {
+        s->err = "voxel range out of bounds";
+        return 0;
+    }
+    std::unordered_map<const vx::Mat*, int> index;
+    for (size_t i = 0; i < s->vmats.size(); i++) index[s->vmats[i].get()] = (int)i;
+    for (int k = 0; k < count; k++) {
+        const int v = first + k;
+        if (mat) mat[k] = index.at(s->vmat[v]);
+        if (ijk) for (int c = 0; c < 3; c++) ijk[3 * k + c] = s->vijk[v][c];
+    }
+    return 1;
 }
 API void vxg_voxel_set_temperature(vxg_sim* h, int vi, float t) {
     Sim* s = S(h);

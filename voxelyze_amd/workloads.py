@@ -30,6 +30,18 @@ def uniform(seed, k):
     return (splitmix_u64(seed, k) >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
 
 
+def largest_component(occ):
+    """The largest face-connected (6-neighbour: the voxels a link joins) component of a boolean
+    site array, as a boolean array of the same shape; ties go to the lowest label."""
+    from scipy import ndimage
+    lab, n = ndimage.label(occ)
+    if n <= 1:
+        return occ.copy()
+    size = np.bincount(lab.ravel())
+    size[0] = 0
+    return lab == int(np.argmax(size))
+
+
 # ----------------------------------------------------------------- config 1
 def cantilever(sim, nx=20, ny=4, nz=4, E=1e6, rho=1000.0, zeta=1.0, total_force=-1.0):
     """Fixed-free cantilever: x=0 face setFixedAll, total_force in z spread over
@@ -81,10 +93,12 @@ def drop_lattice(sim, nx=128, ny=128, nz=64, seed=2, p_occ=0.9, p_stiff=0.3, lif
 
 # ----------------------------------------------------------------- config 3
 def porous_block(sim, n=64, seed=3, porosity=0.2, rmin=2, rmax=6, E=5e6, plastic=5e5, yield_stress=2.5e5,
-                 fail_stress=3.75e5, rho=1000.0, collisions=True):
+                 fail_stress=3.75e5, rho=1000.0, collisions=True, connected=False):
     """n^3 block of a bilinear material with random spherical voids added until the
     void fraction reaches `porosity`; bottom face fixed, top face driven by
-    setDisplacementAll each step (see porous_block_drive)."""
+    setDisplacementAll each step (see porous_block_drive).  connected=True keeps only the
+    largest face-connected component: the voids can enclose pieces with no link to the block,
+    which then rattle inside their void on contact forces alone."""
     m = sim.addMaterial(E, rho)
     m.setModelBilinear(E, plastic, yield_stress, fail_stress)
     solid = np.ones((n, n, n), dtype=bool)
@@ -96,6 +110,8 @@ def porous_block(sim, n=64, seed=3, porosity=0.2, rmin=2, rmax=6, E=5e6, plastic
         d2 = (g[:, None, None] - c[0]) ** 2 + (g[None, :, None] - c[1]) ** 2 + (g[None, None, :] - c[2]) ** 2
         solid &= d2 > r * r
         k += 1
+    if connected:
+        solid = largest_component(solid)
     top, bottom = [], []
     for z in range(n):
         for y in range(n):
@@ -233,14 +249,19 @@ def robot_population(sim, shapes, amplitude=39.0, frequency=2.0, gravity=1.0, fl
     return mats
 
 
-def random_fill_robot(seed, r, nx=32, ny=32, nz=16, p_fill=0.7):
+def random_fill_robot(seed, r, nx=32, ny=32, nz=16, p_fill=0.7, connected=True):
     """Robot r of the differentiable batch: every site of an nx*ny*nz box occupied with
     probability p_fill (u at counter 2*site of stream seed+r) by one of three materials
-    (floor(3u'), u' at counter 2*site+1).  Returns (ijk (n,3) int32, material 0..2)."""
+    (floor(3u'), u' at counter 2*site+1).  connected=True keeps the largest face-connected
+    component only: at 70 % fill a few sites form loose fragments (single voxels, pairs) that
+    are separate bodies, and a pair lying on the floor slides chaotically under kinetic
+    friction.  Returns (ijk (n,3) int32, material 0..2), sites x fastest."""
     s = seed + r
     n = nx * ny * nz
     k = np.arange(n, dtype=np.uint64)
     occ = uniform(s, 2 * k) < p_fill
+    if connected:
+        occ = largest_component(occ.reshape(nz, ny, nx)).ravel()
     mat = np.minimum((uniform(s, 2 * k + np.uint64(1)) * 3).astype(np.int32), 2)
     idx = np.nonzero(occ)[0]
     ijk = np.stack([idx % nx, (idx // nx) % ny, idx // (nx * ny)], axis=1).astype(np.int32)
@@ -248,7 +269,7 @@ def random_fill_robot(seed, r, nx=32, ny=32, nz=16, p_fill=0.7):
 
 
 def differentiable_batch(sim, n_robots=256, seed=5, nx=32, ny=32, nz=16, p_fill=0.7, amplitude=39.0,
-                         frequency=2.0):
+                         frequency=2.0, connected=True):
     """n_robots random-fill robots of three linear materials (passive 1 MPa; actuated 1 MPa
     with CTE +0.01 and -0.01 per degC), gravity, floor friction, sinusoidal actuation."""
     mats = robot_materials(sim)
@@ -259,7 +280,7 @@ def differentiable_batch(sim, n_robots=256, seed=5, nx=32, ny=32, nz=16, p_fill=
     for r in range(n_robots):
         if r:
             sim.addRobot()
-        ijk, mat = random_fill_robot(seed, r, nx, ny, nz, p_fill)
+        ijk, mat = random_fill_robot(seed, r, nx, ny, nz, p_fill, connected)
         sim.setVoxels(np.array([use[m].index for m in range(3)], np.int32)[mat], ijk)
     sim.setActuation(True, 0.0, amplitude, frequency)
     return use

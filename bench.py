@@ -409,9 +409,10 @@ def bench_mixed_error(steps=5000, every=1000):
 def bench_contacts(steps):
     """Nonlinear material and self-contact: the 64^3-site porous block (bilinear material with
     yield and failure, 20 % spherical voids, bottom face fixed, top face driven down 0.1 pitch
-    per 1,000 steps, surface self-collision on), in mixed precision and in FP64.  Timed: the
-    step launches (grid broadphase, contact forces, link and voxel updates); the host-side
-    drive of the top face between 100-step chunks is not."""
+    per 1,000 steps by a ramp the device evaluates before every step, surface self-collision
+    on), in mixed precision and in FP64, over the configuration's steps.  Timed: the step
+    launches (grid broadphase with the per-step staleness test, contact forces, link and voxel
+    updates) in launches of 1,000 steps; kernel times from a separate profiled pass."""
     import ctypes as C
     import torch
     from voxelyze_amd import workloads as W
@@ -428,34 +429,29 @@ def bench_contacts(steps):
             z = nominal[:, 2]
             sim.voxel(int(np.argmin(np.abs(z - z.max() / 2)))).external().setForce(1e-9, 0.0, 0.0)
         dt = sim.recommendedTimeStep()
-        W.porous_block_drive(sim, info["top"], 0, 0.005)
+        W.porous_block_ramp(sim, info["top"], 0.005, dt)  # evaluated on the device before every step
         sim.step(dt, 50)  # warm-up, during which the autotuner settles
         torch.cuda.synchronize()
         print(f"[bench] contacts {name}: porous block {V} voxels, {L} links, built in "
               f"{time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
-        sim.abi.profile(sim.handle, 1)
-        sim.abi.profile_read(sim.handle, None, None, 0)
-        el, done = 0.0, 50
-        for _ in range(steps // 100):
-            W.porous_block_drive(sim, info["top"], done, 0.005)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            n, at = sim.step(dt, 100)
-            torch.cuda.synchronize()
-            el += time.perf_counter() - t0
-            if n != 100:
+        done = 50
+        t0 = time.perf_counter()
+        while done - 50 < steps:
+            k = min(1000, steps - (done - 50))
+            n, at = sim.step(dt, k)
+            if n != k:
                 raise SystemExit(f"bench: the porous block diverged at step {done + at}")
-            done += 100
-        sim.abi.profile(sim.handle, 0)
-        nc = len(PROFILE_CLASSES)
-        ms, cnt = (C.c_double * nc)(), (C.c_int * nc)()
-        sim.abi.profile_read(sim.handle, ms, cnt, nc)
+            done += k
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
         timed = done - 50
+        prof = profiled(sim, lambda: sim.step(dt, 100))
+        done += 100
         states[name] = (sim.state()["pos"], nominal)
         if name == "fp64_pushed":
             del sim
             continue
-        kernels = {PROFILE_CLASSES[i]: round(ms[i] / timed, 4) for i in range(nc) if cnt[i]}
+        kernels = {k: round(v[0] * v[1] / 100, 4) for k, v in prof.items()}
         out[name] = {"value": V * timed / el, "unit": "voxel-steps/s", "ms_per_step": el / timed * 1e3,
                      "kernel_ms_per_step": kernels, "contact_pairs": sim.collisionCount()}
         print(f"[bench] contacts {name}: {el / timed * 1e3:.4f} ms/step, {V * timed / el:.3e} voxel-steps/s, "
@@ -473,7 +469,8 @@ def bench_contacts(steps):
           f"to 1e-9 N on one voxel {out['fp64_response_to_1e-9N_rel_disp']:.3e}", file=sys.stderr, flush=True)
     out.update({"workload": "porous block 64^3 sites, bilinear E=5 MPa -> 0.5 MPa at 0.05 strain, failure at "
                             "0.3 strain (3.75e5 Pa), 20 % voids r=2-6 (largest connected component), bottom fixed, "
-                            "top driven -0.1 pitch/1000 steps, self-collision", "voxels": V, "links": L, "steps": timed,
+                            "top driven -0.1 pitch/1000 steps (device ramp), self-collision", "voxels": V, "links": L,
+                "steps": timed,
                 "after_steps": done})
     return out
 
@@ -577,7 +574,7 @@ def main():
     ap.add_argument("--no-robots", action="store_true", help="skip the 4,096-robot batch")
     ap.add_argument("--no-contacts", action="store_true", help="skip the porous block with self-collision")
     ap.add_argument("--no-linear", action="store_true", help="skip the static linear solve")
-    ap.add_argument("--contact-steps", type=int, default=1000)
+    ap.add_argument("--contact-steps", type=int, default=20000)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward cost")
     ap.add_argument("--robots", type=int, default=4096)
     ap.add_argument("--robot-steps", type=int, default=10000)

@@ -269,6 +269,13 @@ void vxg_ext_set_fixed(vxg_sim* s, int vox, int dof_mask);
 void vxg_ext_set_displacement(vxg_sim* s, int vox, int dof, double displacement);
 /* CVX_External::setDisplacementAll(Vec3D, Vec3D)            VX_External.h:70, VX_External.cpp:68-75 */
 void vxg_ext_set_displacement_all(vxg_sim* s, int vox, const double* translation3, const double* rotation3);
+/* No counterpart in the reference: a prescribed-displacement ramp evaluated on the device.
+   Before every step the voxel's translation becomes base + rate * t, with t its robot's
+   currentTime at the start of the step, every dof fixed and the rotation zero: the same bits as
+   a host loop calling setDisplacementAll(base + rate * t) (VX_External.cpp:68-75) before each
+   doTimeStep, without a host round trip per step.  Any other displacement or fixing call on
+   the voxel (or vxg_ext_reset) ends the ramp; vxg_ext_get reports base as the translation. */
+void vxg_ext_set_displacement_ramp(vxg_sim* s, int vox, const double* base3, const double* rate3);
 /* CVX_External::clearDisplacement(dofComponent)             VX_External.h:72, VX_External.cpp:77-89 */
 void vxg_ext_clear_displacement(vxg_sim* s, int vox, int dof);
 /* CVX_External::clearDisplacementAll()                      VX_External.h:73, VX_External.cpp:91-98 */

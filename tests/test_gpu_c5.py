@@ -8,8 +8,8 @@ actuation phase.
 Checks:
   * the mixed step's adjoint against the FP64 fused step's adjoint (the FP64 reference of
     the mixed step);
-  * central differences along 16 seeded directions, for the FP64 fused step and for the mixed
-    step.  With floor friction the loss is piecewise smooth (voxels switch between static and
+  * central differences along 16 seeded directions for the mixed step, and the first 8 of them
+    for the FP64 fused step.  With floor friction the loss is piecewise smooth (voxels switch between static and
     kinetic friction on force comparisons), so a difference quotient over h = 1e-3 does not
     converge to the derivative; FP64's own discrepancy is the yardstick the mixed step is held
     to;
@@ -118,9 +118,9 @@ def fd_discrepancies(d, ndir=16, h=1e-3):
 
 @pytest.mark.gpu
 def test_c5_central_differences_fp64(c5):
-    r = fd_discrepancies(c5[2])
+    r = fd_discrepancies(c5[2], ndir=8)
     c5["fd_fp64"] = r
-    print(f"[c5] FP64 fused: |FD - adjoint| / size over 16 directions: median {np.median(r):.2e}, worst "
+    print(f"[c5] FP64 fused: |FD - adjoint| / size over 8 directions: median {np.median(r):.2e}, worst "
           f"{r.max():.2e}", flush=True)
     assert r.max() <= 0.2
 
@@ -130,7 +130,8 @@ def test_c5_central_differences_mixed(c5):
     r = fd_discrepancies(c5[1])
     ref = c5.get("fd_fp64")
     print(f"[c5] mixed: |FD - adjoint| / size over 16 directions: median {np.median(r):.2e}, worst {r.max():.2e}"
-          + ("" if ref is None else f" (FP64 fused: median {np.median(ref):.2e}, worst {ref.max():.2e})"), flush=True)
+          + ("" if ref is None else f" (FP64 fused over 8: median {np.median(ref):.2e}, worst {ref.max():.2e})"),
+          flush=True)
     if ref is not None:
         assert np.median(r) <= 2 * np.median(ref) + 1e-5
     assert r.max() <= 0.2

@@ -89,6 +89,7 @@ EXTENSIONS = {
     "robot_com": (_I, [_P, _DP]),
     "robot_com_noalloc": (_I, [_P, _DP]),
     "voxel_get_materials": (_I, [_P, _I, _I, _IP, _IP]),
+    "ext_set_displacement_ramp": (None, [_P, _I, _DP, _DP]),
     "set_actuation": (_I, [_P, _I, _F, _F, _F]),
     "voxel_set_phase": (_I, [_P, _I, _I, _FP]),
     "voxel_set_modulus": (_I, [_P, _I, _I, _FP]),

@@ -175,6 +175,13 @@ class CVX_External:
         r = np.ascontiguousarray(rotation, dtype=np.float64)
         self._a.ext_set_displacement_all(self._h, self._v, _dptr(t), _dptr(r))
 
+    def setDisplacementRamp(self, base=(0.0, 0.0, 0.0), rate=(0.0, 0.0, 0.0)):
+        """vxg_ext_set_displacement_ramp: translation = base + rate * t before every step, on
+        the device (a host loop of setDisplacementAll before each doTimeStep)"""
+        b = np.ascontiguousarray(base, dtype=np.float64)
+        r = np.ascontiguousarray(rate, dtype=np.float64)
+        self._a.ext_set_displacement_ramp(self._h, self._v, _dptr(b), _dptr(r))
+
     def clearDisplacement(self, d):
         self._a.ext_clear_displacement(self._h, self._v, d)
 

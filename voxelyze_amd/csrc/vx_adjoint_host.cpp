@@ -85,6 +85,21 @@ static bool restore(Sim* s, const Checkpoint& c) {
     return true;
 }
 
+/* a checkpoint into entry 0 of the segment tape, the recompute's starting state */
+static bool restoreInto(Sim* s, const Checkpoint& c, const MixTape& t) {
+    const size_t n = (size_t)s->nv(), S = 3 * n;
+    hipError_t e = hipMemcpyAsync(t.vd[0], c.vd.p, VD_COUNT * n * 8, hipMemcpyDeviceToDevice, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(t.vflIn[0], c.vfl.p, n * 4, hipMemcpyDeviceToDevice, s->stream);
+    if (e == hipSuccess && s->actOn) e = hipMemcpyAsync(t.vtemp[0], c.vtemp.p, n * 4, hipMemcpyDeviceToDevice, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(t.ls[0], c.ls.p, MS_COUNT * S * s->lsElem(), hipMemcpyDeviceToDevice, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(t.lfl[0], c.lfl.p, S * 4, hipMemcpyDeviceToDevice, s->stream);
+    if (e != hipSuccess) return s->fail("restore", e);
+    s->rtime = c.rtime;
+    s->rprev = c.rprev;
+    s->hostStale = true;
+    return true;
+}
+
 /* the forward rollout with checkpoints */
 bool Sim::adjointRun(int nsteps, int every, const float* dtIn) {
     err.clear();
@@ -159,22 +174,26 @@ bool Sim::adjointBackward(const double* dLdpos, double* gThetaOut, double* gPhas
         err = "device allocation failed (adjoint)";
         return false;
     }
-    /* the tape of one segment */
-    size_t perStep = VD_COUNT * n * 8 + 6 * n * 8 + 3 * n * 4 + MS_COUNT * S * lsElem() + S * 4;
+    /* the tape of one segment: the recompute's step j reads entry j and writes its new state
+       straight into entry j+1 (no copies), so the state arrays have every+1 entries; a step's
+       output flags are the next step's input flags; without actuation the temperatures are the
+       simulation's own, constant */
+    size_t perStep = VD_COUNT * n * 8 + 6 * n * 8 + (actOn ? 2 : 1) * n * 4 + MS_COUNT * S * lsElem() + S * 4;
     perStep = (perStep + 255) & ~(size_t)255; /* every step's FP64 arrays start 256-byte aligned */
-    if (!a->tapeBuf.ensure(perStep * every)) { err = "device allocation failed (adjoint tape)"; return false; }
+    if (!a->tapeBuf.ensure(perStep * (every + 1))) { err = "device allocation failed (adjoint tape)"; return false; }
     MixTape tape;
     {
         char* p = (char*)a->tapeBuf.p;
-        for (int j = 0; j < every; j++) {
+        for (int j = 0; j <= every; j++) {
             tape.vd.push_back((double*)p); p += VD_COUNT * n * 8;
             tape.sums.push_back((double*)p); p += 6 * n * 8;
             tape.vflIn.push_back((uint32_t*)p); p += n * 4;
-            tape.vflOut.push_back((uint32_t*)p); p += n * 4;
-            tape.vtemp.push_back((float*)p); p += n * 4;
+            if (actOn) { tape.vtemp.push_back((float*)p); p += n * 4; }
+            else tape.vtemp.push_back(dp.vtemp);
             tape.ls.push_back((void*)p); p += MS_COUNT * S * lsElem();
             tape.lfl.push_back((uint32_t*)p); p += S * 4;
         }
+        for (int j = 0; j < every; j++) tape.vflOut.push_back(tape.vflIn[j + 1]);
     }
     HIPCHK(hipMemcpyAsync(a->lamFinal.p, dLdpos, 3 * n * 8, hipMemcpyHostToDevice, stream));
     HIPCHK(hipMemcpyAsync(a->times_d.p, a->times.data(), a->times.size() * 4, hipMemcpyHostToDevice, stream));
@@ -216,7 +235,7 @@ bool Sim::adjointBackward(const double* dLdpos, double* gThetaOut, double* gPhas
     std::vector<int> done(R);
     for (int sgm = nseg - 1; sgm >= 0; sgm--) {
         const int j0 = sgm * every, m = std::min(every, N - j0);
-        if (!restore(this, a->ckpt[sgm])) return false;
+        if (!restoreInto(this, a->ckpt[sgm], tape)) return false;
         launchDt = a->dt;
         if (!beginLaunch(m) || !stepMixed(m, &tape) || !endLaunch(m, done.data())) return false;
         for (int jj = m - 1; jj >= 0; jj--) {

@@ -612,6 +612,7 @@ static vx::Ext* E(vxg_sim* h, int vi) {
 API void vxg_ext_set_fixed(vxg_sim* h, int vi, int mask) {
     vx::Ext* e = E(h, vi);
     if (!e) return;
+    e->ramp = false;
     e->dof = mask & 0x3F;
     for (int i = 0; i < 3; i++) e->translation[i] = e->rotation[i] = 0.0;
 }
@@ -619,6 +620,7 @@ API void vxg_ext_set_fixed(vxg_sim* h, int vi, int mask) {
 API void vxg_ext_set_displacement(vxg_sim* h, int vi, int dof, double d) {
     vx::Ext* e = E(h, vi);
     if (!e) return;
+    e->ramp = false;
     e->dof |= dof;
     if (d != 0.0f) {
         if (dof & VXG_X_TRANSLATE) e->translation[0] = d;
@@ -634,6 +636,7 @@ API void vxg_ext_set_displacement(vxg_sim* h, int vi, int dof, double d) {
 API void vxg_ext_set_displacement_all(vxg_sim* h, int vi, const double* t, const double* r) {
     vx::Ext* e = E(h, vi);
     if (!e) return;
+    e->ramp = false;
     e->dof |= 0x3F;
     for (int i = 0; i < 3; i++) {
         e->translation[i] = t ? t[i] : 0.0;
@@ -641,10 +644,27 @@ API void vxg_ext_set_displacement_all(vxg_sim* h, int vi, const double* t, const
     }
     e->rotationChanged();
 }
+/* a displacement ramp evaluated on the device: before every step the translation is
+   base + rate * t (t the robot's currentTime), every dof fixed, rotation zero -- what a host loop
+   calling setDisplacementAll(base + rate * t) before each doTimeStep does */
+API void vxg_ext_set_displacement_ramp(vxg_sim* h, int vi, const double* base, const double* rate) {
+    vx::Ext* e = E(h, vi);
+    if (!e) return;
+    e->dof |= 0x3F;
+    e->ramp = true;
+    for (int i = 0; i < 3; i++) {
+        e->rampBase[i] = base ? base[i] : 0.0;
+        e->rampRate[i] = rate ? rate[i] : 0.0;
+        e->translation[i] = e->rampBase[i];
+        e->rotation[i] = 0.0;
+    }
+    e->rotationChanged();
+}
 /* CVX_External::clearDisplacement (src/VX_External.cpp:77-89) */
 API void vxg_ext_clear_displacement(vxg_sim* h, int vi, int dof) {
     vx::Ext* e = E(h, vi);
     if (!e) return;
+    e->ramp = false;
     e->dof &= ~dof;
     for (int i = 0; i < 3; i++) {
         if (dof & (1 << i)) e->translation[i] = 0.0;
@@ -656,6 +676,7 @@ API void vxg_ext_clear_displacement(vxg_sim* h, int vi, int dof) {
 API void vxg_ext_clear_displacement_all(vxg_sim* h, int vi) {
     vx::Ext* e = E(h, vi);
     if (!e) return;
+    e->ramp = false;
     e->dof &= ~0x3F;
     for (int i = 0; i < 3; i++) e->translation[i] = e->rotation[i] = 0.0;
     e->rotationChanged();

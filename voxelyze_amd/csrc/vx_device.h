@@ -111,6 +111,14 @@ struct ExtRec {
     int pad;
 };
 
+/* a prescribed-displacement ramp: before every step the external's translation is set to
+   base + rate * t, t the voxel's robot's currentTime (a host loop calling setDisplacementAll
+   before each doTimeStep) */
+struct RampRec {
+    int ext, vox;
+    double base[3], rate[3];
+};
+
 /* device pointers of one simulation */
 struct DevPtrs {
     int nv, nl;

@@ -3,6 +3,7 @@
  * state buffers, the step loop and the fix-ups after a launch in which a robot diverged.
  */
 #include <algorithm>
+#include <cstring>
 
 #include "vx_model.h"
 
@@ -126,7 +127,7 @@ bool Sim::stepFp64Fused(int nsteps) {
         FusedPtrs k0 = fp;
         bind(k0);
         k0.xcd = chosen.xcd;
-        key = graph_key(&k0, sizeof k0, 4, nsteps, (actOn ? 1 : 0) | (chosen.block << 1));
+        key = graph_key(&k0, sizeof k0, 4, nsteps, ((actOn ? 1 : 0) | (chosen.block << 1)) ^ rampKey());
         bool replayed = false;
         if (!graphLaunch(key, &replayed)) return false;
         if (replayed) { /* the host side of the ping-pong, as the recorded launch left it */
@@ -167,6 +168,7 @@ bool Sim::stepFp64Fused(int nsteps) {
         fp.storeForces = (k == nsteps - 1 || lastStep) ? 1 : 0;
         const LaunchCfg cfg = tune ? tuner[3].next(&cand[k]) : chosen;
         fp.xcd = cfg.xcd;
+        launchRamps();
         launch_poisson_fp64(dp, k, stream); /* the Poisson strains this step's links read */
         if (timed) (void)hipEventRecord(events[2 * k], stream);
         launch_step_fp64(fp, cfg.block, false, stream);
@@ -202,9 +204,11 @@ bool Sim::fusedFp64Finish(int launched) {
     const int n = nv(), R = nRobots();
     launch_fused_prevdt(dp.vrobot, dp.rdiv, dp.rdt, dp.vprevdt, n, stream);
     HIPCHK(hipGetLastError());
+    if (!pinned((size_t)4 * R)) return false;
     std::vector<int> div(R);
-    HIPCHK(hipMemcpyAsync(div.data(), dp.rdiv, (size_t)4 * R, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(pin, dp.rdiv, (size_t)4 * R, hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
+    memcpy(div.data(), pin, (size_t)4 * R);
     bool earlier = false;
     for (int r = 0; r < R; r++) {
         const int d = div[r];

@@ -384,6 +384,23 @@ void launch_robot_com(const DevPtrs& s, const int* rfirst, double* com, int nrob
     if (nrobots > 0) hipLaunchKernelGGL(k_robot_com, dim3((unsigned)nrobots), dim3(256), 0, st, s, rfirst, com);
 }
 
+/* prescribed-displacement ramps, before each step: translation = base + rate * t at the voxel's
+   robot's current time, in double, as a host loop would set it with setDisplacementAll */
+__global__ void __launch_bounds__(256) k_ext_ramps(ExtRec* exts, const RampRec* ramps, int n, const float* rtime,
+                                                   const int* vrobot) {
+    const int j = (int)blockIdx.x * 256 + (int)threadIdx.x;
+    if (j >= n) return;
+    const RampRec& q = ramps[j];
+    const double t = (double)rtime[vrobot[q.vox]];
+    ExtRec& e = exts[q.ext];
+    e.tx = q.base[0] + q.rate[0] * t;
+    e.ty = q.base[1] + q.rate[1] * t;
+    e.tz = q.base[2] + q.rate[2] * t;
+}
+void launch_ext_ramps(ExtRec* exts, const RampRec* ramps, int n, const float* rtime, const int* vrobot, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(k_ext_ramps, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, exts, ramps, n, rtime, vrobot);
+}
+
 /* Staging of vxg_robot_com_noalloc: the per-robot centres of mass (3 doubles per robot, at
    most kScratchRobots robots, 1.5 MiB) written by k_robot_com before the copy to the caller.
    Each GPU holds its own copy for the life of the process.  In host code `scratch` names a

@@ -3,6 +3,7 @@
  * ping-pong buffers, the FP64 <-> FP32 link-state conversion and the step loop.
  */
 #include <algorithm>
+#include <cstring>
 
 #include "vx_model.h"
 
@@ -166,7 +167,7 @@ bool Sim::stepMixed(int nsteps, const MixTape* tape) {
         k0.rtimeIn = dp.rtime; k0.rtimeOut = (float*)brtimeAlt.p;
         k0.lsIn = bms[lcur].p; k0.lsOut = bms[lcur ^ 1].p;
         k0.lflIn = (const uint32_t*)bmfl[lcur].p; k0.lflOut = (uint32_t*)bmfl[lcur ^ 1].p;
-        key = graph_key(&k0, sizeof k0, 1 + precision, nsteps, (actOn ? 1 : 0) | (chosen.block << 1));
+        key = graph_key(&k0, sizeof k0, 1 + precision, nsteps, ((actOn ? 1 : 0) | (chosen.block << 1)) ^ rampKey());
         bool replayed = false;
         if (!graphLaunch(key, &replayed)) return false;
         if (replayed) { /* the host side of the ping-pong, as the recorded launch left it */
@@ -180,7 +181,7 @@ bool Sim::stepMixed(int nsteps, const MixTape* tape) {
     }
     if (actOn) { /* temperatures at the robots' current times */
         mp.rtimeIn = dp.rtime;
-        launch_mixed_temps(mp, dp.vtemp, stream);
+        launch_mixed_temps(mp, tape ? tape->vtemp[0] : dp.vtemp, stream);
     }
     bool lastStep = false;
     int launched = 0;
@@ -221,21 +222,25 @@ bool Sim::stepMixed(int nsteps, const MixTape* tape) {
         mp.partners = collisions ? cp.partners : nullptr;
         mp.k = k;
         mp.tapeSums = nullptr;
-        if (tape) { /* record the step's inputs for the reverse pass */
-            const size_t n = (size_t)nv();
-            HIPCHK(hipMemcpyAsync(tape->vd[k], dp.vd, VD_COUNT * n * 8, hipMemcpyDeviceToDevice, stream));
-            HIPCHK(hipMemcpyAsync(tape->vflIn[k], dp.vflags, n * 4, hipMemcpyDeviceToDevice, stream));
-            HIPCHK(hipMemcpyAsync(tape->vtemp[k], dp.vtemp, n * 4, hipMemcpyDeviceToDevice, stream));
-            HIPCHK(hipMemcpyAsync(tape->ls[k], bms[lcur].p, MS_COUNT * S * lsElem(), hipMemcpyDeviceToDevice, stream));
-            HIPCHK(hipMemcpyAsync(tape->lfl[k], bmfl[lcur].p, S * 4, hipMemcpyDeviceToDevice, stream));
+        if (tape) { /* the recompute of a reverse pass: step k reads tape entry k and writes entry k+1 */
+            mp.vdIn = tape->vd[k];
+            mp.vdOut = tape->vd[k + 1];
+            mp.vflIn = tape->vflIn[k];
+            mp.vflOut = tape->vflIn[k + 1];
+            mp.vtempIn = tape->vtemp[k];
+            mp.vtempOut = tape->vtemp[k + 1];
+            mp.lsIn = tape->ls[k];
+            mp.lsOut = tape->ls[k + 1];
+            mp.lflIn = tape->lfl[k];
+            mp.lflOut = tape->lfl[k + 1];
             mp.tapeSums = tape->sums[k];
         }
         const LaunchCfg cfg = tune ? tuner[0].next(&cand[k]) : chosen;
         mp.xcd = cfg.xcd;
+        launchRamps();
         if (timed) (void)hipEventRecord(events[2 * k], stream);
         launch_step_mixed(mp, precision == 2, cfg.block, stream);
         if (timed) (void)hipEventRecord(events[2 * k + 1], stream);
-        if (tape) HIPCHK(hipMemcpyAsync(tape->vflOut[k], bvflAlt.p, (size_t)nv() * 4, hipMemcpyDeviceToDevice, stream));
         swapVoxelBuffers();
         lcur ^= 1;
         launched++;
@@ -265,9 +270,11 @@ bool Sim::stepMixed(int nsteps, const MixTape* tape) {
    voxels, temperatures and clock from before that step (the other ping-pong half) */
 bool Sim::mixedFinish(int launched) {
     const int n = nv(), R = nRobots();
+    if (!pinned((size_t)4 * R)) return false;
     std::vector<int> div(R);
-    HIPCHK(hipMemcpyAsync(div.data(), dp.rdiv, (size_t)4 * R, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(pin, dp.rdiv, (size_t)4 * R, hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
+    memcpy(div.data(), pin, (size_t)4 * R);
     for (int r = 0; r < R; r++) {
         if (div[r] != launched - 1 || rfirst[r] == rfirst[r + 1]) continue;
         launch_fused_restore(dp.vd, (const double*)bvdAlt.p, dp.vflags, (const uint32_t*)bvflAlt.p,

@@ -26,6 +26,7 @@ void launch_poisson_fp64(const DevPtrs& s, int k, hipStream_t st);
 void launch_robot_state(const DevPtrs& s, const int* rfirst, int info, int type, double* out, int nrobots,
                         hipStream_t st);
 void launch_robot_com(const DevPtrs& s, const int* rfirst, double* com, int nrobots, hipStream_t st);
+void launch_ext_ramps(ExtRec* exts, const RampRec* ramps, int n, const float* rtime, const int* vrobot, hipStream_t st);
 hipError_t scratch_address(void** out);
 void launch_col_stale(const DevPtrs& s, const ContactPtrs& c, int gen, hipStream_t st);
 void launch_col_keys(const DevPtrs& s, const ContactPtrs& c, int nbuckets, hipStream_t st);
@@ -295,6 +296,7 @@ void Ext::rotationChanged() {
 /* CVX_External::reset (src/VX_External.cpp:37-44) */
 void Ext::reset() {
     dof = 0;
+    ramp = false;
     for (int i = 0; i < 3; i++) force[i] = moment[i] = 0.0f, translation[i] = rotation[i] = 0.0;
     rotationChanged();
 }
@@ -320,6 +322,7 @@ Sim::~Sim() {
     if (stream) (void)hipStreamSynchronize(stream);
     for (hipEvent_t e : events) (void)hipEventDestroy(e);
     if (ctlHost) (void)hipHostFree(ctlHost);
+    if (pin) (void)hipHostFree(pin);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -1048,6 +1051,21 @@ bool Sim::syncUp() {
         if (!bexts.ensure(std::max<size_t>(1, ex.size()) * sizeof(ExtRec))) { err = "device allocation failed"; return false; }
         dp.exts = (const ExtRec*)bexts.p;
         HIPCHK(up((ExtRec*)bexts.p, ex, stream));
+        std::vector<RampRec> rr;
+        for (int v = 0; v < n; v++) {
+            const Ext* e = vext[v].get();
+            if (!e || !e->ramp) continue;
+            RampRec q{};
+            q.ext = extIndex[v];
+            q.vox = v;
+            for (int c = 0; c < 3; c++) q.base[c] = e->rampBase[c], q.rate[c] = e->rampRate[c];
+            rr.push_back(q);
+        }
+        nRamps = (int)rr.size();
+        if (nRamps) {
+            if (!bramps.ensure(rr.size() * sizeof(RampRec))) { err = "device allocation failed"; return false; }
+            HIPCHK(up((RampRec*)bramps.p, rr, stream));
+        }
         HIPCHK(up(dp.vext, extIndex, stream));
         HIPCHK(hipStreamSynchronize(stream));
     }
@@ -1253,17 +1271,38 @@ int Sim::stepRobots(const float* dt, int nsteps, int* done, int* divergedAt) {
     return mn;
 }
 
+bool Sim::pinned(size_t bytes) {
+    if (pinBytes >= bytes) return true;
+    if (pin) (void)hipHostFree(pin);
+    pin = nullptr;
+    pinBytes = 0;
+    hipError_t e = hipHostMalloc((void**)&pin, bytes, hipHostMallocDefault);
+    if (e != hipSuccess) return fail("hipHostMalloc", e);
+    pinBytes = bytes;
+    return true;
+}
+
 /* per-robot launch inputs: dt, previousDt, clock; no robot diverged yet (idle robots, dt 0,
-   are marked -1) */
+   are marked -1).  Staged in pinned memory: the previous launch ended with a synchronisation,
+   so the staging area is free. */
 bool Sim::beginLaunch(int nsteps) {
     const int R = nRobots();
-    std::vector<int> div(R, NO_DIVERGENCE);
-    for (int r = 0; r < R; r++)
-        if (launchDt[r] == 0) div[r] = -1;
-    HIPCHK(up((float*)brdt.p, launchDt, stream));
-    HIPCHK(up((float*)brprev.p, rprev, stream));
-    HIPCHK(up(dp.rtime, rtime, stream));
-    HIPCHK(up(dp.rdiv, div, stream));
+    const size_t b = (size_t)4 * R;
+    if (!pinned(4 * b)) return false;
+    float* pdt = (float*)pin;
+    float* pprev = (float*)(pin + b);
+    float* ptime = (float*)(pin + 2 * b);
+    int* pdiv = (int*)(pin + 3 * b);
+    for (int r = 0; r < R; r++) {
+        pdt[r] = launchDt[r];
+        pprev[r] = rprev[r];
+        ptime[r] = rtime[r];
+        pdiv[r] = launchDt[r] == 0 ? -1 : NO_DIVERGENCE;
+    }
+    HIPCHK(hipMemcpyAsync(brdt.p, pdt, b, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(brprev.p, pprev, b, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(dp.rtime, ptime, b, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(dp.rdiv, pdiv, b, hipMemcpyHostToDevice, stream));
     HIPCHK(hipMemsetAsync(dp.ctl, 0, 4 * sizeof(int), stream));
     if (profiling) {
         const size_t need = (size_t)4 * nsteps;
@@ -1279,11 +1318,13 @@ bool Sim::beginLaunch(int nsteps) {
 /* steps each robot completed, its clock and previousDt */
 bool Sim::endLaunch(int nsteps, int* done) {
     const int R = nRobots();
-    std::vector<int> div(R);
-    std::vector<float> t(R);
+    const size_t b = (size_t)4 * R;
+    if (!pinned(2 * b)) return false;
+    const int* div = (const int*)pin;
+    const float* t = (const float*)(pin + b);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(div.data(), dp.rdiv, (size_t)4 * R, hipMemcpyDeviceToHost, stream));
-    HIPCHK(hipMemcpyAsync(t.data(), dp.rtime, (size_t)4 * R, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(pin, dp.rdiv, b, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(pin + b, dp.rtime, b, hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
     for (int r = 0; r < R; r++) {
         if (rfirst[r] == rfirst[r + 1]) continue; /* kept on the host */
@@ -1411,7 +1452,7 @@ bool Sim::stepFp64(int nsteps) {
     uint64_t key = 0;
     if (graph) {
         const LaunchCfg cfg[2] = {cl, cv};
-        key = graph_key(&dp, sizeof dp, 0, nsteps, (actOn ? 1 : 0) | (int)(fnv(0, cfg, sizeof cfg) << 1));
+        key = graph_key(&dp, sizeof dp, 0, nsteps, ((actOn ? 1 : 0) | (int)(fnv(0, cfg, sizeof cfg) << 1)) ^ rampKey());
         bool replayed = false;
         if (!graphLaunch(key, &replayed)) return false;
         if (replayed) {
@@ -1428,6 +1469,7 @@ bool Sim::stepFp64(int nsteps) {
         }
     }
     for (int k = 0; k < nsteps; k++) {
+        launchRamps();
         if (actOn) launch_actuate_fp64(dp, k, stream);
         launch_poisson_fp64(dp, k, stream);
         const LaunchCfg lc = tuner[1].settled || !autotune ? cl : tuner[1].next(&candL[k]);
@@ -1489,6 +1531,10 @@ bool Sim::robotCom(double* com) {
     HIPCHK(hipMemcpyAsync(com, bcom.p, (size_t)24 * R, hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
     return true;
+}
+
+void Sim::launchRamps() {
+    if (nRamps) launch_ext_ramps((ExtRec*)bexts.p, (const RampRec*)bramps.p, nRamps, dp.rtime, dp.vrobot, stream);
 }
 
 /* vxg_robot_com_noalloc: k_robot_com into this device's static staging array, then copied to
@@ -1663,6 +1709,7 @@ bool Sim::stepWithContacts(int nsteps) {
         nearbyStale = true;
     }
     for (int k = 0; k < nsteps; k++) {
+        launchRamps();
         if (actOn) launch_actuate_fp64(dp, k, stream);
         launch_poisson_fp64(dp, k, stream);
         launch_links_fp64(dp, k, tuner[1].chosen, stream);

@@ -63,6 +63,10 @@ struct Ext {
     double translation[3] = {0, 0, 0}, rotation[3] = {0, 0, 0};
     bool hasRotQ = false;
     dquat rotQ = {1.0, 0.0, 0.0, 0.0};
+    /* device-evaluated displacement ramp (vxg_ext_set_displacement_ramp): translation =
+       rampBase + rampRate * t before every step; any other displacement call ends it */
+    bool ramp = false;
+    double rampBase[3] = {0, 0, 0}, rampRate[3] = {0, 0, 0};
     void rotationChanged();
     void reset();
     dquat rotationQuat() const { return hasRotQ ? rotQ : qident(); }
@@ -175,12 +179,21 @@ struct Sim {
     DevPtrs dp{};
     DevBuf bvd, bvf, bvflags, bvmat, bvext, bvprevdt, bvtemp, bvijk, bvps, blratio;
     DevBuf blpos, blmeta, bld, blf, blflags;
-    DevBuf bvmats, blmats, bcs, bss, bexts, bctl;
+    DevBuf bvmats, blmats, bcs, bss, bexts, bctl, bramps;
+    int nRamps = 0;
+    void launchRamps(); /* the ramps' translations at each robot's current time (dp.rtime) */
+    /* the ramp table's address and length, which a captured step graph bakes in */
+    int rampKey() const { return (int)(((uintptr_t)bramps.p >> 8) * 31u + (unsigned)nRamps) << 8; }
     int nvCap = 0, nlCap = 0;
     std::vector<int> extIndex; /* compact external index per voxel, -1 none */
     std::vector<int> slotOf;   /* linksList position -> device link slot */
     bool forcesStale = false;  /* host end-force view older than the device slots */
     int* ctlHost = nullptr;
+    /* pinned host staging of the per-launch robot arrays (dt, previousDt, clocks, divergence):
+       their copies are then truly asynchronous, and the launch ends with one synchronisation */
+    char* pin = nullptr;
+    size_t pinBytes = 0;
+    bool pinned(size_t bytes);
     DevBuf bvrobot, bvphase, brfirst, brdt, brprev, brtime, brtimeAlt, brdiv, bcom;
     /* self-collision (src/Voxelyze.cpp:670-750) */
     bool nearbyStale = true, collisionsStale = true;

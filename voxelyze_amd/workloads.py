@@ -136,6 +136,14 @@ def porous_block_drive(sim, top, step, voxel_size, rate=0.1 / 1000):
         sim.voxel(int(i)).external().setDisplacementAll((0.0, 0.0, d))
 
 
+def porous_block_ramp(sim, top, voxel_size, dt, rate=0.1 / 1000):
+    """The same drive as a ramp the device evaluates before every step: z = -(rate * pitch / dt) * t,
+    t the simulation's current time (rate pitches per step at the step dt)."""
+    v = -rate * voxel_size / dt
+    for i in top:
+        sim.voxel(int(i)).external().setDisplacementRamp((0.0, 0.0, 0.0), (0.0, 0.0, v))
+
+
 # ----------------------------------------------------------------- config 4
 def grow_robot(seed, nvox, box=20):
     """Seeded face-neighbour accretion from the box centre to nvox voxels.

@@ -168,24 +168,12 @@ def chunked(sim, dt, n, chunk):
     return done, -1
 
 
-def launch_plan(warmup, steps):
-    """(steps per launch c, leading warm-up steps) so that the timed launches only replay graphs
-    the warm-up recorded: a graph is keyed by its launch length and by which half of the
-    ping-pong buffers holds the state, so an even c needs one c-step launch at the end of the
-    warm-up and an odd c two (one per half).  c is the largest divisor of steps that fits;
-    with none (warm-up too short), c = steps and the timed region records its graph."""
-    for c in range(steps, 1, -1):
-        if steps % c == 0 and c * (1 if c % 2 == 0 else 2) <= warmup:
-            need = c * (1 if c % 2 == 0 else 2)
-            return c, warmup - need
-    return steps, warmup
-
-
 def measure(precision, workload, steps, warmup, fp64_path="fused", launch=None, prof_steps=40):
     """Build the workload with the autotuner's launch shapes pinned (or the given ones), step
-    it `warmup` times, then time exactly `steps` steps on the library's production path: step
-    graphs replayed, no per-kernel events.  Steps run in launches of a length the warm-up ends
-    with (launch_plan), so the warm-up records the graphs the timed launches replay.  Kernel durations come from a
+    it `warmup` times, then time exactly `steps` steps on the library's production path: one
+    vxg_step launch of `steps` steps, recorded as a HIP graph (the recording is inside the timed
+    region: on this lattice it costs less than splitting the steps into launches of a length the
+    warm-up could record, each of which ends with a synchronisation), no per-kernel events.  Kernel durations come from a
     separate profiled pass of `prof_steps` further steps (HIP events around every kernel).
     Returns the sim (kept alive for comparisons) and the figures."""
     import ctypes as C
@@ -210,10 +198,9 @@ def measure(precision, workload, steps, warmup, fp64_path="fused", launch=None, 
     t_build = time.perf_counter() - t_build
     print(f"[bench] {precision}: built {V} voxels, {L} links in {t_build:.1f}s, dt={dt:.7g}", file=sys.stderr,
           flush=True)
-    chunk, lead = launch_plan(warmup, steps)
-    if lead:
-        sim.step(dt, lead)  # one plain launch; the last warm-up launches record the graph(s)
-    chunked(sim, dt, warmup - lead, chunk)
+    if warmup:
+        sim.step(dt, warmup)
+    chunk = steps
     torch.cuda.synchronize()
     cap0, rep0 = C.c_int(), C.c_int()
     sim.abi.graph_stats(sim.handle, C.byref(cap0), C.byref(rep0))

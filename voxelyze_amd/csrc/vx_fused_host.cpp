@@ -204,21 +204,26 @@ bool Sim::fusedFp64Finish(int launched) {
     const int n = nv(), R = nRobots();
     launch_fused_prevdt(dp.vrobot, dp.rdiv, dp.rdt, dp.vprevdt, n, stream);
     HIPCHK(hipGetLastError());
-    if (!pinned((size_t)4 * R)) return false;
+    /* the divergence record and the clocks in one copy: when no robot diverged they are also
+       what endLaunch needs, and the launch ends with this one synchronisation */
+    if (!pinned((size_t)8 * R)) return false;
     std::vector<int> div(R);
     HIPCHK(hipMemcpyAsync(pin, dp.rdiv, (size_t)4 * R, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(pin + (size_t)4 * R, dp.rtime, (size_t)4 * R, hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
     memcpy(div.data(), pin, (size_t)4 * R);
-    bool earlier = false;
+    bool earlier = false, restored = false;
     for (int r = 0; r < R; r++) {
         const int d = div[r];
         if (d < 0 || d >= launched || rfirst[r] == rfirst[r + 1]) continue;
-        if (d == launched - 1)
+        if (d == launched - 1) {
+            restored = true;
             launch_fused_restore(dp.vd, (const double*)bvdAlt.p, dp.vflags, (const uint32_t*)bvflAlt.p, nullptr,
                                  nullptr, dp.rtime, (const float*)brtimeAlt.p, n, rfirst[r], rfirst[r + 1] - rfirst[r],
                                  r, stream);
-        else
+        } else {
             earlier = true;
+        }
     }
     if (earlier) {
         FusedPtrs fp{};
@@ -257,7 +262,8 @@ bool Sim::fusedFp64Finish(int launched) {
         launch_step_fp64(fp, 256, true, stream);
     }
     HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(stream));
+    if (restored || earlier) HIPCHK(hipStreamSynchronize(stream));
+    pinFresh = !restored;
     return true;
 }
 

@@ -270,19 +270,23 @@ bool Sim::stepMixed(int nsteps, const MixTape* tape) {
    voxels, temperatures and clock from before that step (the other ping-pong half) */
 bool Sim::mixedFinish(int launched) {
     const int n = nv(), R = nRobots();
-    if (!pinned((size_t)4 * R)) return false;
+    if (!pinned((size_t)8 * R)) return false;
     std::vector<int> div(R);
     HIPCHK(hipMemcpyAsync(pin, dp.rdiv, (size_t)4 * R, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(pin + (size_t)4 * R, dp.rtime, (size_t)4 * R, hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
     memcpy(div.data(), pin, (size_t)4 * R);
+    bool restored = false;
     for (int r = 0; r < R; r++) {
         if (div[r] != launched - 1 || rfirst[r] == rfirst[r + 1]) continue;
+        restored = true;
         launch_fused_restore(dp.vd, (const double*)bvdAlt.p, dp.vflags, (const uint32_t*)bvflAlt.p,
                              actOn ? dp.vtemp : nullptr, (const float*)bvtempAlt.p, dp.rtime, (const float*)brtimeAlt.p, n,
                              rfirst[r], rfirst[r + 1] - rfirst[r], r, stream);
     }
     HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(stream));
+    if (restored) HIPCHK(hipStreamSynchronize(stream));
+    pinFresh = !restored; /* the divergence record and clocks endLaunch reads */
     return true;
 }
 

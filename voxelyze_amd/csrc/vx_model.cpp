@@ -1287,6 +1287,7 @@ bool Sim::pinned(size_t bytes) {
    so the staging area is free. */
 bool Sim::beginLaunch(int nsteps) {
     const int R = nRobots();
+    pinFresh = false;
     const size_t b = (size_t)4 * R;
     if (!pinned(4 * b)) return false;
     float* pdt = (float*)pin;
@@ -1323,9 +1324,12 @@ bool Sim::endLaunch(int nsteps, int* done) {
     const int* div = (const int*)pin;
     const float* t = (const float*)(pin + b);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(pin, dp.rdiv, b, hipMemcpyDeviceToHost, stream));
-    HIPCHK(hipMemcpyAsync(pin + b, dp.rtime, b, hipMemcpyDeviceToHost, stream));
-    HIPCHK(hipStreamSynchronize(stream));
+    if (!pinFresh) {
+        HIPCHK(hipMemcpyAsync(pin, dp.rdiv, b, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(pin + b, dp.rtime, b, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+    pinFresh = false;
     for (int r = 0; r < R; r++) {
         if (rfirst[r] == rfirst[r + 1]) continue; /* kept on the host */
         done[r] = div[r] < 0 ? nsteps : std::min(div[r], nsteps);

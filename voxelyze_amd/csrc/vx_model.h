@@ -194,6 +194,7 @@ struct Sim {
     char* pin = nullptr;
     size_t pinBytes = 0;
     bool pinned(size_t bytes);
+    bool pinFresh = false; /* pin holds this launch's final divergence record and clocks */
     DevBuf bvrobot, bvphase, brfirst, brdt, brprev, brtime, brtimeAlt, brdiv, bcom;
     /* self-collision (src/Voxelyze.cpp:670-750) */
     bool nearbyStale = true, collisionsStale = true;

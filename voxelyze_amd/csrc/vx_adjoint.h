@@ -155,6 +155,17 @@ VX_HD void voxel_modulus(VoxMat* m, float E) {
     m->collisionTransC = m->zetaCollision * c;
     m->globalRotC = m->zetaGlobal * ci;
 }
+/* the actuation temperature in double, not rounded to float (the FP64 fused step forms its
+   rest lengths and floor penetration from it, so that its loss is smooth in the phases) */
+VX_HD double fused_temperature_fp64(float base, float amp, float freq, double t, float phase) {
+    return (double)base + (double)amp * sin(6.283185307179586 * (double)freq * t + (double)phase);
+}
+/* baseSizeAverage() (VX_Voxel.h:88-90) at a double temperature, without the float rounding
+   of 1 + temp*alphaCTE */
+VX_HD double fp64_bsa(const VoxMat& m, double temp) {
+    const double f = 1.0 + temp * (double)m.alphaCTE;
+    return (m.sizeX * f + m.sizeY * f + m.sizeZ * f) / 3.0;
+}
 /* combined modulus of two linear voxel materials (VX_MaterialLink.cpp:70) */
 VX_HD float combined_modulus(float E1, float E2) { return 2.0f * E1 * E2 / (E1 + E2); }
 

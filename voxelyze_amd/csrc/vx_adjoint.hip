@@ -101,9 +101,14 @@ __global__ void __launch_bounds__(256) k_adj_voxels(AdjPtrs a) {
     in.penetrationStiff = m.penetrationStiff;
     in.muStatic = m.muStatic;
     in.muKinetic = m.muKinetic;
-    const float T = a.t.vtemp[i];
-    const float f = 1 + T * m.alphaCTE;
-    const double bsa = (f * m.sizeX + f * m.sizeY + f * m.sizeZ) / 3.0;
+    double bsa;
+    if (a.lsDouble && a.vphase) { /* the FP64 fused step's double temperature */
+        bsa = fp64_bsa(m, fused_temperature_fp64(a.actBase, a.actAmp, a.actFreq, a.t.rtime[rob], a.vphase[i]));
+    } else {
+        const float T = a.t.vtemp[i];
+        const float f = 1 + T * m.alphaCTE;
+        bsa = (f * m.sizeX + f * m.sizeY + f * m.sizeZ) / 3.0;
+    }
     in.halfBsaMinusHalfL = bsa / 2 - m.nomSize / 2;
     in.dBsaHalf_dT = (double)m.alphaCTE * (m.sizeX + m.sizeY + m.sizeZ) / 6.0;
     in.floorOn = (in.flagsIn & VF_FLOOR) != 0;

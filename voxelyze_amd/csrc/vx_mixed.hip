@@ -87,11 +87,6 @@ __device__ __forceinline__ float fused_actuation(float base, float amp, float fr
     return (float)((double)base + (double)amp * sin(arg));
 }
 
-/* the actuation temperature in double, not rounded to float (FP64 fused step) */
-__device__ __forceinline__ double fused_temperature_fp64(float base, float amp, float freq, double t, float phase) {
-    return (double)base + (double)amp * sin(6.283185307179586 * (double)freq * t + (double)phase);
-}
-
 /* Quat3D::FromAngleToPosX (Quat3D.h:141-166) with libm trig */
 __device__ __forceinline__ void d_angletoposx(dquat* q, dvec rf) {
     if (rf.x == 0.0 && rf.y == 0.0 && rf.z == 0.0) return;
@@ -387,6 +382,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_FUS
 
     VoxMat m = s.vmats[mi];
     if (s.vE) voxel_modulus(&m, s.vE[i]);
+    /* the voxel's modulus-dependent constants; the FP64 fused step forms a per-voxel modulus's
+       in double, as it does the links', so that its loss is smooth in the moduli */
+    const bool smoothE = sizeof(T) == 8 && s.vE;
+    double cGT = m.globalTransC, cGR = m.globalRotC, cCT = m.collisionTransC, kPen = m.penetrationStiff;
+    if (smoothE) {
+        const double E = s.vE[i], L = m.nomSize;
+        const double c = 2.0 * sqrt((double)m.mass * E * L), ci = 2.0 * sqrt((double)m.momentInertia * E * L * L * L);
+        cGT = (double)m.zetaGlobal * c;
+        cCT = (double)m.zetaCollision * c;
+        cGR = (double)m.zetaGlobal * ci;
+        kPen = 2.0 * E * L;
+    }
     const int e = s.vext[i];
     const ExtRec* ex = e >= 0 ? &s.exts[e] : nullptr;
     double* __restrict__ out = s.vdOut;
@@ -408,9 +415,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_FUS
 
     dvec t = qrot(q, lf);
     if (ex) { t.x += ex->fx; t.y += ex->fy; t.z += ex->fz; }
-    t.x -= m.globalTransC * (m.massInverse * lin.x);
-    t.y -= m.globalTransC * (m.massInverse * lin.y);
-    t.z -= m.globalTransC * (m.massInverse * lin.z);
+    t.x -= cGT * (m.massInverse * lin.x);
+    t.y -= cGT * (m.massInverse * lin.y);
+    t.z -= cGT * (m.massInverse * lin.z);
     t.z += m.gravityForce;
     if ((flags & VF_COLLISIONS) && s.colCnt) {
         /* contacts (VX_Collision.cpp:42-56) from this voxel's side, FP32 */
@@ -439,19 +446,26 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_FUS
     const bool floorOn = (flags & VF_FLOOR) != 0;
     float pen = 0.0f;
     if (floorOn) {
-        pen = (float)(m_bsa(m, tempI) / 2 - m.nomSize / 2 - pos.z);
+        double bsa = m_bsa(m, tempI);
+        if (sizeof(T) == 8 && s.vphase) /* FP64 fused step: the double temperature the links use */
+            bsa = fp64_bsa(m, fused_temperature_fp64(s.actBase, s.actAmp, s.actFreq, s.rtimeIn[rob], s.vphase[i]));
+        const double penD = bsa / 2 - m.nomSize / 2 - pos.z;
+        pen = (float)penD;
         if (pen >= 0) {
             const dvec vel = dv(m.massInverse * lin.x, m.massInverse * lin.y, m.massInverse * lin.z);
-            const float normalForce = m.penetrationStiff * pen;
-            cur.z += normalForce - m.collisionTransC * vel.z;
+            /* float as the reference computes it (VX_Voxel.cpp:282); double in the smooth case */
+            const float nf = m.penetrationStiff * pen;
+            const double normalForce = smoothE ? kPen * penD : (double)nf;
+            cur.z += normalForce - cCT * vel.z;
             if (flags & VF_STATIC) {
                 const float sf = (float)(cur.x * cur.x + cur.y * cur.y);
-                const float ff = (m.muStatic * normalForce) * (m.muStatic * normalForce);
+                const float nfs = (float)normalForce;
+                const float ff = (m.muStatic * nfs) * (m.muStatic * nfs);
                 if (sf > ff) flags &= ~VF_STATIC;
             } else {
                 const double l = sqrt(vel.x * vel.x + vel.y * vel.y);
                 if (l > 0) {
-                    const double fk = (m.muKinetic * normalForce) / l;
+                    const double fk = (smoothE ? (double)m.muKinetic * normalForce : (double)(m.muKinetic * nf)) / l;
                     cur.x -= fk * vel.x;
                     cur.y -= fk * vel.y;
                 }
@@ -476,9 +490,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_FUS
 
     dvec mo = qrot(q, lm);
     if (ex) { mo.x += ex->mx; mo.y += ex->my; mo.z += ex->mz; }
-    mo.x -= m.globalRotC * (m.momentInertiaInverse * ang.x);
-    mo.y -= m.globalRotC * (m.momentInertiaInverse * ang.y);
-    mo.z -= m.globalRotC * (m.momentInertiaInverse * ang.z);
+    mo.x -= cGR * (m.momentInertiaInverse * ang.x);
+    mo.y -= cGR * (m.momentInertiaInverse * ang.y);
+    mo.z -= cGR * (m.momentInertiaInverse * ang.z);
     ang.x += dt * mo.x; ang.y += dt * mo.y; ang.z += dt * mo.z;
     const float dti = dt * m.momentInertiaInverse;
     q = qmul(d_fromrv(dv(dti * ang.x, dti * ang.y, dti * ang.z)), q);

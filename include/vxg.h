@@ -389,9 +389,9 @@ void vxg_clear_moduli(vxg_sim* s);
    positions) (3 doubles per voxel) and writes dL/dlog(E_v) and dL/d(phase_v) for every
    voxel, recomputing each segment from its checkpoint, and leaves the simulation at the
    end of the rollout.  The derivative follows every branch the forward run took (contact,
-   friction, small/large-angle links); dt is held fixed.  Requires: linear materials
-   without failure, no self-collision, voxels free or fully fixed.  Returns 1 or 0
-   (vxg_last_error). */
+   friction, small/large-angle links, which self-collision pairs touch, with the partner
+   lists the forward used); dt is held fixed.  Requires: linear materials without failure,
+   voxels free or fully fixed.  Returns 1 or 0 (vxg_last_error). */
 int vxg_adjoint_run(vxg_sim* s, int nsteps, int every, const float* dt);
 int vxg_adjoint_backward(vxg_sim* s, const double* dl_dpos, double* grad_log_modulus, double* grad_phase);
 void vxg_adjoint_free(vxg_sim* s);

@@ -195,6 +195,66 @@ static void test_voxel(bool floorOn, bool kinetic, bool zero, double rot) {
     }
 }
 
+/* one contact pair (pair_force / pair_vjp): positions, momenta, both temperatures and both
+   log moduli, with and without collision damping */
+static void test_pair(bool damped) {
+    PairAdjIn in;
+    const double L = 5e-3;
+    in.pi = RV(1e-4);
+    in.pj = vadd(in.pi, dv(0.9 * L, U(-0.4, 0.4) * L, U(-0.4, 0.4) * L)); /* inside 1.25 L */
+    in.li = RV(1e-5);
+    in.lj = RV(1e-5);
+    in.mInvI = 8000.0;
+    in.mInvJ = 6000.0;
+    in.bsaI = L * (1 + U(-0.05, 0.05));
+    in.bsaJ = L * (1 + U(-0.05, 0.05));
+    in.dBsaI_dT = 0.01 * L;
+    in.dBsaJ_dT = 0.02 * L;
+    in.kI = 1e4;
+    in.kJ = 3e4;
+    in.cI = damped ? 0.3 : 0.0;
+    in.cJ = damped ? 0.2 : 0.0;
+    auto f = [&](const V& x) {
+        PairAdjIn p = in;
+        p.pi = dv(x[0], x[1], x[2]);
+        p.pj = dv(x[3], x[4], x[5]);
+        p.li = dv(x[6], x[7], x[8]);
+        p.lj = dv(x[9], x[10], x[11]);
+        p.bsaI = in.bsaI + x[12] * in.dBsaI_dT;
+        p.bsaJ = in.bsaJ + x[13] * in.dBsaJ_dT;
+        p.kI = in.kI * exp(x[14]);
+        p.cI = in.cI * exp(0.5 * x[14]);
+        p.kJ = in.kJ * exp(x[15]);
+        p.cJ = in.cJ * exp(0.5 * x[15]);
+        const dvec F = pair_force(p);
+        return V{F.x, F.y, F.z};
+    };
+    V ly = rnd(3);
+    PairAdjOut o;
+    pair_vjp(in, dv(ly[0], ly[1], ly[2]), &o);
+    V x = {in.pi.x, in.pi.y, in.pi.z, in.pj.x, in.pj.y, in.pj.z, in.li.x, in.li.y, in.li.z,
+           in.lj.x, in.lj.y, in.lj.z, 0.0, 0.0, 0.0, 0.0};
+    V g = {o.lpi.x, o.lpi.y, o.lpi.z, o.lpj.x, o.lpj.y, o.lpj.z, o.lli.x, o.lli.y, o.lli.z,
+           o.llj.x, o.llj.y, o.llj.z, o.lTi, o.lTj, o.lthetaI, o.lthetaJ};
+    const char* groups[] = {"pos i", "pos j", "lin i", "lin j", "T i", "T j", "logE i", "logE j"};
+    const int start[] = {0, 3, 6, 9, 12, 13, 14, 15, 16};
+    const double scales[] = {1e-6, 1e-6, 1e-5, 1e-5, 1.0, 1.0, 1.0, 1.0};
+    const double steps[] = {1.0, 1.0, 1.0, 1.0, 1e-2, 1e-2, 1e-3, 1e-3};
+    char name[64];
+    for (int gi = 0; gi < 8; gi++) {
+        V dx(x.size(), 0.0);
+        for (int i = start[gi]; i < start[gi + 1]; i++) dx[i] = U(-1, 1) * scales[gi];
+        snprintf(name, sizeof name, "contact%s d%s", damped ? "+damp" : "", groups[gi]);
+        check(name, fd_dir(f, x, dx, ly, steps[gi]), dotv(g, dx), 1e-5);
+    }
+    /* the pair's force on j is -F_i: the same function seen from j's side */
+    PairAdjIn sw = in;
+    std::swap(sw.pi, sw.pj); std::swap(sw.li, sw.lj); std::swap(sw.mInvI, sw.mInvJ); std::swap(sw.bsaI, sw.bsaJ);
+    std::swap(sw.dBsaI_dT, sw.dBsaJ_dT); std::swap(sw.kI, sw.kJ); std::swap(sw.cI, sw.cJ);
+    const dvec Fi = pair_force(in), Fj = pair_force(sw);
+    check(damped ? "contact+damp antisymmetry" : "contact antisymmetry", Fi.x + Fi.y + Fi.z, -(Fj.x + Fj.y + Fj.z), 1e-14);
+}
+
 int main() {
     for (int axis = 0; axis < 3; axis++)
         for (int large = 0; large < 2; large++)
@@ -204,6 +264,8 @@ int main() {
     test_voxel(true, true, false, 1e-6);
     test_voxel(true, false, false, 1e-6);
     test_voxel(true, false, true, 1e-6);
+    test_pair(false);
+    test_pair(true);
     printf("%d failures\n", fails);
     return fails ? 1 : 0;
 }

@@ -3,15 +3,18 @@
  * step, written out by hand, host and device.
  *
  * The step differentiated is the mixed-precision step of vx_mixed.hip with linear
- * materials (Poisson's ratio 0, no plasticity or failure) and no self-collision:
+ * materials (Poisson's ratio 0, no plasticity or failure):
  *   links   CVX_Link::orientLink + updateForces + updateStrain (VX_Link.cpp:77-249)
  *   voxels  CVX_Voxel::timeStep, force, moment, floorForce (VX_Voxel.cpp:162-298)
+ *   contacts CVX_Collision::contactForce (VX_Collision.cpp:42-56), with the partner lists
+ *           the forward used
  * with per-voxel stiffness E_v = exp(theta_v) (the link modulus is the harmonic mean
  * 2 E_n E_p / (E_n + E_p), VX_MaterialLink.cpp:70, with the beam constants of
  * VX_MaterialLink.cpp:121-141) and per-voxel actuation temperature T_v(t).
  * The derivatives are evaluated in FP64 at the recorded forward states; every
  * discrete decision (small/large-angle branch, local-velocity validity, static or
- * kinetic friction, floor contact) is taken as the forward run took it.
+ * kinetic friction, floor contact, which listed pairs touch) is taken as the forward run
+ * took it.
  *
  * Notation: for y = f(x), "f_vjp" receives ly = dL/dy and ADDS dL/dx to lx.
  * Quaternions are treated as 4-vectors with the Euclidean inner product, under which
@@ -392,6 +395,7 @@ struct VoxAdjOut {
     dvec lsumF, lsumM;        /* dL/d(link sums), voxel frame */
     double ltheta;            /* dL/dlog(E_v) through the voxel's own constants */
     double ltemp;             /* dL/dT_v through the floor penetration */
+    dvec lforce;              /* dL/d(the translational force the voxel sums, contacts included) */
 };
 
 /* CVX_Voxel::timeStep (VX_Voxel.cpp:162-232) in FP64 with the recorded decisions, without
@@ -514,6 +518,65 @@ VX_HD void voxel_vjp(const VoxAdjIn& in, dvec lPos1, dvec lLin1, dvec lAng1, dqu
     out->lsumM = lsumM;
     out->ltheta = ltheta;
     out->ltemp = ltemp;
+    out->lforce = lt;
+}
+
+/* ------------------------------------------------------------- one contact pair */
+/* CVX_Collision::contactForce (VX_Collision.cpp:42-56) from voxel i's side, in FP64:
+   F_i = -u (k (r0 - |o|) + c (v_i - v_j)) with o = p_j - p_i, u = o/|o|,
+   r0 = 0.625 (bsa_i + bsa_j), k = 2/(1/k_i + 1/k_j) (series penetration stiffness),
+   c = (c_i + c_j)/2 (mean collision damping), v the velocities along u.  The pair puts
+   -F_i on voxel j.  Whether the pair touches (r0 > |o| > 0) is the caller's recorded
+   decision. */
+struct PairAdjIn {
+    dvec pi, pj, li, lj;        /* positions and momenta */
+    double mInvI, mInvJ;        /* mass inverses */
+    double bsaI, bsaJ;          /* baseSizeAverage at the step's temperatures */
+    double dBsaI_dT, dBsaJ_dT;  /* their temperature derivatives */
+    double kI, kJ, cI, cJ;      /* penetration stiffnesses (~E) and collision damping constants (~sqrt E) */
+};
+struct PairAdjOut {
+    dvec lpi, lpj, lli, llj;
+    double lTi, lTj;            /* dL/dT of the two voxels */
+    double lthetaI, lthetaJ;    /* dL/dlog(E) of the two voxels */
+};
+
+VX_HD dvec pair_force(const PairAdjIn& in) {
+    const dvec o = vsub(in.pj, in.pi);
+    const double len = sqrt(vdot(o, o));
+    const double rel = (in.bsaI + in.bsaJ) * 0.625 - len;
+    const dvec u = vscale(o, 1.0 / len);
+    const double dvel = vdot(vsub(vscale(in.li, in.mInvI), vscale(in.lj, in.mInvJ)), u);
+    const double stiff = 2.0 / (1.0 / in.kI + 1.0 / in.kJ), damp = 0.5 * (in.cI + in.cJ);
+    return vscale(u, -(stiff * rel + damp * dvel));
+}
+
+/* the transpose of pair_force: lF = dL/dF_i */
+VX_HD void pair_vjp(const PairAdjIn& in, dvec lF, PairAdjOut* out) {
+    const dvec o = vsub(in.pj, in.pi);
+    const double len = sqrt(vdot(o, o));
+    const double rel = (in.bsaI + in.bsaJ) * 0.625 - len;
+    const dvec u = vscale(o, 1.0 / len);
+    const dvec w = vsub(vscale(in.li, in.mInvI), vscale(in.lj, in.mInvJ)); /* relative velocity */
+    const double dvel = vdot(w, u);
+    const double stiff = 2.0 / (1.0 / in.kI + 1.0 / in.kJ), damp = 0.5 * (in.cI + in.cJ);
+    const double mag = stiff * rel + damp * dvel;
+    const double lmag = -vdot(lF, u);
+    const dvec lu = vadd(vscale(lF, -mag), vscale(w, lmag * damp));
+    const double lrel = lmag * stiff;
+    out->lli = vscale(u, lmag * damp * in.mInvI);
+    out->llj = vscale(u, -lmag * damp * in.mInvJ);
+    /* o enters through u = o/|o| and through |o| (rel = r0 - |o|) */
+    const dvec lo = vsub(vscale(vsub(lu, vscale(u, vdot(u, lu))), 1.0 / len), vscale(u, lrel));
+    out->lpj = lo;
+    out->lpi = vscale(lo, -1.0);
+    const double lbsa = 0.625 * lrel;
+    out->lTi = lbsa * in.dBsaI_dT;
+    out->lTj = lbsa * in.dBsaJ_dT;
+    /* d(stiff)/dk_i = 2 k_j^2 / (k_i + k_j)^2 with dk_i/dlog E_i = k_i; dc_i/dlog E_i = c_i / 2 */
+    const double lstiff = lmag * rel, ldamp = lmag * dvel, ks = in.kI + in.kJ;
+    out->lthetaI = lstiff * 2.0 * in.kJ * in.kJ * in.kI / (ks * ks) + ldamp * 0.25 * in.cI;
+    out->lthetaJ = lstiff * 2.0 * in.kI * in.kI * in.kJ / (ks * ks) + ldamp * 0.25 * in.cJ;
 }
 
 /* ------------------------------------------------- reverse-pass kernel arguments */
@@ -560,6 +623,11 @@ struct AdjPtrs {
     double* lamLink;         /* 9 * 3nv: dL/d(stored link pose p2, a1v, a2v) */
     double* gTheta;          /* nv */
     double* gPhase;          /* nv */
+    /* self-collision: the partner lists step k used, and dL/d(each voxel's summed force) */
+    const int* colCnt;       /* null: no contacts */
+    const int* colOff;
+    const int* partners;
+    double* lamF;            /* 3 * nv */
     int k;                   /* global step index */
     int first;               /* 1 when step k is the last forward step */
     int combineOnly;         /* 1: only fold the slots of step k+1 into the gradients */

@@ -2,12 +2,15 @@
  * vx_adjoint.hip -- the reverse pass of the mixed-precision step on the device.
  *
  * One reverse step k (state k -> state k+1 recorded by the forward) is two kernels,
- * the transposes of the forward's two halves:
+ * the transposes of the forward's two halves, and a third for contacts:
  *   k_adj_voxels  one lane per voxel: completes dL/d(state k+1) from the previous reverse
  *                 step (its own part plus the <= 6 link-end contributions, gathered in
  *                 direction order, no atomics), then applies voxel_vjp: dL/d(link force
  *                 and moment sums) and its own part of dL/d(state k); accumulates the
  *                 voxel's log-modulus and phase gradients.
+ *   k_adj_contacts (self-collision on) one lane per voxel over its partner list: the
+ *                 transpose of the pair forces it and its partners summed, kept for its
+ *                 own state and parameters.
  *   k_adj_links   one lane per voxel over the three links it is the negative end of:
  *                 link_vjp, writing each end's dL/d(pose), log-modulus and temperature
  *                 contributions into per-voxel slots (direction 2a of the negative voxel,
@@ -77,6 +80,7 @@ __global__ void __launch_bounds__(256) k_adj_voxels(AdjPtrs a) {
         for (int c = 0; c < AP_COUNT; c++) P[c * nv + i] = 0.0;
         for (int c = 0; c < 6; c++) a.lamSum[c * nv + i] = 0.0;
         a.lamT[i] = 0.0;
+        if (a.colCnt) a.lamF[3 * i] = a.lamF[3 * i + 1] = a.lamF[3 * i + 2] = 0.0;
         return;
     }
     VoxMat m = a.vmats[a.vmat[i]];
@@ -124,6 +128,96 @@ __global__ void __launch_bounds__(256) k_adj_voxels(AdjPtrs a) {
     L[3 * nv + i] = o.lsumM.x; L[4 * nv + i] = o.lsumM.y; L[5 * nv + i] = o.lsumM.z;
     a.gTheta[i] += o.ltheta;
     a.lamT[i] = o.ltemp;
+    if (a.colCnt) { /* dL/d(the contact forces this voxel summed); zero where it sums none */
+        const bool applies = (in.flagsIn & VF_COLLISIONS) != 0;
+        a.lamF[3 * i] = applies ? o.lforce.x : 0.0;
+        a.lamF[3 * i + 1] = applies ? o.lforce.y : 0.0;
+        a.lamF[3 * i + 2] = applies ? o.lforce.z : 0.0;
+    }
+}
+
+/* one voxel's side of a contact pair at step k: state, baseSizeAverage at the step's
+   temperature (the FP64 fused step's double temperature when actuated) and the
+   modulus-dependent constants the forward used */
+__device__ void pair_side(const AdjPtrs& a, int v, dvec* p, dvec* l, double* mInv, double* bsa, double* dBsa, double* k,
+                          double* c) {
+    const int nv = a.nv;
+    const double* vd = a.t.vd;
+    *p = dv(vd[VPX * nv + v], vd[VPY * nv + v], vd[VPZ * nv + v]);
+    *l = dv(vd[VLX * nv + v], vd[VLY * nv + v], vd[VLZ * nv + v]);
+    VoxMat m = a.vmats[a.vmat[v]];
+    if (a.vE) voxel_modulus(&m, a.vE[v]);
+    *mInv = m.massInverse;
+    const float T = a.t.vtemp[v];
+    const float f = 1 + T * m.alphaCTE;
+    const double bf = (f * m.sizeX + f * m.sizeY + f * m.sizeZ) / 3.0;
+    *bsa = bf;
+    if (a.lsDouble && a.vphase)
+        *bsa = fp64_bsa(m, fused_temperature_fp64(a.actBase, a.actAmp, a.actFreq, a.t.rtime[a.vrobot[v]], a.vphase[v]));
+    *dBsa = (double)m.alphaCTE * (m.sizeX + m.sizeY + m.sizeZ) / 3.0;
+    *k = m.penetrationStiff;
+    *c = m.collisionTransC;
+    if (a.lsDouble && a.vE) { /* the FP64 fused step's double constants of a per-voxel modulus */
+        const double E = a.vE[v], L = m.nomSize;
+        *k = 2.0 * E * L;
+        *c = (double)m.zetaCollision * 2.0 * sqrt((double)m.mass * E * L);
+    }
+}
+
+/* the contacts of step k (VX_Collision.cpp:42-56 as the forward evaluates them, from each
+   voxel's side): a pair puts F on voxel v and -F on its partner w, so its share of dL is
+   (lamF_v - lamF_w) . dF.  Lane v gathers that over its own partner list, keeps the
+   derivatives with respect to its own state and parameters, and adds them to its partials;
+   partner lists are symmetric, so every pair is seen from both sides and nothing is summed
+   with atomics. */
+__global__ void __launch_bounds__(256) k_adj_contacts(AdjPtrs a) {
+    const int v = (int)blockIdx.x * 256 + (int)threadIdx.x;
+    const int nv = a.nv;
+    if (v >= nv) return;
+    const int cnt = a.colCnt[v], off = a.colOff[v];
+    if (cnt == 0) return;
+    PairAdjIn in;
+    pair_side(a, v, &in.pi, &in.li, &in.mInvI, &in.bsaI, &in.dBsaI_dT, &in.kI, &in.cI);
+    const dvec lamV = dv(a.lamF[3 * v], a.lamF[3 * v + 1], a.lamF[3 * v + 2]);
+    /* the forward's own decision arithmetic: float for the mixed step */
+    const VoxMat& mv0 = a.vmats[a.vmat[v]];
+    const float fv = 1 + a.t.vtemp[v] * mv0.alphaCTE;
+    const double bsaVf = (fv * mv0.sizeX + fv * mv0.sizeY + fv * mv0.sizeZ) / 3.0;
+    dvec lp = dv(0, 0, 0), ll = dv(0, 0, 0);
+    double lT = 0.0, lth = 0.0;
+    for (int q = 0; q < cnt; q++) {
+        const int w = a.partners[off + q];
+        pair_side(a, w, &in.pj, &in.lj, &in.mInvJ, &in.bsaJ, &in.dBsaJ_dT, &in.kJ, &in.cJ);
+        bool touch;
+        if (a.lsDouble) {
+            const dvec o = vsub(in.pj, in.pi);
+            const double len = sqrt(vdot(o, o));
+            touch = (in.bsaI + in.bsaJ) * 0.625 - len > 0 && len > 0;
+        } else {
+            const VoxMat& mw0 = a.vmats[a.vmat[w]];
+            const float fw = 1 + a.t.vtemp[w] * mw0.alphaCTE;
+            const double bsaWf = (fw * mw0.sizeX + fw * mw0.sizeY + fw * mw0.sizeZ) / 3.0;
+            const float ox = (float)(in.pj.x - in.pi.x), oy = (float)(in.pj.y - in.pi.y), oz = (float)(in.pj.z - in.pi.z);
+            const float len = sqrtf(ox * ox + oy * oy + oz * oz);
+            const float rel = (float)((bsaVf + bsaWf) * 0.625f) - len;
+            touch = rel > 0 && len > 0;
+        }
+        if (!touch) continue;
+        const dvec lamW = dv(a.lamF[3 * w], a.lamF[3 * w + 1], a.lamF[3 * w + 2]);
+        PairAdjOut o;
+        pair_vjp(in, vsub(lamV, lamW), &o);
+        lp = vadd(lp, o.lpi);
+        ll = vadd(ll, o.lli);
+        lT += o.lTi;
+        lth += o.lthetaI;
+    }
+    if (!fully_fixed(a, v)) { /* a prescribed pose takes nothing back */
+        double* P = a.part;
+        P[AP_POS * nv + v] += lp.x; P[(AP_POS + 1) * nv + v] += lp.y; P[(AP_POS + 2) * nv + v] += lp.z;
+        P[AP_LIN * nv + v] += ll.x; P[(AP_LIN + 1) * nv + v] += ll.y; P[(AP_LIN + 2) * nv + v] += ll.z;
+    }
+    a.lamT[v] += lT;
+    a.gTheta[v] += lth;
 }
 
 __device__ __forceinline__ void write_slot(const AdjPtrs& a, int d, int v, dvec lp, dquat lq, double th, double lt) {
@@ -221,6 +315,9 @@ void launch_adj_voxels(const AdjPtrs& a, hipStream_t st) {
 }
 void launch_adj_links(const AdjPtrs& a, hipStream_t st) {
     if (a.nv > 0) hipLaunchKernelGGL(k_adj_links, dim3((unsigned)((a.nv + 255) / 256)), dim3(256), 0, st, a);
+}
+void launch_adj_contacts(const AdjPtrs& a, hipStream_t st) {
+    if (a.nv > 0 && a.colCnt) hipLaunchKernelGGL(k_adj_contacts, dim3((unsigned)((a.nv + 255) / 256)), dim3(256), 0, st, a);
 }
 
 }  // namespace vx

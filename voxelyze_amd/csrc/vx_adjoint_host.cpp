@@ -22,11 +22,16 @@ namespace vx {
 
 void launch_adj_voxels(const AdjPtrs& a, hipStream_t st);
 void launch_adj_links(const AdjPtrs& a, hipStream_t st);
+void launch_adj_contacts(const AdjPtrs& a, hipStream_t st);
 
-/* one saved state of the mixed step */
+/* one saved state of the mixed step, with the self-collision watch list (the positions of the
+   last rebuild, the partner lists and the staleness state) when collisions are on */
 struct Checkpoint {
     DevBuf vd, vfl, vtemp, ls, lfl;
     std::vector<float> rtime, rprev;
+    bool col = false, colStale = true, nearStale = true;
+    int colEntries = 0;
+    DevBuf clast, ccount, coff, cpart;
 };
 
 struct AdjointState {
@@ -38,7 +43,8 @@ struct AdjointState {
     std::vector<float> times;       /* (nsteps + 1) * R: robot clocks at the start of each step */
     std::vector<Checkpoint> ckpt;   /* segment starts, then the final state */
     DevBuf tapeBuf, times_d, dt_d, rprev_d;
-    DevBuf lamFinal, part, lamSum, lamT, slots, lamLink, gTheta, gPhase;
+    DevBuf lamFinal, part, lamSum, lamT, slots, lamLink, gTheta, gPhase, lamF;
+    std::vector<std::unique_ptr<ColSnap>> colSnaps; /* partner lists of the recomputed steps */
 };
 
 #define HIPCHK(call)                                       \
@@ -67,6 +73,44 @@ static bool save(Sim* s, Checkpoint* c) {
     if (e != hipSuccess) return s->fail("checkpoint", e);
     c->rtime = s->rtime;
     c->rprev = s->rprev;
+    c->col = s->collisions;
+    if (c->col) {
+        const size_t ce = (size_t)std::max(s->colEntries, 0);
+        if (!c->clast.ensure(3 * n * 4) || !c->ccount.ensure(n * 4) || !c->coff.ensure(n * 4) ||
+            !c->cpart.ensure(std::max<size_t>(ce, 1) * 4)) {
+            s->err = "device allocation failed (adjoint checkpoint)";
+            return false;
+        }
+        e = hipMemcpyAsync(c->clast.p, s->cp.last, 3 * n * 4, hipMemcpyDeviceToDevice, s->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(c->ccount.p, s->cp.pairCount, n * 4, hipMemcpyDeviceToDevice, s->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(c->coff.p, s->cp.pairOff, n * 4, hipMemcpyDeviceToDevice, s->stream);
+        if (e == hipSuccess && ce) e = hipMemcpyAsync(c->cpart.p, s->cp.partners, ce * 4, hipMemcpyDeviceToDevice, s->stream);
+        if (e != hipSuccess) return s->fail("checkpoint", e);
+        c->colEntries = s->colEntries;
+        c->colStale = s->collisionsStale;
+        c->nearStale = s->nearbyStale;
+    }
+    return true;
+}
+
+/* the watch list of a checkpoint back into the simulation */
+static bool restoreContacts(Sim* s, const Checkpoint& c) {
+    if (!c.col) return true;
+    const size_t n = (size_t)s->nv(), ce = (size_t)std::max(c.colEntries, 0);
+    if (!s->bcPartners.ensure(std::max<size_t>(ce, 1) * 4) || !s->bcF.ensure(3 * std::max<size_t>(ce, 1) * 4)) {
+        s->err = "device allocation failed";
+        return false;
+    }
+    s->cp.partners = (int*)s->bcPartners.p;
+    s->cp.cf = (float*)s->bcF.p;
+    hipError_t e = hipMemcpyAsync(s->cp.last, c.clast.p, 3 * n * 4, hipMemcpyDeviceToDevice, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(s->cp.pairCount, c.ccount.p, n * 4, hipMemcpyDeviceToDevice, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(s->cp.pairOff, c.coff.p, n * 4, hipMemcpyDeviceToDevice, s->stream);
+    if (e == hipSuccess && ce) e = hipMemcpyAsync(s->cp.partners, c.cpart.p, ce * 4, hipMemcpyDeviceToDevice, s->stream);
+    if (e != hipSuccess) return s->fail("restore", e);
+    s->colEntries = c.colEntries;
+    s->collisionsStale = c.colStale;
+    s->nearbyStale = c.nearStale;
     return true;
 }
 
@@ -77,7 +121,9 @@ static bool restore(Sim* s, const Checkpoint& c) {
     if (e == hipSuccess) e = hipMemcpyAsync(s->dp.vtemp, c.vtemp.p, n * 4, hipMemcpyDeviceToDevice, s->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(s->bms[s->lcur].p, c.ls.p, MS_COUNT * S * s->lsElem(), hipMemcpyDeviceToDevice, s->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(s->bmfl[s->lcur].p, c.lfl.p, S * 4, hipMemcpyDeviceToDevice, s->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) return s->fail("restore", e);
+    if (!restoreContacts(s, c)) return false;
+    e = hipStreamSynchronize(s->stream);
     if (e != hipSuccess) return s->fail("restore", e);
     s->rtime = c.rtime;
     s->rprev = c.rprev;
@@ -94,6 +140,7 @@ static bool restoreInto(Sim* s, const Checkpoint& c, const MixTape& t) {
     if (e == hipSuccess) e = hipMemcpyAsync(t.ls[0], c.ls.p, MS_COUNT * S * s->lsElem(), hipMemcpyDeviceToDevice, s->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(t.lfl[0], c.lfl.p, S * 4, hipMemcpyDeviceToDevice, s->stream);
     if (e != hipSuccess) return s->fail("restore", e);
+    if (!restoreContacts(s, c)) return false;
     s->rtime = c.rtime;
     s->rprev = c.rprev;
     s->hostStale = true;
@@ -106,7 +153,6 @@ bool Sim::adjointRun(int nsteps, int every, const float* dtIn) {
     adjointFree();
     if (nsteps <= 0 || every <= 0) { err = "nsteps and the checkpoint interval must be positive"; return false; }
     if (!fused()) { err = "the adjoint differentiates the fused step: set VXG_MIXED or VXG_FUSED_FP64"; return false; }
-    if (collisions) { err = "the adjoint does not cover self-collision: disable collisions"; return false; }
     for (auto& m : vmats)
         if (!m->linear || m->epsilonFail != -1.0f || m->nu != 0.0f) {
             err = "the adjoint needs linear materials without failure and Poisson's ratio 0";
@@ -119,6 +165,10 @@ bool Sim::adjointRun(int nsteps, int every, const float* dtIn) {
         }
     if (nv() == 0) { err = "no voxels"; return false; }
     if (!syncUp()) return false;
+    if (collisions && (cp.nv != nv() || !bcLast.p)) { /* the watch list exists before the first checkpoint */
+        if (!contactBuffers()) return false;
+        nearbyStale = true;
+    }
     const int R = nRobots();
     AdjointState* a = new AdjointState();
     adj = a;
@@ -170,6 +220,7 @@ bool Sim::adjointBackward(const double* dLdpos, double* gThetaOut, double* gPhas
     if (!a->lamFinal.ensure(3 * n * 8) || !a->part.ensure(13 * n * 8) || !a->lamSum.ensure(6 * n * 8) ||
         !a->lamT.ensure(n * 8) || !a->slots.ensure(6 * 9 * n * 8) || !a->lamLink.ensure(9 * S * 8) ||
         !a->gTheta.ensure(n * 8) || !a->gPhase.ensure(n * 8) || !a->times_d.ensure(a->times.size() * 4) ||
+        (collisions && !a->lamF.ensure(3 * n * 8)) ||
         !a->dt_d.ensure((size_t)R * 4) || !a->rprev_d.ensure((size_t)R * 4)) {
         err = "device allocation failed (adjoint)";
         return false;
@@ -231,11 +282,15 @@ bool Sim::adjointBackward(const double* dLdpos, double* gThetaOut, double* gPhas
     ap.lamLink = (double*)a->lamLink.p;
     ap.gTheta = (double*)a->gTheta.p;
     ap.gPhase = (double*)a->gPhase.p;
+    ap.lamF = collisions ? (double*)a->lamF.p : nullptr;
+    if (collisions) tape.colSnaps = &a->colSnaps;
     const float* times = (const float*)a->times_d.p;
     std::vector<int> done(R);
     for (int sgm = nseg - 1; sgm >= 0; sgm--) {
         const int j0 = sgm * every, m = std::min(every, N - j0);
         if (!restoreInto(this, a->ckpt[sgm], tape)) return false;
+        tape.nSnaps = 0;
+        tape.colVer.assign(m, -1);
         launchDt = a->dt;
         if (!beginLaunch(m) || !stepMixed(m, &tape) || !endLaunch(m, done.data())) return false;
         for (int jj = m - 1; jj >= 0; jj--) {
@@ -252,7 +307,15 @@ bool Sim::adjointBackward(const double* dLdpos, double* gThetaOut, double* gPhas
             ap.k = j;
             ap.first = j == N - 1;
             ap.combineOnly = 0;
+            ap.colCnt = ap.colOff = ap.partners = nullptr;
+            if (collisions && tape.colVer[jj] >= 0) {
+                const ColSnap& cs = *a->colSnaps[tape.colVer[jj]];
+                ap.colCnt = (const int*)cs.cnt.p;
+                ap.colOff = (const int*)cs.off.p;
+                ap.partners = (const int*)cs.part.p;
+            }
             launch_adj_voxels(ap, stream);
+            launch_adj_contacts(ap, stream);
             launch_adj_links(ap, stream);
         }
         HIPCHK(hipGetLastError());
@@ -260,6 +323,7 @@ bool Sim::adjointBackward(const double* dLdpos, double* gThetaOut, double* gPhas
     /* the link contributions of step 0 */
     ap.first = 0;
     ap.combineOnly = 1;
+    ap.colCnt = ap.colOff = ap.partners = nullptr;
     ap.rtimeNext = times;
     launch_adj_voxels(ap, stream);
     HIPCHK(hipGetLastError());

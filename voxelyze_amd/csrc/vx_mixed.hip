@@ -5,7 +5,8 @@
  * extracted in FP64 (so that subtracting the rest length keeps its digits), the
  * beam forces, damping and the link's own state are FP32, and the force/moment
  * sums, momentum and quaternion integration, floor contact and the state in HBM
- * stay FP64.  Contact forces are FP32, as in the reference (VX_Collision.cpp:45-53).
+ * stay FP64.  Contact forces are FP32, as in the reference (VX_Collision.cpp:45-53); the
+ * FP64 fused variant (T = double) evaluates them in FP64.
  *
  * Fusion: one lane per voxel evaluates the <= 6 links it touches (every link is
  * therefore evaluated by both of its voxels, with identical inputs and code, so
@@ -419,7 +420,41 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_FUS
     t.y -= cGT * (m.massInverse * lin.y);
     t.z -= cGT * (m.massInverse * lin.z);
     t.z += m.gravityForce;
-    if ((flags & VF_COLLISIONS) && s.colCnt) {
+    if ((flags & VF_COLLISIONS) && s.colCnt && sizeof(T) == 8) {
+        /* contacts (VX_Collision.cpp:42-56) from this voxel's side, FP64 (the FP64 fused step:
+           smooth in the state, the temperatures and a per-voxel modulus, as its links are) */
+        const int cnt = s.colCnt[i], off = s.colOff[i];
+        const double bsai = s.vphase ? fp64_bsa(m, fused_temperature_fp64(s.actBase, s.actAmp, s.actFreq, s.rtimeIn[rob],
+                                                                          s.vphase[i]))
+                                     : m_bsa(m, tempI);
+        for (int k = 0; k < cnt; k++) {
+            const int j = s.partners[off + k];
+            VoxMat mj = s.vmats[s.vmat[j]];
+            double kj = mj.penetrationStiff, cj = mj.collisionTransC;
+            if (s.vE) {
+                voxel_modulus(&mj, s.vE[j]);
+                const double E = s.vE[j], L = mj.nomSize;
+                kj = 2.0 * E * L;
+                cj = (double)mj.zetaCollision * 2.0 * sqrt((double)mj.mass * E * L);
+            }
+            const double bsaj = s.vphase ? fp64_bsa(mj, fused_temperature_fp64(s.actBase, s.actAmp, s.actFreq,
+                                                                               s.rtimeIn[s.vrobot[j]], s.vphase[j]))
+                                         : m_bsa(mj, s.vtempIn[j]);
+            const dvec o = dv(vd[VPX * nv + j] - pos.x, vd[VPY * nv + j] - pos.y, vd[VPZ * nv + j] - pos.z);
+            const double len = sqrt(o.x * o.x + o.y * o.y + o.z * o.z);
+            const double rel = (bsai + bsaj) * 0.625 - len;
+            if (rel > 0 && len > 0) {
+                const double inv = 1.0 / len;
+                const dvec u = dv(o.x * inv, o.y * inv, o.z * inv);
+                const double vi = m.massInverse * (lin.x * u.x + lin.y * u.y + lin.z * u.z);
+                const double vj = mj.massInverse * (vd[VLX * nv + j] * u.x + vd[VLY * nv + j] * u.y + vd[VLZ * nv + j] * u.z);
+                const double stiff = 2.0 / (1.0 / kPen + 1.0 / kj);
+                const double damp = 0.5 * (cCT + cj);
+                const double mag = stiff * rel + damp * (vi - vj);
+                t.x -= u.x * mag; t.y -= u.y * mag; t.z -= u.z * mag;
+            }
+        }
+    } else if ((flags & VF_COLLISIONS) && s.colCnt) {
         /* contacts (VX_Collision.cpp:42-56) from this voxel's side, FP32 */
         const int cnt = s.colCnt[i], off = s.colOff[i];
         const double bsai = m_bsa(m, tempI);

@@ -114,8 +114,31 @@ void Sim::swapVoxelBuffers() {
     dp.rtime = (float*)brtime.p;
 }
 
+/* the partner lists step k of a recompute uses: a new snapshot at the first step and after a
+   rebuild, else the previous step's */
+bool Sim::snapPartners(MixTape* tape, int k, bool rebuilt) {
+    if ((size_t)k >= tape->colVer.size()) tape->colVer.resize(k + 1, -1);
+    if (k > 0 && !rebuilt) {
+        tape->colVer[k] = tape->colVer[k - 1];
+        return true;
+    }
+    std::vector<std::unique_ptr<ColSnap>>& v = *tape->colSnaps;
+    if ((size_t)tape->nSnaps == v.size()) v.emplace_back(new ColSnap());
+    ColSnap& c = *v[tape->nSnaps];
+    const size_t n = (size_t)nv(), ce = (size_t)std::max(colEntries, 0);
+    if (!c.cnt.ensure(n * 4) || !c.off.ensure(n * 4) || !c.part.ensure(std::max<size_t>(ce, 1) * 4)) {
+        err = "device allocation failed (contact lists of the tape)";
+        return false;
+    }
+    HIPCHK(hipMemcpyAsync(c.cnt.p, cp.pairCount, n * 4, hipMemcpyDeviceToDevice, stream));
+    HIPCHK(hipMemcpyAsync(c.off.p, cp.pairOff, n * 4, hipMemcpyDeviceToDevice, stream));
+    if (ce) HIPCHK(hipMemcpyAsync(c.part.p, cp.partners, ce * 4, hipMemcpyDeviceToDevice, stream));
+    tape->colVer[k] = tape->nSnaps++;
+    return true;
+}
+
 /* nsteps steps with the fused mixed-precision kernel (one launch per step) */
-bool Sim::stepMixed(int nsteps, const MixTape* tape) {
+bool Sim::stepMixed(int nsteps, MixTape* tape) {
     if (collisions && (cp.nv != nv() || !bcLast.p)) {
         if (!contactBuffers()) return false;
         nearbyStale = true;
@@ -187,7 +210,14 @@ bool Sim::stepMixed(int nsteps, const MixTape* tape) {
     int launched = 0;
     for (int k = 0; k < nsteps && !lastStep; k++) {
         if (collisions) {
-            bool stale = false;
+            /* the recompute's state lives in the tape: the watch list is tested and rebuilt on it */
+            double* const vdCur = dp.vd;
+            uint32_t* const vflCur = dp.vflags;
+            if (tape) {
+                dp.vd = tape->vd[k];
+                dp.vflags = tape->vflIn[k];
+            }
+            bool stale = false, rebuilt = false;
             if (!staleCheck(&stale)) return false;
             bool all = false;
             if (ctlHost[0] && !anyDiverged(k - 1, &all)) return false;
@@ -202,7 +232,15 @@ bool Sim::stepMixed(int nsteps, const MixTape* tape) {
                     collisionsStale = true;
                 }
                 if (stale) collisionsStale = true;
-                if (collisionsStale && !regenerateCollisions()) return false;
+                if (collisionsStale) {
+                    if (!regenerateCollisions()) return false;
+                    rebuilt = true;
+                }
+            }
+            if (tape) {
+                dp.vd = vdCur;
+                dp.vflags = vflCur;
+                if (tape->colSnaps && !snapPartners(tape, k, rebuilt)) return false;
             }
         }
         mp.vdIn = dp.vd;

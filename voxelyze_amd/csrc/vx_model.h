@@ -80,12 +80,22 @@ struct DevBuf {
     ~DevBuf();
 };
 
+/* a copy of the self-collision partner lists (count and offset per voxel, partners) */
+struct ColSnap {
+    DevBuf cnt, off, part;
+};
+
 /* per-step record of the mixed forward for the reverse pass (device pointers, one per step) */
 struct MixTape {
     std::vector<double*> vd, sums;
     std::vector<uint32_t*> vflIn, vflOut, lfl;
     std::vector<float*> vtemp;
     std::vector<void*> ls;   /* float (VXG_MIXED) or double (VXG_FUSED_FP64) link state */
+    /* self-collision: the recompute keeps a snapshot of the partner lists at its first step and
+       after every rebuild, and colVer[k] names the snapshot step k used */
+    std::vector<std::unique_ptr<ColSnap>>* colSnaps = nullptr;
+    int nSnaps = 0;
+    std::vector<int> colVer;
 };
 
 struct StateCheckpoint;
@@ -252,7 +262,8 @@ struct Sim {
     bool uploadMixedLinks();
     bool downloadMixedLinks();
     void swapVoxelBuffers();
-    bool stepMixed(int nsteps, const struct MixTape* tape = nullptr);
+    bool stepMixed(int nsteps, struct MixTape* tape = nullptr);
+    bool snapPartners(struct MixTape* tape, int k, bool rebuilt);
     bool mixedFinish(int launched);
     /* one launch sequence of nsteps steps, each robot with its own dt; done[r] receives the
        steps robot r completed (nsteps unless it diverged) */

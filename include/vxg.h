@@ -390,8 +390,8 @@ void vxg_clear_moduli(vxg_sim* s);
    voxel, recomputing each segment from its checkpoint, and leaves the simulation at the
    end of the rollout.  The derivative follows every branch the forward run took (contact,
    friction, small/large-angle links, which self-collision pairs touch, with the partner
-   lists the forward used); dt is held fixed.  Requires: linear materials without failure,
-   voxels free or fully fixed.  Returns 1 or 0 (vxg_last_error). */
+   lists the forward used, fixed degrees of freedom); dt is held fixed.  Requires: linear
+   materials without failure.  Returns 1 or 0 (vxg_last_error). */
 int vxg_adjoint_run(vxg_sim* s, int nsteps, int every, const float* dt);
 int vxg_adjoint_backward(vxg_sim* s, const double* dl_dpos, double* grad_log_modulus, double* grad_phase);
 void vxg_adjoint_free(vxg_sim* s);

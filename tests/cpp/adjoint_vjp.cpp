@@ -130,7 +130,7 @@ static void test_link(int axis, bool large, bool damp) {
     check(name, fd_dir(fE, V{0.0}, V{1.0}, lyF, 1e-2), o2.lthetaLink, 2e-4);
 }
 
-static void test_voxel(bool floorOn, bool kinetic, bool zero, double rot) {
+static void test_voxel(bool floorOn, bool kinetic, bool zero, double rot, uint32_t dof = 0) {
     VoxAdjIn in;
     in.pos = dv(0.01, 0.02, floorOn ? -1e-8 : 0.01); /* small penetration: its float rounding stays below the FD steps */
     in.lin = RV(1e-4);
@@ -153,6 +153,9 @@ static void test_voxel(bool floorOn, bool kinetic, bool zero, double rot) {
     in.halfBsaMinusHalfL = 1e-8;
     in.dBsaHalf_dT = 1e-5;
     in.floorOn = floorOn;
+    in.dof = dof;
+    in.fixedPos = dv(0.01, 0.02, 0.0);
+    in.fixedQ = RQ(0.1);
     auto f = [&](const V& x) {
         VoxAdjIn v = in;
         v.pos = dv(x[0], x[1], x[2]);
@@ -184,13 +187,15 @@ static void test_voxel(bool floorOn, bool kinetic, bool zero, double rot) {
     const char* groups[] = {"pos", "lin", "ang", "q", "sumF", "sumM", "T", "logE"};
     const int start[] = {0, 3, 6, 9, 13, 16, 19, 20, 21};
     const double scales[] = {1e-9, 1e-6, 1e-9, 1e-4, 1e-4, 1e-8, 1e-4, 1.0};
-    const double steps[] = {1.0, 1e-3, 1e-3, 1e-3, 1.0, 1.0, 1.0, 1e-2}; /* linear dependences take a unit step */
+    double steps[] = {1.0, 1e-3, 1e-3, 1e-3, 1.0, 1.0, 1.0, 1e-2}; /* linear dependences take a unit step */
+    if ((dof & 56) == 56) steps[3] = 1.0; /* q then only rotates the force: quadratic, and its effect on the position
+                                               is too small for a 1e-3 step to resolve in FP64 */
     char name[64];
     for (int gi = 0; gi < 8; gi++) {
         V dx(x.size(), 0.0);
         for (int i = start[gi]; i < start[gi + 1]; i++) dx[i] = U(-1, 1) * scales[gi];
-        snprintf(name, sizeof name, "voxel %s%s%s rot%.0e d%s", floorOn ? "floor" : "free", kinetic ? "+kin" : "",
-                 zero ? "+static" : "", rot, groups[gi]);
+        snprintf(name, sizeof name, "voxel %s%s%s rot%.0e fix%02o d%s", floorOn ? "floor" : "free", kinetic ? "+kin" : "",
+                 zero ? "+static" : "", rot, dof, groups[gi]);
         check(name, fd_dir(f, x, dx, ly, steps[gi]), dotv(g, dx), gi == 7 ? 1e-4 : 1e-5);
     }
 }
@@ -264,6 +269,12 @@ int main() {
     test_voxel(true, true, false, 1e-6);
     test_voxel(true, false, false, 1e-6);
     test_voxel(true, false, true, 1e-6);
+    /* partially fixed voxels (VX_Voxel.cpp:210-228): translations, some rotation axes, all of them */
+    test_voxel(false, false, false, 1e-2, 1 | 4);
+    test_voxel(false, false, false, 1e-2, 8 | 32);
+    test_voxel(false, false, false, 1e-2, 16);
+    test_voxel(false, false, false, 1e-2, 56);
+    test_voxel(true, true, false, 1e-6, 2 | 8);
     test_pair(false);
     test_pair(true);
     printf("%d failures\n", fails);

@@ -157,6 +157,60 @@ def test_adjoint_matches_central_differences(precision):
     assert worst <= bar, (worst, bar)
 
 
+@pytest.mark.gpu
+def test_adjoint_with_partially_fixed_voxels_matches_central_differences():
+    """Partially fixed voxels (VX_Voxel.cpp:210-228; vx_adjoint.h voxel_vjp): in each robot the
+    voxels of the lowest x layer keep their x coordinate and their y and z rotation axes, those
+    of the highest x layer their y and z coordinates and x rotation axis (prescribed coordinates,
+    zeroed momenta, fixed axes zeroed in the rotation vector). FP64 fused step, no floor,
+    5 degC, 100 steps; Richardson-extrapolated central differences (1e-2, 5e-3) along 8 seeded
+    directions within 1e-6 of the derivative's size, as for free robots."""
+    sim = build(shapes(), 2)
+    sim.enableFloor(False)
+    sim.setGravity(0.0)
+    sim.setActuation(True, 0.0, 5.0, FREQ)
+    pos = sim.state()["pos"]
+    fixed = 0
+    for r in range(sim.robotCount()):
+        f, c = sim.robotVoxels(r)
+        x = pos[f:f + c, 0]
+        for i in np.nonzero(x == x.min())[0]:
+            sim.voxel(int(f + i)).external().setFixed(True, False, False, False, True, True)
+            fixed += 1
+        for i in np.nonzero(x == x.max())[0]:
+            sim.voxel(int(f + i)).external().setFixed(False, True, True, True, False, False)
+            fixed += 1
+    theta, phi = params(sim)
+    n = sim.voxelCount()
+    w = (W.uniform(79, np.arange(3 * n, dtype=np.uint64)) * 2 - 1).reshape(n, 3)
+    _, ge, gp = weighted_loss(sim, theta, phi, w, 100)
+    end = sim.state()["pos"]
+    for r in range(sim.robotCount()):  # the prescribed coordinates held
+        f, c = sim.robotVoxels(r)
+        lo = np.nonzero(pos[f:f + c, 0] == pos[f:f + c, 0].min())[0] + f
+        assert np.abs(end[lo, 0] - pos[lo, 0]).max() <= 1e-15
+    worst = 0.0
+    for d in range(8):
+        u = W.uniform(300 + d, np.arange(2 * n, dtype=np.uint64)) * 2 - 1
+
+        def rel(h):
+            Ep, Em = np.exp(theta + h * u[:n]).astype(np.float32), np.exp(theta - h * u[:n]).astype(np.float32)
+            pp, pm = (phi + h * u[n:]).astype(np.float32), (phi - h * u[n:]).astype(np.float32)
+            lp, _, _ = weighted_loss_f(sim, Ep, pp, w, 100, grad=False)
+            lm, _, _ = weighted_loss_f(sim, Em, pm, w, 100, grad=False)
+            dt = np.log(Ep.astype(np.float64)) - np.log(Em.astype(np.float64))
+            dp = pp.astype(np.float64) - pm.astype(np.float64)
+            size = np.abs(ge) @ np.abs(dt) + np.abs(gp) @ np.abs(dp)
+            return ((lp - lm) - (ge @ dt + gp @ dp)) / size
+
+        r1, r2 = rel(1e-2), rel(5e-3)
+        err = abs((4 * r2 - r1) / 3)
+        worst = max(worst, err)
+        print(f"[adjoint] partial fixes ({fixed} voxels), direction {d}: {r1:.2e} {r2:.2e} extrapolated {err:.2e}",
+              flush=True)
+    assert worst <= 1e-6, worst
+
+
 def floor_central_differences(label, mu_kinetic, amp):
     """The floor's normal force in the derivative (vx_adjoint.h voxel_vjp: penetration, its
     dependence on the voxel's modulus and temperature): the robots under gravity on a floor

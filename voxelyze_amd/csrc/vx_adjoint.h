@@ -388,6 +388,10 @@ struct VoxAdjIn {
     double halfBsaMinusHalfL; /* bsa(T)/2 - nomSize/2 */
     double dBsaHalf_dT;       /* d(bsa/2)/dT */
     bool floorOn;
+    uint32_t dof;             /* fixed degrees of freedom of a partially fixed voxel (ExtRec::dof
+                                 bits: 1 2 4 translation x y z, 8 16 32 rotation), 0 when free */
+    dvec fixedPos;            /* its prescribed position and orientation (host checks only: */
+    dquat fixedQ;             /* constants, they do not enter the derivatives) */
 };
 struct VoxAdjOut {
     dvec lpos, lin, lang;
@@ -430,6 +434,21 @@ VX_HD void voxel_forward(const VoxAdjIn& in, dvec* pos1, dvec* lin1, dvec* ang1,
     const dvec mo = vsub(qrot(in.q, in.sumM), vscale(in.ang, (double)in.globalRotC * in.momentInertiaInverse));
     *ang1 = vadd(in.ang, vscale(mo, dt));
     *q1 = qmul(qfromrv(vscale(*ang1, dt * in.momentInertiaInverse)), in.q);
+    /* partial fixes (VX_Voxel.cpp:210-228): prescribed coordinates, zeroed momenta; fixed
+       rotation axes zeroed in the rotation vector */
+    if (in.dof & 1) { pos1->x = in.fixedPos.x; lin1->x = 0.0; }
+    if (in.dof & 2) { pos1->y = in.fixedPos.y; lin1->y = 0.0; }
+    if (in.dof & 4) { pos1->z = in.fixedPos.z; lin1->z = 0.0; }
+    if ((in.dof & 56) == 56) {
+        *q1 = in.fixedQ;
+        *ang1 = dv(0.0, 0.0, 0.0);
+    } else if (in.dof & 56) {
+        dvec rv = qtorv(*q1);
+        if (in.dof & 8) { rv.x = 0.0; ang1->x = 0.0; }
+        if (in.dof & 16) { rv.y = 0.0; ang1->y = 0.0; }
+        if (in.dof & 32) { rv.z = 0.0; ang1->z = 0.0; }
+        *q1 = qfromrv(rv);
+    }
 }
 
 /* CVX_Voxel::timeStep in the mixed step's arithmetic, recomputed in FP64, then its VJP.
@@ -464,6 +483,28 @@ VX_HD void voxel_vjp(const VoxAdjIn& in, dvec lPos1, dvec lLin1, dvec lAng1, dqu
     const double dti = dt * in.momentInertiaInverse;
     const dvec w = vscale(ang1, dti);
     const dquat r = qfromrv(w);
+
+    /* ---- backward: the partial fixes the step ends with */
+    if (in.dof & 1) { lPos1.x = 0.0; lLin1.x = 0.0; }
+    if (in.dof & 2) { lPos1.y = 0.0; lLin1.y = 0.0; }
+    if (in.dof & 4) { lPos1.z = 0.0; lLin1.z = 0.0; }
+    if ((in.dof & 56) == 56) {
+        lQ1 = qzero();
+        lAng1 = dv(0.0, 0.0, 0.0);
+    } else if (in.dof & 56) { /* q1 = FromRotationVector(P ToRotationVector(r q)), P zeroing fixed axes */
+        const dquat qf = qmul(r, in.q);
+        dvec rv = qtorv(qf);
+        if (in.dof & 8) { rv.x = 0.0; lAng1.x = 0.0; }
+        if (in.dof & 16) { rv.y = 0.0; lAng1.y = 0.0; }
+        if (in.dof & 32) { rv.z = 0.0; lAng1.z = 0.0; }
+        dvec lrv = dv(0.0, 0.0, 0.0);
+        qfromrv_vjp(rv, lQ1, &lrv);
+        if (in.dof & 8) lrv.x = 0.0;
+        if (in.dof & 16) lrv.y = 0.0;
+        if (in.dof & 32) lrv.z = 0.0;
+        lQ1 = qzero();
+        qtorv_vjp(qf, lrv, &lQ1);
+    }
 
     /* ---- backward: rotation */
     dquat lq = qzero(), lr = qzero();

@@ -116,6 +116,8 @@ __global__ void __launch_bounds__(256) k_adj_voxels(AdjPtrs a) {
     in.halfBsaMinusHalfL = bsa / 2 - m.nomSize / 2;
     in.dBsaHalf_dT = (double)m.alphaCTE * (m.sizeX + m.sizeY + m.sizeZ) / 6.0;
     in.floorOn = (in.flagsIn & VF_FLOOR) != 0;
+    const int e = a.vext[i];
+    in.dof = e >= 0 ? (a.exts[e].dof & 0x3Fu) : 0u; /* partial fixes (a full one returned above) */
     VoxAdjOut o;
     voxel_vjp(in, lPos1, lLin1, lAng1, lQ1, &o);
     P[AP_POS * nv + i] = o.lpos.x; P[(AP_POS + 1) * nv + i] = o.lpos.y; P[(AP_POS + 2) * nv + i] = o.lpos.z;

@@ -158,11 +158,6 @@ bool Sim::adjointRun(int nsteps, int every, const float* dtIn) {
             err = "the adjoint needs linear materials without failure and Poisson's ratio 0";
             return false;
         }
-    for (auto& e : vext)
-        if (e && (e->dof & 0x3F) != 0 && (e->dof & 0x3F) != 0x3F) {
-            err = "the adjoint supports free or fully fixed voxels only";
-            return false;
-        }
     if (nv() == 0) { err = "no voxels"; return false; }
     if (!syncUp()) return false;
     if (collisions && (cp.nv != nv() || !bcLast.p)) { /* the watch list exists before the first checkpoint */

@@ -47,11 +47,17 @@ ALG_BYTES = {
     "voxel": lambda V, L: 244.0 * V + 96.0 * L,
     "step_mixed": lambda V, L: 300.0 * V + 212.0 * L,
     "step_fp64": lambda V, L: 300.0 * V + 384.0 * L,
+    # reverse pass of the mixed step, actuated (per voxel: its partials, six adjoint slots, the
+    # recorded state and link sums, flags, temperature, indices, gradients; per link: the
+    # positive end's pose, adjoint sums and indices, the link's recorded FP32 pose and flags,
+    # its pose adjoint read and written; every voxel's six slots are written once)
+    "adj_voxels": lambda V, L: 920.0 * V,
+    "adj_links": lambda V, L: 576.0 * V + 304.0 * L,
 }
 # profile classes of vxg_profile_read
-PROFILE_CLASSES = ["link", "voxel", "contact", "step_mixed", "step_fp64"]
+PROFILE_CLASSES = ["link", "voxel", "contact", "step_mixed", "step_fp64", "adj_voxels", "adj_links", "adj_contacts"]
 KERNEL_NAMES = {"link": "k_links_fp64", "voxel": "k_voxels_fp64", "step_mixed": "k_step_mixed",
-                "step_fp64": "k_step_fp64"}
+                "step_fp64": "k_step_fp64", "adj_voxels": "k_adj_voxels", "adj_links": "k_adj_links"}
 FP64_PATHS = {"fused": 0, "split": 1}
 
 
@@ -513,7 +519,23 @@ def bench_adjoint(n_robots, steps):
     sim.setPrecision(MIXED)
     V = sim.voxelCount()
     sim.robotStep(20)  # warm-up, during which the autotuner settles
+    dl = np.zeros((V, 3))
+    for r in range(sim.robotCount()):
+        f, c = sim.robotVoxels(r)
+        dl[f:f + c, 0] = -1.0 / c
+    # the adjoint's device buffers (checkpoints of the whole horizon, the segment tape) are
+    # allocated by a first rollout and kept for the next, as in a training loop
     sim.resetTime()
+    sim.adjointRun(steps, 100)
+    sim.resetTime()
+    sim.adjointRun(min(steps, 100), 100)
+    sim.adjointBackward(dl)
+    sim.resetTime()
+    sim.robotStep(steps)  # records the forward's step graph outside the timing
+    # resetTime rewrites the state on the host; robotCom uploads it (outside the timing) so that
+    # both timed forwards start from a resident state, as the backward does
+    sim.resetTime()
+    sim.robotCom()
     torch.cuda.synchronize()
     print(f"[bench] adjoint: {n_robots} robots, {V} voxels, built in {time.perf_counter() - t0:.1f}s", file=sys.stderr,
           flush=True)
@@ -522,21 +544,44 @@ def bench_adjoint(n_robots, steps):
     torch.cuda.synchronize()
     fwd = time.perf_counter() - t0
     sim.resetTime()
+    sim.robotCom()
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
     sim.adjointRun(steps, 100)
     torch.cuda.synchronize()
     fwd_ckpt = time.perf_counter() - t0
-    dl = np.zeros((V, 3))
-    for r in range(sim.robotCount()):
-        f, c = sim.robotVoxels(r)
-        dl[f:f + c, 0] = -1.0 / c
     t0 = time.perf_counter()
     ge, gp = sim.adjointBackward(dl)
     torch.cuda.synchronize()
     bwd = time.perf_counter() - t0
-    sim.adjointFree()
     print(f"[bench] adjoint: forward {fwd:.3f}s, forward with checkpoints {fwd_ckpt:.3f}s, backward {bwd:.3f}s, "
           f"gradient/forward {(fwd_ckpt + bwd) / fwd:.2f}", file=sys.stderr, flush=True)
+    # kernel durations of the backward: a separate profiled backward of the same rollout (HIP
+    # events around every recompute step and reverse kernel; the link kernel's cost grows as
+    # more links take the large-angle branch, so the whole horizon is averaged)
+    import ctypes as C
+    L = sim.linkCount()
+    sim.resetTime()
+    sim.adjointRun(steps, 100)
+    sim.abi.profile(sim.handle, 1)
+    sim.abi.profile_read(sim.handle, None, None, 0)
+    sim.adjointBackward(dl)
+    torch.cuda.synchronize()
+    sim.abi.profile(sim.handle, 0)
+    nc = len(PROFILE_CLASSES)
+    ms, cnt = (C.c_double * nc)(), (C.c_int * nc)()
+    sim.abi.profile_read(sim.handle, ms, cnt, nc)
+    sim.adjointFree()
+    kernels = {}
+    for cls in ("step_mixed", "adj_voxels", "adj_links"):
+        i = PROFILE_CLASSES.index(cls)
+        if cnt[i]:
+            avg_s = ms[i] / cnt[i] / 1e3
+            b = ALG_BYTES[cls](V, L)
+            kernels[KERNEL_NAMES[cls]] = {"avg_us": round(avg_s * 1e6, 1), "alg_bytes": b,
+                                          "GB/s": round(b / avg_s / 1e9, 1), "frac": round(b / avg_s / 1e9 / HBM_PEAK_GBS, 4)}
+    print("[bench] adjoint kernels: " + ", ".join(f"{k} {v['avg_us']:.1f} us {v['GB/s']:.0f} GB/s"
+                                                   for k, v in kernels.items()), file=sys.stderr, flush=True)
     return {"workload": f"{n_robots} robots 32x32x16 70% fill, 3 materials, +-39 degC 2 Hz actuation, gravity + floor "
                         "friction, mixed precision, checkpoints every 100 steps",
             "voxels": V, "steps": steps, "forward_s": fwd, "forward_with_checkpoints_s": fwd_ckpt, "backward_s": bwd,
@@ -544,6 +589,8 @@ def bench_adjoint(n_robots, steps):
             # recompute + reverse) / forward; the backward call does the recompute and reverse
             "adjoint_over_forward": (fwd_ckpt + bwd) / fwd,
             "backward_over_forward": bwd / fwd,
+            "links": L,
+            "kernels_per_step": kernels,
             "gradient_finite": bool(np.isfinite(ge).all() and np.isfinite(gp).all())}
 
 

@@ -165,11 +165,12 @@ int vxg_set_fp64_path(vxg_sim* s, int path);
 int vxg_fp64_path(vxg_sim* s);
 
 /* Kernel timing (no reference counterpart; measurement hook for benchmarks).
-   When enabled, every kernel of later vxg_step calls is bracketed by HIP events on the
-   library's stream; vxg_profile_read returns, per kernel class (0 = FP64 link kernel,
-   1 = FP64 voxel kernel, 2 = contact kernels, 3 = fused mixed-precision step, 4 = fused
-   FP64 step), the summed milliseconds and launch counts since the last read, then clears
-   them.  Returns the number of classes written. */
+   When enabled, every kernel of later vxg_step calls (and of vxg_adjoint_backward) is
+   bracketed by HIP events on the library's stream; vxg_profile_read returns, per kernel
+   class (0 = FP64 link kernel, 1 = FP64 voxel kernel, 2 = contact kernels, 3 = fused
+   mixed-precision step, 4 = fused FP64 step, 5 = adjoint voxel kernel, 6 = adjoint link
+   kernel, 7 = adjoint contact kernel), the summed milliseconds and launch counts since the
+   last read, then clears them.  Returns the number of classes written (at most cap, 8). */
 int vxg_profile(vxg_sim* s, int enable);
 /* Step graphs (no reference counterpart): a multi-step vxg_step / vxg_robot_step launch
    without self-collision is recorded once as a HIP graph and replayed while its buffers and
@@ -394,6 +395,8 @@ void vxg_clear_moduli(vxg_sim* s);
    materials without failure.  Returns 1 or 0 (vxg_last_error). */
 int vxg_adjoint_run(vxg_sim* s, int nsteps, int every, const float* dt);
 int vxg_adjoint_backward(vxg_sim* s, const double* dl_dpos, double* grad_log_modulus, double* grad_phase);
+/* The rollout's checkpoints, tape and adjoint buffers stay allocated for the next
+   vxg_adjoint_run (which reuses them) until vxg_adjoint_free releases them. */
 void vxg_adjoint_free(vxg_sim* s);
 
 #ifdef __cplusplus

@@ -11,8 +11,9 @@
  *   k_adj_contacts (self-collision on) one lane per voxel over its partner list: the
  *                 transpose of the pair forces it and its partners summed, kept for its
  *                 own state and parameters.
- *   k_adj_links   one lane per voxel over the three links it is the negative end of:
- *                 link_vjp, writing each end's dL/d(pose), log-modulus and temperature
+ *   k_adj_links   one lane per voxel and axis (grid.y), for the link it is the negative
+ *                 end of along that axis: link_vjp, writing each end's dL/d(pose), log-modulus
+ *                 and temperature
  *                 contributions into per-voxel slots (direction 2a of the negative voxel,
  *                 2a+1 of the positive one, as the forward hands forces over) and
  *                 dL/d(the link's stored local pose) for the step before.
@@ -299,24 +300,38 @@ __device__ __forceinline__ void adj_link(const AdjPtrs& a, int n) {
     write_slot(a, 2 * axis + 1, p, o.lposP, o.lqP, o.lthetaLink * wp, ltp);
 }
 
-/* 2 waves per SIMD (256 VGPRs, a few spilled): 9 % faster than 1 wave without the hint and 45 %
-   faster than 3 waves on the 2.9M-voxel differentiable batch; the gradient bits do not change */
+/* One lane per link at 2 waves per SIMD (256 VGPRs, a few spilled): on the 2.9M-voxel
+   differentiable batch over 2,000 steps, 1.72 ms per reverse step against 1.91 ms for one lane
+   doing a voxel's three links, 2.76 ms at 1 wave and 3.27 ms at 3 waves; the gradient bits do
+   not change (every link writes its own slots). */
 #ifndef VX_ADJ_WAVES
 #define VX_ADJ_WAVES 2
+#endif
+#ifndef VX_ADJ_LANE_PER_LINK
+#define VX_ADJ_LANE_PER_LINK 1
 #endif
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_ADJ_WAVES, 8))) k_adj_links(AdjPtrs a) {
     const int n = (int)blockIdx.x * 256 + (int)threadIdx.x;
     if (n >= a.nv) return;
+#if VX_ADJ_LANE_PER_LINK
+    switch (blockIdx.y) {
+        case 0: adj_link<0>(a, n); break;
+        case 1: adj_link<1>(a, n); break;
+        default: adj_link<2>(a, n); break;
+    }
+#else
     adj_link<0>(a, n);
     adj_link<1>(a, n);
     adj_link<2>(a, n);
+#endif
 }
 
 void launch_adj_voxels(const AdjPtrs& a, hipStream_t st) {
     if (a.nv > 0) hipLaunchKernelGGL(k_adj_voxels, dim3((unsigned)((a.nv + 255) / 256)), dim3(256), 0, st, a);
 }
 void launch_adj_links(const AdjPtrs& a, hipStream_t st) {
-    if (a.nv > 0) hipLaunchKernelGGL(k_adj_links, dim3((unsigned)((a.nv + 255) / 256)), dim3(256), 0, st, a);
+    if (a.nv > 0)
+        hipLaunchKernelGGL(k_adj_links, dim3((unsigned)((a.nv + 255) / 256), VX_ADJ_LANE_PER_LINK ? 3 : 1), dim3(256), 0, st, a);
 }
 void launch_adj_contacts(const AdjPtrs& a, hipStream_t st) {
     if (a.nv > 0 && a.colCnt) hipLaunchKernelGGL(k_adj_contacts, dim3((unsigned)((a.nv + 255) / 256)), dim3(256), 0, st, a);

@@ -8,10 +8,12 @@
  * checkpoint, re-runs the segment recording every step's inputs (MixTape), then runs the
  * reverse kernels of vx_adjoint.hip step by step.  The simulation is left in the final
  * state of the rollout.  Device memory: (nsteps/every + 1) checkpoints plus `every`
- * recorded steps.
+ * recorded steps; the buffers stay with the simulation for the next rollout (a training
+ * loop allocates them once) until vxg_adjoint_free.
  */
 #include <algorithm>
 #include <cmath>
+#include <deque>
 
 #include "vx_adjoint.h"
 #include "vx_model.h"
@@ -41,7 +43,11 @@ struct AdjointState {
     std::vector<float> dt;          /* per robot */
     std::vector<float> rprev0;      /* per robot previousDt at the start */
     std::vector<float> times;       /* (nsteps + 1) * R: robot clocks at the start of each step */
-    std::vector<Checkpoint> ckpt;   /* segment starts, then the final state */
+    std::deque<Checkpoint> ckpt;    /* segment starts, then the final state: entries 0..nseg (a
+                                       deque: growing it keeps the existing checkpoints' buffers in
+                                       place; a shorter rollout leaves the later ones allocated) */
+    int nseg = 0;
+    bool valid = false;             /* a complete rollout is recorded */
     DevBuf tapeBuf, times_d, dt_d, rprev_d;
     DevBuf lamFinal, part, lamSum, lamT, slots, lamLink, gTheta, gPhase, lamF;
     std::vector<std::unique_ptr<ColSnap>> colSnaps; /* partner lists of the recomputed steps */
@@ -150,7 +156,7 @@ static bool restoreInto(Sim* s, const Checkpoint& c, const MixTape& t) {
 /* the forward rollout with checkpoints */
 bool Sim::adjointRun(int nsteps, int every, const float* dtIn) {
     err.clear();
-    adjointFree();
+    if (adj) adj->valid = false; /* a new rollout replaces the recorded one, in its buffers */
     if (nsteps <= 0 || every <= 0) { err = "nsteps and the checkpoint interval must be positive"; return false; }
     if (!fused()) { err = "the adjoint differentiates the fused step: set VXG_MIXED or VXG_FUSED_FP64"; return false; }
     for (auto& m : vmats)
@@ -165,8 +171,8 @@ bool Sim::adjointRun(int nsteps, int every, const float* dtIn) {
         nearbyStale = true;
     }
     const int R = nRobots();
-    AdjointState* a = new AdjointState();
-    adj = a;
+    if (!adj) adj = new AdjointState();
+    AdjointState* a = adj;
     a->nsteps = nsteps;
     a->every = every;
     a->topoGen = topoGen;
@@ -184,7 +190,8 @@ bool Sim::adjointRun(int nsteps, int every, const float* dtIn) {
         }
     }
     const int nseg = (nsteps + every - 1) / every;
-    a->ckpt.resize(nseg + 1);
+    a->nseg = nseg;
+    if (a->ckpt.size() < (size_t)nseg + 1) a->ckpt.resize(nseg + 1);
     std::vector<int> done(R);
     for (int sgm = 0; sgm < nseg; sgm++) {
         if (!save(this, &a->ckpt[sgm])) return false;
@@ -196,13 +203,14 @@ bool Sim::adjointRun(int nsteps, int every, const float* dtIn) {
     }
     if (!save(this, &a->ckpt[nseg])) return false;
     HIPCHK(hipStreamSynchronize(stream));
+    a->valid = true;
     return true;
 }
 
 bool Sim::adjointBackward(const double* dLdpos, double* gThetaOut, double* gPhaseOut) {
     err.clear();
     AdjointState* a = adj;
-    if (!a) { err = "no recorded rollout: call the adjoint run first"; return false; }
+    if (!a || !a->valid) { err = "no recorded rollout: call the adjoint run first"; return false; }
     if (a->topoGen != topoGen || a->precision != precision) {
         err = "the topology or the precision mode changed since the rollout was recorded";
         return false;
@@ -210,7 +218,7 @@ bool Sim::adjointBackward(const double* dLdpos, double* gThetaOut, double* gPhas
     if (!syncUp()) return false;
     const size_t n = (size_t)nv(), S = 3 * n;
     const int R = nRobots(), N = a->nsteps, every = a->every;
-    const int nseg = (int)a->ckpt.size() - 1;
+    const int nseg = a->nseg;
     /* adjoint buffers */
     if (!a->lamFinal.ensure(3 * n * 8) || !a->part.ensure(13 * n * 8) || !a->lamSum.ensure(6 * n * 8) ||
         !a->lamT.ensure(n * 8) || !a->slots.ensure(6 * 9 * n * 8) || !a->lamLink.ensure(9 * S * 8) ||
@@ -288,6 +296,12 @@ bool Sim::adjointBackward(const double* dLdpos, double* gThetaOut, double* gPhas
         tape.colVer.assign(m, -1);
         launchDt = a->dt;
         if (!beginLaunch(m) || !stepMixed(m, &tape) || !endLaunch(m, done.data())) return false;
+        const bool prof = profiling; /* HIP events around every reverse kernel, read per segment */
+        while (prof && events.size() < (size_t)4 * m) {
+            hipEvent_t ev;
+            HIPCHK(hipEventCreate(&ev));
+            events.push_back(ev);
+        }
         for (int jj = m - 1; jj >= 0; jj--) {
             const int j = j0 + jj;
             ap.t.vd = tape.vd[jj];
@@ -309,11 +323,26 @@ bool Sim::adjointBackward(const double* dLdpos, double* gThetaOut, double* gPhas
                 ap.colOff = (const int*)cs.off.p;
                 ap.partners = (const int*)cs.part.p;
             }
+            hipEvent_t* ev = prof ? &events[(size_t)4 * jj] : nullptr;
+            if (prof) (void)hipEventRecord(ev[0], stream);
             launch_adj_voxels(ap, stream);
+            if (prof) (void)hipEventRecord(ev[1], stream);
             launch_adj_contacts(ap, stream);
+            if (prof) (void)hipEventRecord(ev[2], stream);
             launch_adj_links(ap, stream);
+            if (prof) (void)hipEventRecord(ev[3], stream);
         }
         HIPCHK(hipGetLastError());
+        if (prof) {
+            HIPCHK(hipStreamSynchronize(stream));
+            for (int jj = 0; jj < m; jj++) {
+                const hipEvent_t* ev = &events[(size_t)4 * jj];
+                float a = 0, b = 0, c = 0;
+                if (hipEventElapsedTime(&a, ev[0], ev[1]) == hipSuccess) { profMs[5] += a; profCount[5]++; }
+                if (collisions && hipEventElapsedTime(&b, ev[1], ev[2]) == hipSuccess) { profMs[7] += b; profCount[7]++; }
+                if (hipEventElapsedTime(&c, ev[2], ev[3]) == hipSuccess) { profMs[6] += c; profCount[6]++; }
+            }
+        }
     }
     /* the link contributions of step 0 */
     ap.first = 0;

@@ -262,12 +262,12 @@ API int vxg_set_launch_config(vxg_sim* h, int kernel, int block, int xcd) {
 }
 API int vxg_profile_read(vxg_sim* h, double* ms, int* launches, int cap) {
     Sim* s = S(h);
-    int n = cap < 5 ? cap : 5;
+    int n = cap < Sim::kProfClasses ? cap : Sim::kProfClasses;
     for (int i = 0; i < n; i++) {
         if (ms) ms[i] = s->profMs[i];
         if (launches) launches[i] = s->profCount[i];
     }
-    for (int i = 0; i < 5; i++) { s->profMs[i] = 0; s->profCount[i] = 0; }
+    for (int i = 0; i < Sim::kProfClasses; i++) { s->profMs[i] = 0; s->profCount[i] = 0; }
     return n;
 }
 

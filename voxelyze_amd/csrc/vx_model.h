@@ -253,8 +253,9 @@ struct Sim {
     /* kernel timing */
     bool profiling = false;
     std::vector<hipEvent_t> events;
-    double profMs[5] = {0, 0, 0, 0, 0};
-    int profCount[5] = {0, 0, 0, 0, 0};
+    static constexpr int kProfClasses = 8;
+    double profMs[kProfClasses] = {};
+    int profCount[kProfClasses] = {};
     /* mixed precision (vx_mixed.hip): second voxel buffer, FP32 link-state pairs */
     DevBuf bvdAlt, bvflAlt, bvtempAlt, bvnbr, bms[2], bmfl[2];
     int lcur = 0;          /* mixed link-state buffer that holds the current state */

@@ -1,0 +1,51 @@
+"""Contact steps in batches without host round trips (vx_mixed_host.cpp, vx_fused_host.cpp):
+a multi-step launch runs up to 64 contact steps between host synchronisations and halts on
+the device where the watch list goes stale (k_col_stale_halt), then rebuilds it and resumes.
+The watch list must be tested and rebuilt exactly where the step-by-step loop does it
+(src/Voxelyze.cpp:686-698): one launch of N steps equals N one-step launches bit for bit, on
+every precision path, while a soft cube falls onto a fixed plate (moving several half
+pitches, so the list goes stale many times inside batches) and lands on it.
+"""
+import numpy as np
+import pytest
+
+import backends as B
+
+
+def drop(precision):
+    sim = B.maker(B.product_abi())(0.001)
+    soft = sim.addMaterial(2e4, 1000.0)
+    soft.setCollisionDamping(0.2)
+    for x in range(4):
+        for y in range(4):
+            sim.setVoxel(soft, x, y, 0).external().setFixedAll()
+    for x in range(1, 3):
+        for y in range(1, 3):
+            for z in range(5, 7):
+                sim.setVoxel(soft, x, y, z)
+    sim.setGravity(1.0)
+    sim.enableCollisions(True)
+    sim.setPrecision(precision)
+    return sim
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1, 2])
+def test_one_launch_equals_single_steps_with_contacts(precision):
+    a, b = drop(precision), drop(precision)
+    dt = a.recommendedTimeStep()
+    n = 900
+    z0 = a.state()["pos"][:, 2].max()
+    done, _ = a.step(dt, n)
+    assert done == n
+    for _ in range(n):
+        b.step(dt, 1)
+    sa, sb = a.state(), b.state()
+    fall = z0 - sa["pos"][:, 2].max()
+    print(f"[contact batches] precision {precision}: top fell {fall / 0.001:.2f} pitch, "
+          f"{a.collisionCount()} pairs listed")
+    assert fall > 2.0 * 0.001  # the cube fell onto the plate: the list went stale many times
+    assert a.collisionCount() == b.collisionCount()
+    for k in ("pos", "orient", "linMom", "angMom"):
+        if k in sa:
+            assert sa[k].tobytes() == sb[k].tobytes(), k

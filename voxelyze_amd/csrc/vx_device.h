@@ -261,6 +261,7 @@ struct MixPtrs {
     const float* vE;        /* per-voxel Young's modulus overriding the materials' (linear), null: off */
     double* tapeSums;       /* when set, each voxel records its link force and moment sums (6 * nv) */
     int k;                  /* this step's index in the launch */
+    const int* halt;        /* contact batches: the step returns at once when halt[0] <= k */
     int xcd;                /* launch configuration: XCD-contiguous block mapping */
 };
 
@@ -310,6 +311,7 @@ struct FusedPtrs {
     const float* vphase;
     float actBase, actAmp, actFreq;
     int k;                    /* this step's index in the launch */
+    const int* halt;          /* contact batches: the step returns at once when halt[0] <= k */
     int storeForces;          /* write the end forces (the launch's last step) */
     int xcd;
     int poisson;              /* a material has Poisson's ratio != 0 */

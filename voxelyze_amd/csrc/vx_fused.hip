@@ -162,6 +162,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_FUS
     const int i = lane_index(s.xcd);
     const int nv = s.nv, nl = s.nl;
     if (i >= nv) return;
+    if (!ForcesOnly && s.halt && s.halt[0] <= s.k) return; /* a contact batch halted before this step */
     const int rob = s.vrobot[i];
     const int d = s.rdiv[rob];
     const float dt = s.rdt[rob];

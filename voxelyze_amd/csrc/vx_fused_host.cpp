@@ -17,6 +17,8 @@ void launch_fused_restore(double* vd, const double* vdPrev, uint32_t* vfl, const
                           const float* vtempPrev, float* rtime, const float* rtimePrev, int nv, int first, int count,
                           int rob, hipStream_t st);
 void launch_col_stale(const DevPtrs& s, const ContactPtrs& c, int gen, hipStream_t st);
+void launch_col_stale_halt(const DevPtrs& s, const ContactPtrs& c, int* halt, int step, hipStream_t st);
+void launch_set_halt(int* halt, int v, hipStream_t st);
 void launch_poisson_fp64(const DevPtrs& s, int k, hipStream_t st);
 uint64_t graph_key(const void* args, size_t n, int path, int nsteps, int flags);
 
@@ -138,8 +140,14 @@ bool Sim::stepFp64Fused(int nsteps) {
     }
     bool lastStep = false;
     int launched = 0;
+    /* contact steps in batches without host round trips, halted on the device where the watch
+       list goes stale (as in stepMixed) */
+    const bool batched = collisions && !timed;
+    int* const halt = dp.ctl + 4;
+    int batchStart = 0, batchEnd = 0;
     for (int k = 0; k < nsteps && !lastStep; k++) {
-        if (collisions) {
+        if (collisions && batched && k > batchStart && k < batchEnd) launch_col_stale_halt(dp, cp, halt, k, stream);
+        if (collisions && k >= batchEnd) {
             /* the watch list from the positions this step starts from (the link phase does
                not move voxels, so this is what updateCollisions sees, Voxelyze.cpp:271) */
             bool stale = false;
@@ -159,12 +167,18 @@ bool Sim::stepFp64Fused(int nsteps) {
                 if (stale) collisionsStale = true;
                 if (collisionsStale && !regenerateCollisions()) return false;
             }
+            if (batched && !lastStep) {
+                batchStart = k;
+                batchEnd = std::min(nsteps, k + kContactBatch);
+                launch_set_halt(halt, INT_MAX, stream);
+            }
         }
         bind(fp);
         fp.colCnt = collisions ? cp.pairCount : nullptr;
         fp.colOff = collisions ? cp.pairOff : nullptr;
         fp.partners = collisions ? cp.partners : nullptr;
         fp.k = k;
+        fp.halt = batched ? halt : nullptr;
         fp.storeForces = (k == nsteps - 1 || lastStep) ? 1 : 0;
         const LaunchCfg cfg = tune ? tuner[3].next(&cand[k]) : chosen;
         fp.xcd = cfg.xcd;
@@ -175,6 +189,18 @@ bool Sim::stepFp64Fused(int nsteps) {
         if (timed) (void)hipEventRecord(events[2 * k + 1], stream);
         swapFp64Buffers();
         launched++;
+        if (batched && k + 1 == batchEnd && k > batchStart) { /* where did the batch stop? */
+            HIPCHK(hipMemcpyAsync(ctlHost + 4, halt, sizeof(int), hipMemcpyDeviceToHost, stream));
+            HIPCHK(hipStreamSynchronize(stream));
+            const int h = ctlHost[4];
+            if (h <= k) { /* steps h..k did nothing: their swaps are taken back and h runs next */
+                const int skipped = k + 1 - h;
+                if (skipped & 1) swapFp64Buffers();
+                launched -= skipped;
+                k = h - 1;
+                batchEnd = h;
+            }
+        }
     }
     if (graph) {
         hipGraph_t g = nullptr;

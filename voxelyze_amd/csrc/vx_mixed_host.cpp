@@ -15,6 +15,8 @@ void launch_step_mixed(const MixPtrs& s, bool fp64, int block, hipStream_t st);
 void launch_mixed_temps(const MixPtrs& s, float* temp, hipStream_t st);
 uint64_t graph_key(const void* args, size_t n, int path, int nsteps, int flags);
 void launch_col_stale(const DevPtrs& s, const ContactPtrs& c, int gen, hipStream_t st);
+void launch_col_stale_halt(const DevPtrs& s, const ContactPtrs& c, int* halt, int step, hipStream_t st);
+void launch_set_halt(int* halt, int v, hipStream_t st);
 void launch_fused_restore(double* vd, const double* vdPrev, uint32_t* vfl, const uint32_t* vflPrev, float* vtemp,
                           const float* vtempPrev, float* rtime, const float* rtimePrev, int nv, int first, int count,
                           int rob, hipStream_t st);
@@ -208,8 +210,18 @@ bool Sim::stepMixed(int nsteps, MixTape* tape) {
     }
     bool lastStep = false;
     int launched = 0;
+    /* Contact steps run in batches of up to kContactBatch steps with no host round trip: the
+       first step of a batch follows the host's stale test and rebuild; before each later one,
+       k_col_stale_halt tests the watch list on the device and, if it went stale, halts the
+       batch there (the remaining step kernels return at once).  At the batch's end the host
+       reads the halt step, takes back the ping-pong swaps of the steps that did nothing and
+       resumes from there, so every step sees the watch list the step-by-step loop would. */
+    const bool batched = collisions && !tape && !timed;
+    int* const halt = dp.ctl + 4;
+    int batchStart = 0, batchEnd = 0;
     for (int k = 0; k < nsteps && !lastStep; k++) {
-        if (collisions) {
+        if (collisions && batched && k > batchStart && k < batchEnd) launch_col_stale_halt(dp, cp, halt, k, stream);
+        if (collisions && k >= batchEnd) {
             /* the recompute's state lives in the tape: the watch list is tested and rebuilt on it */
             double* const vdCur = dp.vd;
             uint32_t* const vflCur = dp.vflags;
@@ -242,6 +254,11 @@ bool Sim::stepMixed(int nsteps, MixTape* tape) {
                 dp.vflags = vflCur;
                 if (tape->colSnaps && !snapPartners(tape, k, rebuilt)) return false;
             }
+            if (batched && !lastStep) {
+                batchStart = k;
+                batchEnd = std::min(nsteps, k + kContactBatch);
+                launch_set_halt(halt, INT_MAX, stream);
+            }
         }
         mp.vdIn = dp.vd;
         mp.vdOut = (double*)bvdAlt.p;
@@ -259,6 +276,7 @@ bool Sim::stepMixed(int nsteps, MixTape* tape) {
         mp.colOff = collisions ? cp.pairOff : nullptr;
         mp.partners = collisions ? cp.partners : nullptr;
         mp.k = k;
+        mp.halt = batched ? halt : nullptr;
         mp.tapeSums = nullptr;
         if (tape) { /* the recompute of a reverse pass: step k reads tape entry k and writes entry k+1 */
             mp.vdIn = tape->vd[k];
@@ -282,6 +300,21 @@ bool Sim::stepMixed(int nsteps, MixTape* tape) {
         swapVoxelBuffers();
         lcur ^= 1;
         launched++;
+        if (batched && k + 1 == batchEnd && k > batchStart) { /* where did the batch stop? */
+            HIPCHK(hipMemcpyAsync(ctlHost + 4, halt, sizeof(int), hipMemcpyDeviceToHost, stream));
+            HIPCHK(hipStreamSynchronize(stream));
+            const int h = ctlHost[4];
+            if (h <= k) { /* steps h..k did nothing: their swaps are taken back and h runs next */
+                const int skipped = k + 1 - h;
+                if (skipped & 1) {
+                    swapVoxelBuffers();
+                    lcur ^= 1;
+                }
+                launched -= skipped;
+                k = h - 1;
+                batchEnd = h;
+            }
+        }
     }
     if (graph) {
         hipGraph_t g = nullptr;

@@ -853,9 +853,9 @@ bool Sim::ensureDevice() {
     if (stream) return true;
     hipError_t e = hipStreamCreate(&stream);
     if (e != hipSuccess) return fail("hipStreamCreate", e);
-    e = hipHostMalloc((void**)&ctlHost, 4 * sizeof(int), hipHostMallocDefault);
+    e = hipHostMalloc((void**)&ctlHost, 8 * sizeof(int), hipHostMallocDefault);
     if (e != hipSuccess) return fail("hipHostMalloc", e);
-    if (!bctl.ensure(4 * sizeof(int))) return fail("hipMalloc", hipErrorOutOfMemory);
+    if (!bctl.ensure(8 * sizeof(int))) return fail("hipMalloc", hipErrorOutOfMemory);
     /* the address of this device's copy of the no-alloc staging array (vx_kernels.hip) */
     e = scratch_address(&scratchPtr);
     if (e != hipSuccess) return fail("hipGetSymbolAddress(scratch)", e);

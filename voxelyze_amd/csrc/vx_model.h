@@ -263,6 +263,7 @@ struct Sim {
     bool uploadMixedLinks();
     bool downloadMixedLinks();
     void swapVoxelBuffers();
+    static constexpr int kContactBatch = 64; /* contact steps per host round trip */
     bool stepMixed(int nsteps, struct MixTape* tape = nullptr);
     bool snapPartners(struct MixTape* tape, int k, bool rebuilt);
     bool mixedFinish(int launched);

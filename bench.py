@@ -338,8 +338,10 @@ def bench_robots(n_robots, steps, fp64_value, subset=64):
         W.robot_population(sub, shapes[:subset])
         sub.setPrecision(prec)
         nominal = sub.state()["pos"]
-        if name == "fp64_pushed":
-            sub.voxel(0).external().setForce(1e-9, 0.0, 0.0)
+        firsts = [sub.robotVoxels(r) for r in range(sub.robotCount())]
+        if name == "fp64_pushed":  # 1e-9 N on the first voxel of every robot
+            for f, _ in firsts:
+                sub.voxel(int(f)).external().setForce(1e-9, 0.0, 0.0)
         left = after
         while left > 0:
             k = min(500, left)
@@ -348,10 +350,16 @@ def bench_robots(n_robots, steps, fp64_value, subset=64):
         pos[name] = sub.state()["pos"]
         del sub
     scale = max(np.abs(pos["fp64"] - nominal).max(), 1e-300)
-    err = float(np.abs(pos["mixed"] - pos["fp64"]).max() / scale)
-    resp = float(np.abs(pos["fp64_pushed"] - pos["fp64"]).max() / scale)
-    print(f"[bench] robots: mixed vs FP64 on {subset} robots after {after} steps {err:.3e} (FP64's response to "
-          f"1e-9 N on one voxel {resp:.3e})", file=sys.stderr, flush=True)
+
+    def per_robot(name):
+        d = np.abs(pos[name] - pos["fp64"]).max(axis=1)
+        return np.array([d[f:f + c].max() / scale for f, c in firsts])
+
+    pr_mixed, pr_push = per_robot("mixed"), per_robot("fp64_pushed")
+    err, resp = float(pr_mixed.max()), float(pr_push.max())
+    print(f"[bench] robots: mixed vs FP64 on {subset} robots after {after} steps {err:.3e} (median robot "
+          f"{np.median(pr_mixed):.2e}, {(pr_mixed > 1e-4).sum()} robots above 1e-4); FP64's response to 1e-9 N on one "
+          f"voxel of every robot {resp:.3e} (median robot {np.median(pr_push):.2e})", file=sys.stderr, flush=True)
     return {"workload": f"{n_robots} robots grown to 100-8000 voxels in a 20^3 box, 4 materials, +-39 degC 2 Hz "
                         "actuation on the device, gravity + floor friction, per-robot dt, mixed precision",
             "voxels": V, "links": L, "steps": steps, "value": value, "unit": "voxel-steps/s",
@@ -360,7 +368,10 @@ def bench_robots(n_robots, steps, fp64_value, subset=64):
             "com_x_travel_m": {"mean": float(travel.mean()), "min": float(travel.min()), "max": float(travel.max()),
                                "median_abs": float(np.median(np.abs(travel))), "after_steps": after - 20},
             "mixed_vs_fp64": {"robots": subset, "after_steps": after, "rel_disp_err": err,
-                              "fp64_response_to_1e-9N_rel_disp": resp}}
+                              "median_robot_rel_disp_err": float(np.median(pr_mixed)),
+                              "robots_above_1e-4": int((pr_mixed > 1e-4).sum()),
+                              "fp64_response_to_1e-9N_on_every_robot_rel_disp": resp,
+                              "fp64_response_median_robot": float(np.median(pr_push))}}
 
 
 def bench_mixed_error(steps=5000, every=1000):

@@ -49,3 +49,37 @@ def test_one_launch_equals_single_steps_with_contacts(precision):
     for k in ("pos", "orient", "linMom", "angMom"):
         if k in sa:
             assert sa[k].tobytes() == sb[k].tobytes(), k
+
+
+def robots_with_contacts(precision):
+    """four robots with self-collision; robot 1 is blown apart by a 1e4 N force on one voxel, so
+    its watch list goes stale every few steps and it diverges inside a batch"""
+    from voxelyze_amd import workloads as W
+    sim = B.maker(B.product_abi())(0.005)
+    W.robot_population(sim, W.robot_shapes(51, 4, vmin=40, vmax=220, box=8))
+    sim.enableCollisions(True)
+    sim.setPrecision(precision)
+    f1, _ = sim.robotVoxels(1)
+    sim.voxel(f1).external().setForce(1e4, 0.0, 0.0)
+    return sim
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1, 2])
+def test_batches_with_a_diverging_robot_equal_single_steps(precision):
+    """Halts of contact batches and a robot's divergence inside them: one launch of N robot
+    steps equals N one-step launches bit for bit (state, clocks, steps done per robot)."""
+    a, b = robots_with_contacts(precision), robots_with_contacts(precision)
+    n = 300
+    done_a = a.robotStep(n)
+    done_b = np.zeros(b.robotCount(), np.int64)
+    for _ in range(n):
+        done_b += b.robotStep(1)
+    print(f"[contact batches] precision {precision}: steps done {done_a} (one launch), {done_b} (single steps)")
+    assert done_a[1] < n and all(done_a[r] == n for r in (0, 2, 3))
+    assert np.array_equal(done_a, done_b)
+    sa, sb = a.state(), b.state()
+    for k in ("pos", "orient", "linMom", "angMom"):
+        assert sa[k].tobytes() == sb[k].tobytes(), k
+    for r in range(a.robotCount()):
+        assert a.robotTime(r) == b.robotTime(r)

@@ -312,6 +312,7 @@ struct FusedPtrs {
     float actBase, actAmp, actFreq;
     int k;                    /* this step's index in the launch */
     const int* halt;          /* contact batches: the step returns at once when halt[0] <= k */
+    int nvmats, nlmats;       /* entries of the material tables */
     int storeForces;          /* write the end forces (the launch's last step) */
     int xcd;
     int poisson;              /* a material has Poisson's ratio != 0 */

@@ -83,6 +83,8 @@ bool Sim::stepFp64Fused(int nsteps) {
     fp.vtemp = dp.vtemp;
     fp.vmats = dp.vmats;
     fp.lmats = dp.lmats;
+    fp.nvmats = (int)vmats.size();
+    fp.nlmats = (int)lmats.size();
     fp.curveStrain = dp.curveStrain;
     fp.curveStress = dp.curveStress;
     fp.exts = dp.exts;

@@ -313,6 +313,8 @@ __device__ void frozen_copy(const MixPtrs& s, int i, int rob, int d) {
 #ifndef VX_FUSED_WAVES
 #define VX_FUSED_WAVES 3
 #endif
+/* (Staging the material tables in LDS, as the FP64 step does, made this kernel 50 % slower on
+   the drop lattice at the same registers and occupancy: 192 -> 291 us.) */
 template <typename T>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_FUSED_WAVES, 8))) k_step_mixed(MixPtrs s) {
     const int i = lane_index(s.xcd);

@@ -90,6 +90,7 @@ struct LinPtrs {
     const int* vnbr;        /* 6 * nv neighbour table, -1 for none */
     const int* lmeta;       /* link material of each slot (axis * nv + negative voxel) */
     const LinkMat* lmats;
+    int nlmats;             /* entries of the link material table */
     const uint8_t* fixed;   /* bit j: degree of freedom j of the voxel is prescribed */
     const double* xfix;     /* 6 * nv: prescribed values (only where fixed) */
     const double* f;        /* 6 * nv: external force and moment (only where free) */

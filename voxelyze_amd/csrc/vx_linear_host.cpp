@@ -81,6 +81,7 @@ bool Sim::linearSolve(double tol, int maxIter, int* itersOut, double* residualOu
     lp.vnbr = dp.vnbr;
     lp.lmeta = dp.lmeta;
     lp.lmats = dp.lmats;
+    lp.nlmats = (int)lmats.size();
     double* dxfix = (double*)carve(V * 48);
     double* df = (double*)carve(V * 48);
     lp.xfix = dxfix;

@@ -83,3 +83,30 @@ def test_batches_with_a_diverging_robot_equal_single_steps(precision):
         assert sa[k].tobytes() == sb[k].tobytes(), k
     for r in range(a.robotCount()):
         assert a.robotTime(r) == b.robotTime(r)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1, 2])
+def test_batches_when_every_robot_stops(precision):
+    """Every robot diverges inside the first batch: the launch ends with the step that settles
+    the ping-pong buffers, which runs outside any batch (a halt the batch recorded must not make
+    it return early).  One launch equals one-step launches bit for bit."""
+    def blown(prec):
+        sim = robots_with_contacts(prec)
+        for r in range(sim.robotCount()):
+            f, _ = sim.robotVoxels(r)
+            sim.voxel(f).external().setForce(1e4, 0.0, 0.0)
+        return sim
+    a, b = blown(precision), blown(precision)
+    n = 200
+    done_a = a.robotStep(n)
+    done_b = np.zeros(b.robotCount(), np.int64)
+    for _ in range(n):
+        done_b += b.robotStep(1)
+    print(f"[contact batches] precision {precision}: every robot stops: {done_a} (one launch), {done_b} (single steps)")
+    assert (done_a < n).all() and np.array_equal(done_a, done_b)
+    sa, sb = a.state(), b.state()
+    for k in ("pos", "orient", "linMom", "angMom"):
+        assert sa[k].tobytes() == sb[k].tobytes(), k
+    for r in range(a.robotCount()):
+        assert a.robotTime(r) == b.robotTime(r)

@@ -180,7 +180,7 @@ bool Sim::stepFp64Fused(int nsteps) {
         fp.colOff = collisions ? cp.pairOff : nullptr;
         fp.partners = collisions ? cp.partners : nullptr;
         fp.k = k;
-        fp.halt = batched ? halt : nullptr;
+        fp.halt = batched && k < batchEnd ? halt : nullptr; /* only a step of an open batch */
         fp.storeForces = (k == nsteps - 1 || lastStep) ? 1 : 0;
         const LaunchCfg cfg = tune ? tuner[3].next(&cand[k]) : chosen;
         fp.xcd = cfg.xcd;

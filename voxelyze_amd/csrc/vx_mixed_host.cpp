@@ -276,7 +276,9 @@ bool Sim::stepMixed(int nsteps, MixTape* tape) {
         mp.colOff = collisions ? cp.pairOff : nullptr;
         mp.partners = collisions ? cp.partners : nullptr;
         mp.k = k;
-        mp.halt = batched ? halt : nullptr;
+        /* only a step of an open batch reads the halt (the step that settles the ping-pong
+           buffers after every robot stopped runs outside any batch) */
+        mp.halt = batched && k < batchEnd ? halt : nullptr;
         mp.tapeSums = nullptr;
         if (tape) { /* the recompute of a reverse pass: step k reads tape entry k and writes entry k+1 */
             mp.vdIn = tape->vd[k];

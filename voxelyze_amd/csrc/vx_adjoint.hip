@@ -157,10 +157,11 @@ __device__ void pair_side(const AdjPtrs& a, int v, dvec* p, dvec* l, double* mIn
     VoxMat m = a.vmats[a.vmat[v]];
     if (a.vE) voxel_modulus(&m, a.vE[v]);
     *mInv = m.massInverse;
+    /* the step's float temperature through the float factor 1 + T*alphaCTE, as the mixed step
+       forms it; the FP64 fused step's double temperature when actuated */
     const float T = a.t.vtemp[v];
     const float f = 1 + T * m.alphaCTE;
-    const double bf = (f * m.sizeX + f * m.sizeY + f * m.sizeZ) / 3.0;
-    *bsa = bf;
+    *bsa = (f * m.sizeX + f * m.sizeY + f * m.sizeZ) / 3.0;
     if (a.lsDouble && a.vphase)
         *bsa = fp64_bsa(m, fused_temperature_fp64(a.actBase, a.actAmp, a.actFreq, a.t.rtime[a.vrobot[v]], a.vphase[v]));
     *dBsa = (double)m.alphaCTE * (m.sizeX + m.sizeY + m.sizeZ) / 3.0;
@@ -188,27 +189,21 @@ __global__ void __launch_bounds__(256) k_adj_contacts(AdjPtrs a) {
     PairAdjIn in;
     pair_side(a, v, &in.pi, &in.li, &in.mInvI, &in.bsaI, &in.dBsaI_dT, &in.kI, &in.cI);
     const dvec lamV = dv(a.lamF[3 * v], a.lamF[3 * v + 1], a.lamF[3 * v + 2]);
-    /* the forward's own decision arithmetic: float for the mixed step */
-    const VoxMat& mv0 = a.vmats[a.vmat[v]];
-    const float fv = 1 + a.t.vtemp[v] * mv0.alphaCTE;
-    const double bsaVf = (fv * mv0.sizeX + fv * mv0.sizeY + fv * mv0.sizeZ) / 3.0;
     dvec lp = dv(0, 0, 0), ll = dv(0, 0, 0);
     double lT = 0.0, lth = 0.0;
     for (int q = 0; q < cnt; q++) {
         const int w = a.partners[off + q];
         pair_side(a, w, &in.pj, &in.lj, &in.mInvJ, &in.bsaJ, &in.dBsaJ_dT, &in.kJ, &in.cJ);
+        /* whether the pair touches, in the forward's own arithmetic (FP32 in the mixed step) */
         bool touch;
         if (a.lsDouble) {
             const dvec o = vsub(in.pj, in.pi);
             const double len = sqrt(vdot(o, o));
             touch = (in.bsaI + in.bsaJ) * 0.625 - len > 0 && len > 0;
         } else {
-            const VoxMat& mw0 = a.vmats[a.vmat[w]];
-            const float fw = 1 + a.t.vtemp[w] * mw0.alphaCTE;
-            const double bsaWf = (fw * mw0.sizeX + fw * mw0.sizeY + fw * mw0.sizeZ) / 3.0;
             const float ox = (float)(in.pj.x - in.pi.x), oy = (float)(in.pj.y - in.pi.y), oz = (float)(in.pj.z - in.pi.z);
             const float len = sqrtf(ox * ox + oy * oy + oz * oz);
-            const float rel = (float)((bsaVf + bsaWf) * 0.625f) - len;
+            const float rel = (float)((in.bsaI + in.bsaJ) * 0.625f) - len;
             touch = rel > 0 && len > 0;
         }
         if (!touch) continue;

@@ -337,10 +337,10 @@ bool Sim::adjointBackward(const double* dLdpos, double* gThetaOut, double* gPhas
             HIPCHK(hipStreamSynchronize(stream));
             for (int jj = 0; jj < m; jj++) {
                 const hipEvent_t* ev = &events[(size_t)4 * jj];
-                float a = 0, b = 0, c = 0;
-                if (hipEventElapsedTime(&a, ev[0], ev[1]) == hipSuccess) { profMs[5] += a; profCount[5]++; }
-                if (collisions && hipEventElapsedTime(&b, ev[1], ev[2]) == hipSuccess) { profMs[7] += b; profCount[7]++; }
-                if (hipEventElapsedTime(&c, ev[2], ev[3]) == hipSuccess) { profMs[6] += c; profCount[6]++; }
+                float tVox = 0, tCol = 0, tLink = 0;
+                if (hipEventElapsedTime(&tVox, ev[0], ev[1]) == hipSuccess) { profMs[5] += tVox; profCount[5]++; }
+                if (collisions && hipEventElapsedTime(&tCol, ev[1], ev[2]) == hipSuccess) { profMs[7] += tCol; profCount[7]++; }
+                if (hipEventElapsedTime(&tLink, ev[2], ev[3]) == hipSuccess) { profMs[6] += tLink; profCount[6]++; }
             }
         }
     }

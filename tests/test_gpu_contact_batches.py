@@ -1,6 +1,7 @@
 """Contact steps in batches without host round trips (vx_mixed_host.cpp, vx_fused_host.cpp):
 a multi-step launch runs up to 64 contact steps between host synchronisations and halts on
-the device where the watch list goes stale (k_col_stale_halt), then rebuilds it and resumes.
+the device where the watch list goes stale (the step kernels test it on the positions they
+write), then rebuilds it and resumes.
 The watch list must be tested and rebuilt exactly where the step-by-step loop does it
 (src/Voxelyze.cpp:686-698): one launch of N steps equals N one-step launches bit for bit, on
 every precision path, while a soft cube falls onto a fixed plate (moving several half

@@ -51,22 +51,7 @@ __global__ void k_col_stale(DevPtrs s, ContactPtrs c, int gen) {
     if (dx * dx + dy * dy + dz * dz > c.recalcSq) s.ctl[3] = gen; /* every writer stores the same value */
 }
 
-/* The same stale test inside a batch of contact steps run without host round trips: after
-   step `step - 1` it tests the state step `step` starts from and, if the watch list is stale
-   there, records `step` as the batch's halt step (halt[0]); the step kernels of steps >= halt
-   then return at once, and the host rebuilds the list and resumes at that step.  A test
-   after the batch has halted does nothing, so halt keeps the first stale step (every writer
-   stores the same value). */
-__global__ void k_col_stale_halt(DevPtrs s, ContactPtrs c, int* halt, int step) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= s.nv || (s.vflags[i] & VF_INTERIOR)) return;
-    if (halt[0] <= step) return;
-    const int nv = s.nv;
-    const double dx = s.vd[VPX * nv + i] - (double)c.last[i];
-    const double dy = s.vd[VPY * nv + i] - (double)c.last[nv + i];
-    const double dz = s.vd[VPZ * nv + i] - (double)c.last[2 * nv + i];
-    if (dx * dx + dy * dy + dz * dz > c.recalcSq) halt[0] = step;
-}
+/* opens a batch of contact steps run without host round trips: no halt recorded yet */
 __global__ void k_set_halt(int* halt, int v) {
     if (threadIdx.x == 0) halt[0] = v;
 }
@@ -186,9 +171,6 @@ __global__ void k_col_forces(DevPtrs s, ContactPtrs c) {
 
 static inline unsigned nblk(int n) { return (unsigned)((n + 255) / 256); }
 
-void launch_col_stale_halt(const DevPtrs& s, const ContactPtrs& c, int* halt, int step, hipStream_t st) {
-    hipLaunchKernelGGL(k_col_stale_halt, dim3(nblk(s.nv)), dim3(256), 0, st, s, c, halt, step);
-}
 void launch_set_halt(int* halt, int v, hipStream_t st) { hipLaunchKernelGGL(k_set_halt, dim3(1), dim3(64), 0, st, halt, v); }
 void launch_col_stale(const DevPtrs& s, const ContactPtrs& c, int gen, hipStream_t st) {
     hipLaunchKernelGGL(k_col_stale, dim3(nblk(s.nv)), dim3(256), 0, st, s, c, gen);

@@ -192,6 +192,19 @@ __device__ __forceinline__ void st_nt(T* p, T v) { __builtin_nontemporal_store(v
 template <class T>
 __device__ __forceinline__ T ld_nt(const T* p) { return __builtin_nontemporal_load(p); }
 
+/* The watch list's stale test (CVoxelyze::updateCollisions, src/Voxelyze.cpp:686-696) made by
+   the step of a contact batch on the position it leaves for voxel i (the position the next
+   step starts from): if a surface voxel moved farther than recalcDist since the last rebuild,
+   the next step `next` halts the batch (every writer stores the same value). */
+__device__ __forceinline__ void halt_if_stale(int* halt, int next, const float* last, float recalcSq,
+                                              const uint32_t* vfl, int nv, int i, double x, double y, double z) {
+    if (vfl[i] & VF_INTERIOR) return;
+    const double dx = x - (double)last[i];
+    const double dy = y - (double)last[nv + i];
+    const double dz = z - (double)last[2 * nv + i];
+    if (dx * dx + dy * dy + dz * dz > recalcSq) halt[0] = next;
+}
+
 /* the lane's element under a launch configuration (workgroup size, XCD-contiguous mapping) */
 __device__ __forceinline__ int lane_index(int xcd) {
     return (xcd ? xcd_block() : (int)blockIdx.x) * (int)blockDim.x + (int)threadIdx.x;
@@ -262,7 +275,11 @@ struct MixPtrs {
     const float* vE;        /* per-voxel Young's modulus overriding the materials' (linear), null: off */
     double* tapeSums;       /* when set, each voxel records its link force and moment sums (6 * nv) */
     int k;                  /* this step's index in the launch */
-    const int* halt;        /* contact batches: the step returns at once when halt[0] <= k */
+    int* halt;              /* contact batches: the step returns at once when halt[0] <= k */
+    int haltTest;           /* 1: the next step is in the same batch; a lane whose new position
+                               makes the watch list stale records k + 1 in halt[0] */
+    const float* colLast;   /* positions at the watch list's last rebuild (3 x nv) */
+    float recalcSq;         /* recalcDist^2, as the stale test compares */
     int xcd;                /* launch configuration: XCD-contiguous block mapping */
 };
 
@@ -312,7 +329,10 @@ struct FusedPtrs {
     const float* vphase;
     float actBase, actAmp, actFreq;
     int k;                    /* this step's index in the launch */
-    const int* halt;          /* contact batches: the step returns at once when halt[0] <= k */
+    int* halt;                /* contact batches: the step returns at once when halt[0] <= k */
+    int haltTest;             /* 1: the next step is in the same batch (see MixPtrs) */
+    const float* colLast;
+    float recalcSq;
     int nvmats, nlmats;       /* entries of the material tables */
     int storeForces;          /* write the end forces (the launch's last step) */
     int xcd;

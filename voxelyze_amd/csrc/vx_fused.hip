@@ -195,6 +195,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_FUS
     } else {
         if (d < s.k) {
             frozen_fp64(s, i, rob, d);
+            if (s.haltTest)
+                halt_if_stale(s.halt, s.k + 1, s.colLast, s.recalcSq, s.vflIn, nv, i, s.vdOut[VPX * nv + i],
+                              s.vdOut[VPY * nv + i], s.vdOut[VPZ * nv + i]);
             return;
         }
         prevDt = s.k == 0 ? s.rprev[rob] : dt;
@@ -242,6 +245,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_FUS
     if (ex && (ex->dof & 0x3F) == 0x3F) {
         voxel_pinned(m, ex, s.vijk, nv, iv, s.vdOut);
         s.vflOut[iv] = flags;
+        if (s.haltTest)
+            halt_if_stale(s.halt, s.k + 1, s.colLast, s.recalcSq, s.vflIn, nv, iv, s.vdOut[VPX * nv + iv],
+                          s.vdOut[VPY * nv + iv], s.vdOut[VPZ * nv + iv]);
         return;
     }
     /* the momenta only this lane reads: loaded past the caches */
@@ -265,6 +271,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_FUS
     });
     store_voxel(s.vdOut, nv, iv, pos, lin, ang, q);
     st_nt(&s.vflOut[iv], flags & ~(uint32_t)VF_PS_VALID); /* poissonsStrainInvalid = true (VX_Voxel.cpp:231) */
+    if (s.haltTest) halt_if_stale(s.halt, s.k + 1, s.colLast, s.recalcSq, s.vflIn, nv, iv, pos.x, pos.y, pos.z);
 }
 
 /* after a launch: previousDt of every voxel whose robot completed at least one step */

@@ -17,7 +17,6 @@ void launch_fused_restore(double* vd, const double* vdPrev, uint32_t* vfl, const
                           const float* vtempPrev, float* rtime, const float* rtimePrev, int nv, int first, int count,
                           int rob, hipStream_t st);
 void launch_col_stale(const DevPtrs& s, const ContactPtrs& c, int gen, hipStream_t st);
-void launch_col_stale_halt(const DevPtrs& s, const ContactPtrs& c, int* halt, int step, hipStream_t st);
 void launch_set_halt(int* halt, int v, hipStream_t st);
 void launch_poisson_fp64(const DevPtrs& s, int k, hipStream_t st);
 uint64_t graph_key(const void* args, size_t n, int path, int nsteps, int flags);
@@ -148,7 +147,6 @@ bool Sim::stepFp64Fused(int nsteps) {
     int* const halt = dp.ctl + 4;
     int batchStart = 0, batchEnd = 0;
     for (int k = 0; k < nsteps && !lastStep; k++) {
-        if (collisions && batched && k > batchStart && k < batchEnd) launch_col_stale_halt(dp, cp, halt, k, stream);
         if (collisions && k >= batchEnd) {
             /* the watch list from the positions this step starts from (the link phase does
                not move voxels, so this is what updateCollisions sees, Voxelyze.cpp:271) */
@@ -181,6 +179,9 @@ bool Sim::stepFp64Fused(int nsteps) {
         fp.partners = collisions ? cp.partners : nullptr;
         fp.k = k;
         fp.halt = batched && k < batchEnd ? halt : nullptr; /* only a step of an open batch */
+        fp.haltTest = batched && k + 1 < batchEnd ? 1 : 0;   /* the next step is in the same batch */
+        fp.colLast = collisions ? cp.last : nullptr;
+        fp.recalcSq = cp.recalcSq;
         fp.storeForces = (k == nsteps - 1 || lastStep) ? 1 : 0;
         const LaunchCfg cfg = tune ? tuner[3].next(&cand[k]) : chosen;
         fp.xcd = cfg.xcd;

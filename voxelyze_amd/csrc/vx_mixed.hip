@@ -325,6 +325,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_FUS
     const int d = s.rdiv[rob];
     if (d < s.k) {
         frozen_copy<T>(s, i, rob, d);
+        if (s.haltTest)
+            halt_if_stale(s.halt, s.k + 1, s.colLast, s.recalcSq, s.vflIn, nv, i, s.vdOut[VPX * nv + i],
+                          s.vdOut[VPY * nv + i], s.vdOut[VPZ * nv + i]);
         return;
     }
     const float dt = s.rdt[rob];
@@ -410,6 +413,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_FUS
         out[VLX * nv + i] = 0.0; out[VLY * nv + i] = 0.0; out[VLZ * nv + i] = 0.0;
         out[VAX * nv + i] = 0.0; out[VAY * nv + i] = 0.0; out[VAZ * nv + i] = 0.0;
         s.vflOut[i] = flags;
+        if (s.haltTest)
+            halt_if_stale(s.halt, s.k + 1, s.colLast, s.recalcSq, s.vflIn, nv, i, out[VPX * nv + i], out[VPY * nv + i],
+                          out[VPZ * nv + i]);
         return;
     }
     /* the momenta only this lane reads: loaded past the caches */
@@ -559,6 +565,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_FUS
     st_nt(&out[VQW * nv + i], q.w); st_nt(&out[VQX * nv + i], q.x); st_nt(&out[VQY * nv + i], q.y);
     st_nt(&out[VQZ * nv + i], q.z);
     st_nt(&s.vflOut[i], flags);
+    if (s.haltTest) halt_if_stale(s.halt, s.k + 1, s.colLast, s.recalcSq, s.vflIn, nv, i, pos.x, pos.y, pos.z);
 }
 
 /* temperatures at each robot's current time, before the first step of an actuated launch */

@@ -15,7 +15,6 @@ void launch_step_mixed(const MixPtrs& s, bool fp64, int block, hipStream_t st);
 void launch_mixed_temps(const MixPtrs& s, float* temp, hipStream_t st);
 uint64_t graph_key(const void* args, size_t n, int path, int nsteps, int flags);
 void launch_col_stale(const DevPtrs& s, const ContactPtrs& c, int gen, hipStream_t st);
-void launch_col_stale_halt(const DevPtrs& s, const ContactPtrs& c, int* halt, int step, hipStream_t st);
 void launch_set_halt(int* halt, int v, hipStream_t st);
 void launch_fused_restore(double* vd, const double* vdPrev, uint32_t* vfl, const uint32_t* vflPrev, float* vtemp,
                           const float* vtempPrev, float* rtime, const float* rtimePrev, int nv, int first, int count,
@@ -211,16 +210,16 @@ bool Sim::stepMixed(int nsteps, MixTape* tape) {
     bool lastStep = false;
     int launched = 0;
     /* Contact steps run in batches of up to kContactBatch steps with no host round trip: the
-       first step of a batch follows the host's stale test and rebuild; before each later one,
-       k_col_stale_halt tests the watch list on the device and, if it went stale, halts the
-       batch there (the remaining step kernels return at once).  At the batch's end the host
-       reads the halt step, takes back the ping-pong swaps of the steps that did nothing and
-       resumes from there, so every step sees the watch list the step-by-step loop would. */
+       first step of a batch follows the host's stale test and rebuild; every step of the batch
+       but its last tests the watch list on the positions it leaves (halt_if_stale, the same
+       test) and, if it went stale, halts the batch at the next step (the remaining step
+       kernels return at once).  At the batch's end the host reads the halt step, takes back the
+       ping-pong swaps of the steps that did nothing and resumes from there, so every step sees
+       the watch list the step-by-step loop would. */
     const bool batched = collisions && !tape && !timed;
     int* const halt = dp.ctl + 4;
     int batchStart = 0, batchEnd = 0;
     for (int k = 0; k < nsteps && !lastStep; k++) {
-        if (collisions && batched && k > batchStart && k < batchEnd) launch_col_stale_halt(dp, cp, halt, k, stream);
         if (collisions && k >= batchEnd) {
             /* the recompute's state lives in the tape: the watch list is tested and rebuilt on it */
             double* const vdCur = dp.vd;
@@ -279,6 +278,9 @@ bool Sim::stepMixed(int nsteps, MixTape* tape) {
         /* only a step of an open batch reads the halt (the step that settles the ping-pong
            buffers after every robot stopped runs outside any batch) */
         mp.halt = batched && k < batchEnd ? halt : nullptr;
+        mp.haltTest = batched && k + 1 < batchEnd ? 1 : 0;
+        mp.colLast = collisions ? cp.last : nullptr;
+        mp.recalcSq = cp.recalcSq;
         mp.tapeSums = nullptr;
         if (tape) { /* the recompute of a reverse pass: step k reads tape entry k and writes entry k+1 */
             mp.vdIn = tape->vd[k];

@@ -258,8 +258,9 @@ void vxg_voxel_set_temperature(vxg_sim* s, int vox, float temperature);
 float vxg_voxel_temperature(vxg_sim* s, int vox);    /* temperature()  VX_Voxel.h:111 */
 /* Bulk state of voxels [first, first+count): position() (VX_Voxel.h:79), orientation()
    (VX_Voxel.h:92), linear and angular momentum (VX_Voxel.h:164,166; velocity() = linMom/m)
-   and the voxel flag bits.  Any output pointer may be NULL.  Layouts: pos/lin/ang 3 per
-   voxel, orient 4 per voxel (w,x,y,z). */
+   and the voxel flag bits.  Any output pointer may be NULL; after steps, a read that leaves
+   any of pos/orient/lin/ang NULL copies only the requested device arrays.  Layouts:
+   pos/lin/ang 3 per voxel, orient 4 per voxel (w,x,y,z). */
 void vxg_voxel_get_state(vxg_sim* s, int first, int count, double* pos, double* orient,
                          double* lin_mom, double* ang_mom, uint32_t* flags);
 

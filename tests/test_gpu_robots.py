@@ -18,6 +18,7 @@ import pytest
 
 import backends as B
 from voxelyze_amd import workloads as W
+from voxelyze_amd.api import _dptr
 
 AMP, FREQ = 39.0, 2.0
 
@@ -203,6 +204,26 @@ def test_robot_com_is_the_mass_weighted_mean():
         m = np.full(c, mass)  # every material here has rho = 1000
         ref = (pos[f:f + c] * m[:, None]).sum(0) / m.sum()
         assert np.allclose(com[r], ref, rtol=0, atol=1e-14)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1])
+def test_partial_state_reads_equal_the_full_read(precision):
+    """After steps, a read of some fields copies only those device arrays (positions() for the
+    whole population, orientations of a range); a read of every field refreshes the host
+    mirror.  Both give the same bits."""
+    sim = batch(shapes(n=4, seed=71), precision, collisions=True)
+    sim.robotStep(200)
+    n = sim.voxelCount()
+    pos = sim.positions()
+    q = np.zeros((n - 7, 4))
+    sim.abi.voxel_get_state(sim.handle, 7, n - 7, None, _dptr(q), None, None, None)
+    mid = sim.positions(n // 3, n // 2)
+    full = sim.state()
+    assert pos.tobytes() == full["pos"].tobytes()
+    assert q.tobytes() == full["orient"][7:].tobytes()
+    assert mid.tobytes() == full["pos"][n // 3:n // 3 + n // 2].tobytes()
+    assert sim.positions().tobytes() == full["pos"].tobytes()  # from the refreshed mirror
 
 
 @pytest.mark.gpu

@@ -24,7 +24,7 @@ class Rollout(torch.autograd.Function):
         sim.adjointRun(nsteps, every, dt)
         ctx.sim = sim
         ctx.device = log_modulus.device
-        return torch.from_numpy(sim.state()["pos"].copy()).to(log_modulus.device)
+        return torch.from_numpy(sim.positions()).to(log_modulus.device)
 
     @staticmethod
     def backward(ctx, grad_pos):

@@ -608,6 +608,15 @@ class CVoxelyze:
             self.abi.voxel_get_state(self.handle, first, n, _dptr(p), _dptr(q), _dptr(lm), _dptr(am), _uptr(fl))
         return {"pos": p, "orient": q, "linMom": lm, "angMom": am, "flags": fl}
 
+    def positions(self, first=0, count=None):
+        """position() (VX_Voxel.h:79) of voxels [first, first+count) as an (n, 3) array: after
+        steps only the three position arrays leave the device"""
+        n = self.voxelCount() - first if count is None else count
+        p = np.zeros((n, 3))
+        if n:
+            self.abi.voxel_get_state(self.handle, first, n, _dptr(p), None, None, None, None)
+        return p
+
     def link_transverse(self, first=0, count=None):
         """currentTransverseArea and currentTransverseStrainSum of links (VX_Link.h:107-108)."""
         n = self.linkCount() - first if count is None else count
@@ -630,6 +639,3 @@ class CVoxelyze:
                                     _fptr(out["maxStrain"]), _fptr(out["strainOffset"]), _uptr(out["flags"]),
                                     _dptr(out["local"]), _dptr(out["forces"]), _dptr(out["restLength"]))
         return out
-
-    def positions(self):
-        return self.state()["pos"]

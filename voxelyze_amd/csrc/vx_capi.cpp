@@ -582,7 +582,9 @@ API void vxg_voxel_get_state(vxg_sim* h, int first, int count, double* pos, doub
                              uint32_t* flags) {
     Sim* s = S(h);
     if (first < 0 || count <= 0 || first + count > s->nv()) return;
-    if (s->hostStale && (size_t)count * 4 < (size_t)s->nv()) {
+    /* after steps, a small range or a read of only some of the fields moves just those
+       device arrays; a whole-population read of everything refreshes the host mirror */
+    if (s->hostStale && ((size_t)count * 4 < (size_t)s->nv() || !(pos && orient && lin && ang))) {
         s->readVoxelRange(first, count, pos, orient, lin, ang, flags);
         if (flags)
             for (int k = 0; k < count; k++) flags[k] &= 0xFFu;

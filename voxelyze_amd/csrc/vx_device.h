@@ -109,7 +109,7 @@ struct ExtRec {
     float fx, fy, fz;         /* extForce */
     float mx, my, mz;         /* extMoment */
     int dof;                  /* dofFixed */
-    int pad;
+    int ramp;                 /* index of its prescribed-displacement ramp, -1 for none */
 };
 
 /* a prescribed-displacement ramp: before every step the external's translation is set to
@@ -203,6 +203,23 @@ __device__ __forceinline__ void halt_if_stale(int* halt, int next, const float* 
     const double dy = y - (double)last[nv + i];
     const double dz = z - (double)last[2 * nv + i];
     if (dx * dx + dy * dy + dz * dz > recalcSq) halt[0] = next;
+}
+
+/* A prescribed-displacement ramp evaluated by the lane of the external's voxel (each external
+   belongs to one voxel): the translation becomes base + rate * t, t the robot's time before the
+   step, in double -- the arithmetic of k_ext_ramps, as a host loop calling setDisplacementAll
+   before each doTimeStep sets it.  The record is rewritten in place, so the step's own reads and
+   every later reader (the linear solve, the next launch) see the ramped translation. */
+__device__ __forceinline__ void apply_ramp(const ExtRec* exts, const RampRec* ramps, int e, float t) {
+    if (!ramps || e < 0) return;
+    ExtRec* ex = const_cast<ExtRec*>(&exts[e]);
+    const int r = ex->ramp;
+    if (r < 0) return;
+    const RampRec& q = ramps[r];
+    const double td = (double)t;
+    ex->tx = q.base[0] + q.rate[0] * td;
+    ex->ty = q.base[1] + q.rate[1] * td;
+    ex->tz = q.base[2] + q.rate[2] * td;
 }
 
 /* the lane's element under a launch configuration (workgroup size, XCD-contiguous mapping) */
@@ -313,7 +330,8 @@ struct FusedPtrs {
     const LinkMat* lmats;
     const float* curveStrain;
     const float* curveStress;
-    const ExtRec* exts;
+    const ExtRec* exts;       /* a ramped voxel's lane rewrites its own record's translation */
+    const RampRec* ramps;     /* prescribed-displacement ramps evaluated in the step (null: none) */
     const int* colCnt;        /* contacts (null when off) */
     const int* colOff;
     const int* partners;

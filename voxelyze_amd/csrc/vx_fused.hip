@@ -202,6 +202,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_FUS
         }
         prevDt = s.k == 0 ? s.rprev[rob] : dt;
         if (i == 0 || s.vrobot[i - 1] != rob) s.rtimeOut[rob] = s.rtimeIn[rob] + dt; /* currentTime += dt */
+        /* here, before the link loop, where few registers are live; read back further down */
+        apply_ramp(s.exts, s.ramps, s.vext[i], s.rtimeIn[rob]);
     }
     const double* __restrict__ vd = s.vdIn;
     const dvec pos0 = dv(vd[VPX * nv + i], vd[VPY * nv + i], vd[VPZ * nv + i]);

@@ -87,6 +87,7 @@ bool Sim::stepFp64Fused(int nsteps) {
     fp.curveStrain = dp.curveStrain;
     fp.curveStress = dp.curveStress;
     fp.exts = dp.exts;
+    fp.ramps = nRamps ? (const RampRec*)bramps.p : nullptr; /* evaluated by the step kernel */
     fp.vf = dp.vf;
     fp.vrobot = dp.vrobot;
     fp.rdt = dp.rdt;
@@ -185,7 +186,6 @@ bool Sim::stepFp64Fused(int nsteps) {
         fp.storeForces = (k == nsteps - 1 || lastStep) ? 1 : 0;
         const LaunchCfg cfg = tune ? tuner[3].next(&cand[k]) : chosen;
         fp.xcd = cfg.xcd;
-        launchRamps();
         launch_poisson_fp64(dp, k, stream); /* the Poisson strains this step's links read */
         if (timed) (void)hipEventRecord(events[2 * k], stream);
         launch_step_fp64(fp, cfg.block, false, stream);
@@ -273,6 +273,7 @@ bool Sim::fusedFp64Finish(int launched) {
         fp.curveStrain = dp.curveStrain;
         fp.curveStress = dp.curveStress;
         fp.exts = dp.exts;
+        fp.ramps = nullptr;
         fp.vf = dp.vf;
         fp.vrobot = dp.vrobot;
         fp.rdt = dp.rdt;

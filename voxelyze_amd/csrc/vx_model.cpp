@@ -1034,11 +1034,13 @@ bool Sim::syncUp() {
     }
     if (dirty & D_EXT) {
         std::vector<ExtRec> ex;
+        std::vector<RampRec> rr;
         extIndex.assign(n, -1);
         for (int v = 0; v < n; v++) {
             const Ext* e = vext[v].get();
             if (!e) continue;
             ExtRec r{};
+            r.ramp = -1;
             r.tx = e->translation[0]; r.ty = e->translation[1]; r.tz = e->translation[2];
             dquat q = e->rotationQuat();
             r.qw = q.w; r.qx = q.x; r.qy = q.y; r.qz = q.z;
@@ -1046,21 +1048,19 @@ bool Sim::syncUp() {
             r.mx = e->moment[0]; r.my = e->moment[1]; r.mz = e->moment[2];
             r.dof = e->dof;
             extIndex[v] = (int)ex.size();
+            if (e->ramp) {
+                RampRec q{};
+                q.ext = extIndex[v];
+                q.vox = v;
+                for (int c = 0; c < 3; c++) q.base[c] = e->rampBase[c], q.rate[c] = e->rampRate[c];
+                r.ramp = (int)rr.size();
+                rr.push_back(q);
+            }
             ex.push_back(r);
         }
         if (!bexts.ensure(std::max<size_t>(1, ex.size()) * sizeof(ExtRec))) { err = "device allocation failed"; return false; }
         dp.exts = (const ExtRec*)bexts.p;
         HIPCHK(up((ExtRec*)bexts.p, ex, stream));
-        std::vector<RampRec> rr;
-        for (int v = 0; v < n; v++) {
-            const Ext* e = vext[v].get();
-            if (!e || !e->ramp) continue;
-            RampRec q{};
-            q.ext = extIndex[v];
-            q.vox = v;
-            for (int c = 0; c < 3; c++) q.base[c] = e->rampBase[c], q.rate[c] = e->rampRate[c];
-            rr.push_back(q);
-        }
         nRamps = (int)rr.size();
         if (nRamps) {
             if (!bramps.ensure(rr.size() * sizeof(RampRec))) { err = "device allocation failed"; return false; }

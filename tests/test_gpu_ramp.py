@@ -36,13 +36,20 @@ def porous(make):
     return sim, info["top"]
 
 
+def porous_contacts(make):
+    """the porous block with self-collision (contact steps in batches on the device)"""
+    sim, top = porous(make)
+    sim.enableCollisions(True)
+    return sim, top
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("path", [0, 1], ids=["fused", "split"])
-@pytest.mark.parametrize("model", ["cantilever", "porous"])
+@pytest.mark.parametrize("model", ["cantilever", "porous", "porous-contacts"])
 def test_device_ramp_equals_host_loop(model, path):
     print(f"[ramp] {model} path {path}", flush=True)
-    build = cantilever if model == "cantilever" else porous
-    base, rate = (0.0, 0.0, 1e-5), (0.0, 0.0, -0.05) if model == "porous" else (0.02, 0.0, 0.04)
+    build = {"cantilever": cantilever, "porous": porous, "porous-contacts": porous_contacts}[model]
+    base, rate = (0.0, 0.0, 1e-5), (0.0, 0.0, -0.05) if model != "cantilever" else (0.02, 0.0, 0.04)
     ref, rv = build(B.maker(B.oracle_cr_abi()))
     dt = ref.recommendedTimeStep()
     host_loop(ref, rv, base, rate, dt, 240)

@@ -20,7 +20,7 @@
  * robot's lanes copy whichever ping-pong buffer is stale, so the robot ends the launch with
  * its voxels as before the diverging step and its links as after it.  The first of those
  * copies also saves the link state from before the diverging step, from which the host
- * recomputes that step's end forces after the launch (k_step_fp64<true>).
+ * recomputes that step's end forces after the launch (k_step_fp64<true, ...>).
  *
  * End forces (what CVX_Link::force() returns) are written to the split step's force slots
  * on the launch's last step only.
@@ -166,7 +166,9 @@ __device__ __forceinline__ void link_end(const FusedPtrs& s, int i, int o, int d
 #endif
 constexpr int kLdsVoxMats = 32, kLdsLinkMats = 64;
 
-template <bool ForcesOnly, bool Poisson, bool MatsLds>
+/* Driven: instances for contact-batch steps (the watch-list test on the positions written) and
+   prescribed-displacement ramps; the others carry neither, and none of their registers. */
+template <bool ForcesOnly, bool Poisson, bool MatsLds, bool Driven>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_FUSED64_WAVES, 8))) k_step_fp64(FusedPtrs s0) {
     FusedPtrs s = s0;
     if (MatsLds) {
@@ -195,7 +197,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_FUS
     } else {
         if (d < s.k) {
             frozen_fp64(s, i, rob, d);
-            if (s.haltTest)
+            if (Driven && s.haltTest)
                 halt_if_stale(s.halt, s.k + 1, s.colLast, s.recalcSq, s.vflIn, nv, i, s.vdOut[VPX * nv + i],
                               s.vdOut[VPY * nv + i], s.vdOut[VPZ * nv + i]);
             return;
@@ -203,7 +205,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_FUS
         prevDt = s.k == 0 ? s.rprev[rob] : dt;
         if (i == 0 || s.vrobot[i - 1] != rob) s.rtimeOut[rob] = s.rtimeIn[rob] + dt; /* currentTime += dt */
         /* here, before the link loop, where few registers are live; read back further down */
-        apply_ramp(s.exts, s.ramps, s.vext[i], s.rtimeIn[rob]);
+        if (Driven) apply_ramp(s.exts, s.ramps, s.vext[i], s.rtimeIn[rob]);
     }
     const double* __restrict__ vd = s.vdIn;
     const dvec pos0 = dv(vd[VPX * nv + i], vd[VPY * nv + i], vd[VPZ * nv + i]);
@@ -247,7 +249,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_FUS
     if (ex && (ex->dof & 0x3F) == 0x3F) {
         voxel_pinned(m, ex, s.vijk, nv, iv, s.vdOut);
         s.vflOut[iv] = flags;
-        if (s.haltTest)
+        if (Driven && s.haltTest)
             halt_if_stale(s.halt, s.k + 1, s.colLast, s.recalcSq, s.vflIn, nv, iv, s.vdOut[VPX * nv + iv],
                           s.vdOut[VPY * nv + iv], s.vdOut[VPZ * nv + iv]);
         return;
@@ -273,7 +275,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_FUS
     });
     store_voxel(s.vdOut, nv, iv, pos, lin, ang, q);
     st_nt(&s.vflOut[iv], flags & ~(uint32_t)VF_PS_VALID); /* poissonsStrainInvalid = true (VX_Voxel.cpp:231) */
-    if (s.haltTest) halt_if_stale(s.halt, s.k + 1, s.colLast, s.recalcSq, s.vflIn, nv, iv, pos.x, pos.y, pos.z);
+    if (Driven && s.haltTest) halt_if_stale(s.halt, s.k + 1, s.colLast, s.recalcSq, s.vflIn, nv, iv, pos.x, pos.y, pos.z);
 }
 
 /* after a launch: previousDt of every voxel whose robot completed at least one step */
@@ -305,15 +307,23 @@ void launch_step_fp64(const FusedPtrs& s, int block, bool forcesOnly, hipStream_
     if (s.nv <= 0) return;
     const dim3 grid((unsigned)((s.nv + block - 1) / block));
     const bool lds = VX_MATS_LDS && !forcesOnly && s.nvmats <= kLdsVoxMats && s.nlmats <= kLdsLinkMats;
-    if (s.poisson) {
-        if (forcesOnly) hipLaunchKernelGGL((k_step_fp64<true, true, false>), grid, dim3(block), 0, st, s);
-        else if (lds) hipLaunchKernelGGL((k_step_fp64<false, true, true>), grid, dim3(block), 0, st, s);
-        else hipLaunchKernelGGL((k_step_fp64<false, true, false>), grid, dim3(block), 0, st, s);
-    } else {
-        if (forcesOnly) hipLaunchKernelGGL((k_step_fp64<true, false, false>), grid, dim3(block), 0, st, s);
-        else if (lds) hipLaunchKernelGGL((k_step_fp64<false, false, true>), grid, dim3(block), 0, st, s);
-        else hipLaunchKernelGGL((k_step_fp64<false, false, false>), grid, dim3(block), 0, st, s);
+    const bool driven = s.haltTest || s.ramps;
+    if (forcesOnly) {
+        if (s.poisson) hipLaunchKernelGGL((k_step_fp64<true, true, false, false>), grid, dim3(block), 0, st, s);
+        else hipLaunchKernelGGL((k_step_fp64<true, false, false, false>), grid, dim3(block), 0, st, s);
+        return;
     }
+#define VX_STEP64(P, L)                                                                                  \
+    if (driven) hipLaunchKernelGGL((k_step_fp64<false, P, L, true>), grid, dim3(block), 0, st, s);      \
+    else hipLaunchKernelGGL((k_step_fp64<false, P, L, false>), grid, dim3(block), 0, st, s)
+    if (s.poisson) {
+        if (lds) { VX_STEP64(true, true); }
+        else { VX_STEP64(true, false); }
+    } else {
+        if (lds) { VX_STEP64(false, true); }
+        else { VX_STEP64(false, false); }
+    }
+#undef VX_STEP64
 }
 void launch_fused_prevdt(const int* vrobot, const int* rdiv, const float* rdt, float* vprevdt, int nv, hipStream_t st) {
     if (nv > 0)

@@ -315,7 +315,9 @@ __device__ void frozen_copy(const MixPtrs& s, int i, int rob, int d) {
 #endif
 /* (Staging the material tables in LDS, as the FP64 step does, made this kernel 50 % slower on
    the drop lattice at the same registers and occupancy: 192 -> 291 us.) */
-template <typename T>
+/* Batch: instances for contact-batch steps whose successor is in the batch, which test the
+   watch list on the positions they write; the others carry no part of that test. */
+template <typename T, bool Batch>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_FUSED_WAVES, 8))) k_step_mixed(MixPtrs s) {
     const int i = lane_index(s.xcd);
     const int nv = s.nv;
@@ -325,7 +327,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_FUS
     const int d = s.rdiv[rob];
     if (d < s.k) {
         frozen_copy<T>(s, i, rob, d);
-        if (s.haltTest)
+        if (Batch && s.haltTest)
             halt_if_stale(s.halt, s.k + 1, s.colLast, s.recalcSq, s.vflIn, nv, i, s.vdOut[VPX * nv + i],
                           s.vdOut[VPY * nv + i], s.vdOut[VPZ * nv + i]);
         return;
@@ -413,7 +415,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_FUS
         out[VLX * nv + i] = 0.0; out[VLY * nv + i] = 0.0; out[VLZ * nv + i] = 0.0;
         out[VAX * nv + i] = 0.0; out[VAY * nv + i] = 0.0; out[VAZ * nv + i] = 0.0;
         s.vflOut[i] = flags;
-        if (s.haltTest)
+        if (Batch && s.haltTest)
             halt_if_stale(s.halt, s.k + 1, s.colLast, s.recalcSq, s.vflIn, nv, i, out[VPX * nv + i], out[VPY * nv + i],
                           out[VPZ * nv + i]);
         return;
@@ -565,7 +567,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_FUS
     st_nt(&out[VQW * nv + i], q.w); st_nt(&out[VQX * nv + i], q.x); st_nt(&out[VQY * nv + i], q.y);
     st_nt(&out[VQZ * nv + i], q.z);
     st_nt(&s.vflOut[i], flags);
-    if (s.haltTest) halt_if_stale(s.halt, s.k + 1, s.colLast, s.recalcSq, s.vflIn, nv, i, pos.x, pos.y, pos.z);
+    if (Batch && s.haltTest) halt_if_stale(s.halt, s.k + 1, s.colLast, s.recalcSq, s.vflIn, nv, i, pos.x, pos.y, pos.z);
 }
 
 /* temperatures at each robot's current time, before the first step of an actuated launch */
@@ -583,8 +585,13 @@ void launch_mixed_temps(const MixPtrs& s, float* temp, hipStream_t st) {
 void launch_step_mixed(const MixPtrs& s, bool fp64, int block, hipStream_t st) {
     if (s.nv <= 0) return;
     const dim3 grid((unsigned)((s.nv + block - 1) / block));
-    if (fp64) hipLaunchKernelGGL(k_step_mixed<double>, grid, dim3(block), 0, st, s);
-    else hipLaunchKernelGGL(k_step_mixed<float>, grid, dim3(block), 0, st, s);
+    if (fp64) {
+        if (s.haltTest) hipLaunchKernelGGL((k_step_mixed<double, true>), grid, dim3(block), 0, st, s);
+        else hipLaunchKernelGGL((k_step_mixed<double, false>), grid, dim3(block), 0, st, s);
+    } else {
+        if (s.haltTest) hipLaunchKernelGGL((k_step_mixed<float, true>), grid, dim3(block), 0, st, s);
+        else hipLaunchKernelGGL((k_step_mixed<float, false>), grid, dim3(block), 0, st, s);
+    }
 }
 
 }  // namespace vx

@@ -353,6 +353,8 @@ struct FusedPtrs {
     float recalcSq;
     int nvmats, nlmats;       /* entries of the material tables */
     int storeForces;          /* write the end forces (the launch's last step) */
+    int restConst;            /* 1: rest lengths are constant and held by both halves of the link
+                                 state, so the step does not rewrite them */
     int xcd;
     int poisson;              /* a material has Poisson's ratio != 0 */
     const float* vps;         /* the voxels' Poisson strains for this step (3 * nv) */

@@ -132,7 +132,7 @@ __device__ __forceinline__ void link_end(const FusedPtrs& s, int i, int o, int d
         st_nt(&ld[LA1Z * nl + slot], e.a1v.z);
         st_nt(&ld[LA2X * nl + slot], e.a2v.x); st_nt(&ld[LA2Y * nl + slot], e.a2v.y);
         st_nt(&ld[LA2Z * nl + slot], e.a2v.z);
-        st_nt(&ld[LREST * nl + slot], rest);
+        if (!s.restConst) st_nt(&ld[LREST * nl + slot], rest);
         float* lf = s.lfOut;
         st_nt(&lf[LSTRAIN * nl + slot], e.strain);
         st_nt(&lf[LMAXSTRAIN * nl + slot], e.maxStrain);

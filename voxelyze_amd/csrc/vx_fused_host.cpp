@@ -126,6 +126,9 @@ bool Sim::stepFp64Fused(int nsteps) {
             events.push_back(ev);
         }
     }
+    /* without actuation the rest lengths are constant: the launch copies them into the other
+       half of the link state once (8 B per link slot) and its steps stop rewriting them */
+    fp.restConst = !actOn && nsteps >= 4 && nv() > 0 ? 1 : 0;
     uint64_t key = 0;
     if (graph) {
         FusedPtrs k0 = fp;
@@ -139,6 +142,10 @@ bool Sim::stepFp64Fused(int nsteps) {
             return fusedFp64Finish(nsteps);
         }
         HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    }
+    if (fp.restConst) {
+        const size_t S = (size_t)3 * nv();
+        HIPCHK(hipMemcpyAsync((double*)bldAlt.p + LREST * S, dp.ld + LREST * S, S * 8, hipMemcpyDeviceToDevice, stream));
     }
     bool lastStep = false;
     int launched = 0;

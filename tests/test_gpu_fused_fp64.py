@@ -235,3 +235,55 @@ def test_checkpoint_refuses_a_count_preserving_topology_edit(edit):
     assert sim.checkpointSave(1)
     sim.step(sim.recommendedTimeStep(), 3)
     assert sim.checkpointRestore(1)
+
+
+def bimorph(make):
+    """a 6-voxel-long thermal bimorph (one expanding layer), fixed at x = 0"""
+    sim = make(0.001)
+    m1 = sim.addMaterial(1e6, 1e3)
+    m1.setInternalDamping(1.0)
+    m1.setCte(0.01)
+    m2 = sim.addMaterial(1e7, 1e3)
+    m2.setInternalDamping(1.0)
+    for i in range(6):
+        a = sim.setVoxel(m1, i, 0, 0)
+        b = sim.setVoxel(m2, i, 0, 1)
+        if i == 0:
+            a.external().setFixedAll()
+            b.external().setFixedAll()
+    return sim
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("path", [FUSED, SPLIT], ids=["fused", "split"])
+def test_temperatures_set_between_multi_step_launches(path):
+    """Temperatures set on the host change rest lengths (CVX_Voxel::setTemperature,
+    VX_Voxel.cpp:108-114); a multi-step launch without actuation keeps them in both halves of
+    the fused step's link state instead of rewriting them every step.  Launches of 40, 41, 7,
+    1, 1, 1 and 10 steps with ambient and per-voxel temperature changes between them give the
+    bits of the oracle stepped one doTimeStep at a time."""
+    dev = bimorph(B.maker(B.product_abi(), path))
+    ref = bimorph(B.maker(B.oracle_cr_abi()))
+    dt = ref.recommendedTimeStep()
+    assert dev.recommendedTimeStep() == dt
+
+    def edit(sim, phase):
+        if phase == 0:
+            sim.setAmbientTemperature(5, True)
+        elif phase == 1:
+            sim.voxel(3).setTemperature(12.0)
+            sim.voxel(5).setTemperature(-4.0)
+        elif phase == 2:
+            sim.setAmbientTemperature(-3, True)
+        elif phase == 4:
+            sim.voxel(4).setTemperature(20.0)
+
+    for phase, n in enumerate([40, 41, 7, 1, 1, 1, 10]):
+        edit(dev, phase)
+        edit(ref, phase)
+        done, _ = dev.step(dt, n)
+        assert done == n
+        for _ in range(n):
+            assert ref.doTimeStep(dt)
+        same_state(dev, ref, f"after launch {phase} ({n} steps)")
+    print(f"[fused64] path {path}: temperatures between launches, bit-exact", flush=True)

@@ -355,6 +355,7 @@ struct FusedPtrs {
     int storeForces;          /* write the end forces (the launch's last step) */
     int restConst;            /* 1: rest lengths are constant and held by both halves of the link
                                  state, so the step does not rewrite them */
+    int offsetConst;          /* 1: the same for the strain offsets (every link material linear) */
     int xcd;
     int poisson;              /* a material has Poisson's ratio != 0 */
     const float* vps;         /* the voxels' Poisson strains for this step (3 * nv) */

@@ -136,7 +136,7 @@ __device__ __forceinline__ void link_end(const FusedPtrs& s, int i, int o, int d
         float* lf = s.lfOut;
         st_nt(&lf[LSTRAIN * nl + slot], e.strain);
         st_nt(&lf[LMAXSTRAIN * nl + slot], e.maxStrain);
-        st_nt(&lf[LOFFSET * nl + slot], e.strainOffset);
+        if (!s.offsetConst) st_nt(&lf[LOFFSET * nl + slot], e.strainOffset);
         st_nt(&lf[LSTRESS * nl + slot], e.stress);
         if (Poisson) {
             lf[LTAREA * nl + slot] = old.tArea;

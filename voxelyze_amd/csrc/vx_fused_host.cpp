@@ -129,6 +129,11 @@ bool Sim::stepFp64Fused(int nsteps) {
     /* without actuation the rest lengths are constant: the launch copies them into the other
        half of the link state once (8 B per link slot) and its steps stop rewriting them */
     fp.restConst = !actOn && nsteps >= 4 && nv() > 0 ? 1 : 0;
+    /* a linear material never changes a link's strain offset (VX_Link.cpp:220-249): with every
+       link material linear the offsets are copied the same way (4 B per link slot) */
+    bool allLinear = true;
+    for (const auto& m : lmats) allLinear = allLinear && m->linear;
+    fp.offsetConst = allLinear && nsteps >= 4 && nv() > 0 ? 1 : 0;
     uint64_t key = 0;
     if (graph) {
         FusedPtrs k0 = fp;
@@ -146,6 +151,10 @@ bool Sim::stepFp64Fused(int nsteps) {
     if (fp.restConst) {
         const size_t S = (size_t)3 * nv();
         HIPCHK(hipMemcpyAsync((double*)bldAlt.p + LREST * S, dp.ld + LREST * S, S * 8, hipMemcpyDeviceToDevice, stream));
+    }
+    if (fp.offsetConst) {
+        const size_t S = (size_t)3 * nv();
+        HIPCHK(hipMemcpyAsync((float*)blfAlt.p + LOFFSET * S, dp.lf + LOFFSET * S, S * 4, hipMemcpyDeviceToDevice, stream));
     }
     bool lastStep = false;
     int launched = 0;
